@@ -1,0 +1,268 @@
+/*
+ * sttopt_hip.h -- C ABI of the MI355X (gfx950) solve-and-sensitivity library.
+ *
+ * Every entry point takes plain device pointers, element counts and a HIP stream
+ * (passed as void*), and returns an int status (STTOPT_OK or a negative code; the
+ * message of the last failure on the calling thread is sttopt_last_error()).  No
+ * entry point allocates device memory: the caller owns every buffer, sized with the
+ * *_bytes / *_count helpers below.  All vectors are batched: row b of a batched
+ * array starts at base + b * stride, and a stride of 0 shares one row across the
+ * batch.
+ *
+ * Mesh conventions (sttopt/fem.py:60-95, sttopt/conventions.md:24-53): an
+ * nelx x nely grid of unit Q4 elements, element e at (row e / nelx, col e % nelx)
+ * with row 0 at the top, node (r, c) numbered r * (nelx + 1) + c, d dofs per node
+ * interleaved (dof = d * node + component), element-local node order
+ * bottom-left, bottom-right, top-right, top-left.  Free-dof masks are packed bits:
+ * dof i is free when (mask[i >> 5] >> (i & 31)) & 1.
+ *
+ * Each declaration names the function of the reference (a pure PyTorch package)
+ * whose work it replaces.  The reference has no FFI of its own; the Python modules
+ * of this package bind these entry points with ctypes behind the reference's own
+ * call signatures.
+ */
+#ifndef STTOPT_HIP_H
+#define STTOPT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define STTOPT_OK 0
+#define STTOPT_EINVAL (-1)     /* bad sizes, null pointers, unsupported mode */
+#define STTOPT_EHIP (-2)       /* a HIP runtime call failed */
+#define STTOPT_ENOCONV (-3)    /* PCG reached max_iter with rows above rtol */
+
+#define STTOPT_MAX_LEVELS 16
+
+/* dtype codes */
+#define STTOPT_F64 0
+#define STTOPT_F32 1
+
+/* operator modes for sttopt_apply */
+#define STTOPT_MODE_MATVEC 0   /* y = K u (no projection)                 */
+#define STTOPT_MODE_OPERATOR 1 /* y = P (K P u + rdiag * P u)              */
+
+const char* sttopt_last_error(void);
+int sttopt_version(void);
+
+/* ---------------------------------------------------------------- masks ---- */
+
+/* Pack a bool/uint8 free-dof mask (1 = free) into bits.  Words = (ndof+31)/32.
+ * Replaces torch_fem.free_mask's bool tensor on the device (torch_fem.py:101-107). */
+int sttopt_pack_mask(const uint8_t* mask, int64_t ndof, uint32_t* bits, void* stream);
+
+/* Coarse mask = fine mask sampled at every kx-th node column / ky-th node row
+ * (torch_mg.py:376). */
+int sttopt_coarsen_mask(const uint32_t* fine_bits, int32_t nelx, int32_t nely,
+                        int32_t d, int32_t kx, int32_t ky, uint32_t* coarse_bits,
+                        void* stream);
+
+/* ------------------------------------------------------------ operators ---- */
+
+/* Fine-level matrix-free apply y = sum_e density_e A_e^T KE A_e u, optionally
+ * projected and with a lumped reaction diagonal.
+ * Replaces torch_fem.matvec (torch_fem.py:62-83) in mode MATVEC and
+ * torch_fem.operator (torch_fem.py:135-159) in mode OPERATOR.
+ * ke: (4d)x(4d) row-major doubles (host memory).  mask/rdiag may be NULL. */
+int sttopt_apply(int32_t dtype, int32_t mode, int32_t d, int32_t nelx, int32_t nely,
+                 int32_t rows, const void* u, int64_t u_stride, const void* density,
+                 int64_t density_stride, const double* ke, const uint32_t* mask,
+                 const void* rdiag, int64_t rdiag_stride, void* y, int64_t y_stride,
+                 void* stream);
+
+/* diag(K) (+ rdiag), with 1 at fixed dofs when mask != NULL.
+ * Replaces torch_fem.matvec_diagonal / jacobi_preconditioner_diag
+ * (torch_fem.py:86-98, 162-181). */
+int sttopt_diagonal(int32_t dtype, int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                    const void* density, int64_t density_stride, const double* ke,
+                    const uint32_t* mask, const void* rdiag, int64_t rdiag_stride,
+                    void* out, int64_t out_stride, void* stream);
+
+/* Lumped reaction diagonal: each element adds reaction_e / 4 to each of its dofs.
+ * Replaces torch_fem.lumped_reaction (torch_fem.py:117-132). */
+int sttopt_lumped_reaction(int32_t dtype, int32_t d, int32_t nelx, int32_t nely,
+                           int32_t rows, const void* reaction, int64_t reaction_stride,
+                           void* out, int64_t out_stride, void* stream);
+
+/* Coarse-level apply with one explicit (4d)x(4d) matrix per element, stored
+ * entry-major (structure of arrays): keff[row][a * 4d + b][e].
+ * Replaces torch_mg.keff_matvec + _Level.__call__ (torch_mg.py:179-193, 239-255). */
+int sttopt_apply_keff(int32_t dtype, int32_t d, int32_t nelx, int32_t nely,
+                      int32_t rows, const void* u, int64_t u_stride, const void* keff,
+                      int64_t keff_stride, const uint32_t* mask, void* y,
+                      int64_t y_stride, void* stream);
+
+/* --------------------------------------------------------- multigrid -------- */
+
+typedef struct sttopt_level {
+  int32_t nelx, nely;     /* elements of this level                              */
+  int32_t kx, ky;         /* coarsening factors to the next level; 1,1 at last   */
+  const uint32_t* mask;   /* packed free-dof bits of this level                  */
+  /* level 0: SIMP density (+ optional lumped reaction diagonal), in the V-cycle
+   * precision; coarser levels: entry-major element matrices keff                */
+  const void* density;
+  int64_t density_stride;
+  const void* rdiag;
+  int64_t rdiag_stride;
+  const void* keff;
+  int64_t keff_stride;
+  const void* invdiag;    /* 1 / diag at free dofs, 0 at fixed, V-cycle precision */
+  int64_t invdiag_stride;
+  const double* cinv;     /* coarsest level: dense inverse [rows][n][n], fp64     */
+  int64_t cinv_stride;
+  /* per-level V-cycle work vectors, each [rows][ndof] in the V-cycle precision   */
+  void* vb;
+  void* vx;
+  void* vx2;
+  void* vr;
+  void* ve;               /* ve, vw: only needed when gamma > 1                   */
+  void* vw;
+} sttopt_level;
+
+typedef struct sttopt_mg {
+  int32_t rows;           /* batch rows                                           */
+  int32_t d;              /* dofs per node (2 elasticity, 1 scalar)               */
+  int32_t nlevels;
+  int32_t vdtype;         /* V-cycle precision: STTOPT_F64 or STTOPT_F32          */
+  int32_t n_smooth;
+  int32_t gamma;
+  double omega;
+  double ke[64];          /* reference element matrix, (4d)x(4d) row-major        */
+  /* the fp64 outer operator (level 0)                                            */
+  const double* density64;
+  int64_t density64_stride;
+  const double* rdiag64;
+  int64_t rdiag64_stride;
+  sttopt_level levels[STTOPT_MAX_LEVELS];
+  /* device scratch sized by sttopt_mg_scratch_bytes                              */
+  void* scratch;
+} sttopt_mg;
+
+/* Bytes of device scratch the PCG / V-cycle need for this hierarchy (per-row
+ * scalars, reduction partials, flags).  levels[*].nelx/nely and rows must be set. */
+int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg);
+
+/* Galerkin coarse element matrices of level (l+1) from level l, fp64, entry-major.
+ * Level 0 is given implicitly by density * KE (+ reaction * I/4): pass
+ * fine_keff = NULL and the child products G = S^T KE S (g) and GM = S^T (I/4) S
+ * (gm, may be NULL) as device arrays of (kx*ky) x (4d)^2 doubles, with
+ * density/reaction per fine element.  For coarser levels pass fine_keff and the
+ * interpolation matrices S (device, (kx*ky) x (4d)^2, S[m][p][a]: child m's local
+ * dof p from the coarse element's local dof a).
+ * Replaces the einsums of torch_mg.build_hierarchy (torch_mg.py:355-370). */
+int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t ky,
+                    int32_t rows, const double* density, int64_t density_stride,
+                    const double* reaction, int64_t reaction_stride,
+                    const double* g, const double* gm, const double* fine_keff,
+                    int64_t fine_keff_stride, const double* s, double* coarse_keff,
+                    int64_t coarse_keff_stride, void* stream);
+
+/* Jacobi diagonal of a keff level: 1/diag at free dofs and 0 at fixed (inv != 0),
+ * or diag with 1 at fixed dofs (inv == 0).  Replaces torch_mg.py:377-378. */
+int sttopt_keff_diagonal(int32_t dtype_out, int32_t d, int32_t nelx, int32_t nely,
+                         int32_t rows, const double* keff, int64_t keff_stride,
+                         const uint32_t* mask, int32_t inv, void* out,
+                         int64_t out_stride, void* stream);
+
+/* Inverse of a diagonal with the fixed-dof convention above, with dtype cast. */
+int sttopt_invert_diagonal(int32_t dtype_out, int64_t n, int32_t rows,
+                           const double* diag, int64_t diag_stride,
+                           const uint32_t* mask, void* out, int64_t out_stride,
+                           void* stream);
+
+/* Dense projected operator of the coarsest level: symmetrized, identity rows and
+ * columns at fixed dofs.  out: [rows][n][n], n = ndof of the level.  Either keff
+ * (a coarse level) or density with ke (a device array here) and optional rdiag
+ * (an uncoarsened fine level).
+ * Replaces torch_mg._dense_cholesky's assembly (torch_mg.py:269-289). */
+int sttopt_dense_assemble(int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                          const double* keff, int64_t keff_stride,
+                          const double* density, int64_t density_stride,
+                          const double* rdiag, int64_t rdiag_stride,
+                          const double* ke, const uint32_t* mask, double* out,
+                          int64_t out_stride, void* stream);
+
+/* Grid transfers on the node grid (torch_mg._on_node_grid with _restrict_axis /
+ * _interp_axis, torch_mg.py:75-109).  restrict: coarse = P_c (R fine);
+ * prolong_add: fine += P_f (P coarse).  Masks may be NULL. */
+int sttopt_restrict(int32_t dtype, int32_t d, int32_t cnelx, int32_t cnely,
+                    int32_t kx, int32_t ky, int32_t rows, const void* fine,
+                    int64_t fine_stride, const uint32_t* cmask, void* coarse,
+                    int64_t coarse_stride, void* stream);
+int sttopt_prolong_add(int32_t dtype, int32_t d, int32_t cnelx, int32_t cnely,
+                       int32_t kx, int32_t ky, int32_t rows, const void* coarse,
+                       int64_t coarse_stride, const uint32_t* fmask, void* fine,
+                       int64_t fine_stride, void* stream);
+
+/* One V-cycle z = M(r) on every active row (torch_mg.VCycle.__call__,
+ * torch_mg.py:440-476).  r, z fp64 [rows][ndof0]. */
+int sttopt_vcycle(const sttopt_mg* mg, const double* r, int64_t r_stride, double* z,
+                  int64_t z_stride, void* stream);
+
+/* Apply level l of the hierarchy in the V-cycle precision (torch_mg._Level.__call__). */
+int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v,
+                       int64_t v_stride, void* y, int64_t y_stride, void* stream);
+
+/* Batched multigrid-preconditioned CG on rows of b (torch_fem.pcg with the
+ * torch_mg levels, torch_fem.py:201-343).  x holds the warm start on entry (zeros
+ * for a cold start; its fixed dofs are ignored) and the solution on return.
+ * Rows retire as they converge (per-row device flags): iters[row] receives each
+ * row's iteration count and resid[row] its final relative residual.  flexible != 0
+ * uses the Polak-Ribiere beta (for a V-cycle in fp32).  check_every: iterations
+ * between host checks of the active count.  Returns STTOPT_ENOCONV when a row is
+ * still above rtol after max_iter iterations (x, iters, resid are still written). */
+int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double* x,
+                 int64_t x_stride, double rtol, int32_t max_iter, int32_t flexible,
+                 int32_t check_every, double* work, int32_t* iters, double* resid,
+                 int32_t* n_iter, void* stream);
+
+/* Work vectors sttopt_mgpcg needs: returns the count of doubles ([rows][ndof0] x 5). */
+int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg);
+
+/* ---------------------------------------------------------- adjoints -------- */
+
+/* FemSolve.backward element terms (torch_solve.py:144-152):
+ * grad_density[row][e] = -(lam_e KE) . U_e (if grad_density != NULL),
+ * grad_reaction[row][e] = -sum(lam_e * U_e) / 4 (if grad_reaction != NULL). */
+int sttopt_adjoint_grad(int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                        const double* u, int64_t u_stride, const double* lam,
+                        int64_t lam_stride, const double* ke, double* grad_density,
+                        int64_t gd_stride, double* grad_reaction, int64_t gr_stride,
+                        void* stream);
+
+/* Element strain energy c_e = U_e^T KE U_e, per row (compliance.py:49-59). */
+int sttopt_strain_energy(int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                         const double* u, int64_t u_stride, const double* ke,
+                         double* ce, int64_t ce_stride, void* stream);
+
+/* Space-time compliance prologue (compliance.py:33-46, 137-161, 226-243):
+ * row 0: density = SIMP(x), load = F_ext; row k = 1..nstage: m = H_beta(t_k - t),
+ * xt = x * m, density = SIMP(xt), load_y = -fe/4 * sum over the node's elements
+ * of xt, load_x = 0.  density: [1+nstage][nel], load: [1+nstage][ndof]. */
+int sttopt_stage_prologue(int32_t nelx, int32_t nely, int32_t nstage,
+                          const double* x, const double* t, const double* stage_times,
+                          double beta, double emin, double emax, double penal,
+                          double fe, const double* f_ext, double* density,
+                          double* load, void* stream);
+
+/* Space-time compliance epilogue and its hand-derived adjoint
+ * (compliance.py:244-253; tests/reference/compliance.py:24-110).
+ * comp[row] = sum_e SIMP(xt_row)_e * c_e(U_row).  When dx != NULL, also
+ * dx[e] = sum_rows g[row] dc_row/dx_e and dt[e] = sum_rows g[row] dc_row/dt_e with
+ * the closed forms of the self-adjoint compliance (lambda = 2U), including the
+ * gravity load term -2 (C^T U_y)_e.  scratch: sttopt_stage_scratch_count doubles. */
+int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage,
+                          const double* x, const double* t, const double* stage_times,
+                          double beta, double emin, double emax, double penal,
+                          double fe, const double* ke, const double* u,
+                          const double* g, double* comp, double* dx, double* dt,
+                          double* scratch, void* stream);
+int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* STTOPT_HIP_H */

@@ -1,0 +1,722 @@
+// Matrix-free Q4 operator kernels for gfx950.
+//
+// One thread owns one node and gathers the contributions of the (up to) four
+// elements around it, so the scatter of torch_fem.matvec's index_add_
+// (torch_fem.py:76-83) becomes a deterministic, atomic-free gather.  The element
+// dof map is never read: node and element indices follow from the structured grid.
+// A workgroup is a 64 x 4 node tile (one wavefront per node row); the 3 x 3 node
+// neighbourhood each thread reads is shared with its neighbours through L1, and the
+// tile order is remapped so that each XCD streams a contiguous band of the grid.
+// SIMP scaling, the lumped reaction diagonal, boundary projection and the
+// damped-Jacobi / residual epilogues of the V-cycle are fused into the same pass.
+#include <cstdio>
+#include <mutex>
+
+#include "stt_kernels.h"
+
+namespace stt {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int hip_status(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return STTOPT_EHIP;
+}
+
+template <typename T, typename TB, typename TY>
+struct StencilArgs {
+  int nelx, nely, nnx, nny, ntx, tiles, total;
+  int mode, proj_in, dot;
+  const T* u;
+  long long su;
+  const T* rho;
+  long long sr;
+  const T* keff;
+  long long sk;
+  const T* rdiag;
+  long long srd;
+  const uint32_t* mask;
+  const TB* b;
+  long long sb;
+  const T* invd;
+  long long sd;
+  TY* y;
+  long long sy;
+  T omega;
+  const int* active;
+  double* partial;
+  KE<T> ke;
+};
+
+template <typename T, int D>
+struct Vec {
+  T v[D];
+};
+
+template <typename T, int D>
+__device__ __forceinline__ Vec<T, D> load_node(const T* base, long long node) {
+  Vec<T, D> out;
+  if constexpr (D == 2) {
+    using V2 = HIP_vector_type<T, 2>;
+    const V2 w = *reinterpret_cast<const V2*>(base + 2 * node);
+    out.v[0] = w.x;
+    out.v[1] = w.y;
+  } else {
+    out.v[0] = base[node];
+  }
+  return out;
+}
+
+template <typename T, typename TB, typename TY, int D, bool KEFF>
+__global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> a) {
+  constexpr int K = 4 * D;
+  __shared__ double red[kBlock / kWave];
+  const int t = xcd_remap(blockIdx.x, a.total);
+  const int row = t / a.tiles;
+  const int tt = t - row * a.tiles;
+  if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
+  const int by = tt / a.ntx, bx = tt - by * a.ntx;
+  const int c = bx * kTileX + (threadIdx.x & (kTileX - 1));
+  const int r = by * kTileY + (threadIdx.x / kTileX);
+  double dot = 0.0;
+
+  if (c < a.nnx && r < a.nny) {
+    const T* u = a.u + row * a.su;
+    // 3 x 3 node neighbourhood, zero outside the grid (and at fixed dofs when the
+    // input has to be projected).
+    T U[3][3][D];
+#pragma unroll
+    for (int dr = 0; dr < 3; ++dr) {
+#pragma unroll
+      for (int dc = 0; dc < 3; ++dc) {
+        const int rr = r + dr - 1, cc = c + dc - 1;
+        const bool ok = rr >= 0 && rr < a.nny && cc >= 0 && cc < a.nnx;
+        const long long node = (long long)rr * a.nnx + cc;
+        Vec<T, D> w;
+        if (ok) {
+          w = load_node<T, D>(u, node);
+        } else {
+#pragma unroll
+          for (int k = 0; k < D; ++k) w.v[k] = T(0);
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          T val = w.v[k];
+          if (a.proj_in && ok && !mask_free(a.mask, node * D + k)) val = T(0);
+          U[dr][dc][k] = val;
+        }
+      }
+    }
+
+    T acc[D];
+#pragma unroll
+    for (int i = 0; i < D; ++i) acc[i] = T(0);
+    // q: element up-left, up-right, down-left, down-right of the node; the node is
+    // that element's local BR, BL, TR, TL corner respectively.
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+      if (er < 0 || er >= a.nely || ec < 0 || ec >= a.nelx) continue;
+      const long long e = (long long)er * a.nelx + ec;
+      constexpr int locs[4] = {1, 0, 2, 3};
+      const int loc = locs[q];
+      if constexpr (!KEFF) {
+        T part[D];
+#pragma unroll
+        for (int i = 0; i < D; ++i) part[i] = T(0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int ur = (q >> 1) + loc_dr(j), uc = (q & 1) + loc_dc(j);
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+              part[i] += a.ke.v[(loc * D + i) * K + j * D + k] * U[ur][uc][k];
+          }
+        }
+        const T s = a.rho[row * a.sr + e];
+#pragma unroll
+        for (int i = 0; i < D; ++i) acc[i] += s * part[i];
+      } else {
+        const long long nel = (long long)a.nelx * a.nely;
+        const T* ke_e = a.keff + row * a.sk + e;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int ur = (q >> 1) + loc_dr(j), uc = (q & 1) + loc_dc(j);
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+              acc[i] += ke_e[(long long)((loc * D + i) * K + j * D + k) * nel] * U[ur][uc][k];
+          }
+        }
+      }
+    }
+
+    const long long node = (long long)r * a.nnx + c;
+    if (a.rdiag != nullptr) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) acc[i] += a.rdiag[row * a.srd + node * D + i] * U[1][1][i];
+    }
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const long long dof = node * D + i;
+      const bool fr = a.mode == kMatvec || mask_free(a.mask, dof);
+      T out = T(0);
+      TB bval = TB(0);
+      if (a.mode == kSmooth || a.mode == kResid) bval = a.b[row * a.sb + dof];
+      if (fr) {
+        if (a.mode == kMatvec || a.mode == kOperator) {
+          out = acc[i];
+        } else if (a.mode == kResid) {
+          out = static_cast<T>(bval) - acc[i];
+        } else {  // kSmooth
+          out = U[1][1][i] + a.omega * a.invd[row * a.sd + dof] * (static_cast<T>(bval) - acc[i]);
+        }
+      }
+      const TY yo = static_cast<TY>(out);
+      a.y[row * a.sy + dof] = yo;
+      if (a.dot == kDotU) dot += (double)U[1][1][i] * (double)yo;
+      if (a.dot == kDotB) dot += (double)bval * (double)yo;
+    }
+  }
+  if (a.dot != kNoDot) {
+    const double s = block_sum(dot, red);
+    if (threadIdx.x == 0) a.partial[(long long)row * a.tiles + tt] = s;
+  }
+}
+
+template <typename T, typename TB, typename TY>
+int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
+            long long su, const TB* b, long long sb, const T* invd, long long sd,
+            TY* y, long long sy, double omega, const int* active, double* partial,
+            hipStream_t s) {
+  if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
+  StencilArgs<T, TB, TY> a;
+  a.nelx = op.nelx;
+  a.nely = op.nely;
+  a.nnx = op.nnx();
+  a.nny = op.nny();
+  a.ntx = op.ntx();
+  a.tiles = op.tiles();
+  const long long total = (long long)a.tiles * rows;
+  if (total > 0x7fffffffLL) {
+    set_error("stencil: grid too large");
+    return STTOPT_EINVAL;
+  }
+  a.total = (int)total;
+  a.mode = mode;
+  a.proj_in = proj_in ? 1 : 0;
+  a.dot = dot;
+  a.u = u;
+  a.su = su;
+  a.rho = static_cast<const T*>(op.rho);
+  a.sr = op.sr;
+  a.keff = static_cast<const T*>(op.keff);
+  a.sk = op.sk;
+  a.rdiag = static_cast<const T*>(op.rdiag);
+  a.srd = op.srd;
+  a.mask = op.mask;
+  a.b = b;
+  a.sb = sb;
+  a.invd = invd;
+  a.sd = sd;
+  a.y = y;
+  a.sy = sy;
+  a.omega = static_cast<T>(omega);
+  a.active = active;
+  a.partial = partial;
+  a.ke = make_ke<T>(op.ke, op.d);
+  const bool keff = op.keff != nullptr;
+  if (!keff && op.rho == nullptr) {
+    set_error("stencil: level has neither density nor keff");
+    return STTOPT_EINVAL;
+  }
+  dim3 grid((unsigned)a.total), block(kBlock);
+  if (op.d == 2) {
+    if (keff)
+      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 2, true>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 2, false>), grid, block, 0, s, a);
+  } else if (op.d == 1) {
+    if (keff)
+      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 1, true>), grid, block, 0, s, a);
+    else
+      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 1, false>), grid, block, 0, s, a);
+  } else {
+    set_error("stencil: d must be 1 or 2");
+    return STTOPT_EINVAL;
+  }
+  STT_CHECK_LAUNCH("stencil_kernel");
+  return STTOPT_OK;
+}
+
+template int stencil<double, double, double>(const OpDesc&, int, int, bool, int,
+                                             const double*, long long, const double*,
+                                             long long, const double*, long long,
+                                             double*, long long, double, const int*,
+                                             double*, hipStream_t);
+template int stencil<float, float, float>(const OpDesc&, int, int, bool, int,
+                                          const float*, long long, const float*,
+                                          long long, const float*, long long, float*,
+                                          long long, double, const int*, double*,
+                                          hipStream_t);
+template int stencil<float, double, double>(const OpDesc&, int, int, bool, int,
+                                            const float*, long long, const double*,
+                                            long long, const float*, long long,
+                                            double*, long long, double, const int*,
+                                            double*, hipStream_t);
+
+// ------------------------------------------------------------------ diagonal --
+
+template <typename TO, int D, bool KEFF>
+__global__ __launch_bounds__(256) void diagonal_kernel(int nelx, int nely, int rows,
+                                                       const double* rho, long long sr,
+                                                       const double* keff, long long sk,
+                                                       const double* rdiag, long long srd,
+                                                       const uint32_t* mask, KE<double> ke,
+                                                       int out_mode, TO* out, long long so) {
+  constexpr int K = 4 * D;
+  const int nnx = nelx + 1, nny = nely + 1;
+  const long long nnodes = (long long)nnx * nny;
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (node >= nnodes) return;
+  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
+  double acc[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) acc[i] = 0.0;
+  const long long nel = (long long)nelx * nely;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+    if (er < 0 || er >= nely || ec < 0 || ec >= nelx) continue;
+    const long long e = (long long)er * nelx + ec;
+    constexpr int locs[4] = {1, 0, 2, 3};
+    const int loc = locs[q];
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const int a = loc * D + i;
+      if constexpr (KEFF)
+        acc[i] += keff[row * sk + (long long)(a * K + a) * nel + e];
+      else
+        acc[i] += rho[row * sr + e] * ke.v[a * K + a];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const long long dof = node * D + i;
+    double v = acc[i];
+    if (rdiag != nullptr) v += rdiag[row * srd + dof];
+    const bool fr = mask_free(mask, dof);
+    if (out_mode == 1 && !fr) v = 1.0;
+    if (out_mode == 2) v = fr ? 1.0 / v : 0.0;
+    out[row * so + dof] = static_cast<TO>(v);
+  }
+}
+
+template <typename TO>
+int diagonal(const OpDesc& op, int rows, int out_mode, TO* out, long long so,
+             hipStream_t s) {
+  if (rows <= 0) return STTOPT_OK;
+  dim3 grid(ceil_div(op.nnodes(), 256), rows), block(256);
+  const KE<double> ke = make_ke<double>(op.ke, op.d);
+  const double* rho = static_cast<const double*>(op.rho);
+  const double* keff = static_cast<const double*>(op.keff);
+  const double* rd = static_cast<const double*>(op.rdiag);
+#define STT_DIAG(D, KF)                                                                 \
+  hipLaunchKernelGGL((diagonal_kernel<TO, D, KF>), grid, block, 0, s, op.nelx, op.nely, \
+                     rows, rho, op.sr, keff, op.sk, rd, op.srd, op.mask, ke, out_mode,  \
+                     out, so)
+  if (op.d == 2) {
+    if (keff) STT_DIAG(2, true); else STT_DIAG(2, false);
+  } else {
+    if (keff) STT_DIAG(1, true); else STT_DIAG(1, false);
+  }
+#undef STT_DIAG
+  STT_CHECK_LAUNCH("diagonal_kernel");
+  return STTOPT_OK;
+}
+template int diagonal<double>(const OpDesc&, int, int, double*, long long, hipStream_t);
+template int diagonal<float>(const OpDesc&, int, int, float*, long long, hipStream_t);
+
+// ---------------------------------------------------------- lumped reaction --
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void lumped_kernel(int nelx, int nely, const T* react,
+                                                     long long sr, T* out, long long so) {
+  const int nnx = nelx + 1, nny = nely + 1;
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (node >= (long long)nnx * nny) return;
+  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
+  T acc = T(0);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+    if (er < 0 || er >= nely || ec < 0 || ec >= nelx) continue;
+    acc += react[row * sr + (long long)er * nelx + ec] / T(4);
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) out[row * so + node * D + i] = acc;
+}
+
+// ------------------------------------------------------------- elementwise --
+
+template <typename T, typename TB>
+__global__ __launch_bounds__(256) void jacobi_first_kernel(long long n, const TB* b,
+                                                           long long sb, const T* invd,
+                                                           long long sd, T omega,
+                                                           const uint32_t* mask, T* x,
+                                                           long long sx, T* bcopy,
+                                                           long long sbc, const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const T bv = static_cast<T>(b[row * sb + i]);
+    if (bcopy != nullptr) bcopy[row * sbc + i] = bv;
+    x[row * sx + i] = mask_free(mask, i) ? omega * invd[row * sd + i] * bv : T(0);
+  }
+}
+
+static inline dim3 elem_grid(long long n, int rows) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  return dim3((unsigned)blocks, rows);
+}
+
+template <typename T, typename TB>
+int jacobi_first(long long n, int rows, const TB* b, long long sb, const T* invd,
+                 long long sd, double omega, const uint32_t* mask, T* x, long long sx,
+                 T* bcopy, long long sbc, const int* active, hipStream_t s) {
+  if (rows <= 0 || n <= 0) return STTOPT_OK;
+  hipLaunchKernelGGL((jacobi_first_kernel<T, TB>), elem_grid(n, rows), dim3(256), 0, s, n,
+                     b, sb, invd, sd, static_cast<T>(omega), mask, x, sx, bcopy, sbc,
+                     active);
+  STT_CHECK_LAUNCH("jacobi_first_kernel");
+  return STTOPT_OK;
+}
+template int jacobi_first<double, double>(long long, int, const double*, long long,
+                                          const double*, long long, double,
+                                          const uint32_t*, double*, long long, double*,
+                                          long long, const int*, hipStream_t);
+template int jacobi_first<float, float>(long long, int, const float*, long long,
+                                        const float*, long long, double, const uint32_t*,
+                                        float*, long long, float*, long long, const int*,
+                                        hipStream_t);
+template int jacobi_first<float, double>(long long, int, const double*, long long,
+                                         const float*, long long, double, const uint32_t*,
+                                         float*, long long, float*, long long, const int*,
+                                         hipStream_t);
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void convert_kernel(long long n, const TI* in, long long si,
+                                                      TO* out, long long so,
+                                                      const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[row * so + i] = static_cast<TO>(in[row * si + i]);
+}
+
+template <typename TI, typename TO>
+int convert(long long n, int rows, const TI* in, long long si, TO* out, long long so,
+            const int* active, hipStream_t s) {
+  if (rows <= 0 || n <= 0) return STTOPT_OK;
+  hipLaunchKernelGGL((convert_kernel<TI, TO>), elem_grid(n, rows), dim3(256), 0, s, n, in,
+                     si, out, so, active);
+  STT_CHECK_LAUNCH("convert_kernel");
+  return STTOPT_OK;
+}
+template int convert<double, double>(long long, int, const double*, long long, double*,
+                                     long long, const int*, hipStream_t);
+template int convert<float, double>(long long, int, const float*, long long, double*,
+                                    long long, const int*, hipStream_t);
+template int convert<double, float>(long long, int, const double*, long long, float*,
+                                    long long, const int*, hipStream_t);
+template int convert<float, float>(long long, int, const float*, long long, float*,
+                                   long long, const int*, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void fill_kernel(long long n, T value, T* x, long long sx,
+                                                   const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    x[row * sx + i] = value;
+}
+
+template <typename T>
+int fill(long long n, int rows, T value, T* x, long long sx, const int* active,
+         hipStream_t s) {
+  if (rows <= 0 || n <= 0) return STTOPT_OK;
+  hipLaunchKernelGGL((fill_kernel<T>), elem_grid(n, rows), dim3(256), 0, s, n, value, x, sx,
+                     active);
+  STT_CHECK_LAUNCH("fill_kernel");
+  return STTOPT_OK;
+}
+template int fill<double>(long long, int, double, double*, long long, const int*,
+                          hipStream_t);
+template int fill<float>(long long, int, float, float*, long long, const int*, hipStream_t);
+template int fill<int>(long long, int, int, int*, long long, const int*, hipStream_t);
+
+template <typename T>
+__global__ __launch_bounds__(256) void axpy_kernel(long long n, const T* xin, long long si,
+                                                   T* y, long long sy, const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    y[row * sy + i] += xin[row * si + i];
+}
+
+template <typename T>
+int axpy(long long n, int rows, const T* xin, long long si, T* y, long long sy,
+         const int* active, hipStream_t s) {
+  if (rows <= 0 || n <= 0) return STTOPT_OK;
+  hipLaunchKernelGGL((axpy_kernel<T>), elem_grid(n, rows), dim3(256), 0, s, n, xin, si, y,
+                     sy, active);
+  STT_CHECK_LAUNCH("axpy_kernel");
+  return STTOPT_OK;
+}
+template int axpy<double>(long long, int, const double*, long long, double*, long long,
+                          const int*, hipStream_t);
+template int axpy<float>(long long, int, const float*, long long, float*, long long,
+                         const int*, hipStream_t);
+
+__global__ __launch_bounds__(256) void project_kernel(long long n, const uint32_t* mask,
+                                                      double* x, long long sx) {
+  const int row = blockIdx.y;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    if (!mask_free(mask, i)) x[row * sx + i] = 0.0;
+}
+
+int project_f64(long long n, int rows, const uint32_t* mask, double* x, long long sx,
+                hipStream_t s) {
+  if (rows <= 0 || n <= 0 || mask == nullptr) return STTOPT_OK;
+  hipLaunchKernelGGL(project_kernel, elem_grid(n, rows), dim3(256), 0, s, n, mask, x, sx);
+  STT_CHECK_LAUNCH("project_kernel");
+  return STTOPT_OK;
+}
+
+// ---------------------------------------------------------------- masks ------
+
+__global__ __launch_bounds__(256) void pack_mask_kernel(const uint8_t* m, long long n,
+                                                        uint32_t* bits) {
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long words = (n + 31) >> 5;
+  if (w >= words) return;
+  uint32_t v = 0;
+  for (int k = 0; k < 32; ++k) {
+    const long long i = (w << 5) + k;
+    if (i < n && m[i] != 0) v |= (1u << k);
+  }
+  bits[w] = v;
+}
+
+__global__ __launch_bounds__(256) void coarsen_mask_kernel(const uint32_t* fine, int nelx,
+                                                           int d, int kx, int ky,
+                                                           int cnnx, int cnny,
+                                                           uint32_t* coarse) {
+  const long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long n = (long long)cnnx * cnny * d;
+  const long long words = (n + 31) >> 5;
+  if (w >= words) return;
+  uint32_t v = 0;
+  for (int k = 0; k < 32; ++k) {
+    const long long i = (w << 5) + k;
+    if (i >= n) break;
+    const long long node = i / d;
+    const int comp = (int)(i - node * d);
+    const int R = (int)(node / cnnx), C = (int)(node - (long long)R * cnnx);
+    const long long fnode = (long long)(R * ky) * (nelx + 1) + (long long)C * kx;
+    if (mask_free(fine, fnode * d + comp)) v |= (1u << k);
+  }
+  coarse[w] = v;
+}
+
+}  // namespace stt
+
+// ===================================================================== C ABI ==
+
+using namespace stt;
+
+extern "C" {
+
+const char* sttopt_last_error(void) { return g_last_error.c_str(); }
+
+int sttopt_version(void) { return 1; }
+
+int sttopt_pack_mask(const uint8_t* mask, int64_t ndof, uint32_t* bits, void* stream) {
+  if (mask == nullptr || bits == nullptr || ndof < 0) {
+    set_error("sttopt_pack_mask: null pointer or negative size");
+    return STTOPT_EINVAL;
+  }
+  const long long words = (ndof + 31) >> 5;
+  if (words == 0) return STTOPT_OK;
+  hipLaunchKernelGGL(pack_mask_kernel, dim3(ceil_div(words, 256)), dim3(256), 0,
+                     (hipStream_t)stream, mask, (long long)ndof, bits);
+  STT_CHECK_LAUNCH("pack_mask_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_coarsen_mask(const uint32_t* fine_bits, int32_t nelx, int32_t nely, int32_t d,
+                        int32_t kx, int32_t ky, uint32_t* coarse_bits, void* stream) {
+  if (fine_bits == nullptr || coarse_bits == nullptr || kx < 1 || ky < 1 ||
+      nelx % kx != 0 || nely % ky != 0 || (d != 1 && d != 2)) {
+    set_error("sttopt_coarsen_mask: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  const int cnnx = nelx / kx + 1, cnny = nely / ky + 1;
+  const long long words = ((long long)cnnx * cnny * d + 31) >> 5;
+  hipLaunchKernelGGL(coarsen_mask_kernel, dim3(ceil_div(words, 256)), dim3(256), 0,
+                     (hipStream_t)stream, fine_bits, nelx, d, kx, ky, cnnx, cnny,
+                     coarse_bits);
+  STT_CHECK_LAUNCH("coarsen_mask_kernel");
+  return STTOPT_OK;
+}
+
+static int check_grid(int32_t d, int32_t nelx, int32_t nely, int32_t rows, const char* who) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || rows < 0) {
+    set_error(std::string(who) + ": invalid grid (d, nelx, nely, rows)");
+    return STTOPT_EINVAL;
+  }
+  return STTOPT_OK;
+}
+
+int sttopt_apply(int32_t dtype, int32_t mode, int32_t d, int32_t nelx, int32_t nely,
+                 int32_t rows, const void* u, int64_t u_stride, const void* density,
+                 int64_t density_stride, const double* ke, const uint32_t* mask,
+                 const void* rdiag, int64_t rdiag_stride, void* y, int64_t y_stride,
+                 void* stream) {
+  if (int st = check_grid(d, nelx, nely, rows, "sttopt_apply")) return st;
+  if (u == nullptr || density == nullptr || ke == nullptr || y == nullptr ||
+      (mode != STTOPT_MODE_MATVEC && mode != STTOPT_MODE_OPERATOR)) {
+    set_error("sttopt_apply: null pointer or bad mode");
+    return STTOPT_EINVAL;
+  }
+  OpDesc op;
+  op.d = d;
+  op.nelx = nelx;
+  op.nely = nely;
+  op.rho = density;
+  op.sr = density_stride;
+  op.rdiag = mode == STTOPT_MODE_OPERATOR ? rdiag : nullptr;
+  op.srd = rdiag_stride;
+  op.mask = mode == STTOPT_MODE_OPERATOR ? mask : nullptr;
+  for (int i = 0; i < 16 * d * d; ++i) op.ke[i] = ke[i];
+  const int m = mode == STTOPT_MODE_MATVEC ? kMatvec : kOperator;
+  const bool proj = mode == STTOPT_MODE_OPERATOR && mask != nullptr;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == STTOPT_F64)
+    return stencil<double, double, double>(op, rows, m, proj, kNoDot,
+                                           static_cast<const double*>(u), u_stride, nullptr,
+                                           0, nullptr, 0, static_cast<double*>(y), y_stride,
+                                           0.0, nullptr, nullptr, s);
+  if (dtype == STTOPT_F32)
+    return stencil<float, float, float>(op, rows, m, proj, kNoDot,
+                                        static_cast<const float*>(u), u_stride, nullptr, 0,
+                                        nullptr, 0, static_cast<float*>(y), y_stride, 0.0,
+                                        nullptr, nullptr, s);
+  set_error("sttopt_apply: dtype must be STTOPT_F64 or STTOPT_F32");
+  return STTOPT_EINVAL;
+}
+
+int sttopt_apply_keff(int32_t dtype, int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                      const void* u, int64_t u_stride, const void* keff, int64_t keff_stride,
+                      const uint32_t* mask, void* y, int64_t y_stride, void* stream) {
+  if (int st = check_grid(d, nelx, nely, rows, "sttopt_apply_keff")) return st;
+  if (u == nullptr || keff == nullptr || y == nullptr) {
+    set_error("sttopt_apply_keff: null pointer");
+    return STTOPT_EINVAL;
+  }
+  OpDesc op;
+  op.d = d;
+  op.nelx = nelx;
+  op.nely = nely;
+  op.keff = keff;
+  op.sk = keff_stride;
+  op.mask = mask;
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == STTOPT_F64)
+    return stencil<double, double, double>(op, rows, kOperator, mask != nullptr, kNoDot,
+                                           static_cast<const double*>(u), u_stride, nullptr,
+                                           0, nullptr, 0, static_cast<double*>(y), y_stride,
+                                           0.0, nullptr, nullptr, s);
+  if (dtype == STTOPT_F32)
+    return stencil<float, float, float>(op, rows, kOperator, mask != nullptr, kNoDot,
+                                        static_cast<const float*>(u), u_stride, nullptr, 0,
+                                        nullptr, 0, static_cast<float*>(y), y_stride, 0.0,
+                                        nullptr, nullptr, s);
+  set_error("sttopt_apply_keff: dtype must be STTOPT_F64 or STTOPT_F32");
+  return STTOPT_EINVAL;
+}
+
+int sttopt_diagonal(int32_t dtype, int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                    const void* density, int64_t density_stride, const double* ke,
+                    const uint32_t* mask, const void* rdiag, int64_t rdiag_stride, void* out,
+                    int64_t out_stride, void* stream) {
+  if (int st = check_grid(d, nelx, nely, rows, "sttopt_diagonal")) return st;
+  if (density == nullptr || ke == nullptr || out == nullptr || dtype != STTOPT_F64) {
+    set_error("sttopt_diagonal: null pointer or dtype other than fp64");
+    return STTOPT_EINVAL;
+  }
+  OpDesc op;
+  op.d = d;
+  op.nelx = nelx;
+  op.nely = nely;
+  op.rho = density;
+  op.sr = density_stride;
+  op.rdiag = rdiag;
+  op.srd = rdiag_stride;
+  op.mask = mask;
+  for (int i = 0; i < 16 * d * d; ++i) op.ke[i] = ke[i];
+  return diagonal<double>(op, rows, mask != nullptr ? 1 : 0, static_cast<double*>(out),
+                          out_stride, (hipStream_t)stream);
+}
+
+int sttopt_lumped_reaction(int32_t dtype, int32_t d, int32_t nelx, int32_t nely,
+                           int32_t rows, const void* reaction, int64_t reaction_stride,
+                           void* out, int64_t out_stride, void* stream) {
+  if (int st = check_grid(d, nelx, nely, rows, "sttopt_lumped_reaction")) return st;
+  if (reaction == nullptr || out == nullptr || rows == 0) {
+    if (rows == 0) return STTOPT_OK;
+    set_error("sttopt_lumped_reaction: null pointer");
+    return STTOPT_EINVAL;
+  }
+  const long long nnodes = (long long)(nelx + 1) * (nely + 1);
+  dim3 grid(ceil_div(nnodes, 256), rows), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == STTOPT_F64) {
+    if (d == 2)
+      hipLaunchKernelGGL((lumped_kernel<double, 2>), grid, block, 0, s, nelx, nely,
+                         static_cast<const double*>(reaction), reaction_stride,
+                         static_cast<double*>(out), out_stride);
+    else
+      hipLaunchKernelGGL((lumped_kernel<double, 1>), grid, block, 0, s, nelx, nely,
+                         static_cast<const double*>(reaction), reaction_stride,
+                         static_cast<double*>(out), out_stride);
+  } else if (dtype == STTOPT_F32) {
+    if (d == 2)
+      hipLaunchKernelGGL((lumped_kernel<float, 2>), grid, block, 0, s, nelx, nely,
+                         static_cast<const float*>(reaction), reaction_stride,
+                         static_cast<float*>(out), out_stride);
+    else
+      hipLaunchKernelGGL((lumped_kernel<float, 1>), grid, block, 0, s, nelx, nely,
+                         static_cast<const float*>(reaction), reaction_stride,
+                         static_cast<float*>(out), out_stride);
+  } else {
+    set_error("sttopt_lumped_reaction: bad dtype");
+    return STTOPT_EINVAL;
+  }
+  STT_CHECK_LAUNCH("lumped_kernel");
+  return STTOPT_OK;
+}
+
+}  // extern "C"

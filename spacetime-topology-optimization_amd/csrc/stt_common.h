@@ -1,0 +1,84 @@
+// Shared device helpers for the gfx950 solve-and-sensitivity kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/sttopt_hip.h"
+
+namespace stt {
+
+constexpr int kWave = 64;
+// Node tiles of the stencil kernels: 64 nodes along x (one wavefront) by 4 node
+// rows (one wavefront each).  A tile's four waves share their halo rows in L1.
+constexpr int kTileX = 64;
+constexpr int kTileY = 4;
+constexpr int kBlock = kTileX * kTileY;
+
+void set_error(const std::string& msg);
+int hip_status(hipError_t e, const char* what);
+
+#define STT_CHECK_LAUNCH(what)                                    \
+  do {                                                            \
+    hipError_t e__ = hipGetLastError();                           \
+    if (e__ != hipSuccess) return ::stt::hip_status(e__, what);   \
+  } while (0)
+
+// (4d) x (4d) reference element matrix, passed by value in the kernel arguments.
+template <typename T>
+struct KE {
+  T v[64];
+};
+
+template <typename T>
+inline KE<T> make_ke(const double* ke, int d) {
+  KE<T> k{};
+  const int n = 4 * d;
+  for (int i = 0; i < n * n; ++i) k.v[i] = static_cast<T>(ke[i]);
+  return k;
+}
+
+__device__ __forceinline__ bool mask_free(const uint32_t* bits, long long dof) {
+  return bits == nullptr || ((bits[dof >> 5] >> (dof & 31)) & 1u);
+}
+
+// XCD-aware workgroup order.  Workgroups are handed to the eight XCDs
+// round-robin; this remaps the launch index so that each XCD walks one contiguous
+// run of tiles, which keeps the halo rows a tile shares with its neighbours in
+// that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int L, int total) {
+  const int xcd = L & 7, local = L >> 3;
+  const int q = total >> 3, r = total & 7;
+  return xcd * q + min(xcd, r) + local;
+}
+
+// Element-local node order BL, BR, TR, TL as (row, col) offsets from the
+// element's top-left node.
+__device__ __forceinline__ int loc_dr(int j) { return (j == 0 || j == 1) ? 1 : 0; }
+__device__ __forceinline__ int loc_dc(int j) { return (j == 1 || j == 2) ? 1 : 0; }
+
+// Wave and block reductions in fp64 (fixed order: deterministic).
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;
+}
+
+// Sum over a block of kBlock threads; result valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* lds /* >= kBlock/kWave */) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  if (lane == 0) lds[w] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0) {
+    for (int i = 0; i < (int)(blockDim.x / kWave); ++i) s += lds[i];
+  }
+  return s;
+}
+
+inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace stt

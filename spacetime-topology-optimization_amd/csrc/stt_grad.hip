@@ -1,0 +1,346 @@
+// Element-wise adjoint and objective kernels: the FemSolve.backward element terms
+// (torch_solve.py:144-152), the strain energy (compliance.py:49-59), and the fused
+// space-time compliance prologue / epilogue with its hand-derived adjoint
+// (compliance.py:33-46, 137-161, 198-253; tests/reference/compliance.py:24-110).
+#include "stt_kernels.h"
+
+namespace stt {
+
+// Gathers the 4d element values of element (er, ec) in local order BL, BR, TR, TL.
+template <int D>
+__device__ __forceinline__ void gather_elem(const double* u, int nnx, int er, int ec,
+                                            double* ue) {
+  const long long tl = (long long)er * nnx + ec;
+  const long long nodes[4] = {tl + nnx, tl + nnx + 1, tl + 1, tl};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int k = 0; k < D; ++k) ue[j * D + k] = u[nodes[j] * D + k];
+  }
+}
+
+// sum_i (sum_j a_j KE[j][i]) b_i  -- the reference's (a_e @ KE) . b_e
+template <int D>
+__device__ __forceinline__ double ke_form(const KE<double>& ke, const double* a,
+                                          const double* b) {
+  constexpr int K = 4 * D;
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    double t = 0.0;
+#pragma unroll
+    for (int j = 0; j < K; ++j) t += a[j] * ke.v[j * K + i];
+    s += t * b[i];
+  }
+  return s;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void adjoint_grad_kernel(int nelx, int nely,
+                                                           const double* u, long long su,
+                                                           const double* lam, long long sl,
+                                                           KE<double> ke, double* gd,
+                                                           long long sgd, double* gr,
+                                                           long long sgr) {
+  constexpr int K = 4 * D;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (e >= (long long)nelx * nely) return;
+  const int er = (int)(e / nelx), ec = (int)(e - (long long)er * nelx);
+  double ue[K], le[K];
+  gather_elem<D>(u + row * su, nelx + 1, er, ec, ue);
+  gather_elem<D>(lam + row * sl, nelx + 1, er, ec, le);
+  if (gd != nullptr) gd[row * sgd + e] = -ke_form<D>(ke, le, ue);
+  if (gr != nullptr) {
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) s += le[i] * ue[i];
+    gr[row * sgr + e] = -s / 4.0;
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void strain_energy_kernel(int nelx, int nely,
+                                                            const double* u, long long su,
+                                                            KE<double> ke, double* ce,
+                                                            long long sce) {
+  constexpr int K = 4 * D;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (e >= (long long)nelx * nely) return;
+  const int er = (int)(e / nelx), ec = (int)(e - (long long)er * nelx);
+  double ue[K];
+  gather_elem<D>(u + row * su, nelx + 1, er, ec, ue);
+  ce[row * sce + e] = ke_form<D>(ke, ue, ue);
+}
+
+// ------------------------------------------------------- space-time stages --
+
+struct StageParams {
+  int nelx, nely, nstage;
+  double beta, emin, emax, penal, fe;
+};
+
+// H_beta(t_k - t) (compliance.time_mask) and its t-derivative
+// (tests/reference/compliance.time_mask_derivative), zero where the clamp acts.
+__device__ __forceinline__ void stage_mask(double t, double tk, double beta, double& m,
+                                           double& dm) {
+  const double a = tanh(beta * tk);
+  const double th = tanh(beta * (t - tk));
+  const double den = a + tanh(beta * (1.0 - tk));
+  const double v = 1.0 - (a + th) / den;
+  if (v >= 0.0) {
+    m = v;
+    dm = -beta * (1.0 - th * th) / den;
+  } else {
+    m = 0.0;
+    dm = 0.0;
+  }
+}
+
+__device__ __forceinline__ double simp(double x, const StageParams& p) {
+  return p.emin + pow(x, p.penal) * (p.emax - p.emin);
+}
+
+__global__ __launch_bounds__(256) void stage_density_kernel(StageParams p, const double* x,
+                                                            const double* t,
+                                                            const double* times,
+                                                            double* density) {
+  const long long nel = (long long)p.nelx * p.nely;
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (e >= nel) return;
+  double xe = x[e];
+  if (row > 0) {
+    double m, dm;
+    stage_mask(t[e], times[row - 1], p.beta, m, dm);
+    xe *= m;
+  }
+  density[row * nel + e] = simp(xe, p);
+}
+
+// Self-weight load of stage rows (row 0 copies the external load):
+// F_y(node) = -fe/4 * sum of x*m over the node's elements, F_x = 0.
+__global__ __launch_bounds__(256) void stage_load_kernel(StageParams p, const double* x,
+                                                         const double* t, const double* times,
+                                                         const double* fext, double* load) {
+  const int nnx = p.nelx + 1, nny = p.nely + 1;
+  const long long nnodes = (long long)nnx * nny;
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (node >= nnodes) return;
+  const long long ndof = 2 * nnodes;
+  if (row == 0) {
+    load[2 * node] = fext[2 * node];
+    load[2 * node + 1] = fext[2 * node + 1];
+    return;
+  }
+  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
+  const double tk = times[row - 1];
+  double acc = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+    if (er < 0 || er >= p.nely || ec < 0 || ec >= p.nelx) continue;
+    const long long e = (long long)er * p.nelx + ec;
+    double m, dm;
+    stage_mask(t[e], tk, p.beta, m, dm);
+    acc += (p.fe / 4.0) * (x[e] * m);
+  }
+  load[row * ndof + 2 * node] = 0.0;
+  load[row * ndof + 2 * node + 1] = -acc;
+}
+
+// Per-row compliance partials: sum_e SIMP(xt_row)_e * (U_e^T KE U_e).
+__global__ __launch_bounds__(256) void stage_comp_kernel(StageParams p, const double* x,
+                                                         const double* t, const double* times,
+                                                         KE<double> ke, const double* u,
+                                                         double* part, int nblk) {
+  __shared__ double red[4];
+  const long long nel = (long long)p.nelx * p.nely;
+  const long long ndof = 2LL * (p.nelx + 1) * (p.nely + 1);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  double v = 0.0;
+  if (e < nel) {
+    const int er = (int)(e / p.nelx), ec = (int)(e - (long long)er * p.nelx);
+    double ue[8];
+    gather_elem<2>(u + row * ndof, p.nelx + 1, er, ec, ue);
+    double xe = x[e];
+    if (row > 0) {
+      double m, dm;
+      stage_mask(t[e], times[row - 1], p.beta, m, dm);
+      xe *= m;
+    }
+    v = simp(xe, p) * ke_form<2>(ke, ue, ue);
+  }
+  const double s = block_sum(v, red);
+  if (threadIdx.x == 0) part[(long long)row * nblk + blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void row_finalize_kernel(const double* part, int nblk,
+                                                           double* out) {
+  __shared__ double red[4];
+  const int row = blockIdx.x;
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < nblk; i += blockDim.x) acc += part[(long long)row * nblk + i];
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) out[row] = s;
+}
+
+// Hand-derived adjoint of sum_row g[row] * c_row with lambda = 2 U:
+//   row 0:  dc/dx_e = -p dE x^(p-1) c_e
+//   row k:  with xt = x m, A = -p dE xt^(p-1) c_e, G = -2 (C^T U_y)_e,
+//           dc/dx_e = (A + G) m,  dc/dt_e = (A + G) x m'
+__global__ __launch_bounds__(256) void stage_adjoint_kernel(StageParams p, const double* x,
+                                                            const double* t,
+                                                            const double* times,
+                                                            KE<double> ke, const double* u,
+                                                            const double* g, double* dx,
+                                                            double* dt) {
+  const long long nel = (long long)p.nelx * p.nely;
+  const long long ndof = 2LL * (p.nelx + 1) * (p.nely + 1);
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  const int er = (int)(e / p.nelx), ec = (int)(e - (long long)er * p.nelx);
+  const double de = p.emax - p.emin;
+  const double xe = x[e], te = p.nstage > 0 ? t[e] : 0.0;
+  double ue[8];
+  gather_elem<2>(u, p.nelx + 1, er, ec, ue);
+  double gx = g[0] * (-p.penal * de * pow(xe, p.penal - 1.0) * ke_form<2>(ke, ue, ue));
+  double gt = 0.0;
+  for (int k = 1; k <= p.nstage; ++k) {
+    const double gk = g[k];
+    gather_elem<2>(u + k * ndof, p.nelx + 1, er, ec, ue);
+    double m, dm;
+    stage_mask(te, times[k - 1], p.beta, m, dm);
+    const double xt = xe * m;
+    const double A = -p.penal * de * pow(xt, p.penal - 1.0) * ke_form<2>(ke, ue, ue);
+    // (C^T U_y)_e = fe/4 * sum of the element's four nodal y-displacements
+    const double cu = (p.fe / 4.0) * (ue[1] + ue[3] + ue[5] + ue[7]);
+    const double s = A - 2.0 * cu;
+    gx += gk * s * m;
+    gt += gk * s * xe * dm;
+  }
+  dx[e] = gx;
+  dt[e] = gt;
+}
+
+}  // namespace stt
+
+using namespace stt;
+
+extern "C" {
+
+int sttopt_adjoint_grad(int32_t d, int32_t nelx, int32_t nely, int32_t rows, const double* u,
+                        int64_t u_stride, const double* lam, int64_t lam_stride,
+                        const double* ke, double* grad_density, int64_t gd_stride,
+                        double* grad_reaction, int64_t gr_stride, void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || u == nullptr || lam == nullptr ||
+      ke == nullptr || rows < 0) {
+    set_error("sttopt_adjoint_grad: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0 || (grad_density == nullptr && grad_reaction == nullptr)) return STTOPT_OK;
+  const long long nel = (long long)nelx * nely;
+  dim3 grid(ceil_div(nel, 256), rows), block(256);
+  const KE<double> k = make_ke<double>(ke, d);
+  hipStream_t s = (hipStream_t)stream;
+  if (d == 2)
+    hipLaunchKernelGGL(adjoint_grad_kernel<2>, grid, block, 0, s, nelx, nely, u, u_stride,
+                       lam, lam_stride, k, grad_density, gd_stride, grad_reaction, gr_stride);
+  else
+    hipLaunchKernelGGL(adjoint_grad_kernel<1>, grid, block, 0, s, nelx, nely, u, u_stride,
+                       lam, lam_stride, k, grad_density, gd_stride, grad_reaction, gr_stride);
+  STT_CHECK_LAUNCH("adjoint_grad_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_strain_energy(int32_t d, int32_t nelx, int32_t nely, int32_t rows, const double* u,
+                         int64_t u_stride, const double* ke, double* ce, int64_t ce_stride,
+                         void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || u == nullptr || ke == nullptr ||
+      ce == nullptr || rows < 0) {
+    set_error("sttopt_strain_energy: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  const long long nel = (long long)nelx * nely;
+  dim3 grid(ceil_div(nel, 256), rows), block(256);
+  const KE<double> k = make_ke<double>(ke, d);
+  hipStream_t s = (hipStream_t)stream;
+  if (d == 2)
+    hipLaunchKernelGGL(strain_energy_kernel<2>, grid, block, 0, s, nelx, nely, u, u_stride, k,
+                       ce, ce_stride);
+  else
+    hipLaunchKernelGGL(strain_energy_kernel<1>, grid, block, 0, s, nelx, nely, u, u_stride, k,
+                       ce, ce_stride);
+  STT_CHECK_LAUNCH("strain_energy_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_stage_prologue(int32_t nelx, int32_t nely, int32_t nstage, const double* x,
+                          const double* t, const double* stage_times, double beta,
+                          double emin, double emax, double penal, double fe,
+                          const double* f_ext, double* density, double* load, void* stream) {
+  if (nelx <= 0 || nely <= 0 || nstage < 0 || !x || !f_ext || !density || !load ||
+      (nstage > 0 && (!t || !stage_times))) {
+    set_error("sttopt_stage_prologue: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  StageParams p{nelx, nely, nstage, beta, emin, emax, penal, fe};
+  const long long nel = (long long)nelx * nely;
+  const long long nnodes = (long long)(nelx + 1) * (nely + 1);
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(stage_density_kernel, dim3(ceil_div(nel, 256), 1 + nstage), dim3(256), 0,
+                     s, p, x, t, stage_times, density);
+  hipLaunchKernelGGL(stage_load_kernel, dim3(ceil_div(nnodes, 256), 1 + nstage), dim3(256), 0,
+                     s, p, x, t, stage_times, f_ext, load);
+  STT_CHECK_LAUNCH("stage prologue");
+  return STTOPT_OK;
+}
+
+int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage) {
+  const long long nel = (long long)nelx * nely;
+  return (long long)(1 + nstage) * ceil_div(nel, 256);
+}
+
+int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage, const double* x,
+                          const double* t, const double* stage_times, double beta,
+                          double emin, double emax, double penal, double fe, const double* ke,
+                          const double* u, const double* g, double* comp, double* dx,
+                          double* dt, double* scratch, void* stream) {
+  if (nelx <= 0 || nely <= 0 || nstage < 0 || !x || !ke || !u ||
+      (nstage > 0 && (!t || !stage_times))) {
+    set_error("sttopt_stage_epilogue: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  StageParams p{nelx, nely, nstage, beta, emin, emax, penal, fe};
+  const KE<double> k = make_ke<double>(ke, 2);
+  const long long nel = (long long)nelx * nely;
+  const int nblk = ceil_div(nel, 256);
+  hipStream_t s = (hipStream_t)stream;
+  if (comp != nullptr) {
+    if (scratch == nullptr) {
+      set_error("sttopt_stage_epilogue: compliance needs scratch");
+      return STTOPT_EINVAL;
+    }
+    hipLaunchKernelGGL(stage_comp_kernel, dim3(nblk, 1 + nstage), dim3(256), 0, s, p, x, t,
+                       stage_times, k, u, scratch, nblk);
+    hipLaunchKernelGGL(row_finalize_kernel, dim3(1 + nstage), dim3(256), 0, s, scratch, nblk,
+                       comp);
+    STT_CHECK_LAUNCH("stage compliance");
+  }
+  if (dx != nullptr || dt != nullptr) {
+    if (g == nullptr || dx == nullptr || dt == nullptr) {
+      set_error("sttopt_stage_epilogue: the adjoint needs g, dx and dt");
+      return STTOPT_EINVAL;
+    }
+    hipLaunchKernelGGL(stage_adjoint_kernel, dim3(nblk), dim3(256), 0, s, p, x, t,
+                       stage_times, k, u, g, dx, dt);
+    STT_CHECK_LAUNCH("stage adjoint");
+  }
+  return STTOPT_OK;
+}
+
+}  // extern "C"

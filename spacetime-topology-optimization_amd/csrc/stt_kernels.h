@@ -1,0 +1,83 @@
+// Internal launch interface shared by the translation units of the library.
+#pragma once
+
+#include "stt_common.h"
+
+namespace stt {
+
+enum StencilMode { kMatvec = 0, kOperator = 1, kSmooth = 2, kResid = 3 };
+enum DotKind { kNoDot = 0, kDotU = 1, kDotB = 2 };
+
+// One level's operator, type-erased.  Exactly one of rho (level 0: density * KE)
+// or keff (coarser levels: entry-major element matrices) is set.
+struct OpDesc {
+  int d = 2, nelx = 0, nely = 0;
+  const void* rho = nullptr;
+  long long sr = 0;
+  const void* keff = nullptr;
+  long long sk = 0;
+  const void* rdiag = nullptr;
+  long long srd = 0;
+  const uint32_t* mask = nullptr;
+  double ke[64] = {};
+
+  int nnx() const { return nelx + 1; }
+  int nny() const { return nely + 1; }
+  long long nnodes() const { return (long long)(nelx + 1) * (nely + 1); }
+  long long ndof() const { return nnodes() * d; }
+  long long nel() const { return (long long)nelx * nely; }
+  int ntx() const { return ceil_div(nnx(), kTileX); }
+  int nty() const { return ceil_div(nny(), kTileY); }
+  int tiles() const { return ntx() * nty(); }  // reduction partials per row
+};
+
+// y = mode(u): see StencilMode.  Types: T is the stencil (storage and compute)
+// precision, TB the type of b, TY of y.  proj_in masks the input vector (only
+// needed when it may carry values at fixed dofs).  dot != kNoDot writes one fp64
+// partial sum per tile and row to partial[row * tiles + tile].
+template <typename T, typename TB, typename TY>
+int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
+            long long su, const TB* b, long long sb, const T* invd, long long sd,
+            TY* y, long long sy, double omega, const int* active, double* partial,
+            hipStream_t s);
+
+// Node-centric diagonal of an operator (+ rdiag).  out_mode: 0 raw diagonal,
+// 1 diagonal with 1 at fixed dofs, 2 inverse at free dofs and 0 at fixed.
+template <typename TO>
+int diagonal(const OpDesc& op, int rows, int out_mode, TO* out, long long so,
+             hipStream_t s);
+
+// Elementwise kernels over [rows][n] (n = ndof of a level).
+template <typename T, typename TB>
+int jacobi_first(long long n, int rows, const TB* b, long long sb, const T* invd,
+                 long long sd, double omega, const uint32_t* mask, T* x, long long sx,
+                 T* bcopy, long long sbc, const int* active, hipStream_t s);
+template <typename TI, typename TO>
+int convert(long long n, int rows, const TI* in, long long si, TO* out, long long so,
+            const int* active, hipStream_t s);
+template <typename T>
+int fill(long long n, int rows, T value, T* x, long long sx, const int* active,
+         hipStream_t s);
+template <typename T>
+int axpy(long long n, int rows, const T* xin, long long si, T* y, long long sy,
+         const int* active, hipStream_t s);  // y += x
+int project_f64(long long n, int rows, const uint32_t* mask, double* x, long long sx,
+                hipStream_t s);
+
+// Grid transfers between a level with cnelx x cnely elements and its fine level.
+template <typename T>
+int restrict_to(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* fine,
+                long long sf, const uint32_t* cmask, T* coarse, long long sc,
+                const int* active, hipStream_t s);
+template <typename T>
+int prolong_add(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* coarse,
+                long long sc, const uint32_t* fmask, T* fine, long long sf,
+                const int* active, hipStream_t s);
+
+// Dense coarsest solve x = P (cinv b).
+template <typename T>
+int coarse_solve(long long n, int rows, const double* cinv, long long scinv,
+                 const T* b, long long sb, const uint32_t* mask, T* x, long long sx,
+                 const int* active, hipStream_t s);
+
+}  // namespace stt

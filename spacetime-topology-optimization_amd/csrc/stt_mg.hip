@@ -1,0 +1,458 @@
+// Geometric multigrid on the structured Q4 grid: on-device Galerkin coarse
+// operators, full-weighting restriction / bilinear prolongation, the dense
+// coarsest solve, and the V-cycle driver (torch_mg.py:293-476).
+#include "stt_kernels.h"
+
+namespace stt {
+
+// ------------------------------------------------------------- Galerkin -----
+// One wavefront per (64 coarse elements, one entry (a, b) of the (4d)^2 element
+// matrix, one row): the entry index is uniform across the wave, so the
+// interpolation weights and child products are scalar loads, and the 64 lanes read
+// 64 neighbouring fine elements.
+//   level 1:   keff_E[a][b] = sum_m rho_{child m} G_m[a][b] (+ react GM_m[a][b])
+//   deeper:    keff_E[a][b] = sum_m sum_{p,q} S_m[p][a] keff_{child m}[p][q] S_m[q][b]
+template <int D, bool FROM_DENSITY>
+__global__ __launch_bounds__(64) void galerkin_kernel(int nelx, int nely, int kx, int ky,
+                                                      const double* rho, long long sr,
+                                                      const double* react, long long sre,
+                                                      const double* g, const double* gm,
+                                                      const double* fk, long long sfk,
+                                                      const double* S, double* ck,
+                                                      long long sck) {
+  constexpr int K = 4 * D, KK = K * K;
+  const int cnelx = nelx / kx, cnely = nely / ky;
+  const long long cnel = (long long)cnelx * cnely;
+  const long long fnel = (long long)nelx * nely;
+  const long long E = (long long)blockIdx.x * 64 + threadIdx.x;
+  const int entry = blockIdx.y;
+  const int row = blockIdx.z;
+  if (E >= cnel) return;
+  const int A = entry / K, B = entry - (entry / K) * K;
+  const int cr = (int)(E / cnelx), cc = (int)(E - (long long)cr * cnelx);
+  double acc = 0.0;
+  for (int my = 0; my < ky; ++my) {
+    for (int mx = 0; mx < kx; ++mx) {
+      const int m = my * kx + mx;
+      const long long f = (long long)(cr * ky + my) * nelx + (cc * kx + mx);
+      if constexpr (FROM_DENSITY) {
+        acc += rho[row * sr + f] * g[m * KK + entry];
+        if (react != nullptr) acc += react[row * sre + f] * gm[m * KK + entry];
+      } else {
+        const double* Sm = S + m * KK;
+        for (int p = 0; p < K; ++p) {
+          const double spa = Sm[p * K + A];
+          if (spa == 0.0) continue;
+          double inner = 0.0;
+          for (int q = 0; q < K; ++q) {
+            const double sqb = Sm[q * K + B];
+            if (sqb == 0.0) continue;
+            inner += fk[row * sfk + (long long)(p * K + q) * fnel + f] * sqb;
+          }
+          acc += spa * inner;
+        }
+      }
+    }
+  }
+  ck[row * sck + (long long)entry * cnel + E] = acc;
+}
+
+// ------------------------------------------------------------ transfers -----
+
+template <typename T>
+__global__ __launch_bounds__(256) void restrict_kernel(int d, int cnelx, int cnely, int kx,
+                                                       int ky, const T* fine, long long sf,
+                                                       const uint32_t* cmask, T* coarse,
+                                                       long long sc, const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  const int cnnx = cnelx + 1, cnny = cnely + 1;
+  const int fnnx = cnelx * kx + 1, fnny = cnely * ky + 1;
+  const long long cnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cnode >= (long long)cnnx * cnny) return;
+  const int I = (int)(cnode / cnnx), J = (int)(cnode - (long long)I * cnnx);
+  const T* f = fine + row * sf;
+  for (int comp = 0; comp < d; ++comp) {
+    T acc = T(0);
+    for (int dy = -(ky - 1); dy <= ky - 1; ++dy) {
+      const int R = I * ky + dy;
+      if (R < 0 || R >= fnny) continue;
+      const T wy = T(ky - (dy < 0 ? -dy : dy)) / T(ky);
+      T racc = T(0);
+      for (int dx = -(kx - 1); dx <= kx - 1; ++dx) {
+        const int C = J * kx + dx;
+        if (C < 0 || C >= fnnx) continue;
+        const T wx = T(kx - (dx < 0 ? -dx : dx)) / T(kx);
+        racc += wx * f[((long long)R * fnnx + C) * d + comp];
+      }
+      acc += wy * racc;
+    }
+    const long long dof = cnode * d + comp;
+    coarse[row * sc + dof] = mask_free(cmask, dof) ? acc : T(0);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void prolong_add_kernel(int d, int cnelx, int cnely,
+                                                          int kx, int ky, const T* coarse,
+                                                          long long sc, const uint32_t* fmask,
+                                                          T* fine, long long sf,
+                                                          const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  const int cnnx = cnelx + 1;
+  const int fnnx = cnelx * kx + 1, fnny = cnely * ky + 1;
+  const long long fnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (fnode >= (long long)fnnx * fnny) return;
+  const int R = (int)(fnode / fnnx), C = (int)(fnode - (long long)R * fnnx);
+  const int I0 = R / ky, fy = R - I0 * ky;
+  const int J0 = C / kx, fx = C - J0 * kx;
+  const T wy1 = T(fy) / T(ky), wy0 = T(1) - wy1;
+  const T wx1 = T(fx) / T(kx), wx0 = T(1) - wx1;
+  const T* c = coarse + row * sc;
+  for (int comp = 0; comp < d; ++comp) {
+    const long long dof = fnode * d + comp;
+    if (!mask_free(fmask, dof)) continue;
+    const long long c00 = ((long long)I0 * cnnx + J0) * d + comp;
+    T top = wx0 * c[c00];
+    if (fx > 0) top += wx1 * c[c00 + d];
+    T v = wy0 * top;
+    if (fy > 0) {
+      const long long c10 = c00 + (long long)cnnx * d;
+      T bot = wx0 * c[c10];
+      if (fx > 0) bot += wx1 * c[c10 + d];
+      v += wy1 * bot;
+    }
+    fine[row * sf + dof] += v;
+  }
+}
+
+template <typename T>
+int restrict_to(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* fine,
+                long long sf, const uint32_t* cmask, T* coarse, long long sc,
+                const int* active, hipStream_t s) {
+  if (rows <= 0) return STTOPT_OK;
+  const long long n = (long long)(cnelx + 1) * (cnely + 1);
+  hipLaunchKernelGGL((restrict_kernel<T>), dim3(ceil_div(n, 256), rows), dim3(256), 0, s, d,
+                     cnelx, cnely, kx, ky, fine, sf, cmask, coarse, sc, active);
+  STT_CHECK_LAUNCH("restrict_kernel");
+  return STTOPT_OK;
+}
+
+template <typename T>
+int prolong_add(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* coarse,
+                long long sc, const uint32_t* fmask, T* fine, long long sf,
+                const int* active, hipStream_t s) {
+  if (rows <= 0) return STTOPT_OK;
+  const long long n = (long long)(cnelx * kx + 1) * (cnely * ky + 1);
+  hipLaunchKernelGGL((prolong_add_kernel<T>), dim3(ceil_div(n, 256), rows), dim3(256), 0, s,
+                     d, cnelx, cnely, kx, ky, coarse, sc, fmask, fine, sf, active);
+  STT_CHECK_LAUNCH("prolong_add_kernel");
+  return STTOPT_OK;
+}
+
+template int restrict_to<double>(int, int, int, int, int, int, const double*, long long,
+                                 const uint32_t*, double*, long long, const int*,
+                                 hipStream_t);
+template int restrict_to<float>(int, int, int, int, int, int, const float*, long long,
+                                const uint32_t*, float*, long long, const int*, hipStream_t);
+template int prolong_add<double>(int, int, int, int, int, int, const double*, long long,
+                                 const uint32_t*, double*, long long, const int*,
+                                 hipStream_t);
+template int prolong_add<float>(int, int, int, int, int, int, const float*, long long,
+                                const uint32_t*, float*, long long, const int*, hipStream_t);
+
+// ---------------------------------------------------------- coarse solve ----
+// One wavefront per output dof: the 64 lanes stream one row of the dense
+// inverse (coalesced) and reduce in fp64.
+
+template <typename T>
+__global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const double* cinv,
+                                                           long long scinv, const T* b,
+                                                           long long sb, const uint32_t* mask,
+                                                           T* x, long long sx,
+                                                           const int* active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;
+  const long long i = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (i >= n) return;  // uniform per wave
+  const double* Ai = cinv + row * scinv + i * n;
+  const T* bb = b + row * sb;
+  double acc = 0.0;
+  for (long long j = lane; j < n; j += kWave) acc += Ai[j] * (double)bb[j];
+  acc = wave_sum(acc);
+  if (lane == 0) x[row * sx + i] = mask_free(mask, i) ? static_cast<T>(acc) : T(0);
+}
+
+template <typename T>
+int coarse_solve(long long n, int rows, const double* cinv, long long scinv, const T* b,
+                 long long sb, const uint32_t* mask, T* x, long long sx, const int* active,
+                 hipStream_t s) {
+  if (rows <= 0 || n <= 0) return STTOPT_OK;
+  if (cinv == nullptr) {
+    set_error("coarse_solve: coarsest level has no dense inverse");
+    return STTOPT_EINVAL;
+  }
+  hipLaunchKernelGGL((coarse_solve_kernel<T>), dim3(ceil_div(n, 4), rows), dim3(256), 0, s,
+                     n, cinv, scinv, b, sb, mask, x, sx, active);
+  STT_CHECK_LAUNCH("coarse_solve_kernel");
+  return STTOPT_OK;
+}
+template int coarse_solve<double>(long long, int, const double*, long long, const double*,
+                                  long long, const uint32_t*, double*, long long,
+                                  const int*, hipStream_t);
+template int coarse_solve<float>(long long, int, const double*, long long, const float*,
+                                 long long, const uint32_t*, float*, long long, const int*,
+                                 hipStream_t);
+
+// ------------------------------------------------------- dense assembly -----
+// Entry (i, j) of the coarsest operator is gathered from the (up to four)
+// elements holding both nodes, then symmetrized as (K_ij + K_ji) / 2; fixed dofs
+// get identity rows and columns (torch_mg._dense_cholesky, torch_mg.py:269-289).
+
+template <int D>
+__device__ double dense_entry(int nelx, int nely, int r, int c, int ci, int r2, int c2,
+                              int cj, const double* keff, const double* rho,
+                              const double* ke) {
+  constexpr int K = 4 * D;
+  const long long nel = (long long)nelx * nely;
+  double v = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+    if (er < 0 || er >= nely || ec < 0 || ec >= nelx) continue;
+    const int pr = r2 - er, pc = c2 - ec;  // second node relative to element top-left
+    if (pr < 0 || pr > 1 || pc < 0 || pc > 1) continue;
+    const int jl = pr == 1 ? (pc == 0 ? 0 : 1) : (pc == 1 ? 2 : 3);
+    const int il = q == 0 ? 1 : q == 1 ? 0 : q == 2 ? 2 : 3;
+    const int a = il * D + ci, b = jl * D + cj;
+    const long long e = (long long)er * nelx + ec;
+    if (keff != nullptr)
+      v += keff[(long long)(a * K + b) * nel + e];
+    else
+      v += rho[e] * ke[a * K + b];
+  }
+  return v;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void dense_assemble_kernel(int nelx, int nely,
+                                                             const double* keff, long long sk,
+                                                             const double* rho, long long sr,
+                                                             const double* rdiag, long long srd,
+                                                             const double* ke,
+                                                             const uint32_t* mask,
+                                                             double* out, long long so) {
+  const int row = blockIdx.y;
+  const int nnx = nelx + 1, nny = nely + 1;
+  const long long n = (long long)nnx * nny * D;
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  // one thread per (dof i, neighbour slot): 9 neighbour nodes x D components
+  const long long i = t / (9 * D);
+  if (i >= n) return;
+  const int slot = (int)(t - i * 9 * D);
+  const int nb = slot / D, cj = slot - (slot / D) * D;
+  const long long node = i / D;
+  const int ci = (int)(i - node * D);
+  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
+  const int r2 = r + nb / 3 - 1, c2 = c + nb % 3 - 1;
+  if (r2 < 0 || r2 >= nny || c2 < 0 || c2 >= nnx) return;
+  const long long j = ((long long)r2 * nnx + c2) * D + cj;
+  const double* kf = keff != nullptr ? keff + row * sk : nullptr;
+  const double* rh = rho != nullptr ? rho + row * sr : nullptr;
+  double v = 0.5 * (dense_entry<D>(nelx, nely, r, c, ci, r2, c2, cj, kf, rh, ke) +
+                    dense_entry<D>(nelx, nely, r2, c2, cj, r, c, ci, kf, rh, ke));
+  if (i == j && rdiag != nullptr) v += rdiag[row * srd + i];
+  const bool fi = mask_free(mask, i), fj = mask_free(mask, j);
+  if (!(fi && fj)) v = (i == j && !fi) ? 1.0 : 0.0;
+  out[row * so + i * n + j] = v;
+}
+
+}  // namespace stt
+
+using namespace stt;
+
+extern "C" {
+
+int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t ky,
+                    int32_t rows, const double* density, int64_t density_stride,
+                    const double* reaction, int64_t reaction_stride, const double* g,
+                    const double* gm, const double* fine_keff, int64_t fine_keff_stride,
+                    const double* s, double* coarse_keff, int64_t coarse_keff_stride,
+                    void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || kx < 1 || ky < 1 || kx > 3 ||
+      ky > 3 || nelx % kx || nely % ky || coarse_keff == nullptr || rows < 0) {
+    set_error("sttopt_galerkin: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  const bool from_density = fine_keff == nullptr;
+  if (from_density && (density == nullptr || g == nullptr || (reaction && !gm))) {
+    set_error("sttopt_galerkin: level-1 build needs density and G (and GM with reaction)");
+    return STTOPT_EINVAL;
+  }
+  if (!from_density && s == nullptr) {
+    set_error("sttopt_galerkin: coarse build needs the interpolation matrices S");
+    return STTOPT_EINVAL;
+  }
+  const long long cnel = (long long)(nelx / kx) * (nely / ky);
+  const int K = 4 * d;
+  dim3 grid(ceil_div(cnel, 64), K * K, rows), block(64);
+  hipStream_t st = (hipStream_t)stream;
+#define STT_GAL(D, FD)                                                                   \
+  hipLaunchKernelGGL((galerkin_kernel<D, FD>), grid, block, 0, st, nelx, nely, kx, ky,    \
+                     density, density_stride, reaction, reaction_stride, g, gm, fine_keff, \
+                     fine_keff_stride, s, coarse_keff, coarse_keff_stride)
+  if (d == 2) {
+    if (from_density) STT_GAL(2, true); else STT_GAL(2, false);
+  } else {
+    if (from_density) STT_GAL(1, true); else STT_GAL(1, false);
+  }
+#undef STT_GAL
+  STT_CHECK_LAUNCH("galerkin_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_keff_diagonal(int32_t dtype_out, int32_t d, int32_t nelx, int32_t nely,
+                         int32_t rows, const double* keff, int64_t keff_stride,
+                         const uint32_t* mask, int32_t inv, void* out, int64_t out_stride,
+                         void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || keff == nullptr || out == nullptr) {
+    set_error("sttopt_keff_diagonal: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  OpDesc op;
+  op.d = d;
+  op.nelx = nelx;
+  op.nely = nely;
+  op.keff = keff;
+  op.sk = keff_stride;
+  op.mask = mask;
+  const int mode = inv ? 2 : 1;
+  if (dtype_out == STTOPT_F64)
+    return diagonal<double>(op, rows, mode, static_cast<double*>(out), out_stride,
+                            (hipStream_t)stream);
+  if (dtype_out == STTOPT_F32)
+    return diagonal<float>(op, rows, mode, static_cast<float*>(out), out_stride,
+                           (hipStream_t)stream);
+  set_error("sttopt_keff_diagonal: bad dtype");
+  return STTOPT_EINVAL;
+}
+
+}  // extern "C"
+
+namespace stt {
+template <typename TO>
+__global__ __launch_bounds__(256) void invert_diag_kernel(long long n, const double* diag,
+                                                          long long sd, const uint32_t* mask,
+                                                          TO* out, long long so) {
+  const int row = blockIdx.y;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[row * so + i] =
+        mask_free(mask, i) ? static_cast<TO>(1.0 / diag[row * sd + i]) : TO(0);
+}
+}  // namespace stt
+
+extern "C" {
+
+int sttopt_invert_diagonal(int32_t dtype_out, int64_t n, int32_t rows, const double* diag,
+                           int64_t diag_stride, const uint32_t* mask, void* out,
+                           int64_t out_stride, void* stream) {
+  if (diag == nullptr || out == nullptr || n < 0 || rows < 0) {
+    set_error("sttopt_invert_diagonal: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (n == 0 || rows == 0) return STTOPT_OK;
+  long long blocks = (n + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  dim3 grid((unsigned)blocks, rows);
+  if (dtype_out == STTOPT_F64)
+    hipLaunchKernelGGL((invert_diag_kernel<double>), grid, dim3(256), 0, (hipStream_t)stream,
+                       (long long)n, diag, diag_stride, mask, static_cast<double*>(out),
+                       out_stride);
+  else if (dtype_out == STTOPT_F32)
+    hipLaunchKernelGGL((invert_diag_kernel<float>), grid, dim3(256), 0, (hipStream_t)stream,
+                       (long long)n, diag, diag_stride, mask, static_cast<float*>(out),
+                       out_stride);
+  else {
+    set_error("sttopt_invert_diagonal: bad dtype");
+    return STTOPT_EINVAL;
+  }
+  STT_CHECK_LAUNCH("invert_diag_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_dense_assemble(int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                          const double* keff, int64_t keff_stride, const double* density,
+                          int64_t density_stride, const double* rdiag, int64_t rdiag_stride,
+                          const double* ke, const uint32_t* mask, double* out,
+                          int64_t out_stride, void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || out == nullptr ||
+      (keff == nullptr && (density == nullptr || ke == nullptr))) {
+    set_error("sttopt_dense_assemble: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  const long long n = (long long)(nelx + 1) * (nely + 1) * d;
+  hipStream_t st = (hipStream_t)stream;
+  // ke is a device array here (the kernel reads it per element)
+  if (int e = fill<double>(n * n, rows, 0.0, out, out_stride, nullptr, st)) return e;
+  const long long threads = n * 9 * d;
+  dim3 grid(ceil_div(threads, 256), rows), block(256);
+  if (d == 2)
+    hipLaunchKernelGGL(dense_assemble_kernel<2>, grid, block, 0, st, nelx, nely, keff,
+                       keff_stride, density, density_stride, rdiag, rdiag_stride, ke, mask,
+                       out, out_stride);
+  else
+    hipLaunchKernelGGL(dense_assemble_kernel<1>, grid, block, 0, st, nelx, nely, keff,
+                       keff_stride, density, density_stride, rdiag, rdiag_stride, ke, mask,
+                       out, out_stride);
+  STT_CHECK_LAUNCH("dense_assemble_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_restrict(int32_t dtype, int32_t d, int32_t cnelx, int32_t cnely, int32_t kx,
+                    int32_t ky, int32_t rows, const void* fine, int64_t fine_stride,
+                    const uint32_t* cmask, void* coarse, int64_t coarse_stride,
+                    void* stream) {
+  if ((d != 1 && d != 2) || cnelx <= 0 || cnely <= 0 || kx < 1 || ky < 1 || !fine ||
+      !coarse) {
+    set_error("sttopt_restrict: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == STTOPT_F64)
+    return restrict_to<double>(d, cnelx, cnely, kx, ky, rows,
+                               static_cast<const double*>(fine), fine_stride, cmask,
+                               static_cast<double*>(coarse), coarse_stride, nullptr, s);
+  if (dtype == STTOPT_F32)
+    return restrict_to<float>(d, cnelx, cnely, kx, ky, rows, static_cast<const float*>(fine),
+                              fine_stride, cmask, static_cast<float*>(coarse), coarse_stride,
+                              nullptr, s);
+  set_error("sttopt_restrict: bad dtype");
+  return STTOPT_EINVAL;
+}
+
+int sttopt_prolong_add(int32_t dtype, int32_t d, int32_t cnelx, int32_t cnely, int32_t kx,
+                       int32_t ky, int32_t rows, const void* coarse, int64_t coarse_stride,
+                       const uint32_t* fmask, void* fine, int64_t fine_stride, void* stream) {
+  if ((d != 1 && d != 2) || cnelx <= 0 || cnely <= 0 || kx < 1 || ky < 1 || !fine ||
+      !coarse) {
+    set_error("sttopt_prolong_add: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (dtype == STTOPT_F64)
+    return prolong_add<double>(d, cnelx, cnely, kx, ky, rows,
+                               static_cast<const double*>(coarse), coarse_stride, fmask,
+                               static_cast<double*>(fine), fine_stride, nullptr, s);
+  if (dtype == STTOPT_F32)
+    return prolong_add<float>(d, cnelx, cnely, kx, ky, rows,
+                              static_cast<const float*>(coarse), coarse_stride, fmask,
+                              static_cast<float*>(fine), fine_stride, nullptr, s);
+  set_error("sttopt_prolong_add: bad dtype");
+  return STTOPT_EINVAL;
+}
+
+}  // extern "C"

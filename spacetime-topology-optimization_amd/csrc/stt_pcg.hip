@@ -1,0 +1,679 @@
+// Batched multigrid-preconditioned CG with per-row convergence flags
+// (torch_fem.pcg, torch_fem.py:201-343, with torch_mg.VCycle as M).
+//
+// Every batch row carries a device-side `active` flag.  A row that reaches rtol is
+// frozen at the iteration it converged (its x, r, p are never touched again), which
+// is the reference's row retirement without compacting the batch: workgroups of a
+// retired row exit on their first instruction.  The host reads one integer (the
+// active count) every `check_every` iterations instead of syncing on every
+// iteration.  All inner products and norms are reduced in fp64, in a fixed order
+// (per-tile partials, then one workgroup per row), so results are deterministic.
+#include <vector>
+
+#include "stt_kernels.h"
+
+namespace stt {
+
+// ------------------------------------------------------------ scratch map ---
+
+struct Scratch {
+  double* part[5];  // [rows][tiles] each: pAp, rr, rz, rzo, bb
+  double* bsafe;    // [rows]
+  double* rz;       // [rows]
+  double* alpha;    // [rows]
+  double* beta;     // [rows]
+  int* active;      // [rows]
+  int* counter;     // [1]
+};
+
+static long long align256(long long b) { return (b + 255) & ~255LL; }
+
+static long long scratch_bytes(int rows, int tiles) {
+  long long b = 0;
+  b += 5 * align256(8LL * rows * tiles);
+  b += 4 * align256(8LL * rows);
+  b += align256(4LL * rows);
+  b += align256(16);
+  return b;
+}
+
+static Scratch carve(void* base, int rows, int tiles) {
+  Scratch s;
+  char* p = static_cast<char*>(base);
+  for (int i = 0; i < 5; ++i) {
+    s.part[i] = reinterpret_cast<double*>(p);
+    p += align256(8LL * rows * tiles);
+  }
+  s.bsafe = reinterpret_cast<double*>(p);
+  p += align256(8LL * rows);
+  s.rz = reinterpret_cast<double*>(p);
+  p += align256(8LL * rows);
+  s.alpha = reinterpret_cast<double*>(p);
+  p += align256(8LL * rows);
+  s.beta = reinterpret_cast<double*>(p);
+  p += align256(8LL * rows);
+  s.active = reinterpret_cast<int*>(p);
+  p += align256(4LL * rows);
+  s.counter = reinterpret_cast<int*>(p);
+  return s;
+}
+
+// --------------------------------------------------------- tile kernels -----
+// Elementwise PCG kernels walk the same 64 x 4 node tiles as the stencil kernels,
+// so every fp64 partial-sum array has one slot per (row, tile).
+
+struct TileGeom {
+  int nnx, nny, ntx, tiles, total, d;
+};
+
+__device__ __forceinline__ bool tile_node(const TileGeom& g, int& row, int& tt,
+                                          long long& node) {
+  const int t = xcd_remap(blockIdx.x, g.total);
+  row = t / g.tiles;
+  tt = t - row * g.tiles;
+  const int by = tt / g.ntx, bx = tt - by * g.ntx;
+  const int c = bx * kTileX + (threadIdx.x & (kTileX - 1));
+  const int r = by * kTileY + (threadIdx.x / kTileX);
+  node = (long long)r * g.nnx + c;
+  return c < g.nnx && r < g.nny;
+}
+
+// x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).
+__global__ __launch_bounds__(kBlock) void pcg_update_kernel(TileGeom g, long long n,
+                                                            const double* alpha, double* x,
+                                                            double* r, const double* p,
+                                                            const double* Ap,
+                                                            const double* z,
+                                                            const int* active,
+                                                            double* prr, double* prz) {
+  __shared__ double red[kBlock / kWave];
+  int row, tt;
+  long long node;
+  const bool in = tile_node(g, row, tt, node);
+  if (active[row] == 0) return;
+  double rr = 0.0, rz = 0.0;
+  if (in) {
+    const double a = alpha[row];
+    for (int k = 0; k < g.d; ++k) {
+      const long long i = row * n + node * g.d + k;
+      x[i] += a * p[i];
+      const double rn = r[i] - a * Ap[i];
+      r[i] = rn;
+      rr += rn * rn;
+      if (z != nullptr) rz += rn * z[i];
+    }
+  }
+  double s = block_sum(rr, red);
+  if (threadIdx.x == 0) prr[(long long)row * g.tiles + tt] = s;
+  if (z != nullptr) {
+    __syncthreads();
+    s = block_sum(rz, red);
+    if (threadIdx.x == 0) prz[(long long)row * g.tiles + tt] = s;
+  }
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long long n,
+                                                             const double* beta, double* p,
+                                                             const double* z,
+                                                             const int* active) {
+  int row, tt;
+  long long node;
+  const bool in = tile_node(g, row, tt, node);
+  if (active[row] == 0 || !in) return;
+  const double b = beta[row];
+  for (int k = 0; k < g.d; ++k) {
+    const long long i = row * n + node * g.d + k;
+    p[i] = z[i] + b * p[i];
+  }
+}
+
+// partial sums of a.b over free dofs (b == nullptr: a.a), optionally only of
+// active rows.
+__global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, const double* a,
+                                                     long long sa, const double* b,
+                                                     long long sb, const uint32_t* mask,
+                                                     const int* active, double* part) {
+  __shared__ double red[kBlock / kWave];
+  int row, tt;
+  long long node;
+  const bool in = tile_node(g, row, tt, node);
+  if (active != nullptr && active[row] == 0) return;
+  double acc = 0.0;
+  if (in) {
+    for (int k = 0; k < g.d; ++k) {
+      const long long i = node * g.d + k;
+      if (!mask_free(mask, i)) continue;
+      const double av = a[row * sa + i];
+      acc += av * (b != nullptr ? b[row * sb + i] : av);
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long long)row * g.tiles + tt] = s;
+}
+
+// ---------------------------------------------------- per-row finalizers ----
+
+__device__ double row_sum(const double* part, int tiles, int row, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < tiles; i += blockDim.x) acc += part[(long long)row * tiles + i];
+  return block_sum(acc, red);
+}
+
+__device__ __forceinline__ double safe_div(double num, double den) {
+  return den != 0.0 ? num / den : 0.0;
+}
+
+// Initial residual: b norm, relative residual, active flags.
+__global__ __launch_bounds__(kBlock) void fin_init_kernel(int tiles, const double* pbb,
+                                                          const double* prr, double rtol,
+                                                          double* bsafe, double* resid,
+                                                          int* active, int* iters,
+                                                          int* counter) {
+  __shared__ double red[kBlock / kWave];
+  const int row = blockIdx.x;
+  const double bb = row_sum(pbb, tiles, row, red);
+  __syncthreads();
+  const double rr = row_sum(prr, tiles, row, red);
+  if (threadIdx.x == 0) {
+    const double bn = sqrt(bb);
+    const double bs = bn > 0.0 ? bn : 1.0;
+    const double rel = sqrt(rr) / bs;
+    bsafe[row] = bs;
+    resid[row] = rel;
+    const int act = (rel <= rtol) ? 0 : 1;
+    active[row] = act;
+    iters[row] = 0;
+    if (act) atomicAdd(counter, 1);
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void fin_rz_kernel(int tiles, const double* prz,
+                                                        const int* active, double* rz) {
+  __shared__ double red[kBlock / kWave];
+  const int row = blockIdx.x;
+  if (active[row] == 0) return;
+  const double s = row_sum(prz, tiles, row, red);
+  if (threadIdx.x == 0) rz[row] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void fin_alpha_kernel(int tiles, const double* ppap,
+                                                           const int* active,
+                                                           const double* rz, double* alpha) {
+  __shared__ double red[kBlock / kWave];
+  const int row = blockIdx.x;
+  if (active[row] == 0) return;
+  const double pap = row_sum(ppap, tiles, row, red);
+  if (threadIdx.x == 0) alpha[row] = safe_div(rz[row], pap);
+}
+
+__global__ __launch_bounds__(kBlock) void fin_resid_kernel(int tiles, const double* prr,
+                                                           double rtol, int k,
+                                                           const double* bsafe,
+                                                           double* resid, int* active,
+                                                           int* iters, int* counter) {
+  __shared__ double red[kBlock / kWave];
+  const int row = blockIdx.x;
+  if (active[row] == 0) return;
+  const double rr = row_sum(prr, tiles, row, red);
+  if (threadIdx.x == 0) {
+    const double rel = sqrt(rr) / bsafe[row];
+    resid[row] = rel;
+    iters[row] = k;
+    if (rel <= rtol) {
+      active[row] = 0;
+    } else {
+      atomicAdd(counter, 1);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void fin_beta_kernel(int tiles, const double* prz,
+                                                          const double* przo, int flexible,
+                                                          const int* active, double* rz,
+                                                          double* beta) {
+  __shared__ double red[kBlock / kWave];
+  const int row = blockIdx.x;
+  if (active[row] == 0) return;
+  const double rzn = row_sum(prz, tiles, row, red);
+  double rzo = 0.0;
+  if (flexible) {
+    __syncthreads();
+    rzo = row_sum(przo, tiles, row, red);
+  }
+  if (threadIdx.x == 0) {
+    beta[row] = safe_div(rzn - rzo, rz[row]);
+    rz[row] = rzn;
+  }
+}
+
+// ------------------------------------------------------------- V-cycle ------
+
+template <typename T>
+struct VCycle {
+  const sttopt_mg* mg;
+  hipStream_t s;
+  const int* active;
+  int rows;
+  OpDesc op[STTOPT_MAX_LEVELS];
+  long long n[STTOPT_MAX_LEVELS];
+
+  VCycle(const sttopt_mg* m, hipStream_t st, const int* act) : mg(m), s(st), active(act) {
+    rows = m->rows;
+    for (int l = 0; l < m->nlevels; ++l) {
+      const sttopt_level& L = m->levels[l];
+      OpDesc& o = op[l];
+      o.d = m->d;
+      o.nelx = L.nelx;
+      o.nely = L.nely;
+      o.mask = L.mask;
+      if (l == 0) {
+        o.rho = L.density;
+        o.sr = L.density_stride;
+        o.rdiag = L.rdiag;
+        o.srd = L.rdiag_stride;
+      } else {
+        o.keff = L.keff;
+        o.sk = L.keff_stride;
+      }
+      for (int i = 0; i < 64; ++i) o.ke[i] = m->ke[i];
+      n[l] = o.ndof();
+    }
+  }
+
+  const T* invd(int l) const { return static_cast<const T*>(mg->levels[l].invdiag); }
+  long long sd(int l) const { return mg->levels[l].invdiag_stride; }
+  T* buf(void* p) const { return static_cast<T*>(p); }
+
+  // n_smooth damped-Jacobi sweeps x <- x + w D^-1 (b - A x) starting from x;
+  // returns the buffer holding the result (x or alt).
+  int sweeps(int l, int count, T*& x, T*& alt, const T* b, long long sb) {
+    for (int k = 0; k < count; ++k) {
+      if (int e = stencil<T, T, T>(op[l], rows, kSmooth, false, kNoDot, x, n[l], b, sb,
+                                   invd(l), sd(l), alt, n[l], mg->omega, active, nullptr, s))
+        return e;
+      T* t = x;
+      x = alt;
+      alt = t;
+    }
+    return STTOPT_OK;
+  }
+
+  // Cycle on level l >= 1 with right-hand side b; *out receives the result buffer.
+  int cycle(int l, const T* b, T** out) {
+    const sttopt_level& L = mg->levels[l];
+    const long long nl = n[l];
+    if (l == mg->nlevels - 1) {
+      T* x = buf(L.vx);
+      if (int e = coarse_solve<T>(nl, rows, L.cinv, L.cinv_stride, b, nl, L.mask, x, nl,
+                                  active, s))
+        return e;
+      *out = x;
+      return STTOPT_OK;
+    }
+    T* x = buf(L.vx);
+    T* alt = buf(L.vx2);
+    const int ns = mg->n_smooth;
+    if (ns >= 1) {
+      if (int e = jacobi_first<T, T>(nl, rows, b, nl, invd(l), sd(l), mg->omega, L.mask, x,
+                                     nl, nullptr, 0, active, s))
+        return e;
+      if (int e = sweeps(l, ns - 1, x, alt, b, nl)) return e;
+    } else if (int e = fill<T>(nl, rows, T(0), x, nl, active, s)) {
+      return e;
+    }
+    if (int e = coarse_correct(l, x, b, nl)) return e;
+    if (int e = sweeps(l, ns, x, alt, b, nl)) return e;
+    *out = x;
+    return STTOPT_OK;
+  }
+
+  // x += P (cycle(R (b - A x))), with gamma coarse visits.
+  int coarse_correct(int l, T* x, const T* b, long long sb) {
+    const sttopt_level& L = mg->levels[l];
+    const sttopt_level& C = mg->levels[l + 1];
+    const long long nl = n[l], nc = n[l + 1];
+    T* r = buf(L.vr);
+    if (int e = stencil<T, T, T>(op[l], rows, kResid, false, kNoDot, x, nl, b, sb, nullptr,
+                                 0, r, nl, 0.0, active, nullptr, s))
+      return e;
+    const int cnelx = C.nelx, cnely = C.nely;
+    if (int e = restrict_to<T>(mg->d, cnelx, cnely, L.kx, L.ky, rows, r, nl, C.mask,
+                               buf(C.vb), nc, active, s))
+      return e;
+    T* ec = nullptr;
+    if (int e = cycle(l + 1, buf(C.vb), &ec)) return e;
+    if (mg->gamma > 1) {
+      T* acc = buf(C.ve);
+      if (int e = convert<T, T>(nc, rows, ec, nc, acc, nc, active, s)) return e;
+      for (int g = 1; g < mg->gamma; ++g) {
+        if (int e = stencil<T, T, T>(op[l + 1], rows, kResid, false, kNoDot, acc, nc,
+                                     buf(C.vb), nc, nullptr, 0, buf(C.vw), nc, 0.0, active,
+                                     nullptr, s))
+          return e;
+        T* e2 = nullptr;
+        if (int e = cycle(l + 1, buf(C.vw), &e2)) return e;
+        if (int e = axpy<T>(nc, rows, e2, nc, acc, nc, active, s)) return e;
+      }
+      ec = acc;
+    }
+    return prolong_add<T>(mg->d, cnelx, cnely, L.kx, L.ky, rows, ec, nc, L.mask, x, nl,
+                          active, s);
+  }
+
+  // z = M(r) on level 0.  r, z fp64.  When part != nullptr, writes the tile
+  // partials of r.z (free dofs) for the PCG.
+  int run(const double* r, long long rs, double* z, long long zs, double* part) {
+    const sttopt_level& L = mg->levels[0];
+    const long long nl = n[0];
+    const int ns = mg->n_smooth;
+    if (mg->nlevels == 1) {
+      if (int e = coarse_solve<double>(nl, rows, L.cinv, L.cinv_stride, r, rs, L.mask, z, zs,
+                                       active, s))
+        return e;
+      return part ? dot_rz(r, rs, z, zs, part) : STTOPT_OK;
+    }
+    // fine-level right-hand side in the V-cycle precision
+    const T* b;
+    long long sb;
+    T* x = buf(L.vx);
+    T* alt = buf(L.vx2);
+    if constexpr (sizeof(T) == sizeof(double)) {
+      b = reinterpret_cast<const T*>(r);
+      sb = rs;
+      if (ns >= 1) {
+        if (int e = jacobi_first<T, T>(nl, rows, b, sb, invd(0), sd(0), mg->omega, L.mask, x,
+                                       nl, nullptr, 0, active, s))
+          return e;
+      }
+    } else {
+      b = buf(L.vb);
+      sb = nl;
+      if (ns >= 1) {
+        if (int e = jacobi_first<T, double>(nl, rows, r, rs, invd(0), sd(0), mg->omega,
+                                            L.mask, x, nl, buf(L.vb), nl, active, s))
+          return e;
+      } else if (int e = convert<double, T>(nl, rows, r, rs, buf(L.vb), nl, active, s)) {
+        return e;
+      }
+    }
+    if (ns >= 1) {
+      if (int e = sweeps(0, ns - 1, x, alt, b, sb)) return e;
+    } else if (int e = fill<T>(nl, rows, T(0), x, nl, active, s)) {
+      return e;
+    }
+    if (int e = coarse_correct(0, x, b, sb)) return e;
+    if (ns >= 1) {
+      if (int e = sweeps(0, ns - 1, x, alt, b, sb)) return e;
+      // last sweep writes z in fp64 and the r.z partials in the same pass
+      return stencil<T, double, double>(op[0], rows, kSmooth, false,
+                                        part ? kDotB : kNoDot, x, nl, r, rs, invd(0), sd(0),
+                                        z, zs, mg->omega, active, part, s);
+    }
+    if (int e = convert<T, double>(nl, rows, x, nl, z, zs, active, s)) return e;
+    return part ? dot_rz(r, rs, z, zs, part) : STTOPT_OK;
+  }
+
+  int dot_rz(const double* r, long long rs, const double* z, long long zs, double* part) {
+    TileGeom g{op[0].nnx(), op[0].nny(), op[0].ntx(), op[0].tiles(),
+               op[0].tiles() * rows, mg->d};
+    hipLaunchKernelGGL(dot_kernel, dim3(g.total), dim3(kBlock), 0, s, g, n[0], r, rs, z, zs,
+                       op[0].mask, active, part);
+    STT_CHECK_LAUNCH("dot_kernel");
+    return STTOPT_OK;
+  }
+};
+
+static int validate(const sttopt_mg* mg) {
+  if (mg == nullptr || mg->rows < 0 || (mg->d != 1 && mg->d != 2) || mg->nlevels < 1 ||
+      mg->nlevels > STTOPT_MAX_LEVELS || mg->n_smooth < 0 || mg->gamma < 1 ||
+      (mg->vdtype != STTOPT_F64 && mg->vdtype != STTOPT_F32)) {
+    set_error("sttopt_mg: invalid hierarchy header");
+    return STTOPT_EINVAL;
+  }
+  for (int l = 0; l < mg->nlevels; ++l) {
+    const sttopt_level& L = mg->levels[l];
+    if (L.nelx <= 0 || L.nely <= 0) {
+      set_error("sttopt_mg: level with an empty grid");
+      return STTOPT_EINVAL;
+    }
+    const bool last = l == mg->nlevels - 1;
+    if (last && L.cinv == nullptr) {
+      set_error("sttopt_mg: coarsest level needs its dense inverse");
+      return STTOPT_EINVAL;
+    }
+    if (!last) {
+      const sttopt_level& C = mg->levels[l + 1];
+      if (L.kx < 1 || L.ky < 1 || L.nelx != C.nelx * L.kx || L.nely != C.nely * L.ky) {
+        set_error("sttopt_mg: level grids do not match their coarsening factors");
+        return STTOPT_EINVAL;
+      }
+      if (L.invdiag == nullptr || L.vx == nullptr || L.vx2 == nullptr || L.vr == nullptr) {
+        set_error("sttopt_mg: level is missing its diagonal or work vectors");
+        return STTOPT_EINVAL;
+      }
+      if (l == 0 && L.density == nullptr) {
+        set_error("sttopt_mg: level 0 needs the density in the V-cycle precision");
+        return STTOPT_EINVAL;
+      }
+      if (mg->gamma > 1 && (C.ve == nullptr || C.vw == nullptr)) {
+        set_error("sttopt_mg: gamma > 1 needs ve/vw work vectors");
+        return STTOPT_EINVAL;
+      }
+    } else if (l > 0 && L.vx == nullptr) {
+      set_error("sttopt_mg: coarsest level is missing its work vector");
+      return STTOPT_EINVAL;
+    }
+    if (l > 0 && (L.vb == nullptr || (!last && L.keff == nullptr))) {
+      set_error("sttopt_mg: coarse level is missing keff or its right-hand-side vector");
+      return STTOPT_EINVAL;
+    }
+    if (l == 0 && mg->vdtype == STTOPT_F32 && mg->nlevels > 1 && L.vb == nullptr) {
+      set_error("sttopt_mg: an fp32 V-cycle needs the level-0 vb vector");
+      return STTOPT_EINVAL;
+    }
+  }
+  return STTOPT_OK;
+}
+
+static int* pinned_counter() {
+  static thread_local int* p = nullptr;
+  if (p == nullptr) {
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), 64, hipHostMallocDefault) != hipSuccess)
+      p = nullptr;
+  }
+  return p;
+}
+
+static int read_counter(const int* dev, hipStream_t s, int* out) {
+  int* h = pinned_counter();
+  if (h == nullptr) {
+    set_error("sttopt_mgpcg: hipHostMalloc of the readback word failed");
+    return STTOPT_EHIP;
+  }
+  hipError_t e = hipMemcpyAsync(h, dev, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(counter)");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+  *out = *h;
+  return STTOPT_OK;
+}
+
+template <typename T>
+static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double* x,
+                      long long xs, double rtol, int max_iter, int flexible, int check_every,
+                      double* work, int* iters, double* resid, int* n_iter, hipStream_t s) {
+  const int rows = mg->rows;
+  OpDesc op;
+  op.d = mg->d;
+  op.nelx = mg->levels[0].nelx;
+  op.nely = mg->levels[0].nely;
+  op.mask = mg->levels[0].mask;
+  op.rho = mg->density64;
+  op.sr = mg->density64_stride;
+  op.rdiag = mg->rdiag64;
+  op.srd = mg->rdiag64_stride;
+  for (int i = 0; i < 64; ++i) op.ke[i] = mg->ke[i];
+  const long long n = op.ndof();
+  const int tiles = op.tiles();
+  Scratch sc = carve(mg->scratch, rows, tiles);
+  double* r = work;
+  double* z = work + rows * n;
+  double* p = work + 2 * rows * n;
+  double* Ap = work + 3 * rows * n;
+  TileGeom g{op.nnx(), op.nny(), op.ntx(), tiles, tiles * rows, mg->d};
+  const dim3 tgrid(g.total), tblock(kBlock), rgrid(rows);
+  *n_iter = 0;
+
+  // x <- P x0;  r = P b - A x;  relative residual; active flags
+  if (int e = project_f64(n, rows, op.mask, x, xs, s)) return e;
+  if (int e = stencil<double, double, double>(op, rows, kResid, false, kNoDot, x, xs, b, bs,
+                                              nullptr, 0, r, n, 0.0, nullptr, nullptr, s))
+    return e;
+  hipLaunchKernelGGL(dot_kernel, tgrid, tblock, 0, s, g, n, b, bs, nullptr, 0, op.mask,
+                     nullptr, sc.part[4]);
+  hipLaunchKernelGGL(dot_kernel, tgrid, tblock, 0, s, g, n, r, n, nullptr, 0, op.mask,
+                     nullptr, sc.part[1]);
+  if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
+  hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4], sc.part[1],
+                     rtol, sc.bsafe, resid, sc.active, iters, sc.counter);
+  STT_CHECK_LAUNCH("pcg init");
+  int remaining = 0;
+  if (int e = read_counter(sc.counter, s, &remaining)) return e;
+  if (remaining == 0) return STTOPT_OK;
+
+  VCycle<T> vc(mg, s, sc.active);
+  if (int e = vc.run(r, n, z, n, sc.part[2])) return e;
+  hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.active, sc.rz);
+  if (int e = convert<double, double>(n, rows, z, n, p, n, sc.active, s)) return e;
+  STT_CHECK_LAUNCH("pcg first preconditioner");
+
+  if (check_every < 1) check_every = 1;
+  bool done = false;
+  for (int k = 1; k <= max_iter; ++k) {
+    if (int e = stencil<double, double, double>(op, rows, kOperator, false, kDotU, p, n,
+                                                (const double*)nullptr, 0, nullptr, 0, Ap, n,
+                                                0.0, sc.active, sc.part[0], s))
+      return e;
+    hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
+                       sc.rz, sc.alpha);
+    hipLaunchKernelGGL(pcg_update_kernel, tgrid, tblock, 0, s, g, n, sc.alpha, x, r, p, Ap,
+                       flexible ? z : nullptr, sc.active, sc.part[1], sc.part[3]);
+    if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
+    hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, tiles, sc.part[1], rtol, k,
+                       sc.bsafe, resid, sc.active, iters, sc.counter);
+    STT_CHECK_LAUNCH("pcg update");
+    if (k % check_every == 0 || k == max_iter) {
+      if (int e = read_counter(sc.counter, s, &remaining)) return e;
+      if (remaining == 0) {
+        done = true;
+        break;
+      }
+    }
+    if (k == max_iter) break;
+    if (int e = vc.run(r, n, z, n, sc.part[2])) return e;
+    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.part[3],
+                       flexible, sc.active, sc.rz, sc.beta);
+    hipLaunchKernelGGL(pcg_pupdate_kernel, tgrid, tblock, 0, s, g, n, sc.beta, p, z,
+                       sc.active);
+    STT_CHECK_LAUNCH("pcg direction");
+  }
+  // n_iter: the slowest row's count
+  std::vector<int> h(rows > 0 ? rows : 1);
+  hipError_t e = hipMemcpyAsync(h.data(), iters, sizeof(int) * rows, hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(iters)");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+  int mx = 0;
+  for (int i = 0; i < rows; ++i) mx = h[i] > mx ? h[i] : mx;
+  *n_iter = done ? mx : max_iter;
+  if (!done) {
+    set_error("sttopt_mgpcg: rows above rtol after max_iter iterations");
+    return STTOPT_ENOCONV;
+  }
+  return STTOPT_OK;
+}
+
+}  // namespace stt
+
+using namespace stt;
+
+extern "C" {
+
+int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg) {
+  if (mg == nullptr || mg->nlevels < 1) return 0;
+  OpDesc op;
+  op.d = mg->d;
+  op.nelx = mg->levels[0].nelx;
+  op.nely = mg->levels[0].nely;
+  return scratch_bytes(mg->rows, op.tiles());
+}
+
+int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {
+  if (mg == nullptr || mg->nlevels < 1) return 0;
+  const long long n = (long long)(mg->levels[0].nelx + 1) * (mg->levels[0].nely + 1) * mg->d;
+  return 4LL * mg->rows * n;
+}
+
+int sttopt_vcycle(const sttopt_mg* mg, const double* r, int64_t r_stride, double* z,
+                  int64_t z_stride, void* stream) {
+  if (int e = validate(mg)) return e;
+  if (r == nullptr || z == nullptr) {
+    set_error("sttopt_vcycle: null vector");
+    return STTOPT_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (mg->vdtype == STTOPT_F64) {
+    VCycle<double> vc(mg, s, nullptr);
+    return vc.run(r, r_stride, z, z_stride, nullptr);
+  }
+  VCycle<float> vc(mg, s, nullptr);
+  return vc.run(r, r_stride, z, z_stride, nullptr);
+}
+
+int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v, int64_t v_stride,
+                       void* y, int64_t y_stride, void* stream) {
+  if (int e = validate(mg)) return e;
+  if (level < 0 || level >= mg->nlevels || v == nullptr || y == nullptr) {
+    set_error("sttopt_level_apply: bad level or null vector");
+    return STTOPT_EINVAL;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (mg->vdtype == STTOPT_F64) {
+    VCycle<double> vc(mg, s, nullptr);
+    return stencil<double, double, double>(vc.op[level], mg->rows, kOperator, true, kNoDot,
+                                           static_cast<const double*>(v), v_stride, nullptr,
+                                           0, nullptr, 0, static_cast<double*>(y), y_stride,
+                                           0.0, nullptr, nullptr, s);
+  }
+  VCycle<float> vc(mg, s, nullptr);
+  return stencil<float, float, float>(vc.op[level], mg->rows, kOperator, true, kNoDot,
+                                      static_cast<const float*>(v), v_stride, nullptr, 0,
+                                      nullptr, 0, static_cast<float*>(y), y_stride, 0.0,
+                                      nullptr, nullptr, s);
+}
+
+int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double* x,
+                 int64_t x_stride, double rtol, int32_t max_iter, int32_t flexible,
+                 int32_t check_every, double* work, int32_t* iters, double* resid,
+                 int32_t* n_iter, void* stream) {
+  if (int e = validate(mg)) return e;
+  if (b == nullptr || x == nullptr || work == nullptr || iters == nullptr ||
+      resid == nullptr || n_iter == nullptr || mg->scratch == nullptr ||
+      mg->density64 == nullptr || max_iter < 0) {
+    set_error("sttopt_mgpcg: null pointer or negative max_iter");
+    return STTOPT_EINVAL;
+  }
+  if (mg->rows == 0) {
+    *n_iter = 0;
+    return STTOPT_OK;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  if (mg->vdtype == STTOPT_F64)
+    return mgpcg_impl<double>(mg, b, b_stride, x, x_stride, rtol, max_iter, flexible,
+                              check_every, work, iters, resid, n_iter, s);
+  return mgpcg_impl<float>(mg, b, b_stride, x, x_stride, rtol, max_iter, flexible,
+                           check_every, work, iters, resid, n_iter, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,589 @@
+"""Geometric-multigrid hierarchy and V-cycle on the gfx950 HIP kernels.
+
+Drop-in for the reference's `sttopt.torch_mg` (sttopt/torch_mg.py:50-535): same
+coarsening rule, exact element-wise Galerkin coarse operators, projected transfers,
+damped-Jacobi smoothing with equal pre/post sweeps and a dense coarsest solve, so
+the V-cycle is the same SPD preconditioner.  What moves to the device:
+
+- `build_hierarchy`: Galerkin products (`sttopt_galerkin`), coarse diagonals, the
+  dense coarsest operator (`sttopt_dense_assemble`); its Cholesky factor and
+  inverse come from torch.linalg on the device.  Coarse element matrices are kept
+  entry-major (`[row][entry][element]`) so the coarse apply reads them coalesced.
+- `VCycle.__call__`: one native call (`sttopt_vcycle`).
+- `solve` / `pcg(levels[0], b, VCycle)`: the whole MG-PCG in `sttopt_mgpcg`.
+
+Precision.  The V-cycle runs in `VCYCLE_DTYPE` (fp32 by default): smoother, grid
+transfers and coarse applies in fp32, the coarsest solve through the fp64 dense
+inverse.  The outer PCG (operator, vectors, every inner product and norm) stays
+fp64, and the fp32 V-cycle is paired with the flexible (Polak-Ribiere) beta, which
+tolerates the preconditioner's rounding-level asymmetry.  Set `VCYCLE_DTYPE =
+torch.float64` (or pass `dtype=`) for the all-fp64 cycle of the reference.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import functools
+from dataclasses import dataclass, field
+
+import torch
+from torch import Tensor
+
+from . import _grid, _lib
+from . import fem as _fem
+from . import torch_fem
+
+#: Stop coarsening once a level has at most this many elements (torch_mg.py:47-50).
+MAX_COARSE_ELEMENTS = 700
+
+#: Precision of the V-cycle preconditioner (the outer PCG is always fp64).
+VCYCLE_DTYPE = torch.float32
+
+#: PCG iterations between host reads of the active-row count.
+CHECK_EVERY = 1
+
+
+def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
+    """Per-dimension factor: the smallest of 2, 3 dividing it, else 1 (torch_mg.py:53-72)."""
+
+    def one(n: int) -> int:
+        for k in (2, 3):
+            if n % k == 0:
+                return k
+        return 1
+
+    return one(nelx), one(nely)
+
+
+def _interp_axis(c: Tensor, k: int) -> Tensor:
+    """Linear interpolation by factor `k` along the last axis, `nc+1 -> k*nc+1` nodes."""
+    if k == 1:
+        return c
+    w = torch.arange(k, dtype=c.dtype, device=c.device) / k
+    left, right = c[..., :-1, None], c[..., 1:, None]
+    inner = (left * (1 - w) + right * w).flatten(-2)
+    return torch.cat([inner, c[..., -1:]], dim=-1)
+
+
+def _restrict_axis(f: Tensor, k: int) -> Tensor:
+    """Transpose of `_interp_axis` (full weighting) along the last axis."""
+    if k == 1:
+        return f
+    w = torch.arange(k, dtype=f.dtype, device=f.device) / k
+    blocks = f[..., :-1].unflatten(-1, (-1, k))
+    own = (blocks * (1 - w)).sum(-1)  # coarse node i from fine k*i + j, j < k
+    nxt = (blocks * w).sum(-1)  # coarse node i + 1 from the same fine nodes
+    out = torch.zeros(*f.shape[:-1], own.shape[-1] + 1, dtype=f.dtype, device=f.device)
+    out[..., :-1] += own
+    out[..., 1:] += nxt
+    out[..., -1] += f[..., -1]
+    return out
+
+
+def _on_node_grid(v: Tensor, nnx: int, nny: int, kx: int, ky: int, axis_op) -> Tensor:
+    """Apply `axis_op(t, k)` along x then y of a dof vector on a `(nny, nnx, d)` node grid."""
+    d = v.shape[-1] // (nnx * nny)
+    g = v.reshape(*v.shape[:-1], nny, nnx, d)
+    g = axis_op(g.transpose(-1, -2), kx).transpose(-1, -2)
+    g = axis_op(g.movedim(-3, -1), ky).movedim(-1, -3)
+    return g.reshape(*v.shape[:-1], -1)
+
+
+@functools.lru_cache(maxsize=None)
+def element_dof_map_tensor(nelx: int, nely: int, device, dofs_per_node: int = 2) -> Tensor:
+    """`fem.element_dof_map` as a cached int64 device tensor (torch_mg.py:112-127)."""
+    return torch.tensor(
+        _fem.element_dof_map(nelx, nely, dofs_per_node), dtype=torch.int64, device=device
+    )
+
+
+@functools.lru_cache(maxsize=None)
+def _child_elements(nelx: int, nely: int, kx: int, ky: int, device) -> Tensor:
+    """Fine elements under each coarse element, children ordered row-major in the block."""
+    e = torch.arange(nelx * nely, device=device).reshape(nely // ky, ky, nelx // kx, kx)
+    return e.permute(0, 2, 1, 3).reshape(-1, ky * kx)
+
+
+@functools.lru_cache(maxsize=None)
+def child_interpolation_matrices(
+    kx: int, ky: int, device=None, dtype=torch.float64, dofs_per_node: int = 2
+) -> Tensor:
+    """`S[m]`: child element m's nodal values from its coarse element's, under the
+    same bilinear interpolation as the grid transfer, shape `(kx*ky, 4d, 4d)`
+    (torch_mg.py:140-165).  Derived by running `_interp_axis` on a one-element mesh."""
+    d = dofs_per_node
+    coarse_edof = torch.tensor(_fem.element_dof_map(1, 1, d)[0])
+    fine_edof = torch.tensor(_fem.element_dof_map(kx, ky, d))
+    basis = torch.zeros(4 * d, 4 * d, dtype=torch.float64)
+    basis[torch.arange(4 * d), coarse_edof] = 1.0
+    fine = _on_node_grid(basis, 2, 2, kx, ky, _interp_axis)  # (coarse dof, fine dof)
+    S = fine[:, fine_edof].permute(1, 2, 0)
+    return S.to(device=device, dtype=dtype).contiguous()
+
+
+def _to_soa(keff: Tensor) -> Tensor:
+    """(*b, nel, k, k) -> (rows, k*k, nel) entry-major."""
+    nel, k = keff.shape[-3], keff.shape[-1]
+    return keff.reshape(-1, nel, k * k).transpose(1, 2).contiguous()
+
+
+def _soa_view(soa: Tensor, k: int, dbatch: tuple[int, ...]) -> Tensor:
+    """(rows, k*k, nel) -> (*dbatch, nel, k, k) view."""
+    rows, _, nel = soa.shape
+    return soa.reshape(rows, k, k, nel).permute(0, 3, 1, 2).reshape(*dbatch, nel, k, k)
+
+
+def keff_matvec(U: Tensor, keff: Tensor, edofMat: Tensor, ndof: int) -> Tensor:
+    """Matvec of a level with one matrix per element (torch_mg.py:179-193)."""
+    dev = _lib.require_device(U, keff)
+    k = keff.shape[-1]
+    d = k // 4
+    nelx, nely = _grid.grid_of(edofMat, d, ndof)
+    soa = _to_soa(keff.to(U.dtype))
+    kb = tuple(keff.shape[:-3])
+    batch = _grid.batch_shape(U.shape[:-1], kb)
+    rows = _grid.numel(batch)
+    if soa.shape[0] not in (1, rows):
+        soa = _to_soa(keff.expand(*batch, *keff.shape[-3:]).to(U.dtype))
+    sk = 0 if soa.shape[0] == 1 else soa.stride(0)
+    u2, su = _grid.rows_of(U, ndof, batch)
+    out = torch.empty(rows, ndof, dtype=U.dtype, device=dev)
+    _lib.check(
+        _lib.load().sttopt_apply_keff(
+            _lib.dtype_code(U.dtype), d, nelx, nely, rows, u2.data_ptr(), su,
+            soa.data_ptr(), sk, None, out.data_ptr(), ndof, _lib.stream_handle(dev),
+        ),
+        "sttopt_apply_keff",
+    )
+    return out.reshape(*batch, ndof)
+
+
+def _select_rows(t: Tensor | None, rows: Tensor, core_ndim: int) -> Tensor | None:
+    if t is None or t.ndim == core_ndim:
+        return t
+    if t.ndim != core_ndim + 1:
+        raise AssertionError(f"row selection needs a single batch dim, got shape {tuple(t.shape)}")
+    return t[rows]
+
+
+@dataclass
+class _Level:
+    """One grid of the hierarchy; calling it applies `A(v) = P(K P v)` on that grid
+    (torch_mg.py:211-266).  Level 0 carries `density` (and `reaction_diag`); coarser
+    levels carry `keff`, a `(*batch, nel, k, k)` view of the entry-major storage."""
+
+    nelx: int
+    nely: int
+    ndof: int
+    mask: Tensor
+    diag: Tensor
+    edofMat: Tensor
+    KE: Tensor | None = None
+    density: Tensor | None = None
+    keff: Tensor | None = None
+    reaction_diag: Tensor | None = None
+    kx: int = 1
+    ky: int = 1
+    chol: Tensor | None = None
+    # device storage used by the kernels
+    _soa: Tensor | None = field(default=None, repr=False)  # (rows, k*k, nel) fp64
+    _cinv: Tensor | None = field(default=None, repr=False)  # (rows, n, n) fp64
+    _dbatch: tuple = field(default=(), repr=False)
+    _copies: dict = field(default_factory=dict, repr=False)
+
+    @property
+    def d(self) -> int:
+        return self.ndof // ((self.nelx + 1) * (self.nely + 1))
+
+    @property
+    def rows(self) -> int:
+        return _grid.numel(self._dbatch)
+
+    def bits(self) -> Tensor:
+        return _grid.mask_bits(self.mask)
+
+    def __call__(self, v: Tensor) -> Tensor:
+        if self.keff is None:
+            return torch_fem.operator(
+                v, self.density, self.edofMat, self.KE, self.ndof, self.mask, self.reaction_diag
+            )
+        dev = _lib.require_device(v)
+        batch = _grid.batch_shape(v.shape[:-1], self._dbatch)
+        rows = _grid.numel(batch)
+        soa = self._soa.to(v.dtype)
+        if soa.shape[0] not in (1, rows):
+            raise NotImplementedError("level rows must be 1 or match the vector batch")
+        sk = 0 if soa.shape[0] == 1 else soa.stride(0)
+        u2, su = _grid.rows_of(v, self.ndof, batch)
+        out = torch.empty(rows, self.ndof, dtype=v.dtype, device=dev)
+        _lib.check(
+            _lib.load().sttopt_apply_keff(
+                _lib.dtype_code(v.dtype), self.d, self.nelx, self.nely, rows, u2.data_ptr(),
+                su, soa.data_ptr(), sk, self.bits().data_ptr(), out.data_ptr(), self.ndof,
+                _lib.stream_handle(dev),
+            ),
+            "sttopt_apply_keff",
+        )
+        return out.reshape(*batch, self.ndof)
+
+    def select(self, rows: Tensor) -> "_Level":
+        """This level restricted to batch rows `rows` (shared tensors pass through)."""
+        soa = self._soa
+        dbatch = self._dbatch
+        if len(dbatch) == 1:
+            dbatch = (int(rows.numel()),)
+            if soa is not None:
+                soa = soa[rows]
+        cinv = _select_rows(self._cinv, rows, 2) if self._cinv is not None and len(self._dbatch) == 1 else self._cinv
+        k = 4 * self.d
+        return dataclasses.replace(
+            self,
+            diag=_select_rows(self.diag, rows, 1),
+            density=_select_rows(self.density, rows, 1),
+            reaction_diag=_select_rows(self.reaction_diag, rows, 1),
+            keff=None if soa is None else _soa_view(soa, k, dbatch),
+            chol=_select_rows(self.chol, rows, 2),
+            _soa=soa,
+            _cinv=cinv,
+            _dbatch=dbatch,
+            _copies={},
+        )
+
+    # V-cycle data in a given precision, cached per level
+    def vdata(self, dtype: torch.dtype) -> dict:
+        hit = self._copies.get(dtype)
+        if hit is not None:
+            return hit
+        rows = max(self.rows, 1)
+        out = {}
+        dev = self.mask.device
+        diag = self.diag.detach().to(torch.float64).reshape(rows, self.ndof)
+        inv = torch.empty(rows, self.ndof, dtype=dtype, device=dev)
+        _lib.check(
+            _lib.load().sttopt_invert_diagonal(
+                _lib.dtype_code(dtype), self.ndof, rows, diag.data_ptr(), self.ndof,
+                self.bits().data_ptr(), inv.data_ptr(), self.ndof, _lib.stream_handle(dev),
+            ),
+            "sttopt_invert_diagonal",
+        )
+        out["invdiag"] = inv
+        if self.keff is None:
+            out["density"] = self.density.detach().to(dtype).reshape(rows, -1).contiguous()
+            if self.reaction_diag is not None:
+                out["rdiag"] = self.reaction_diag.detach().to(dtype).reshape(rows, -1).contiguous()
+        else:
+            out["keff"] = self._soa if dtype == torch.float64 else self._soa.to(dtype)
+        self._copies[dtype] = out
+        return out
+
+
+def _dense_factor(level: _Level, rows: int) -> None:
+    """Dense projected operator of `level`, its Cholesky factor and inverse
+    (torch_mg._dense_cholesky, torch_mg.py:269-290)."""
+    dev = level.mask.device
+    n = level.ndof
+    dense = torch.empty(rows, n, n, dtype=torch.float64, device=dev)
+    if level.keff is not None:
+        args = (level._soa.data_ptr(), level._soa.stride(0), None, 0, None, 0, None)
+    else:
+        den = level.density.detach().to(torch.float64).reshape(rows, -1).contiguous()
+        rd = level.reaction_diag
+        rd = None if rd is None else rd.detach().to(torch.float64).reshape(rows, -1).contiguous()
+        ke_dev = level.KE.detach().to(device=dev, dtype=torch.float64).contiguous()
+        level._copies["_dense_keep"] = (den, rd, ke_dev)
+        args = (None, 0, den.data_ptr(), den.stride(0), _lib.ptr(rd),
+                0 if rd is None else rd.stride(0), ke_dev.data_ptr())
+    _lib.check(
+        _lib.load().sttopt_dense_assemble(
+            level.d, level.nelx, level.nely, rows, *args, level.bits().data_ptr(),
+            dense.data_ptr(), n * n, _lib.stream_handle(dev),
+        ),
+        "sttopt_dense_assemble",
+    )
+    chol = torch.linalg.cholesky(dense)
+    level._cinv = torch.cholesky_inverse(chol).contiguous()
+    level._copies.pop("_dense_keep", None)
+    level.chol = chol.reshape(*level._dbatch, n, n)
+
+
+def build_hierarchy(
+    density: Tensor,
+    edofMat: Tensor,
+    KE: Tensor,
+    mask: Tensor,
+    nelx: int,
+    nely: int,
+    *,
+    reaction: Tensor | None = None,
+    max_coarse_elements: int = MAX_COARSE_ELEMENTS,
+) -> list[_Level]:
+    """The multigrid levels for one density field or a batch of them, finest first
+    (torch_mg.py:293-400), built on the device."""
+    dev = _lib.require_device(density, mask, reaction)
+    ndof = mask.shape[-1]
+    d = _grid.dofs_per_node(KE)
+    if _grid.grid_of(edofMat, d, ndof) != (nelx, nely):
+        raise ValueError(f"edofMat does not describe a {nelx}x{nely} grid")
+    dbatch = tuple(density.shape[:-1])
+    rows = max(_grid.numel(dbatch), 1)
+    nel = nelx * nely
+    den64 = density.detach().to(torch.float64).reshape(rows, nel).contiguous()
+    react64 = None
+    reaction_diag = None
+    if reaction is not None:
+        react64 = reaction.detach().to(torch.float64).expand(*dbatch, nel).reshape(rows, nel).contiguous()
+        reaction_diag = torch_fem.lumped_reaction(reaction.detach(), edofMat, ndof)
+    diag = torch_fem.jacobi_preconditioner_diag(
+        density.detach(), edofMat, KE, ndof, mask, reaction_diag
+    )
+    KE64 = KE.detach().to(device=dev, dtype=torch.float64)
+    levels = [
+        _Level(nelx=nelx, nely=nely, ndof=ndof, mask=mask, diag=diag, edofMat=edofMat,
+               KE=KE, density=density.detach(), reaction_diag=reaction_diag, _dbatch=dbatch)
+    ]
+    lib = _lib.load()
+    stream = _lib.stream_handle(dev)
+    k = 4 * d
+    soa = None
+    while True:
+        top = levels[-1]
+        if top.nelx * top.nely <= max_coarse_elements:
+            break
+        kx, ky = coarsening_factors(top.nelx, top.nely)
+        if (kx, ky) == (1, 1):
+            break
+        S = child_interpolation_matrices(kx, ky, dev, torch.float64, d)
+        cnelx, cnely = top.nelx // kx, top.nely // ky
+        csoa = torch.empty(rows, k * k, cnelx * cnely, dtype=torch.float64, device=dev)
+        if soa is None:
+            G = torch.einsum("mpi,pq,mqj->mij", S, KE64, S).contiguous()
+            GM = None
+            if react64 is not None:
+                ME = torch.eye(k, dtype=torch.float64, device=dev) / 4
+                GM = torch.einsum("mpi,pq,mqj->mij", S, ME, S).contiguous()
+            _lib.check(
+                lib.sttopt_galerkin(
+                    d, top.nelx, top.nely, kx, ky, rows, den64.data_ptr(), nel,
+                    _lib.ptr(react64), nel, G.data_ptr(), _lib.ptr(GM), None, 0, None,
+                    csoa.data_ptr(), csoa.stride(0), stream,
+                ),
+                "sttopt_galerkin",
+            )
+        else:
+            _lib.check(
+                lib.sttopt_galerkin(
+                    d, top.nelx, top.nely, kx, ky, rows, None, 0, None, 0, None, None,
+                    soa.data_ptr(), soa.stride(0), S.data_ptr(), csoa.data_ptr(),
+                    csoa.stride(0), stream,
+                ),
+                "sttopt_galerkin",
+            )
+        soa = csoa
+        top.kx, top.ky = kx, ky
+        cndof = d * (cnelx + 1) * (cnely + 1)
+        cmask = top.mask.reshape(top.nely + 1, top.nelx + 1, d)[::ky, ::kx].reshape(-1).contiguous()
+        cdiag = torch.empty(rows, cndof, dtype=torch.float64, device=dev)
+        _lib.check(
+            lib.sttopt_keff_diagonal(
+                _lib.F64, d, cnelx, cnely, rows, soa.data_ptr(), soa.stride(0),
+                _grid.mask_bits(cmask).data_ptr(), 0, cdiag.data_ptr(), cndof, stream,
+            ),
+            "sttopt_keff_diagonal",
+        )
+        levels.append(
+            _Level(
+                nelx=cnelx, nely=cnely, ndof=cndof, mask=cmask,
+                diag=cdiag.reshape(*dbatch, cndof).to(density.dtype),
+                edofMat=element_dof_map_tensor(cnelx, cnely, dev, d),
+                keff=_soa_view(soa, k, dbatch), _soa=soa, _dbatch=dbatch,
+            )
+        )
+    _dense_factor(levels[-1], rows)
+    return levels
+
+
+class VCycle:
+    """Multigrid V-cycle as an SPD preconditioner `z = M(r)` (torch_mg.py:403-476).
+
+    :param levels: `build_hierarchy` output, finest first
+    :param omega: damped-Jacobi factor
+    :param n_smooth: pre- and post-smoothing sweeps per level
+    :param gamma: coarse visits per level (1: V-cycle, 2: W-cycle)
+    :param dtype: V-cycle precision (default `VCYCLE_DTYPE`)
+    """
+
+    def __init__(self, levels, *, omega: float = 0.6, n_smooth: int = 2, gamma: int = 1,
+                 dtype: torch.dtype | None = None) -> None:
+        self.levels = levels
+        self.omega = omega
+        self.n_smooth = n_smooth
+        self.gamma = gamma
+        self.dtype = VCYCLE_DTYPE if dtype is None else dtype
+        self._work: dict = {}
+
+    def select(self, rows: Tensor) -> "VCycle":
+        return VCycle([lv.select(rows) for lv in self.levels], omega=self.omega,
+                      n_smooth=self.n_smooth, gamma=self.gamma, dtype=self.dtype)
+
+    # -- native descriptor --------------------------------------------------
+
+    def _native(self, rows: int):
+        """The `sttopt_mg` descriptor for `rows` batch rows, with every buffer it
+        points at kept alive on this object."""
+        hit = self._work.get(rows)
+        if hit is not None:
+            return hit
+        lv0 = self.levels[0]
+        hrows = max(lv0.rows, 1)
+        if hrows not in (1, rows):
+            raise NotImplementedError(
+                f"hierarchy built for {hrows} rows cannot precondition {rows} rows"
+            )
+        shared = hrows == 1 and rows > 1
+        dev = lv0.mask.device
+        vd = self.dtype
+        mg = _lib.MG()
+        mg.rows, mg.d, mg.nlevels = rows, lv0.d, len(self.levels)
+        mg.vdtype = _lib.dtype_code(vd)
+        mg.n_smooth, mg.gamma, mg.omega = self.n_smooth, self.gamma, self.omega
+        ke = _lib.ke_array(lv0.KE)
+        for i in range(64):
+            mg.ke[i] = ke[i]
+        keep = []
+        den64 = lv0.density.detach().to(torch.float64).reshape(hrows, -1).contiguous()
+        keep.append(den64)
+        mg.density64 = den64.data_ptr()
+        mg.density64_stride = 0 if shared else den64.stride(0)
+        if lv0.reaction_diag is not None:
+            rd64 = lv0.reaction_diag.detach().to(torch.float64).reshape(hrows, -1).contiguous()
+            keep.append(rd64)
+            mg.rdiag64 = rd64.data_ptr()
+            mg.rdiag64_stride = 0 if shared else rd64.stride(0)
+        for l, lv in enumerate(self.levels):
+            L = mg.levels[l]
+            L.nelx, L.nely, L.kx, L.ky = lv.nelx, lv.nely, lv.kx, lv.ky
+            bits = lv.bits()
+            keep.append(bits)
+            L.mask = bits.data_ptr()
+            data = lv.vdata(vd)
+            inv = data["invdiag"]
+            L.invdiag = inv.data_ptr()
+            L.invdiag_stride = 0 if shared else inv.stride(0)
+            if "density" in data:
+                L.density = data["density"].data_ptr()
+                L.density_stride = 0 if shared else data["density"].stride(0)
+                if "rdiag" in data:
+                    L.rdiag = data["rdiag"].data_ptr()
+                    L.rdiag_stride = 0 if shared else data["rdiag"].stride(0)
+            if "keff" in data:
+                L.keff = data["keff"].data_ptr()
+                L.keff_stride = 0 if shared else data["keff"].stride(0)
+            if lv._cinv is not None:
+                L.cinv = lv._cinv.data_ptr()
+                L.cinv_stride = 0 if shared else lv._cinv.stride(0)
+            names = ["vb", "vx", "vx2", "vr"] + (["ve", "vw"] if self.gamma > 1 else [])
+            for name in names:
+                buf = torch.empty(rows, lv.ndof, dtype=vd, device=dev)
+                keep.append(buf)
+                setattr(L, name, buf.data_ptr())
+        nbytes = _lib.load().sttopt_mg_scratch_bytes(ctypes.byref(mg))
+        scratch = torch.empty(max(nbytes, 256), dtype=torch.uint8, device=dev)
+        keep.append(scratch)
+        mg.scratch = scratch.data_ptr()
+        work = torch.empty(_lib.load().sttopt_mgpcg_work_count(ctypes.byref(mg)),
+                           dtype=torch.float64, device=dev)
+        entry = (mg, keep, work)
+        self._work[rows] = entry
+        return entry
+
+    def __call__(self, r: Tensor) -> Tensor:
+        dev = _lib.require_device(r)
+        lv0 = self.levels[0]
+        batch = _grid.batch_shape(r.shape[:-1], lv0._dbatch)
+        rows = max(_grid.numel(batch), 1)
+        mg, _, _ = self._native(rows)
+        r2, rs = _grid.rows_of(r, lv0.ndof, batch, torch.float64)
+        z = torch.empty(rows, lv0.ndof, dtype=torch.float64, device=dev)
+        _lib.check(
+            _lib.load().sttopt_vcycle(ctypes.byref(mg), r2.data_ptr(), rs, z.data_ptr(),
+                                      lv0.ndof, _lib.stream_handle(dev)),
+            "sttopt_vcycle",
+        )
+        return z.reshape(*batch, lv0.ndof).to(r.dtype)
+
+    # -- native PCG (torch_fem.pcg fast path) ----------------------------------
+
+    def _native_pcg(self, apply_A, b: Tensor, *, rtol, max_iter, x0, info):
+        """`torch_fem.pcg(levels[0], b, self)` entirely in `sttopt_mgpcg`; None when
+        `apply_A` is not this hierarchy's level 0."""
+        if apply_A is not self.levels[0] or b.dtype != torch.float64:
+            return None
+        return native_mgpcg(self, b, rtol=rtol, max_iter=max_iter, x0=x0, info=info)
+
+
+def native_mgpcg(vcycle: VCycle, b: Tensor, *, rtol: float, max_iter: int,
+                 x0: Tensor | None = None, info: dict | None = None) -> tuple[Tensor, int]:
+    """Batched MG-PCG of `levels[0] x = b` on the device (`sttopt_mgpcg`).
+
+    :return: `(x, n_iter)`; x has b's shape (fixed dofs 0)
+    :raises CGConvergenceError: rows above `rtol` after `max_iter`
+    """
+    dev = _lib.require_device(b, x0)
+    lv0 = vcycle.levels[0]
+    batch = _grid.batch_shape(b.shape[:-1], lv0._dbatch)
+    rows = max(_grid.numel(batch), 1)
+    mg, _, work = vcycle._native(rows)
+    b2, bs = _grid.rows_of(b, lv0.ndof, batch, torch.float64)
+    if x0 is None:
+        x = torch.zeros(rows, lv0.ndof, dtype=torch.float64, device=dev)
+    else:
+        x = x0.detach().to(torch.float64).expand(*batch, lv0.ndof).reshape(rows, lv0.ndof).clone()
+    iters = torch.zeros(rows, dtype=torch.int32, device=dev)
+    resid = torch.zeros(rows, dtype=torch.float64, device=dev)
+    n_iter = ctypes.c_int32(0)
+    flexible = 1 if vcycle.dtype != torch.float64 else 0
+    status = _lib.load().sttopt_mgpcg(
+        ctypes.byref(mg), b2.data_ptr(), bs, x.data_ptr(), lv0.ndof, float(rtol),
+        int(max_iter), flexible, CHECK_EVERY, work.data_ptr(), iters.data_ptr(),
+        resid.data_ptr(), ctypes.byref(n_iter), _lib.stream_handle(dev),
+    )
+    if info is not None:
+        info["row_iters"] = int(iters.sum())
+        info["row_iter_list"] = iters.tolist()
+    if status == _lib.ENOCONV:
+        raise torch_fem.CGConvergenceError(resid.reshape(batch) if batch else resid[0], int(max_iter), rtol)
+    _lib.check(status, "sttopt_mgpcg")
+    return x.reshape(*batch, lv0.ndof), int(n_iter.value)
+
+
+def solve(
+    F: Tensor,
+    xPhys: Tensor,
+    edofMat: Tensor,
+    KE: Tensor,
+    Emin: float,
+    Emax: float,
+    penal: float,
+    mask: Tensor,
+    nelx: int,
+    nely: int,
+    *,
+    rtol: float = 1e-8,
+    max_iter: int = 500,
+    x0: Tensor | None = None,
+    omega: float = 0.6,
+    n_smooth: int = 2,
+    gamma: int = 1,
+    max_coarse_elements: int = MAX_COARSE_ELEMENTS,
+    info: dict | None = None,
+) -> tuple[Tensor, int]:
+    """Multigrid-preconditioned CG for `K U = F` (torch_mg.py:479-535)."""
+    _lib.require_device(F, xPhys, mask, x0)
+    density = torch_fem.simp_density(xPhys, Emin, Emax, penal)
+    levels = build_hierarchy(density, edofMat, KE, mask, nelx, nely,
+                             max_coarse_elements=max_coarse_elements)
+    precond = VCycle(levels, omega=omega, n_smooth=n_smooth, gamma=gamma)
+    U, n_iter = torch_fem.pcg(levels[0], torch_fem.project(F, mask), precond, rtol=rtol,
+                              max_iter=max_iter, x0=x0, info=info)
+    return torch_fem.project(U, mask), n_iter

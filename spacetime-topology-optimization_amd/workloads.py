@@ -1,0 +1,134 @@
+"""Seeded synthetic problems of the solve path (mesh, loads, densities, time fields).
+
+Every field is generated in fp64 on the host from `numpy.random.default_rng(seed)` and
+then moved to the device, so the same seed gives the same inputs everywhere.
+
+- `mbb`:        plane-stress half MBB beam, grey density clip(0.5 + 0.1 N, 1e-3, 1).
+- `space_time`: bottom edge clamped, staged self-weight (+ optional top-right point
+  load on the whole-structure row); density clip(mean + std N, 1e-3, 1); time field =
+  white noise, Gaussian-smoothed (sigma in elements, scipy "reflect" boundary) and
+  min-max scaled to [0, 1]; stage times t_k = k / n_stage.
+- `heat`:       scalar steady heat with conductivity 1e-3 + rho^3, T = 0 on the bottom
+  nodes, lumped unit volumetric source.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import scipy.ndimage
+import torch
+
+from . import fem as _fem
+
+EMIN, EMAX, PENAL, NU = 1e-9, 1.0, 3.0, 0.3
+
+
+def grey_density(nelx: int, nely: int, mean: float, std: float, rng) -> np.ndarray:
+    return np.clip(mean + std * rng.standard_normal((nely, nelx)), 1e-3, 1.0)
+
+
+def smooth_time_field(nelx: int, nely: int, sigma: float, rng) -> np.ndarray:
+    t = scipy.ndimage.gaussian_filter(rng.standard_normal((nely, nelx)), sigma, mode="reflect")
+    return (t - t.min()) / (t.max() - t.min())
+
+
+@dataclass
+class Problem:
+    nelx: int
+    nely: int
+    KE: torch.Tensor
+    edofMat: torch.Tensor
+    F: torch.Tensor
+    freedofs: torch.Tensor
+    mask: torch.Tensor
+    xPhys: torch.Tensor
+    ndof: int
+    tPhys: torch.Tensor | None = None
+    stage_times: list = field(default_factory=list)
+    beta: float = 16.0
+    C: object = None
+    max_coarse_elements: int = 700
+    name: str = ""
+
+    @property
+    def nel(self) -> int:
+        return self.nelx * self.nely
+
+
+def _problem(nelx, nely, case, x, device, **kw) -> Problem:
+    F, free = _fem.load_case(case, nelx, nely)
+    ndof = F.size
+    mask = torch.zeros(ndof, dtype=torch.bool)
+    mask[torch.as_tensor(free)] = True
+    return Problem(
+        nelx=nelx,
+        nely=nely,
+        KE=torch.tensor(_fem.plane_stress_KE(NU), dtype=torch.float64, device=device),
+        edofMat=torch.tensor(_fem.element_dof_map(nelx, nely), dtype=torch.int64, device=device),
+        F=torch.tensor(F, dtype=torch.float64, device=device),
+        freedofs=torch.tensor(free, dtype=torch.int64, device=device),
+        mask=mask.to(device),
+        xPhys=torch.tensor(x, dtype=torch.float64, device=device),
+        ndof=ndof,
+        **kw,
+    )
+
+
+def mbb(nelx: int = 120, nely: int = 40, seed: int = 0, device="cpu",
+        max_coarse_elements: int = 200) -> Problem:
+    """Plain method case: MBB half beam, one fp64 MG-PCG solve."""
+    rng = np.random.default_rng(seed)
+    x = grey_density(nelx, nely, 0.5, 0.1, rng)
+    return _problem(nelx, nely, "mbb", x, device, max_coarse_elements=max_coarse_elements,
+                    name=f"mbb_{nelx}x{nely}")
+
+
+def space_time(nelx: int, nely: int, n_stage: int, *, beta: float = 16.0, seed: int = 0,
+               mean: float = 0.4, std: float = 0.05, sigma: float = 4.0,
+               point_load: bool = True, device="cpu") -> Problem:
+    """Space-time self-weight case: `1 + n_stage` rows (whole structure + stages)."""
+    rng = np.random.default_rng(seed)
+    x = grey_density(nelx, nely, mean, std, rng)
+    t = smooth_time_field(nelx, nely, sigma, rng)
+    case = "bottom_clamped_tr" if point_load else "bottom_clamped"
+    p = _problem(nelx, nely, case, x, device, beta=beta,
+                 stage_times=[k / n_stage for k in range(1, n_stage + 1)],
+                 name=f"space_time_{nelx}x{nely}_nt{n_stage}")
+    p.tPhys = torch.tensor(t, dtype=torch.float64, device=device)
+    p.C = 1.0 / (nelx * nely)  # fe of the unit-total-weight self-weight stencil
+    return p
+
+
+@dataclass
+class HeatProblem:
+    nelx: int
+    nely: int
+    KE: torch.Tensor
+    edof: torch.Tensor
+    chi: torch.Tensor
+    F: torch.Tensor
+    fixed: torch.Tensor
+    rho: torch.Tensor
+
+
+def heat(nelx: int = 480, nely: int = 240, seed: int = 0, device="cpu") -> HeatProblem:
+    """Steady heat: chi = 1e-3 + rho^3, T = 0 on the bottom nodes, lumped unit source."""
+    rng = np.random.default_rng(seed)
+    rho = grey_density(nelx, nely, 0.5, 0.1, rng).ravel()
+    edof = _fem.element_dof_map(nelx, nely, 1)
+    nn = (nelx + 1) * (nely + 1)
+    F = np.bincount(edof.ravel(), weights=np.full(edof.size, 0.25), minlength=nn)
+    fixed = np.zeros(nn, dtype=bool)
+    fixed[_fem.node_grid(nelx, nely)[-1, :]] = True
+    return HeatProblem(
+        nelx=nelx,
+        nely=nely,
+        KE=torch.tensor(_fem.diffusion_KE(), dtype=torch.float64, device=device),
+        edof=torch.tensor(edof, dtype=torch.int64, device=device),
+        chi=torch.tensor(1e-3 + rho**3, dtype=torch.float64, device=device),
+        F=torch.tensor(F, dtype=torch.float64, device=device),
+        fixed=torch.tensor(fixed, device=device),
+        rho=torch.tensor(rho, dtype=torch.float64, device=device),
+    )
