@@ -1,0 +1,289 @@
+"""Benchmark: batched space-time stage solves + adjoint gradients per second.
+
+Workload (one "step"): the optimizer-step hot path on a 1024 x 512 Q4 plane-stress
+mesh (1,051,650 dofs): `compliance.batched_whole_and_gravity_compliance` with the
+whole-structure row (top-right unit point load) and 16 smoothed-Heaviside
+self-weight stages (beta 16, t_k = k/16) -- 17 rows solved to rel. residual 1e-8 by
+one batched MG-PCG (fp64 outer iteration, fp32 V-cycle, 6 levels) -- followed by the
+hand-written adjoint backward to dc/dx and dc/dt.  Inputs are seeded synthetic
+fields, resident in HBM before the timed region.
+
+value = 17 rows x steps / time  (solves with their gradients per second).
+
+Also reported: the roofline of the dominant kernel (the fp64 K(rho)u apply of the
+outer iteration) from HIP events on the solve stream, the plain fp64 one-row-at-a-
+time GPU path on the same rows, and the CPU baseline (the oracle's torch-CPU fp64
+restatement of the reference algorithm) on a bounded sample.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+NELX, NELY, NSTAGE, BETA = 1024, 512, 16, 16.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--nelx", type=int, default=NELX)
+    ap.add_argument("--nely", type=int, default=NELY)
+    ap.add_argument("--nstage", type=int, default=NSTAGE)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true")
+    return ap.parse_args()
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (rows shard "
+              "across GPUs without communication; see DESIGN.md). Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import __graft_entry__
+
+    __graft_entry__.package()
+    from sttopt_amd import _lib, compliance, torch_fem, workloads
+
+    dev = torch.device("cuda:0")
+    _lib.load()
+    p = workloads.space_time(args.nelx, args.nely, args.nstage, beta=BETA, seed=0, device=dev)
+    rows = 1 + args.nstage
+    E = (workloads.EMIN, workloads.EMAX, workloads.PENAL)
+    print(f"# workload {p.name}: {p.ndof} dofs x {rows} rows", file=sys.stderr, flush=True)
+
+    def step():
+        x = p.xPhys.clone().requires_grad_(True)
+        t = p.tPhys.clone().requires_grad_(True)
+        c, stages, U = compliance.batched_whole_and_gravity_compliance(
+            x, t, p.KE, p.edofMat, *E, p.freedofs, p.F, p.ndof, p.C, p.beta, p.stage_times)
+        (c + sum(stages)).backward()
+        return x.grad, t.grad
+
+    for i in range(args.warmup):
+        step()
+        print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        step()
+        iters.append(dict(compliance.LAST_SOLVE_INFO))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = rows * args.steps / elapsed
+    print(f"# timed {args.steps} steps: {ms_per_step:.2f} ms/step, n_iter {iters[-1]['n_iter']}",
+          file=sys.stderr, flush=True)
+
+    # -- roofline of the dominant kernel: fp64 K(rho)u apply on all rows ----------
+    roof = apply_roofline(torch, _lib, torch_fem, p, rows, dev)
+    print(f"# apply: {roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of HBM peak)",
+          file=sys.stderr, flush=True)
+
+    # -- the plain fp64 one-row-at-a-time GPU path -------------------------------
+    seq = None
+    if not args.no_sequential:
+        seq = sequential_fp64(torch, p, rows, E)
+        print(f"# fp64 one-row-at-a-time: {seq['rows_per_s']:.2f} rows/s", file=sys.stderr,
+              flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(p, E)
+        print(f"# cpu baseline: {cpu['value']:.4f} rows/s on {cpu['cores']} threads",
+              file=sys.stderr, flush=True)
+
+    out = {
+        "metric": "batched stage solves+adjoint grads/s, 1024x512 Q4, rtol 1e-8; % HBM peak",
+        "value": value,
+        "unit": "solves+grads/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"space-time self-weight {args.nelx}x{args.nely} Q4, 1 whole-structure "
+                        f"row + {args.nstage} stages (beta {BETA}), batched MG-PCG rtol 1e-8 "
+                        "+ hand-written adjoint",
+            "rows": rows,
+            "ndof": p.ndof,
+            "preconditioner": "fp32 V-cycle (damped Jacobi 2+2, omega 0.6), fp64 outer PCG",
+            "pcg_iterations": iters[-1]["n_iter"],
+            "row_iterations": iters[-1]["row_iter_list"],
+            "fp64_sequential_rows_per_s": None if seq is None else seq["rows_per_s"],
+            "speedup_vs_fp64_sequential": None if seq is None else value / seq["rows_per_s"],
+        },
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def apply_roofline(torch, _lib, torch_fem, p, rows, dev, launches=50):
+    """Average duration of the fp64 K(rho)u apply over all rows (HIP events on the
+    stream the kernel is launched on) against its algorithmic bytes
+    8 * (2 ndof + nel) per row."""
+    g = torch.Generator(device="cpu").manual_seed(0)
+    u = torch.randn(rows, p.ndof, generator=g, dtype=torch.float64).to(dev)
+    dens = torch.rand(rows, p.nel, generator=g, dtype=torch.float64).to(dev) + 0.1
+    y = torch.empty_like(u)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream(dev)
+    ke = _lib.ke_array(p.KE)
+    from sttopt_amd import _grid
+
+    bits = _grid.mask_bits(p.mask)
+    args = (_lib.F64, _lib.MODE_OPERATOR, 2, p.nelx, p.nely, rows, u.data_ptr(), p.ndof,
+            dens.data_ptr(), p.nel, ke, bits.data_ptr(), None, 0, y.data_ptr(), p.ndof,
+            stream.cuda_stream)
+    for _ in range(3):
+        _lib.check(lib.sttopt_apply(*args), "sttopt_apply")
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(launches):
+        lib.sttopt_apply(*args)
+    end.record(stream)
+    end.synchronize()
+    sec = start.elapsed_time(end) / 1e3 / launches
+    nbytes = rows * 8 * (2 * p.ndof + p.nel)
+    achieved = nbytes / sec / 1e9
+    return {
+        "kernel": "stencil_kernel<double,double,double,2,false> (fp64 K(rho)u, 17 rows)",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": None,
+        "algorithmic_bytes_per_launch": nbytes,
+        "avg_launch_us": sec * 1e6,
+    }
+
+
+def sequential_fp64(torch, p, rows, E):
+    """The plain fp64 path one row per call: SIMP rows and loads as in the fused
+    prologue, then FemSolve (fp64 V-cycle) + backward, one row at a time."""
+    from sttopt_amd import compliance, torch_fem, torch_mg, torch_solve
+
+    old = torch_mg.VCYCLE_DTYPE
+    torch_mg.VCYCLE_DTYPE = torch.float64
+    try:
+        dev = p.xPhys.device
+        mask = p.mask
+        dens_rows, loads = [], []
+        for k in range(rows):
+            if k == 0:
+                dens_rows.append(torch_fem.simp_density(p.xPhys, *E).flatten())
+                loads.append(p.F)
+            else:
+                m = compliance.time_mask(p.tPhys, p.stage_times[k - 1], p.beta)
+                xt = (p.xPhys * m).flatten()
+                dens_rows.append(torch_fem.simp_density(xt, *E))
+                F = torch.zeros(p.ndof, dtype=torch.float64, device=dev)
+                C = torch_sparse_gravity(torch, p)
+                F[1::2] = -(C @ xt)
+                loads.append(F)
+
+        def one(k):
+            d = dens_rows[k].detach().clone().requires_grad_(True)
+            U = torch_solve.femsolve(d, loads[k], p.edofMat, p.KE, mask, p.nelx, p.nely)
+            ce = compliance._element_strain_energy(U, p.edofMat, p.KE, p.nely, p.nelx).reshape(-1)
+            torch.sum(d * ce).backward()
+
+        one(0)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for k in range(rows):
+            one(k)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        return {"rows_per_s": rows / sec, "seconds": sec}
+    finally:
+        torch_mg.VCYCLE_DTYPE = old
+
+
+_C_CACHE = {}
+
+
+def torch_sparse_gravity(torch, p):
+    key = (p.nelx, p.nely)
+    if key not in _C_CACHE:
+        import numpy as np
+
+        from sttopt_amd import fem
+
+        C = fem.gravity_load_matrix(p.nelx, p.nely).tocsr()
+        _C_CACHE[key] = torch.sparse_csr_tensor(
+            torch.as_tensor(C.indptr), torch.as_tensor(C.indices), torch.as_tensor(C.data),
+            size=C.shape, dtype=torch.float64, device=p.xPhys.device)
+        assert np is not None
+    return _C_CACHE[key]
+
+
+def cpu_baseline(p, E):
+    """The oracle's torch-CPU fp64 restatement of the reference algorithm (index_add_
+    matvec, einsum Galerkin, dense Cholesky, retiring PCG, alpha*U adjoint) on a
+    bounded sample: the whole-structure row and one stage row of the same mesh,
+    forward solve + adjoint + element gradient, timed once after a warm-up at a
+    small mesh."""
+    import numpy as np
+    import torch
+
+    from oracle import fem_direct, mgcg_port
+
+    threads = os.cpu_count() or 1
+    torch.set_num_threads(threads)
+    nelx, nely = p.nelx, p.nely
+    edof = torch.as_tensor(fem_direct.element_dofs(nelx, nely))
+    KE = torch.as_tensor(fem_direct.plane_stress_KE(0.3))
+    mask = p.mask.cpu()
+    x = p.xPhys.cpu().numpy()
+    t = p.tPhys.cpu().numpy()
+    m = fem_direct.time_mask(t, p.stage_times[len(p.stage_times) // 2], p.beta).ravel()
+    xt = x.ravel() * m
+    Fg = np.zeros(p.ndof)
+    Fg[1::2] = -(fem_direct.gravity_matrix(nelx, nely) @ xt)
+    dens = torch.as_tensor(np.stack([fem_direct.simp(x.ravel(), *E), fem_direct.simp(xt, *E)]))
+    F = torch.stack([p.F.cpu(), torch.as_tensor(Fg)])
+    sample_rows = 2
+    t0 = time.perf_counter()
+    U, n, levels = mgcg_port.mg_solve(F, dens, edof, KE, mask, nelx, nely, rtol=1e-8)
+    Ue = U[..., edof]
+    ce = torch.einsum("rei,ij,rej->re", Ue, KE, Ue)
+    g = 2 * mgcg_port.matvec(U, dens, edof, KE, p.ndof)
+    mgcg_port.adjoint(levels, U, F, g, edof, KE, rtol=1e-8)
+    sec = time.perf_counter() - t0
+    assert ce.shape[0] == sample_rows
+    return {
+        "value": sample_rows / sec,
+        "unit": "solves+grads/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample_rows} of the {1 + len(p.stage_times)} rows (whole structure + one "
+                  f"stage) at {nelx}x{nely}: hierarchy + MG-PCG to 1e-8 ({n} it) + adjoint + "
+                  f"gradient, {sec:.1f} s",
+        "seconds": sec,
+    }
+
+
+if __name__ == "__main__":
+    main()

@@ -79,6 +79,8 @@ def rows_of(t: torch.Tensor, core: int, batch: tuple[int, ...], dtype=None) -> t
     row stride: 0 when `t` has no batch dims (shared by every row)."""
     if dtype is not None and t.dtype != dtype:
         t = t.to(dtype)
+    if t.data_ptr() % 16:
+        t = t.clone()  # the kernels load node pairs as 2-vectors: keep 16-byte alignment
     if t.ndim == 1:
         return t.contiguous(), 0
     if tuple(t.shape[:-1]) != batch:

@@ -1,0 +1,294 @@
+"""Multigrid and MG-PCG on the device against the fp64 oracles: hierarchy (Galerkin
+exactness, diagonals, coarsening), V-cycle (equal to the reference algorithm,
+symmetric, positive definite), MGCG (direct-solve agreement, true residual,
+reference iteration counts, retirement, warm start, non-convergence) -- the
+protocol of the reference's tests/test_torch_fem.py:322-713."""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import binary_design
+from oracle import fem_direct, mgcg_port
+
+pytestmark = pytest.mark.gpu
+
+EMIN, EMAX, PENAL, NU = 1e-9, 1.0, 3.0, 0.3
+MG_KW = {"max_coarse_elements": 24}
+MG_MESHES = [(12, 8), (18, 12), (24, 16)]
+
+
+def _setup(nelx, nely, dev):
+    from sttopt_amd import fem
+
+    F, free = fem.load_case("cantilever", nelx, nely)
+    ndof = F.size
+    mask = np.zeros(ndof, bool)
+    mask[free] = True
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.asarray(a), dtype=dt, device=dev)  # noqa: E731
+    return dict(F=F, free=free, mask=mask, ndof=ndof, KE=KE, edof=edof, F_t=t(F),
+                mask_t=torch.as_tensor(mask, device=dev), KE_t=t(KE), edof_t=t(edof, torch.int64),
+                t=t)
+
+
+def _fields(nelx, nely, rng):
+    return {
+        "uniform": np.full(nelx * nely, 0.4),
+        "random": rng.uniform(0.05, 1.0, nelx * nely),
+        "binary": binary_design(nelx, nely),
+    }
+
+
+def _port_hierarchy(s, nelx, nely, x, **kw):
+    dens = torch.as_tensor(fem_direct.simp(x, EMIN, EMAX, PENAL))
+    return mgcg_port.build_hierarchy(dens, torch.as_tensor(s["edof"]), torch.as_tensor(s["KE"]),
+                                     torch.as_tensor(s["mask"]), nelx, nely, **{**MG_KW, **kw})
+
+
+def _hip_hierarchy(s, nelx, nely, x, **kw):
+    from sttopt_amd import torch_fem, torch_mg
+
+    dens = torch_fem.simp_density(s["t"](x), EMIN, EMAX, PENAL)
+    return torch_mg.build_hierarchy(dens, s["edof_t"], s["KE_t"], s["mask_t"], nelx, nely,
+                                    **{**MG_KW, **kw})
+
+
+@pytest.mark.parametrize("nelx, nely", [(12, 8), (18, 12), (45, 15)])
+def test_hierarchy_matches_reference_algorithm(dev, nelx, nely):
+    s = _setup(nelx, nely, dev)
+    x = binary_design(nelx, nely)
+    hip = _hip_hierarchy(s, nelx, nely, x, max_coarse_elements=4)
+    ref = _port_hierarchy(s, nelx, nely, x, max_coarse_elements=4)
+    assert len(hip) == len(ref) >= 3
+    for a, b in zip(hip, ref):
+        assert (a.nelx, a.nely, a.ndof, a.kx, a.ky) == (b.nelx, b.nely, b.ndof, b.kx, b.ky)
+        assert torch.equal(a.mask.cpu(), b.mask)
+        np.testing.assert_allclose(a.diag.cpu().numpy(), b.diag.numpy(), rtol=1e-12)
+        if b.keff is not None:
+            got = a.keff.cpu().numpy()
+            want = b.keff.numpy()
+            np.testing.assert_allclose(got, want, rtol=1e-11, atol=1e-13 * np.abs(want).max())
+    np.testing.assert_allclose(hip[-1].chol.cpu().numpy(), ref[-1].chol.numpy(), rtol=1e-9,
+                               atol=1e-12 * float(ref[-1].chol.abs().max()))
+
+
+@pytest.mark.parametrize("nelx, nely", [(12, 8), (18, 12)])
+def test_coarse_operator_is_exact_galerkin(dev, nelx, nely):
+    from sttopt_amd import torch_fem, torch_mg
+
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(1)
+    levels = _hip_hierarchy(s, nelx, nely, binary_design(nelx, nely), max_coarse_elements=4)
+    for fine, coarse in zip(levels, levels[1:]):
+        v = torch_fem.project(s["t"](rng.standard_normal(coarse.ndof)), coarse.mask)
+        Pv = torch_fem.project(torch_mg._on_node_grid(v, coarse.nelx + 1, coarse.nely + 1, fine.kx,
+                                                      fine.ky, torch_mg._interp_axis), fine.mask)
+        RAPv = torch_fem.project(torch_mg._on_node_grid(fine(Pv), fine.nelx + 1, fine.nely + 1,
+                                                        fine.kx, fine.ky, torch_mg._restrict_axis),
+                                 coarse.mask)
+        got, want = coarse(v).cpu().numpy(), RAPv.cpu().numpy()
+        np.testing.assert_allclose(got, want, rtol=1e-10, atol=1e-12 * np.abs(want).max())
+
+
+def test_coarsening_factors_and_level_shapes(dev):
+    from sttopt_amd import torch_mg
+
+    assert torch_mg.coarsening_factors(180, 60) == (2, 2)
+    assert torch_mg.coarsening_factors(45, 15) == (3, 3)
+    assert torch_mg.coarsening_factors(15, 5) == (3, 1)
+    assert torch_mg.coarsening_factors(5, 5) == (1, 1)
+    s = _setup(24, 16, dev)
+    levels = _hip_hierarchy(s, 24, 16, np.full(24 * 16, 0.4), max_coarse_elements=4)
+    for f, c in zip(levels, levels[1:]):
+        assert (c.nelx, c.nely) == (f.nelx // f.kx, f.nely // f.ky)
+    assert torch_mg.coarsening_factors(levels[-1].nelx, levels[-1].nely) == (1, 1)
+
+
+@pytest.mark.parametrize("nelx, nely", MG_MESHES)
+def test_fp64_vcycle_equals_reference_vcycle_and_is_spd(dev, nelx, nely):
+    from sttopt_amd import torch_fem, torch_mg
+
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(2)
+    for x in _fields(nelx, nely, rng).values():
+        hip = _hip_hierarchy(s, nelx, nely, x)
+        ref = _port_hierarchy(s, nelx, nely, x)
+        M = torch_mg.VCycle(hip, dtype=torch.float64)
+        Mref = mgcg_port.VCycle(ref)
+        u = torch_fem.project(s["t"](rng.standard_normal(s["ndof"])), s["mask_t"])
+        v = torch_fem.project(s["t"](rng.standard_normal(s["ndof"])), s["mask_t"])
+        Mu, Mv = M(u), M(v)
+        want = Mref(u.cpu()).numpy()
+        np.testing.assert_allclose(Mu.cpu().numpy(), want, rtol=1e-9, atol=1e-11 * np.abs(want).max())
+        a, b = float((Mu * v).sum()), float((u * Mv).sum())
+        assert abs(a - b) <= 1e-10 * abs(a)
+        assert float((u * Mu).sum()) > 0
+        assert torch.all(Mu[~s["mask_t"]] == 0)
+        for lv in hip:
+            assert torch.all(lv.diag > 0)
+
+
+def test_fp32_vcycle_is_close_to_fp64(dev):
+    from sttopt_amd import torch_fem, torch_mg
+
+    nelx, nely = 48, 32
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(3)
+    hip = _hip_hierarchy(s, nelx, nely, rng.uniform(0.05, 1, nelx * nely))
+    r = torch_fem.project(s["t"](rng.standard_normal((2, s["ndof"]))), s["mask_t"])
+    z64 = torch_mg.VCycle(hip, dtype=torch.float64)(r)
+    z32 = torch_mg.VCycle(hip, dtype=torch.float32)(r)
+    assert z32.dtype == torch.float64
+    err = float((z32 - z64).norm() / z64.norm())
+    assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("vdtype", [torch.float64, torch.float32])
+@pytest.mark.parametrize("nelx, nely", MG_MESHES)
+def test_mgcg_matches_direct_solve(dev, nelx, nely, vdtype):
+    from sttopt_amd import torch_mg
+
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(3)
+    fixed = ~s["mask"]
+    old = torch_mg.VCYCLE_DTYPE
+    torch_mg.VCYCLE_DTYPE = vdtype
+    try:
+        for x in _fields(nelx, nely, rng).values():
+            dens = fem_direct.simp(x, EMIN, EMAX, PENAL)
+            K = fem_direct.assemble(s["KE"], dens, s["edof"], s["ndof"])
+            want = fem_direct.solve(K, s["F"], s["free"])
+            U, n_iter = torch_mg.solve(s["F_t"], s["t"](x), s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL,
+                                       s["mask_t"], nelx, nely, rtol=1e-11, **MG_KW)
+            assert n_iter > 0
+            got = U.cpu().numpy()
+            b = np.where(fixed, 0.0, s["F"])
+            res = b - K @ got
+            res[fixed] = 0
+            assert np.linalg.norm(res) / np.linalg.norm(b) < 1e-10
+            np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
+            assert np.all(got[fixed] == 0.0)
+    finally:
+        torch_mg.VCYCLE_DTYPE = old
+
+
+@pytest.mark.parametrize("nelx, nely", MG_MESHES)
+def test_fp64_iteration_counts_equal_reference_algorithm(dev, nelx, nely):
+    from sttopt_amd import torch_mg
+
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(5)
+    old = torch_mg.VCYCLE_DTYPE
+    torch_mg.VCYCLE_DTYPE = torch.float64
+    try:
+        for x in _fields(nelx, nely, rng).values():
+            _, n_hip = torch_mg.solve(s["F_t"], s["t"](x), s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL,
+                                      s["mask_t"], nelx, nely, rtol=1e-8, **MG_KW)
+            dens = torch.as_tensor(fem_direct.simp(x, EMIN, EMAX, PENAL))
+            _, n_ref, _ = mgcg_port.mg_solve(torch.as_tensor(s["F"]), dens, torch.as_tensor(s["edof"]),
+                                             torch.as_tensor(s["KE"]), torch.as_tensor(s["mask"]),
+                                             nelx, nely, rtol=1e-8, **MG_KW)
+            assert abs(n_hip - n_ref) <= 1, (n_hip, n_ref)
+    finally:
+        torch_mg.VCYCLE_DTYPE = old
+
+
+def test_batched_rows_retire_independently(dev):
+    """Rows of unequal difficulty: each equals its own single solve, n_iter is the
+    slowest row's, and a zero right-hand side row is exactly 0 after 0 iterations."""
+    from sttopt_amd import torch_mg
+
+    nelx, nely = 18, 12
+    s = _setup(nelx, nely, dev)
+    xs = np.stack([binary_design(nelx, nely), np.full(nelx * nely, 0.4), np.full(nelx * nely, 0.9)])
+    F = s["F_t"].expand(3, -1).clone()
+    F[2] = 0
+    info = {}
+    U, n_iter = torch_mg.solve(F, s["t"](xs), s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL,
+                               s["mask_t"], nelx, nely, rtol=1e-11, info=info, **MG_KW)
+    assert torch.all(U[2] == 0)
+    assert info["row_iter_list"][2] == 0
+    singles = []
+    for b in range(2):
+        Ub, nb = torch_mg.solve(s["F_t"], s["t"](xs[b]), s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL,
+                                s["mask_t"], nelx, nely, rtol=1e-11, **MG_KW)
+        singles.append(nb)
+        want = Ub.cpu().numpy()
+        np.testing.assert_allclose(U[b].cpu().numpy(), want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
+    assert n_iter == max(singles)
+    assert info["row_iter_list"][:2] == singles
+
+
+def test_warm_start_and_exact_warm_start(dev):
+    from sttopt_amd import torch_mg
+
+    nelx, nely = 18, 12
+    s = _setup(nelx, nely, dev)
+    x = s["t"](binary_design(nelx, nely))
+    args = (s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL, s["mask_t"], nelx, nely)
+    U, n_cold = torch_mg.solve(s["F_t"], x, *args, rtol=1e-11, **MG_KW)
+    rng = np.random.default_rng(5)
+    x0 = (U + s["t"](rng.standard_normal(s["ndof"]) * 1e-6)) * s["mask_t"]
+    U2, n_warm = torch_mg.solve(s["F_t"], x, *args, rtol=1e-11, x0=x0, **MG_KW)
+    assert n_warm < n_cold
+    U3, n0 = torch_mg.solve(s["F_t"], x, *args, rtol=1e-8, x0=U, **MG_KW)
+    assert n0 == 0 and torch.equal(U3, U)
+
+
+def test_nonconvergence_raises_with_original_row_indices(dev):
+    from sttopt_amd import torch_fem, torch_mg
+
+    nelx, nely = 18, 12
+    s = _setup(nelx, nely, dev)
+    xs = s["t"](np.stack([np.full(nelx * nely, 0.4), binary_design(nelx, nely)]))
+    F = torch.stack([torch.zeros_like(s["F_t"]), s["F_t"]])
+    with pytest.raises(torch_fem.CGConvergenceError) as ei:
+        torch_mg.solve(F, xs, s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL, s["mask_t"], nelx, nely,
+                       rtol=1e-14, max_iter=2, **MG_KW)
+    assert ei.value.n_iter == 2
+    assert ei.value.residuals.shape == (2,)
+    assert "failed batch indices [1]" in str(ei.value)
+
+
+def test_generic_pcg_with_wrapped_operators_retires_rows(dev):
+    """Plain callables with `select` take the reference's compaction schedule."""
+    from sttopt_amd import torch_fem, torch_mg
+
+    class Counting:
+        def __init__(self, op, log):
+            self.op, self.log = op, log
+
+        def __call__(self, v):
+            self.log.append(v.shape[0])
+            return self.op(v)
+
+        def select(self, rows):
+            return Counting(self.op.select(rows), self.log)
+
+    nelx, nely = 18, 12
+    s = _setup(nelx, nely, dev)
+    xs = np.stack([binary_design(nelx, nely), np.full(nelx * nely, 0.4)])
+    dens = torch_fem.simp_density(s["t"](xs), EMIN, EMAX, PENAL)
+    levels = torch_mg.build_hierarchy(dens, s["edof_t"], s["KE_t"], s["mask_t"], nelx, nely, **MG_KW)
+    F = torch.stack([s["F_t"], torch.zeros_like(s["F_t"])])
+    log = []
+    U, _ = torch_fem.pcg(Counting(levels[0], log), torch_fem.project(F, s["mask_t"]),
+                         Counting(torch_mg.VCycle(levels, dtype=torch.float64), log), rtol=1e-11)
+    assert torch.all(U[1] == 0)
+    assert log[0] == 2 and log[1] == 1
+
+
+def test_jacobi_pcg_solve_matches_direct(dev):
+    from sttopt_amd import torch_fem
+
+    nelx, nely = 12, 8
+    s = _setup(nelx, nely, dev)
+    x = np.random.default_rng(2).uniform(0.05, 1.0, nelx * nely)
+    dens = fem_direct.simp(x, EMIN, EMAX, PENAL)
+    want = fem_direct.solve(fem_direct.assemble(s["KE"], dens, s["edof"], s["ndof"]), s["F"], s["free"])
+    U, n = torch_fem.solve(s["F_t"], s["t"](x), s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL, s["mask_t"],
+                           rtol=1e-10)
+    assert n > 0
+    np.testing.assert_allclose(U.cpu().numpy(), want, rtol=1e-6, atol=1e-9 * np.abs(want).max())

@@ -1,0 +1,138 @@
+"""CPU-side checks: the C-ABI library loads and exports every declared entry point;
+host logic (coarsening, transfers, interpolation matrices, workloads, load cases);
+every kernel-backed entry point refuses CPU tensors (there is no CPU path)."""
+
+import re
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import ROOT
+from oracle import mgcg_port
+
+
+def _declared_functions():
+    text = (ROOT / "include" / "sttopt_hip.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sttopt_\w+)\s*\(", text)))
+
+
+def test_library_loads_and_exports_every_declared_symbol():
+    from sttopt_amd import _lib
+
+    lib = _lib.load()
+    declared = _declared_functions()
+    assert len(declared) >= 20
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert declared == _lib.exported_symbols()
+    assert lib.sttopt_version() >= 1
+
+
+def test_library_reports_errors_without_touching_the_gpu():
+    from sttopt_amd import _lib
+
+    lib = _lib.load()
+    st = lib.sttopt_apply(_lib.F64, 0, 3, 4, 4, 1, None, 0, None, 0, None, None, None, 0, None, 0, None)
+    assert st == _lib.EINVAL and "invalid grid" in _lib.last_error()
+    st = lib.sttopt_galerkin(2, 5, 4, 2, 2, 1, None, 0, None, 0, None, None, None, 0, None, None, 0, None)
+    assert st == _lib.EINVAL
+
+
+def test_coarsening_factors():
+    from sttopt_amd import torch_mg
+
+    cases = {(180, 60): (2, 2), (45, 15): (3, 3), (15, 5): (3, 1), (5, 5): (1, 1), (1024, 512): (2, 2)}
+    for (nx, ny), want in cases.items():
+        assert torch_mg.coarsening_factors(nx, ny) == want
+        assert mgcg_port.coarsening_factors(nx, ny) == want
+
+
+@pytest.mark.parametrize("kx, ky", [(1, 1), (2, 2), (3, 3), (3, 1), (2, 3)])
+def test_node_grid_transfers_are_adjoint_and_match_the_port(kx, ky):
+    from sttopt_amd import torch_mg
+
+    rng = np.random.default_rng(0)
+    ncx, ncy = 3, 2
+    c = torch.tensor(rng.standard_normal(2 * (ncx + 1) * (ncy + 1)))
+    f = torch.tensor(rng.standard_normal(2 * (ncx * kx + 1) * (ncy * ky + 1)))
+    Pc = torch_mg._on_node_grid(c, ncx + 1, ncy + 1, kx, ky, torch_mg._interp_axis)
+    Rf = torch_mg._on_node_grid(f, ncx * kx + 1, ncy * ky + 1, kx, ky, torch_mg._restrict_axis)
+    assert abs(float((Pc * f).sum() - (c * Rf).sum())) < 1e-12
+    np.testing.assert_allclose(
+        Pc.numpy(), mgcg_port.on_node_grid(c, ncx + 1, ncy + 1, kx, ky, mgcg_port.interp_axis).numpy(),
+        atol=1e-15)
+    np.testing.assert_allclose(
+        Rf.numpy(),
+        mgcg_port.on_node_grid(f, ncx * kx + 1, ncy * ky + 1, kx, ky, mgcg_port.restrict_axis).numpy(),
+        atol=1e-14)
+
+
+@pytest.mark.parametrize("kx, ky, d", [(2, 2, 2), (3, 3, 2), (2, 3, 1), (3, 1, 2)])
+def test_child_interpolation_matrices(kx, ky, d):
+    from sttopt_amd import torch_mg
+
+    S = torch_mg.child_interpolation_matrices(kx, ky, None, torch.float64, d)
+    np.testing.assert_allclose(S.numpy(), mgcg_port.child_matrices(kx, ky, d).numpy(), atol=1e-15)
+    # interpolation reproduces constants: every row sums to 1 per component
+    np.testing.assert_allclose(S.sum(-1).numpy(), np.ones(S.shape[:2]), atol=1e-15)
+
+
+def test_child_elements_order():
+    from sttopt_amd import torch_mg
+
+    ch = torch_mg._child_elements(4, 2, 2, 2, "cpu")
+    np.testing.assert_array_equal(ch.numpy(), [[0, 1, 4, 5], [2, 3, 6, 7]])
+    np.testing.assert_array_equal(ch.numpy(), mgcg_port.child_elements(4, 2, 2, 2).numpy())
+
+
+def test_workload_sizes_and_determinism():
+    from sttopt_amd import workloads
+
+    assert workloads.mbb(device="cpu").ndof == 9922
+    a = workloads.space_time(240, 120, 10, device="cpu")
+    b = workloads.space_time(240, 120, 10, device="cpu")
+    assert a.ndof == 58322 and torch.equal(a.xPhys, b.xPhys) and torch.equal(a.tPhys, b.tPhys)
+    assert float(a.tPhys.min()) == 0.0 and float(a.tPhys.max()) == 1.0
+    assert a.stage_times[-1] == 1.0 and len(a.stage_times) == 10
+    assert float(a.xPhys.min()) >= 1e-3 and float(a.xPhys.max()) <= 1.0
+    h = workloads.heat(device="cpu")
+    assert h.F.numel() == 115921 and abs(float(h.F.sum()) - 480 * 240) < 1e-6
+
+
+def test_free_mask_project_and_safe_div_are_plain_tensor_algebra():
+    from sttopt_amd import torch_fem
+
+    m = torch_fem.free_mask(6, torch.tensor([0, 2, 5]))
+    assert m.tolist() == [True, False, True, False, False, True]
+    assert torch.equal(torch_fem.project(torch.ones(6), m), m.double())
+    out = torch_fem.safe_div(torch.tensor([1.0, 2.0]), torch.tensor([0.0, 4.0]))
+    assert out.tolist() == [0.0, 0.5]
+
+
+def test_kernel_entry_points_refuse_cpu_tensors():
+    from sttopt_amd import _lib, compliance, torch_fem, torch_mg, torch_solve, workloads
+
+    p = workloads.space_time(8, 6, 2, device="cpu")
+    dens = torch.ones(p.nel, dtype=torch.float64)
+    with pytest.raises(_lib.DeviceError):
+        torch_fem.matvec(torch.zeros(p.ndof, dtype=torch.float64), dens, p.edofMat, p.KE, p.ndof)
+    with pytest.raises(_lib.DeviceError):
+        torch_mg.build_hierarchy(dens, p.edofMat, p.KE, p.mask, p.nelx, p.nely)
+    with pytest.raises(_lib.DeviceError):
+        torch_solve.femsolve(dens, p.F, p.edofMat, p.KE, p.mask, p.nelx, p.nely)
+    with pytest.raises(_lib.DeviceError):
+        compliance.batched_whole_and_gravity_compliance(
+            p.xPhys, p.tPhys, p.KE, p.edofMat, 1e-9, 1.0, 3.0, p.freedofs, p.F, p.ndof, p.C,
+            p.beta, p.stage_times)
+    with pytest.raises(_lib.DeviceError):
+        torch_fem.pcg(lambda v: v, torch.ones(3), lambda r: r)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    out = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
+                         capture_output=True, text=True, timeout=120)
+    assert out.returncode == 2 and "one GPU" in out.stderr

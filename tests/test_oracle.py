@@ -1,0 +1,299 @@
+"""Pins the oracle itself (CPU only): closed forms, brute force on tiny meshes,
+independent derivations and central finite differences, chosen so that a dropped
+term, a wrong sign or index, or a transposed operand fails one of them."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from oracle import fem_direct, mgcg_port
+
+EMIN, EMAX, PENAL, NU = 1e-9, 1.0, 3.0, 0.3
+
+
+def _pkg():
+    import sttopt_amd
+
+    return sttopt_amd
+
+
+# ----------------------------------------------------------- element data ----
+
+
+def test_element_tables_equal_quadrature_derivation():
+    """Closed-form tables (oracle) == 2x2 Gauss quadrature (product): two derivations."""
+    fem = _pkg().fem
+    for nu in (0.0, 0.3, 0.45):
+        np.testing.assert_allclose(fem_direct.plane_stress_KE(nu), fem.plane_stress_KE(nu), atol=1e-14)
+    np.testing.assert_allclose(fem_direct.diffusion_KE(), fem.diffusion_KE(), atol=1e-15)
+
+
+def test_plane_stress_KE_rigid_modes_and_uniform_strain_energy():
+    KE = fem_direct.plane_stress_KE(NU)
+    # corners BL, BR, TR, TL on the unit square (y up)
+    xy = np.array([[0, 0], [1, 0], [1, 1], [0, 1]], float)
+    tx = np.tile([1.0, 0.0], 4)
+    ty = np.tile([0.0, 1.0], 4)
+    rot = np.ravel(np.stack([-xy[:, 1], xy[:, 0]], 1))
+    for mode in (tx, ty, rot):
+        assert np.abs(KE @ mode).max() < 1e-14
+    w = np.linalg.eigvalsh(KE)
+    assert np.sum(np.abs(w) < 1e-12) == 3 and np.all(w[3:] > 1e-3)
+    # u = (x, 0): eps_xx = 1, energy u^T K u = E / (1 - nu^2) (area 1)
+    u = np.ravel(np.stack([xy[:, 0], np.zeros(4)], 1))
+    assert abs(u @ KE @ u - 1 / (1 - NU**2)) < 1e-14
+    # pure shear u = (y, x): gamma = 2, energy = G gamma^2 = 2 (1 - nu) ... / (1 - nu^2)
+    u = np.ravel(np.stack([xy[:, 1], xy[:, 0]], 1))
+    G = 1 / (2 * (1 + NU))
+    assert abs(u @ KE @ u - G * 4) < 1e-14
+
+
+def test_diffusion_KE_constant_and_linear_fields():
+    KE = fem_direct.diffusion_KE()
+    assert np.abs(KE.sum(1)).max() < 1e-15
+    x = np.array([0, 1, 1, 0.0])
+    assert abs(x @ KE @ x - 1.0) < 1e-15  # |grad u|^2 = 1 over a unit area
+
+
+def test_dof_maps_agree():
+    fem = _pkg().fem
+    for nelx, nely, d in [(3, 2, 2), (5, 4, 1), (7, 3, 2)]:
+        np.testing.assert_array_equal(fem_direct.element_dofs(nelx, nely, d),
+                                      fem.element_dof_map(nelx, nely, d))
+
+
+def test_assembly_equals_dense_brute_force():
+    rng = np.random.default_rng(0)
+    nelx, nely = 3, 2
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    ndof = 2 * (nelx + 1) * (nely + 1)
+    dens = rng.uniform(0.1, 1, nelx * nely)
+    dense = np.zeros((ndof, ndof))
+    for e in range(nelx * nely):
+        for a in range(8):
+            for b in range(8):
+                dense[edof[e, a], edof[e, b]] += dens[e] * KE[a, b]
+    np.testing.assert_allclose(fem_direct.assemble(KE, dens, edof, ndof).toarray(), dense, atol=1e-15)
+
+
+# ----------------------------------------------------------- closed forms ----
+
+
+def test_uniaxial_tension_patch_test():
+    """Bar under uniform end traction: u_x = sigma x, u_y = -nu sigma y exactly."""
+    fem = _pkg().fem
+    nelx, nely, P = 8, 3, 2.0
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    nodes = fem.node_grid(nelx, nely)
+    ndof = 2 * nodes.size
+    F = np.zeros(ndof)
+    right_up = nodes[::-1, -1]
+    F[2 * right_up] = P * fem.edge_load_shares(nely + 1, nely)
+    fixed = np.concatenate([2 * nodes[:, 0], [2 * nodes[-1, 0] + 1]])
+    free = np.setdiff1d(np.arange(ndof), fixed)
+    U = fem_direct.solve(fem_direct.assemble(KE, np.ones(nelx * nely), edof, ndof), F, free)
+    sigma = P / nely
+    r, c = np.divmod(np.arange(nodes.size), nelx + 1)
+    y = nely - r
+    np.testing.assert_allclose(U[0::2], sigma * c, atol=1e-12)
+    np.testing.assert_allclose(U[1::2], -NU * sigma * y, atol=1e-12)
+    # compliance = P * u_tip = P * sigma * L
+    assert abs(F @ U - P * sigma * nelx) < 1e-10
+
+
+def test_self_weight_matrix_totals_and_reactions():
+    nelx, nely = 6, 4
+    C = fem_direct.gravity_matrix(nelx, nely)
+    x = np.random.default_rng(1).uniform(0.1, 1, nelx * nely)
+    assert abs((C @ x).sum() - x.sum() / (nelx * nely)) < 1e-15
+    np.testing.assert_allclose(C.T @ np.ones(C.shape[0]), np.full(nelx * nely, 1 / (nelx * nely)))
+    fem = _pkg().fem
+    np.testing.assert_allclose(fem.gravity_load_matrix(nelx, nely).toarray(), C.toarray())
+    # hanging column, bottom clamped, unit density: reactions balance the unit weight
+    F, free = fem.load_case("bottom_clamped", nelx, nely)
+    Fy = np.zeros_like(F)
+    Fy[1::2] = -(C @ np.ones(nelx * nely))
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    K = fem_direct.assemble(KE, np.ones(nelx * nely), edof, F.size)
+    U = fem_direct.solve(K, Fy, free)
+    reaction = K @ U - Fy
+    fixed = np.setdiff1d(np.arange(F.size), free)
+    assert abs(reaction[fixed][1::2].sum() - 1.0) < 1e-10
+
+
+def test_mbb_load_case_dofs():
+    fem = _pkg().fem
+    nelx, nely = 120, 40
+    F, free = fem.load_case("mbb", nelx, nely)
+    assert F.size == 9922 and F.sum() == -1.0 and F[1] == -1.0
+    nodes = fem.node_grid(nelx, nely)
+    fixed = np.setdiff1d(np.arange(F.size), free)
+    want = np.sort(np.concatenate([2 * nodes[:, 0], [2 * nodes[-1, -1] + 1]]))
+    np.testing.assert_array_equal(fixed, want)
+
+
+# ------------------------------------------------------ sensitivities (FD) ----
+
+
+def _fd(f, x, idx, h):
+    xp, xm = x.copy(), x.copy()
+    xp.flat[idx] += h
+    xm.flat[idx] -= h
+    return (f(xp) - f(xm)) / (2 * h)
+
+
+def test_time_mask_derivative_matches_fd():
+    t = np.linspace(0.01, 0.99, 23)
+    for ti, beta in [(0.3, 16.0), (0.7, 4.0)]:
+        d = fem_direct.time_mask_derivative(t, ti, beta)
+        h = 1e-6
+        fd = (fem_direct.time_mask(t + h, ti, beta) - fem_direct.time_mask(t - h, ti, beta)) / (2 * h)
+        np.testing.assert_allclose(d, fd, rtol=1e-6, atol=1e-8)
+
+
+def test_hand_derived_sensitivities_match_fd():
+    """Whole-structure dc/dx and the stage dc/dx, dc/dt (incl. the doubled
+    gravity-load adjoint term) against central differences of the direct solve."""
+    fem = _pkg().fem
+    nelx, nely = 8, 5
+    rng = np.random.default_rng(3)
+    x = rng.uniform(0.3, 0.7, (nely, nelx))
+    # stage well into the build (every element partly active): a well-conditioned
+    # system, so the central difference is not dominated by cancellation
+    t = rng.uniform(0.0, 0.6, (nely, nelx))
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    F, free = fem.load_case("cantilever", nelx, nely)
+    _, dcx, _ = fem_direct.whole_compliance(x, KE, edof, F, free)
+    Fb, freeb = fem.load_case("bottom_clamped", nelx, nely)
+    ti, beta, fe = 0.7, 4.0, 1.0 / (nelx * nely)
+    _, gx, gt, _ = fem_direct.gravity_compliance(x, t, ti, beta, KE, edof, freeb, fe)
+    for e in rng.choice(nelx * nely, 6, replace=False):
+        fd = _fd(lambda xx: fem_direct.whole_compliance(xx, KE, edof, F, free)[0], x, e, 1e-6)
+        assert abs(fd - dcx[e]) <= 1e-5 * abs(dcx[e])
+        fd = _fd(lambda xx: fem_direct.gravity_compliance(xx, t, ti, beta, KE, edof, freeb, fe)[0], x, e, 1e-6)
+        assert abs(fd - gx[e]) <= 1e-5 * max(abs(gx[e]), 1e-3 * np.abs(gx).max())
+        fd = _fd(lambda tt: fem_direct.gravity_compliance(x, tt, ti, beta, KE, edof, freeb, fe)[0], t, e, 1e-6)
+        assert abs(fd - gt[e]) <= 1e-5 * max(abs(gt[e]), 1e-3 * np.abs(gt).max())
+
+
+# ------------------------------------------------------------ the MG port ----
+
+
+def _port_setup(nelx, nely, x):
+    fem = _pkg().fem
+    F, free = fem.load_case("cantilever", nelx, nely)
+    ndof = F.size
+    mask = np.zeros(ndof, bool)
+    mask[free] = True
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    dens = fem_direct.simp(x, EMIN, EMAX, PENAL)
+    return F, free, mask, KE, edof, dens, ndof
+
+
+def test_port_galerkin_adjoint_transfers_and_spd_vcycle():
+    from conftest import binary_design
+
+    nelx, nely = 18, 12
+    F, free, mask, KE, edof, dens, ndof = _port_setup(nelx, nely, binary_design(nelx, nely))
+    T = torch.as_tensor
+    levels = mgcg_port.build_hierarchy(T(dens), T(edof), T(KE), T(mask), nelx, nely, max_coarse_elements=4)
+    assert len(levels) >= 3
+    rng = np.random.default_rng(0)
+    for f, c in zip(levels, levels[1:]):
+        v = T(rng.standard_normal(c.ndof)) * c.mask
+        Pv = mgcg_port.on_node_grid(v, c.nelx + 1, c.nely + 1, f.kx, f.ky, mgcg_port.interp_axis) * f.mask
+        RAPv = mgcg_port.on_node_grid(f(Pv), f.nelx + 1, f.nely + 1, f.kx, f.ky, mgcg_port.restrict_axis) * c.mask
+        np.testing.assert_allclose(c(v).numpy(), RAPv.numpy(), rtol=1e-10, atol=1e-12 * float(RAPv.abs().max()))
+    M = mgcg_port.VCycle(mgcg_port.build_hierarchy(T(dens), T(edof), T(KE), T(mask), nelx, nely,
+                                                   max_coarse_elements=24))
+    u = T(rng.standard_normal(ndof)) * T(mask)
+    w = T(rng.standard_normal(ndof)) * T(mask)
+    a, b = float((M(u) * w).sum()), float((u * M(w)).sum())
+    assert abs(a - b) <= 1e-10 * abs(a) and float((u * M(u)).sum()) > 0
+
+
+def test_port_mgcg_and_jacobi_match_direct():
+    nelx, nely = 18, 12
+    x = np.random.default_rng(2).uniform(0.05, 1, nelx * nely)
+    F, free, mask, KE, edof, dens, ndof = _port_setup(nelx, nely, x)
+    want = fem_direct.solve(fem_direct.assemble(KE, dens, edof, ndof), F, free)
+    T = torch.as_tensor
+    U, n, _ = mgcg_port.mg_solve(T(F), T(dens), T(edof), T(KE), T(mask), nelx, nely, rtol=1e-11,
+                                 max_coarse_elements=24)
+    np.testing.assert_allclose(U.numpy(), want, atol=1e-8 * np.abs(want).max())
+    diag = mgcg_port.diagonal(T(dens), T(edof), T(KE), ndof)
+    diag = torch.where(T(mask), diag, torch.ones(()))
+    Uj, nj = mgcg_port.pcg(lambda v: mgcg_port.operator(v, T(dens), T(edof), T(KE), ndof, T(mask)),
+                           T(F) * T(mask), lambda r: r / diag, rtol=1e-11, max_iter=20000)
+    np.testing.assert_allclose(Uj.numpy(), want, atol=1e-7 * np.abs(want).max())
+    assert n < nj / 5
+
+
+def test_port_adjoint_is_twice_U_for_compliance():
+    nelx, nely = 12, 8
+    x = np.random.default_rng(4).uniform(0.2, 0.9, nelx * nely)
+    F, free, mask, KE, edof, dens, ndof = _port_setup(nelx, nely, x)
+    T = torch.as_tensor
+    U, _, levels = mgcg_port.mg_solve(T(F), T(dens), T(edof), T(KE), T(mask), nelx, nely, rtol=1e-10,
+                                      max_coarse_elements=24)
+    g = 2 * mgcg_port.matvec(U, T(dens), T(edof), T(KE), ndof)
+    lam, grad, n = mgcg_port.adjoint(levels, U, T(F), g, T(edof), T(KE), rtol=1e-8)
+    assert n == 0
+    np.testing.assert_allclose(lam.numpy(), 2 * U.numpy(), rtol=1e-7, atol=1e-12)
+    ce = fem_direct.strain_energy(U.numpy(), edof, KE)
+    np.testing.assert_allclose(grad.numpy(), -2 * ce, rtol=1e-6, atol=1e-14)
+
+
+def test_port_retirement_matches_single_solves():
+    from conftest import binary_design
+
+    nelx, nely = 18, 12
+    xs = [binary_design(nelx, nely), np.full(nelx * nely, 0.4), np.full(nelx * nely, 0.9)]
+    F, free, mask, KE, edof, _, ndof = _port_setup(nelx, nely, xs[0])
+    T = torch.as_tensor
+    dens = T(np.stack([fem_direct.simp(x, EMIN, EMAX, PENAL) for x in xs]))
+    Ub, nb, _ = mgcg_port.mg_solve(T(F).expand(3, -1), dens, T(edof), T(KE), T(mask), nelx, nely,
+                                   rtol=1e-11, max_coarse_elements=24)
+    ns = []
+    for k in range(3):
+        U, n, _ = mgcg_port.mg_solve(T(F), dens[k], T(edof), T(KE), T(mask), nelx, nely, rtol=1e-11,
+                                     max_coarse_elements=24)
+        ns.append(n)
+        np.testing.assert_allclose(Ub[k].numpy(), U.numpy(), atol=1e-6 * float(U.abs().max()))
+    assert nb == max(ns) and min(ns) < max(ns)
+
+
+def test_sparse_gravity_scaling_in_stage_oracle():
+    """gravity_compliance scales C to the requested fe: doubling fe quadruples c."""
+    fem = _pkg().fem
+    nelx, nely = 6, 4
+    rng = np.random.default_rng(5)
+    x = rng.uniform(0.3, 0.7, (nely, nelx))
+    t = rng.uniform(0, 1, (nely, nelx))
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    _, free = fem.load_case("bottom_clamped", nelx, nely)
+    fe = 1 / (nelx * nely)
+    c1 = fem_direct.gravity_compliance(x, t, 0.6, 8.0, KE, edof, free, fe)[0]
+    c2 = fem_direct.gravity_compliance(x, t, 0.6, 8.0, KE, edof, free, 2 * fe)[0]
+    assert abs(c2 / c1 - 4.0) < 1e-10
+    assert isinstance(fem_direct.gravity_matrix(nelx, nely), sp.csr_matrix)
+
+
+@pytest.mark.parametrize("nelx, nely", [(120, 40)])
+def test_c1_mbb_oracle_solution(nelx, nely):
+    """The plain-method configuration (120x40 MBB, 9,922 dofs) through the oracle."""
+    wl = _pkg().workloads
+    p = wl.mbb(nelx, nely, device="cpu")
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    c, dcx, U = fem_direct.whole_compliance(p.xPhys.numpy(), KE, edof, p.F.numpy(), p.freedofs.numpy())
+    assert p.ndof == 9922 and c > 0 and np.all(dcx < 0)
+    # compliance equals F . U (work of the external load)
+    assert abs(c - float(p.F.numpy() @ U)) <= 1e-9 * c
