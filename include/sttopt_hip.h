@@ -47,6 +47,8 @@ extern "C" {
 
 const char* sttopt_last_error(void);
 int sttopt_version(void);
+/* sizeof(sttopt_level) and sizeof(sttopt_mg), for binding checks. */
+int sttopt_abi_sizes(int64_t* level_bytes, int64_t* mg_bytes);
 
 /* ---------------------------------------------------------------- masks ---- */
 
@@ -109,6 +111,11 @@ typedef struct sttopt_level {
   int64_t rdiag_stride;
   const void* keff;
   int64_t keff_stride;
+  /* coarser levels, alternative to keff: keff_E = sum over the pkx x pky level-0
+   * elements g under E of density0_g * ptable[g], ptable: (pkx*pky) x (4d)^2 in the
+   * V-cycle precision (device); the V-cycle then reads only level 0's density      */
+  const void* ptable;
+  int32_t pkx, pky;
   const void* invdiag;    /* 1 / diag at free dofs, 0 at fixed, V-cycle precision */
   int64_t invdiag_stride;
   const double* cinv;     /* coarsest level: dense inverse [rows][n][n], fp64     */

@@ -53,6 +53,9 @@ class Level(ctypes.Structure):
         ("rdiag_stride", c_int64),
         ("keff", c_void_p),
         ("keff_stride", c_int64),
+        ("ptable", c_void_p),
+        ("pkx", c_int32),
+        ("pky", c_int32),
         ("invdiag", c_void_p),
         ("invdiag_stride", c_int64),
         ("cinv", c_void_p),
@@ -92,6 +95,7 @@ _P = c_void_p
 _SIGNATURES = {
     "sttopt_last_error": (ctypes.c_char_p, []),
     "sttopt_version": (c_int32, []),
+    "sttopt_abi_sizes": (c_int32, [_P, _P]),
     "sttopt_pack_mask": (c_int32, [_P, c_int64, _P, _P]),
     "sttopt_coarsen_mask": (c_int32, [_P, c_int32, c_int32, c_int32, c_int32, c_int32, _P, _P]),
     "sttopt_apply": (

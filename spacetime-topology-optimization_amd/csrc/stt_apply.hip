@@ -47,6 +47,10 @@ struct StencilArgs {
   T omega;
   const int* active;
   double* partial;
+  const T* ptab;
+  int pkx, pky, fnelx;
+  const T* prho;
+  long long spr;
   KE<T> ke;
 };
 
@@ -69,7 +73,11 @@ __device__ __forceinline__ Vec<T, D> load_node(const T* base, long long node) {
   return out;
 }
 
-template <typename T, typename TB, typename TY, int D, bool KEFF>
+// SRC: 0 = density * KE (level 0), 1 = explicit entry-major keff, 2 = keff
+// parameterized by the fine densities under each coarse element.
+enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2 };
+
+template <typename T, typename TB, typename TY, int D, int SRC>
 __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> a) {
   constexpr int K = 4 * D;
   __shared__ double red[kBlock / kWave];
@@ -122,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
       const long long e = (long long)er * a.nelx + ec;
       constexpr int locs[4] = {1, 0, 2, 3};
       const int loc = locs[q];
-      if constexpr (!KEFF) {
+      if constexpr (SRC == kSrcDensity) {
         T part[D];
 #pragma unroll
         for (int i = 0; i < D; ++i) part[i] = T(0);
@@ -139,6 +147,31 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
         const T s = a.rho[row * a.sr + e];
 #pragma unroll
         for (int i = 0; i < D; ++i) acc[i] += s * part[i];
+      } else if constexpr (SRC == kSrcParam) {
+        // keff_E rows (loc*D + i) = sum_g rho_g G_g rows: contract each G_g with the
+        // element's nodal values, then weight by the child density.
+        const T* rho = a.prho + row * a.spr;
+        for (int gy = 0; gy < a.pky; ++gy) {
+          const long long frow = (long long)(er * a.pky + gy) * a.fnelx + (long long)ec * a.pkx;
+          for (int gx = 0; gx < a.pkx; ++gx) {
+            const T s = rho[frow + gx];
+            const T* G = a.ptab + (gy * a.pkx + gx) * K * K;
+            T part[D];
+#pragma unroll
+            for (int i = 0; i < D; ++i) part[i] = T(0);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+              const int ur = (q >> 1) + loc_dr(j), uc = (q & 1) + loc_dc(j);
+#pragma unroll
+              for (int i = 0; i < D; ++i) {
+#pragma unroll
+                for (int k = 0; k < D; ++k) part[i] += G[(loc * D + i) * K + j * D + k] * U[ur][uc][k];
+              }
+            }
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc[i] += s * part[i];
+          }
+        }
       } else {
         const long long nel = (long long)a.nelx * a.nely;
         const T* ke_e = a.keff + row * a.sk + e;
@@ -228,23 +261,35 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   a.omega = static_cast<T>(omega);
   a.active = active;
   a.partial = partial;
+  a.ptab = static_cast<const T*>(op.ptab);
+  a.pkx = op.pkx;
+  a.pky = op.pky;
+  a.fnelx = op.fnelx;
+  a.prho = static_cast<const T*>(op.prho);
+  a.spr = op.spr;
   a.ke = make_ke<T>(op.ke, op.d);
-  const bool keff = op.keff != nullptr;
-  if (!keff && op.rho == nullptr) {
+  const int src = op.ptab != nullptr ? kSrcParam : op.keff != nullptr ? kSrcKeff : kSrcDensity;
+  if (src == kSrcDensity && op.rho == nullptr) {
     set_error("stencil: level has neither density nor keff");
     return STTOPT_EINVAL;
   }
+  if (src == kSrcParam && (op.prho == nullptr || op.pkx < 1 || op.pky < 1 ||
+                           op.fnelx != op.nelx * op.pkx)) {
+    set_error("stencil: parameterized level needs the fine density and a matching block");
+    return STTOPT_EINVAL;
+  }
   dim3 grid((unsigned)a.total), block(kBlock);
+#define STT_STENCIL(D, SRC) \
+  hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, SRC>), grid, block, 0, s, a)
   if (op.d == 2) {
-    if (keff)
-      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 2, true>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 2, false>), grid, block, 0, s, a);
+    if (src == kSrcParam) STT_STENCIL(2, kSrcParam);
+    else if (src == kSrcKeff) STT_STENCIL(2, kSrcKeff);
+    else STT_STENCIL(2, kSrcDensity);
   } else if (op.d == 1) {
-    if (keff)
-      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 1, true>), grid, block, 0, s, a);
-    else
-      hipLaunchKernelGGL((stencil_kernel<T, TB, TY, 1, false>), grid, block, 0, s, a);
+    if (src == kSrcParam) STT_STENCIL(1, kSrcParam);
+    else if (src == kSrcKeff) STT_STENCIL(1, kSrcKeff);
+    else STT_STENCIL(1, kSrcDensity);
+#undef STT_STENCIL
   } else {
     set_error("stencil: d must be 1 or 2");
     return STTOPT_EINVAL;
@@ -552,6 +597,12 @@ extern "C" {
 const char* sttopt_last_error(void) { return g_last_error.c_str(); }
 
 int sttopt_version(void) { return 1; }
+
+int sttopt_abi_sizes(int64_t* level_bytes, int64_t* mg_bytes) {
+  if (level_bytes) *level_bytes = (int64_t)sizeof(sttopt_level);
+  if (mg_bytes) *mg_bytes = (int64_t)sizeof(sttopt_mg);
+  return STTOPT_OK;
+}
 
 int sttopt_pack_mask(const uint8_t* mask, int64_t ndof, uint32_t* bits, void* stream) {
   if (mask == nullptr || bits == nullptr || ndof < 0) {

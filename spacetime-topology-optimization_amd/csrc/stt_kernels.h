@@ -20,6 +20,13 @@ struct OpDesc {
   long long srd = 0;
   const uint32_t* mask = nullptr;
   double ke[64] = {};
+  // Density-parameterized coarse level: keff_E = sum over the pkx x pky fine
+  // elements g under E of prho_g * ptab[g] (ptab: (pkx*pky) x (4d)^2 matrices), with
+  // prho the level-0 density on its fnelx-wide grid.
+  const void* ptab = nullptr;
+  int pkx = 1, pky = 1, fnelx = 0;
+  const void* prho = nullptr;
+  long long spr = 0;
 
   int nnx() const { return nelx + 1; }
   int nny() const { return nely + 1; }

@@ -272,6 +272,13 @@ struct VCycle {
         o.sr = L.density_stride;
         o.rdiag = L.rdiag;
         o.srd = L.rdiag_stride;
+      } else if (L.ptable != nullptr) {
+        o.ptab = L.ptable;
+        o.pkx = L.pkx;
+        o.pky = L.pky;
+        o.fnelx = m->levels[0].nelx;
+        o.prho = m->levels[0].density;
+        o.spr = m->levels[0].density_stride;
       } else {
         o.keff = L.keff;
         o.sk = L.keff_stride;
@@ -464,8 +471,14 @@ static int validate(const sttopt_mg* mg) {
       set_error("sttopt_mg: coarsest level is missing its work vector");
       return STTOPT_EINVAL;
     }
-    if (l > 0 && (L.vb == nullptr || (!last && L.keff == nullptr))) {
+    if (l > 0 && (L.vb == nullptr || (!last && L.keff == nullptr && L.ptable == nullptr))) {
       set_error("sttopt_mg: coarse level is missing keff or its right-hand-side vector");
+      return STTOPT_EINVAL;
+    }
+    if (l > 0 && L.ptable != nullptr &&
+        (mg->levels[0].density == nullptr || L.pkx < 1 || L.pky < 1 ||
+         L.nelx * L.pkx != mg->levels[0].nelx || L.nely * L.pky != mg->levels[0].nely)) {
+      set_error("sttopt_mg: parameterized level does not tile level 0");
       return STTOPT_EINVAL;
     }
     if (l == 0 && mg->vdtype == STTOPT_F32 && mg->nlevels > 1 && L.vb == nullptr) {

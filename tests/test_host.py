@@ -2,6 +2,7 @@
 host logic (coarsening, transfers, interpolation matrices, workloads, load cases);
 every kernel-backed entry point refuses CPU tensors (there is no CPU path)."""
 
+import ctypes
 import re
 import subprocess
 import sys
@@ -30,6 +31,9 @@ def test_library_loads_and_exports_every_declared_symbol():
         assert hasattr(lib, name), name
     assert declared == _lib.exported_symbols()
     assert lib.sttopt_version() >= 1
+    lb, mb = ctypes.c_int64(), ctypes.c_int64()
+    assert lib.sttopt_abi_sizes(ctypes.byref(lb), ctypes.byref(mb)) == 0
+    assert lb.value == ctypes.sizeof(_lib.Level) and mb.value == ctypes.sizeof(_lib.MG)
 
 
 def test_library_reports_errors_without_touching_the_gpu():
