@@ -43,6 +43,13 @@ VCYCLE_DTYPE = torch.float32
 #: PCG iterations between host reads of the active-row count.
 CHECK_EVERY = 1
 
+#: Coarse levels whose elements cover at most this many level-0 elements are applied
+#: in the V-cycle from the level-0 densities and fixed per-position matrices
+#: (keff_E = sum_g rho_g G_pos(g), exact by linearity of the Galerkin product) instead
+#: of reading an explicit (4d)^2 matrix per element: at 2x2 coarsening that is 4
+#: densities instead of 64 matrix entries per level-1 element.
+PARAM_MAX_CHILDREN = 16
+
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
     """Per-dimension factor: the smallest of 2, 3 dividing it, else 1 (torch_mg.py:53-72)."""
@@ -191,6 +198,10 @@ class _Level:
     _cinv: Tensor | None = field(default=None, repr=False)  # (rows, n, n) fp64
     _dbatch: tuple = field(default=(), repr=False)
     _copies: dict = field(default_factory=dict, repr=False)
+    # per-position matrices of the level-0 elements under one element of this level
+    # ((pky*pkx, k*k) fp64) and the block size (pkx, pky); None with a reaction term
+    _ptab: Tensor | None = field(default=None, repr=False)
+    _pblock: tuple = field(default=(1, 1), repr=False)
 
     @property
     def d(self) -> int:
@@ -272,6 +283,8 @@ class _Level:
             out["density"] = self.density.detach().to(dtype).reshape(rows, -1).contiguous()
             if self.reaction_diag is not None:
                 out["rdiag"] = self.reaction_diag.detach().to(dtype).reshape(rows, -1).contiguous()
+        elif self._ptab is not None and self._pblock[0] * self._pblock[1] <= PARAM_MAX_CHILDREN:
+            out["ptable"] = self._ptab.to(dtype).contiguous()
         else:
             out["keff"] = self._soa if dtype == torch.float64 else self._soa.to(dtype)
         self._copies[dtype] = out
@@ -346,6 +359,7 @@ def build_hierarchy(
     stream = _lib.stream_handle(dev)
     k = 4 * d
     soa = None
+    ptab = None  # (Ky, Kx, k, k) per-position matrices of the current coarsest level
     while True:
         top = levels[-1]
         if top.nelx * top.nely <= max_coarse_elements:
@@ -370,6 +384,8 @@ def build_hierarchy(
                 ),
                 "sttopt_galerkin",
             )
+            if react64 is None:
+                ptab = G.reshape(ky, kx, k, k)
         else:
             _lib.check(
                 lib.sttopt_galerkin(
@@ -379,6 +395,17 @@ def build_hierarchy(
                 ),
                 "sttopt_galerkin",
             )
+        else_ptab = None
+        if soa is not None and ptab is not None:
+            Ky, Kx = ptab.shape[:2]
+            nxt = torch.empty(ky, Ky, kx, Kx, k, k, dtype=torch.float64, device=dev)
+            for my in range(ky):
+                for mx in range(kx):
+                    Sm = S[my * kx + mx]
+                    nxt[my, :, mx] = torch.einsum("pi,yxpq,qj->yxij", Sm, ptab, Sm)
+            else_ptab = nxt.reshape(ky * Ky, kx * Kx, k, k)
+        if soa is not None:
+            ptab = else_ptab
         soa = csoa
         top.kx, top.ky = kx, ky
         cndof = d * (cnelx + 1) * (cnely + 1)
@@ -397,6 +424,8 @@ def build_hierarchy(
                 diag=cdiag.reshape(*dbatch, cndof).to(density.dtype),
                 edofMat=element_dof_map_tensor(cnelx, cnely, dev, d),
                 keff=_soa_view(soa, k, dbatch), _soa=soa, _dbatch=dbatch,
+                _ptab=None if ptab is None else ptab.reshape(-1, k * k).contiguous(),
+                _pblock=(1, 1) if ptab is None else (ptab.shape[1], ptab.shape[0]),
             )
         )
     _dense_factor(levels[-1], rows)
@@ -479,6 +508,9 @@ class VCycle:
             if "keff" in data:
                 L.keff = data["keff"].data_ptr()
                 L.keff_stride = 0 if shared else data["keff"].stride(0)
+            if "ptable" in data:
+                L.ptable = data["ptable"].data_ptr()
+                L.pkx, L.pky = lv._pblock
             if lv._cinv is not None:
                 L.cinv = lv._cinv.data_ptr()
                 L.cinv_stride = 0 if shared else lv._cinv.stride(0)

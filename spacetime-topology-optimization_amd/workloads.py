@@ -132,3 +132,49 @@ def heat(nelx: int = 480, nely: int = 240, seed: int = 0, device="cpu") -> HeatP
         fixed=torch.tensor(fixed, device=device),
         rho=torch.tensor(rho, dtype=torch.float64, device=device),
     )
+
+
+UNEVEN_SIZES = [(64, 32), (128, 64), (256, 128), (512, 256)]
+
+
+def smooth_density(nelx: int, nely: int, rng, volfrac: float = 0.5, sigma: float = 3.0) -> np.ndarray:
+    """Gaussian-smoothed noise shifted (by bisection) to mean `volfrac` after clipping
+    to [1e-3, 1]."""
+    f = scipy.ndimage.gaussian_filter(rng.standard_normal((nely, nelx)), sigma, mode="reflect")
+    f = (f - f.mean()) / (f.std() + 1e-30) * 0.2
+    lo, hi = -1.0, 2.0
+    for _ in range(60):
+        mid = (lo + hi) / 2
+        if np.clip(f + mid, 1e-3, 1.0).mean() < volfrac:
+            lo = mid
+        else:
+            hi = mid
+    return np.clip(f + (lo + hi) / 2, 1e-3, 1.0)
+
+
+def uneven_batch(n_instances: int = 64, seed: int = 0, device="cpu", sizes=None):
+    """`n_instances` cantilever instances (left edge clamped) with grid sizes drawn from
+    `sizes`, each with two load cases: a uniform -y traction over the whole top edge
+    and a unit point load at a seeded free node in a seeded direction.
+
+    :return: list of dicts with nelx, nely, x (nely, nelx), loads (2, ndof), freedofs
+    """
+    rng = np.random.default_rng(seed)
+    sizes = UNEVEN_SIZES if sizes is None else sizes
+    out = []
+    for _ in range(n_instances):
+        nelx, nely = sizes[rng.integers(len(sizes))]
+        x = smooth_density(nelx, nely, rng)
+        _, free = _fem.load_case("cantilever", nelx, nely)
+        nodes = _fem.node_grid(nelx, nely)
+        ndof = 2 * nodes.size
+        loads = np.zeros((2, ndof))
+        loads[0, 2 * nodes[0, :] + 1] = -_fem.edge_load_shares(nelx + 1, nelx)
+        node = int(rng.integers(nelx + 1, nodes.size))  # off the clamped first column
+        while node % (nelx + 1) == 0:
+            node = int(rng.integers(nelx + 1, nodes.size))
+        ang = rng.uniform(0, 2 * np.pi)
+        loads[1, 2 * node] = np.cos(ang)
+        loads[1, 2 * node + 1] = np.sin(ang)
+        out.append(dict(nelx=nelx, nely=nely, x=x, loads=loads, freedofs=free))
+    return out

@@ -1,0 +1,306 @@
+"""The five workload configurations, end to end on the device, against the fp64
+oracles (assembled direct solves + hand-derived adjoints) at full size where the
+oracle finishes in seconds, and through size-independent properties (true residual,
+fused == composed adjoint, batched == single) at the largest sizes."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import torch
+
+from conftest import rel_err
+from oracle import fem_direct
+
+pytestmark = pytest.mark.gpu
+
+EMIN, EMAX, PENAL, NU = 1e-9, 1.0, 3.0, 0.3
+
+
+def _true_residual(U, dens, p):
+    """||P(F - K U)|| / ||P F|| per row, recomputed in fp64 with the operator."""
+    from sttopt_amd import torch_fem
+
+    r = torch_fem.project(p - torch_fem.matvec(U, dens["density"], dens["edof"], dens["KE"],
+                                               U.shape[-1]), dens["mask"])
+    b = torch_fem.project(p, dens["mask"])
+    return (r.norm(dim=-1) / b.norm(dim=-1)).cpu().numpy()
+
+
+# ------------------------------------------------------------------ C1 -------
+
+
+def test_c1_mbb_plain_fp64_solve_and_sensitivity(dev):
+    """120x40 MBB half beam, one fp64 MG-PCG solve with 4 levels (max_coarse 200),
+    rtol 1e-8, compliance and dc/drho vs the direct oracle."""
+    from sttopt_amd import compliance, torch_fem, torch_mg, workloads
+
+    p = workloads.mbb(device=dev)
+    old = torch_mg.VCYCLE_DTYPE
+    torch_mg.VCYCLE_DTYPE = torch.float64
+    try:
+        x = p.xPhys.clone().requires_grad_(True)
+        dens = torch_fem.simp_density(x.flatten(), EMIN, EMAX, PENAL)
+        info = {}
+        from sttopt_amd import torch_solve
+
+        U = torch_solve.femsolve(dens, p.F, p.edofMat, p.KE, p.mask, p.nelx, p.nely,
+                                 max_coarse_elements=200, info=info)
+        ce = compliance._element_strain_energy(U, p.edofMat, p.KE, p.nely, p.nelx)
+        c = torch.sum((EMIN + x**PENAL * (EMAX - EMIN)) * ce)
+        c.backward()
+        levels = torch_mg.build_hierarchy(dens.detach(), p.edofMat, p.KE, p.mask, p.nelx, p.nely,
+                                          max_coarse_elements=200)
+    finally:
+        torch_mg.VCYCLE_DTYPE = old
+    assert len(levels) == 4 and (levels[-1].nelx, levels[-1].nely) == (15, 5)
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(p.nelx, p.nely)
+    c_ref, dcx_ref, U_ref = fem_direct.whole_compliance(p.xPhys.cpu().numpy(), KE, edof,
+                                                        p.F.cpu().numpy(), p.freedofs.cpu().numpy())
+    print(f"C1: n_iter {info['forward_n_iter']} / {info['backward_n_iter']}, "
+          f"c rel {abs(float(c) - c_ref) / c_ref:.2e}")
+    assert abs(float(c) - c_ref) <= 1e-6 * c_ref
+    assert rel_err(U.detach().cpu().numpy(), U_ref) < 1e-6
+    assert rel_err(x.grad.cpu().numpy().ravel(), dcx_ref) < 1e-5
+    assert info["backward_n_iter"] == 0
+
+
+# ------------------------------------------------------------------ C2 -------
+
+
+def test_c2_space_time_self_weight_vs_oracle_and_fd(dev):
+    """240x120, N_t = 10 stages (beta 16) as one batched fp64 solve (+ the stage-free
+    whole row), compliance and the hand-written adjoint vs the oracle; central FD
+    (h = 1e-6) on 20 seeded elements."""
+    from sttopt_amd import compliance, torch_mg, workloads
+
+    p = workloads.space_time(240, 120, 10, beta=16.0, seed=0, point_load=False, device=dev)
+    args = (p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof, p.C, p.beta, p.stage_times)
+    old = torch_mg.VCYCLE_DTYPE
+    torch_mg.VCYCLE_DTYPE = torch.float64
+    try:
+        x = p.xPhys.clone().requires_grad_(True)
+        t = p.tPhys.clone().requires_grad_(True)
+        c, stages, U = compliance.batched_whole_and_gravity_compliance(x, t, *args)
+        total = c + sum(stages)
+        total.backward()
+        info = dict(compliance.LAST_SOLVE_INFO)
+        KE = fem_direct.plane_stress_KE(NU)
+        edof = fem_direct.element_dofs(p.nelx, p.nely)
+        c_ref, dcx_ref, st_ref, U_ref = fem_direct.space_time(
+            p.xPhys.cpu().numpy(), p.tPhys.cpu().numpy(), p.stage_times, p.beta, KE, edof,
+            p.F.cpu().numpy(), p.freedofs.cpu().numpy(), p.C)
+        tot_ref = c_ref + sum(s[0] for s in st_ref)
+        print(f"C2: rows {info['rows']}, iterations per row {info['row_iter_list']}")
+        assert abs(float(total) - tot_ref) <= 1e-6 * abs(tot_ref)
+        for k, s in enumerate(st_ref):
+            assert abs(float(stages[k]) - s[0]) <= 1e-6 * abs(s[0])
+        gx_ref = dcx_ref + sum(s[1] for s in st_ref)
+        gt_ref = sum(s[2] for s in st_ref)
+        assert rel_err(x.grad.cpu().numpy().ravel(), gx_ref) < 1e-5
+        assert rel_err(t.grad.cpu().numpy().ravel(), gt_ref) < 1e-5
+        # central differences through the whole fused path
+        rng = np.random.default_rng(20)
+        elems = rng.choice(p.nel, 20, replace=False)
+        h = 1e-6
+        worst = 0.0
+        with torch.no_grad():
+            for e in elems:
+                for which, grad in (("x", x.grad), ("t", t.grad)):
+                    vals = []
+                    for sgn in (1, -1):
+                        xx, tt = p.xPhys.clone(), p.tPhys.clone()
+                        (xx if which == "x" else tt).view(-1)[e] += sgn * h
+                        cc, ss, _ = compliance.batched_whole_and_gravity_compliance(xx, tt, *args)
+                        vals.append(float(cc + sum(ss)))
+                    fd = (vals[0] - vals[1]) / (2 * h)
+                    g = float(grad.view(-1)[e])
+                    worst = max(worst, abs(fd - g) / max(abs(g), 1e-3 * float(grad.abs().max())))
+        print(f"C2: worst FD rel error {worst:.2e}")
+        assert worst < 1e-4
+    finally:
+        torch_mg.VCYCLE_DTYPE = old
+
+
+# ------------------------------------------------------------------ C3 -------
+
+
+def _heat_chain(hp, ep, rho, rtol):
+    """rho -> chi = 1e-3 + rho^3 -> T (scalar solve, T = 0 on the bottom, unit lumped
+    source) -> t_e = mean_e(T) / max (the max held constant) -> stage-weighted
+    self-weight compliance of the elastic companion with xPhys = rho."""
+    from sttopt_amd import compliance, virtual_heat
+
+    mesh = virtual_heat.ScalarMesh(hp.nelx, hp.nely, hp.edof, hp.KE)
+    chi = 1e-3 + rho.flatten() ** 3
+    T = mesh.solve(chi, torch.zeros_like(hp.F), hp.fixed, rtol, F=hp.F)
+    tbar = virtual_heat.element_means(T, hp.edof)
+    t = (tbar / tbar.max().detach()).reshape(hp.nely, hp.nelx)
+    c, stages, _ = compliance.batched_whole_and_gravity_compliance(
+        rho, t, ep.KE, ep.edofMat, EMIN, EMAX, PENAL, ep.freedofs, ep.F, ep.ndof, ep.C, ep.beta,
+        ep.stage_times)
+    return T, t, sum(stages)
+
+
+def test_c3_virtual_heat_time_field_and_gradient(dev):
+    """480x240 scalar heat with an fp32 V-cycle inside the fp64 PCG (rtol 1e-10): T, t
+    and the gradient of a stage-weighted compliance through the time field, against
+    an all-fp64 oracle (direct solves + hand-derived heat adjoint)."""
+    from sttopt_amd import workloads
+
+    hp = workloads.heat(480, 240, seed=0, device=dev)
+    ep = workloads.space_time(480, 240, 4, beta=16.0, seed=0, point_load=False, device=dev)
+    rho = hp.rho.reshape(240, 480).clone().requires_grad_(True)
+    T, t, L = _heat_chain(hp, ep, rho, 1e-10)
+    L.backward()
+    # -- oracle
+    nelx, nely = 480, 240
+    r = hp.rho.cpu().numpy()
+    chi = 1e-3 + r**3
+    edof1 = fem_direct.element_dofs(nelx, nely, 1)
+    Kh = fem_direct.assemble(fem_direct.diffusion_KE(), chi, edof1, (nelx + 1) * (nely + 1))
+    fixed = hp.fixed.cpu().numpy()
+    T_ref = fem_direct.lifted_solve(Kh, hp.F.cpu().numpy(), np.zeros(fixed.size), fixed)
+    tbar = T_ref[edof1].mean(1)
+    tmax = tbar.max()
+    t_ref = tbar / tmax
+    KE = fem_direct.plane_stress_KE(NU)
+    edof2 = fem_direct.element_dofs(nelx, nely)
+    dLdx = np.zeros(nelx * nely)
+    dLdt = np.zeros(nelx * nely)
+    Lref = 0.0
+    for ti in ep.stage_times:
+        cg, gx, gt, _ = fem_direct.gravity_compliance(r.reshape(nely, nelx), t_ref.reshape(nely, nelx),
+                                                      ti, ep.beta, KE, edof2,
+                                                      ep.freedofs.cpu().numpy(), ep.C)
+        Lref += cg
+        dLdx += gx
+        dLdt += gt
+    # t_e = mean(T_e) / tmax: dL/dT at nodes, then the heat adjoint lambda = K^-1 dL/dT
+    gT = np.zeros(fixed.size)
+    np.add.at(gT, edof1.ravel(), np.repeat(dLdt / tmax / 4, 4))
+    lam = fem_direct.lifted_solve(Kh, gT, np.zeros(fixed.size), fixed)
+    dchi = -np.einsum("ei,ij,ej->e", lam[edof1], fem_direct.diffusion_KE(), T_ref[edof1])
+    grad_ref = dLdx + dchi * 3 * r**2
+    eT = rel_err(T.detach().cpu().numpy(), T_ref)
+    et = rel_err(t.detach().cpu().numpy().ravel(), t_ref)
+    eg = rel_err(rho.grad.cpu().numpy().ravel(), grad_ref)
+    print(f"C3: rel err T {eT:.2e}, t {et:.2e}, L {abs(float(L) - Lref) / abs(Lref):.2e}, grad {eg:.2e}")
+    assert eT < 1e-8 and et < 1e-8
+    assert abs(float(L) - Lref) <= 1e-6 * abs(Lref)
+    assert eg < 1e-5
+
+
+# ------------------------------------------------------------------ C4 -------
+
+
+def test_c4_uneven_batch_matches_single_instance_solves(dev):
+    """64 instances of four grid sizes x 2 load cases, bucketed batched solves with
+    fp32 V-cycles, compliance gradients for every instance; small instances against
+    the direct oracle, every instance against its own single solve."""
+    from sttopt_amd import compliance, torch_fem, uneven, workloads
+
+    specs = workloads.uneven_batch(64, seed=0)
+    KE = torch.tensor(fem_direct.plane_stress_KE(NU), dtype=torch.float64, device=dev)
+    insts, xs = [], []
+    for s in specs:
+        ndof = s["loads"].shape[1]
+        mask = torch.zeros(ndof, dtype=torch.bool, device=dev)
+        mask[torch.as_tensor(s["freedofs"], device=dev)] = True
+        x = torch.tensor(s["x"], dtype=torch.float64, device=dev).requires_grad_(True)
+        xs.append(x)
+        insts.append(uneven.Instance(
+            s["nelx"], s["nely"], torch_fem.simp_density(x.flatten(), EMIN, EMAX, PENAL),
+            torch.tensor(s["loads"], dtype=torch.float64, device=dev), mask,
+            torch.tensor(fem_direct.element_dofs(s["nelx"], s["nely"]), device=dev), KE))
+    info = {}
+    Us = uneven.solve_instances(insts, info=info)
+    print("C4 buckets:", info["buckets"])
+    total = 0
+    for inst, U in zip(insts, Us):
+        ce = compliance._StrainEnergy.apply(U, KE, inst.nelx, inst.nely)
+        total = total + torch.sum(inst.density * ce)
+    total.backward()
+    checked = 0
+    for s, inst, U, x in zip(specs, insts, Us, xs):
+        F = s["loads"]
+        if s["nelx"] == 64 and checked < 4:
+            checked += 1
+            K = fem_direct.assemble(fem_direct.plane_stress_KE(NU), fem_direct.simp(s["x"].ravel(), EMIN, EMAX, PENAL),
+                                    fem_direct.element_dofs(s["nelx"], s["nely"]), F.shape[1])
+            for k in range(2):
+                want = fem_direct.solve(K, F[k], s["freedofs"])
+                assert rel_err(U[k].detach().cpu().numpy(), want) < 1e-6
+            # compliance gradient of both cases: -p x^(p-1) sum_k c_e
+            ce = sum(fem_direct.strain_energy(fem_direct.solve(K, F[k], s["freedofs"]),
+                                              fem_direct.element_dofs(s["nelx"], s["nely"]),
+                                              fem_direct.plane_stress_KE(NU)) for k in range(2))
+            gref = -PENAL * (EMAX - EMIN) * s["x"].ravel() ** (PENAL - 1) * ce
+            assert rel_err(x.grad.cpu().numpy().ravel(), gref) < 1e-5
+    assert checked == 4
+    # batched == one-instance solve, for one instance of every size
+    seen = set()
+    from sttopt_amd import torch_solve
+
+    for inst, U in zip(insts, Us):
+        if (inst.nelx, inst.nely) in seen:
+            continue
+        seen.add((inst.nelx, inst.nely))
+        U1 = torch_solve.femsolve(inst.density.detach(), inst.loads, inst.edofMat, KE, inst.mask,
+                                  inst.nelx, inst.nely)
+        assert rel_err(U.detach().cpu().numpy(), U1.cpu().numpy()) < 1e-7
+    assert len(seen) == 4
+
+
+# ------------------------------------------------------------------ C5 -------
+
+
+def test_c5_large_space_time_step_size_independent_checks(dev):
+    """2048x1024 (4.2M dofs, 7 levels), 16 stages (beta 32), self-weight + a top-right
+    point load: true fp64 residual of every row <= 1e-8 (with a margin for the
+    recurrence), fused adjoint == reference composition (autograd through FemSolve)."""
+    from sttopt_amd import compliance, torch_fem, torch_mg, workloads
+
+    p = workloads.space_time(2048, 1024, 16, beta=32.0, seed=0, mean=0.45, std=0.05,
+                             point_load=True, device=dev)
+    args = (p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof)
+    x = p.xPhys.clone().requires_grad_(True)
+    t = p.tPhys.clone().requires_grad_(True)
+    c, stages, U = compliance.batched_whole_and_gravity_compliance(x, t, *args, p.C, p.beta,
+                                                                   p.stage_times)
+    info = dict(compliance.LAST_SOLVE_INFO)
+    (c + sum(stages)).backward()
+    print(f"C5: iterations per row {info['row_iter_list']}")
+    # rebuild every row's density and load with the composed torch algebra, then
+    # recompute the true residual of each row in fp64
+    rows_d, rows_F = [torch_fem.simp_density(p.xPhys.flatten(), EMIN, EMAX, PENAL)], [p.F]
+    nel, nnodes = p.nel, (p.nelx + 1) * (p.nely + 1)
+    C = torch.sparse_coo_tensor(
+        torch.stack([torch.as_tensor(fem_direct.element_dofs(p.nelx, p.nely, 1).ravel()),
+                     torch.arange(nel).repeat_interleave(4)]),
+        torch.full((4 * nel,), p.C / 4, dtype=torch.float64), (nnodes, nel)).coalesce().to(dev)
+    for ti in p.stage_times:
+        _, xt, Fg = compliance._gravity_load(p.xPhys, p.tPhys, ti, C, p.beta, p.ndof)
+        rows_d.append(torch_fem.simp_density(xt.flatten(), EMIN, EMAX, PENAL))
+        rows_F.append(Fg)
+    dens = torch.stack(rows_d)
+    Fs = torch.stack(rows_F)
+    res = _true_residual(U, dict(density=dens, edof=p.edofMat, KE=p.KE, mask=p.mask), Fs)
+    print(f"C5: true residuals {res.max():.2e}")
+    assert res.max() < 2e-8
+    # composed reference path (patched hook) for the gradient
+    orig = compliance._solve_fe_batched
+    compliance._solve_fe_batched = lambda *a, **k: orig(*a, **k)
+    try:
+        x2 = p.xPhys.clone().requires_grad_(True)
+        t2 = p.tPhys.clone().requires_grad_(True)
+        c2, s2, _ = compliance.batched_whole_and_gravity_compliance(x2, t2, *args, C, p.beta,
+                                                                     p.stage_times)
+        (c2 + sum(s2)).backward()
+    finally:
+        compliance._solve_fe_batched = orig
+    assert abs(float(c + sum(stages)) - float(c2 + sum(s2))) <= 1e-6 * abs(float(c2 + sum(s2)))
+    ex, et = rel_err(x.grad.cpu().numpy(), x2.grad.cpu().numpy()), rel_err(t.grad.cpu().numpy(), t2.grad.cpu().numpy())
+    print(f"C5: fused vs composed gradient rel L2 {ex:.2e} (x), {et:.2e} (t)")
+    assert ex < 1e-5 and et < 1e-5
+    assert sp is not None

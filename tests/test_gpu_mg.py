@@ -292,3 +292,29 @@ def test_jacobi_pcg_solve_matches_direct(dev):
                            rtol=1e-10)
     assert n > 0
     np.testing.assert_allclose(U.cpu().numpy(), want, rtol=1e-6, atol=1e-9 * np.abs(want).max())
+
+
+@pytest.mark.parametrize("vdtype", [torch.float64, torch.float32])
+def test_density_parameterized_coarse_levels_equal_explicit_keff(dev, vdtype):
+    """The V-cycle that applies the first coarse levels from the level-0 densities
+    (keff_E = sum_g rho_g G_pos(g)) equals the one reading explicit Galerkin keff."""
+    from sttopt_amd import torch_fem, torch_mg
+
+    nelx, nely = 64, 48  # levels 64x48 -> 32x24 -> 16x12 -> 8x6 -> 4x3
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(9)
+    r = torch_fem.project(s["t"](rng.standard_normal((3, s["ndof"]))), s["mask_t"])
+    x = rng.uniform(0.05, 1, (3, nelx * nely))
+    old = torch_mg.PARAM_MAX_CHILDREN
+    try:
+        outs = []
+        for pmax in (0, 16):
+            torch_mg.PARAM_MAX_CHILDREN = pmax
+            levels = _hip_hierarchy(s, nelx, nely, x, max_coarse_elements=12)
+            assert len(levels) >= 4
+            outs.append(torch_mg.VCycle(levels, dtype=vdtype)(r))
+        tol = 1e-12 if vdtype == torch.float64 else 1e-5
+        err = float((outs[0] - outs[1]).norm() / outs[0].norm())
+        assert err < tol, err
+    finally:
+        torch_mg.PARAM_MAX_CHILDREN = old
