@@ -31,6 +31,11 @@ from . import _grid, _lib, torch_fem, torch_mg, torch_solve
 #: Iteration counts of the last fused batched solve: {"n_iter", "row_iters", "rows"}.
 LAST_SOLVE_INFO: dict = {}
 
+#: Coarsening stop of the fused path's hierarchy (None: torch_mg.MAX_COARSE_ELEMENTS).
+#: Stage rows whose active material forms small islands floating on Emin need a finer
+#: coarsest grid than the default (a coarse space that resolves the islands).
+MAX_COARSE_ELEMENTS: int | None = None
+
 
 def time_mask(tPhys: Tensor, ti: float, beta: float) -> Tensor:
     """Smoothed Heaviside `H_beta(ti - t)`: ~1 where `tPhys < ti`, ~0 after
@@ -192,7 +197,9 @@ class _SpaceTimeCompliance(torch.autograd.Function):
             "sttopt_stage_prologue",
         )
         edof = torch_mg.element_dof_map_tensor(nelx, nely, dev, 2)
-        levels = torch_mg.build_hierarchy(density, edof, KE, mask, nelx, nely)
+        mce = torch_mg.MAX_COARSE_ELEMENTS if MAX_COARSE_ELEMENTS is None else MAX_COARSE_ELEMENTS
+        levels = torch_mg.build_hierarchy(density, edof, KE, mask, nelx, nely,
+                                          max_coarse_elements=mce)
         precond = torch_mg.VCycle(levels)
         info: dict = {}
         U, n_iter = torch_mg.native_mgpcg(

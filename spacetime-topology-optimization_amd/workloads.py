@@ -34,6 +34,17 @@ def smooth_time_field(nelx: int, nely: int, sigma: float, rng) -> np.ndarray:
     return (t - t.min()) / (t.max() - t.min())
 
 
+def print_order_field(nelx: int, nely: int, sigma: float, amplitude: float, rng) -> np.ndarray:
+    """A bottom-up print order: element height fraction plus `amplitude` times a
+    Gaussian-smoothed (sigma elements) unit-variance noise, min-max scaled to [0, 1].
+    Every stage's active set then grows from the clamped base."""
+    height = (nely - 0.5 - np.arange(nely))[:, None] / nely * np.ones((1, nelx))
+    n = scipy.ndimage.gaussian_filter(rng.standard_normal((nely, nelx)), sigma, mode="reflect")
+    n /= n.std() + 1e-30
+    t = height + amplitude * n
+    return (t - t.min()) / (t.max() - t.min())
+
+
 @dataclass
 class Problem:
     nelx: int
@@ -87,11 +98,21 @@ def mbb(nelx: int = 120, nely: int = 40, seed: int = 0, device="cpu",
 
 def space_time(nelx: int, nely: int, n_stage: int, *, beta: float = 16.0, seed: int = 0,
                mean: float = 0.4, std: float = 0.05, sigma: float = 4.0,
-               point_load: bool = True, device="cpu") -> Problem:
-    """Space-time self-weight case: `1 + n_stage` rows (whole structure + stages)."""
+               point_load: bool = True, time_field: str = "noise", amplitude: float = 0.05,
+               device="cpu") -> Problem:
+    """Space-time self-weight case: `1 + n_stage` rows (whole structure + stages).
+
+    :param time_field: "noise" (smoothed white noise, sigma elements) or "print"
+        (`print_order_field`: bottom-up ramp + amplitude x smoothed noise)
+    """
     rng = np.random.default_rng(seed)
     x = grey_density(nelx, nely, mean, std, rng)
-    t = smooth_time_field(nelx, nely, sigma, rng)
+    if time_field == "noise":
+        t = smooth_time_field(nelx, nely, sigma, rng)
+    elif time_field == "print":
+        t = print_order_field(nelx, nely, sigma, amplitude, rng)
+    else:
+        raise ValueError(f"unknown time field {time_field!r}")
     case = "bottom_clamped_tr" if point_load else "bottom_clamped"
     p = _problem(nelx, nely, case, x, device, beta=beta,
                  stage_times=[k / n_stage for k in range(1, n_stage + 1)],

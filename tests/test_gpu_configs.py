@@ -71,13 +71,20 @@ def test_c1_mbb_plain_fp64_solve_and_sensitivity(dev):
 def test_c2_space_time_self_weight_vs_oracle_and_fd(dev):
     """240x120, N_t = 10 stages (beta 16) as one batched fp64 solve (+ the stage-free
     whole row), compliance and the hand-written adjoint vs the oracle; central FD
-    (h = 1e-6) on 20 seeded elements."""
+    (h = 1e-6) on 20 seeded elements.
+
+    The smoothed white-noise time field leaves islands of ~10 elements floating on
+    Emin in the early stages; the reference's default hierarchy (coarsest 30x15) does
+    not reach 1e-8 on rows 2-3 in 500 iterations (the oracle's restatement of the
+    reference algorithm fails the same way), so this runs with a 60x30 coarsest grid
+    (max_coarse_elements 2048)."""
     from sttopt_amd import compliance, torch_mg, torch_solve, workloads
 
     p = workloads.space_time(240, 120, 10, beta=16.0, seed=0, point_load=False, device=dev)
     args = (p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof, p.C, p.beta, p.stage_times)
     old = torch_mg.VCYCLE_DTYPE
     torch_mg.VCYCLE_DTYPE = torch.float64
+    compliance.MAX_COARSE_ELEMENTS = 2048
     try:
         x = p.xPhys.clone().requires_grad_(True)
         t = p.tPhys.clone().requires_grad_(True)
@@ -123,6 +130,7 @@ def test_c2_space_time_self_weight_vs_oracle_and_fd(dev):
     finally:
         torch_mg.VCYCLE_DTYPE = old
         torch_solve.DEFAULT_CG_RTOL = 1e-8
+        compliance.MAX_COARSE_ELEMENTS = None
 
 
 # ------------------------------------------------------------------ C3 -------
@@ -259,13 +267,14 @@ def test_c4_uneven_batch_matches_single_instance_solves(dev):
 
 
 def test_c5_large_space_time_step_size_independent_checks(dev):
-    """2048x1024 (4.2M dofs, 7 levels), 16 stages (beta 32), self-weight + a top-right
-    point load: true fp64 residual of every row <= 1e-8 (with a margin for the
-    recurrence), fused adjoint == reference composition (autograd through FemSolve)."""
+    """2048x1024 (4.2M dofs, 7 levels), 16 stages (beta 32) of a bottom-up print-order
+    time field, self-weight + a top-right point load, fp32 V-cycle: true fp64 residual
+    of every row <= 1e-8 (with a margin for the recurrence), fused adjoint ==
+    reference composition (autograd through FemSolve)."""
     from sttopt_amd import compliance, torch_fem, torch_mg, workloads
 
     p = workloads.space_time(2048, 1024, 16, beta=32.0, seed=0, mean=0.45, std=0.05,
-                             point_load=True, device=dev)
+                             point_load=True, time_field="print", sigma=32.0, device=dev)
     args = (p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof)
     x = p.xPhys.clone().requires_grad_(True)
     t = p.tPhys.clone().requires_grad_(True)

@@ -231,13 +231,13 @@ def test_space_time_fused_equals_composed_autograd(dev):
 
 def test_space_time_gradient_matches_central_differences(dev):
     """Hand-written adjoint vs central FD (h = 1e-6) through the whole fused solve on
-    seeded elements (the C2 protocol at a reduced mesh).  The early stages float on
-    Emin (cond ~ 1e9+), so the differenced compliances are solved to rtol 1e-12 with
-    the fp64 V-cycle: at 1e-8 the solve error, not the adjoint, dominates the FD."""
+    seeded elements (the C2 protocol at a reduced mesh, print-order time field).  The
+    differenced compliances are solved to rtol 1e-11 with the fp64 V-cycle: at 1e-8
+    the solve error, not the adjoint, would dominate the difference."""
     from sttopt_amd import compliance, torch_mg, torch_solve, workloads
 
     old = (torch_solve.DEFAULT_CG_RTOL, torch_mg.VCYCLE_DTYPE)
-    torch_solve.DEFAULT_CG_RTOL, torch_mg.VCYCLE_DTYPE = 1e-12, torch.float64
+    torch_solve.DEFAULT_CG_RTOL, torch_mg.VCYCLE_DTYPE = 1e-11, torch.float64
     try:
         _fd_check(compliance, workloads, dev)
     finally:
@@ -246,7 +246,7 @@ def test_space_time_gradient_matches_central_differences(dev):
 
 def _fd_check(compliance, workloads, dev):
 
-    p = workloads.space_time(60, 30, 10, beta=16.0, seed=1, device=dev)
+    p = workloads.space_time(60, 30, 10, beta=16.0, seed=1, time_field="print", device=dev)
     args = (p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof, p.C, p.beta, p.stage_times)
 
     def total(x, t):
