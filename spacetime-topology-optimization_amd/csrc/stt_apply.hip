@@ -77,7 +77,9 @@ __device__ __forceinline__ Vec<T, D> load_node(const T* base, long long node) {
 // parameterized by the fine densities under each coarse element.
 enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2 };
 
-template <typename T, typename TB, typename TY, int D, int SRC>
+// Coarse-level stencil (SRC 1 or 2): one node per thread on 64 x 4 node tiles.
+// PKX, PKY > 0 fix the parameterized block at compile time (fully unrolled).
+template <typename T, typename TB, typename TY, int D, int SRC, int PKX = 0, int PKY = 0>
 __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> a) {
   constexpr int K = 4 * D;
   __shared__ double red[kBlock / kWave];
@@ -87,7 +89,7 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
   const int by = tt / a.ntx, bx = tt - by * a.ntx;
   const int c = bx * kTileX + (threadIdx.x & (kTileX - 1));
-  const int r = by * kTileY + (threadIdx.x / kTileX);
+  const int r = by * kSmallTileY + (threadIdx.x / kTileX);
   double dot = 0.0;
 
   if (c < a.nnx && r < a.nny) {
@@ -151,11 +153,14 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
         // keff_E rows (loc*D + i) = sum_g rho_g G_g rows: contract each G_g with the
         // element's nodal values, then weight by the child density.
         const T* rho = a.prho + row * a.spr;
-        for (int gy = 0; gy < a.pky; ++gy) {
-          const long long frow = (long long)(er * a.pky + gy) * a.fnelx + (long long)ec * a.pkx;
-          for (int gx = 0; gx < a.pkx; ++gx) {
+        const int pkx = PKX > 0 ? PKX : a.pkx, pky = PKY > 0 ? PKY : a.pky;
+#pragma unroll
+        for (int gy = 0; gy < pky; ++gy) {
+          const long long frow = (long long)(er * pky + gy) * a.fnelx + (long long)ec * pkx;
+#pragma unroll
+          for (int gx = 0; gx < pkx; ++gx) {
             const T s = rho[frow + gx];
-            const T* G = a.ptab + (gy * a.pkx + gx) * K * K;
+            const T* G = a.ptab + (gy * pkx + gx) * K * K;
             T part[D];
 #pragma unroll
             for (int i = 0; i < D; ++i) part[i] = T(0);
@@ -221,6 +226,138 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
   }
 }
 
+// Level-0 stencil (density * KE): one 64 x 16 node tile per workgroup.  The tile's
+// nodes plus a one-node halo (66 x 18) and the 65 x 17 surrounding element
+// densities are staged in LDS with coalesced loads (zeros outside the grid, so the
+// compute loop has no boundary branches); each thread then walks 4 consecutive
+// node rows with a 3-row register window.
+constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2;
+constexpr int kElemX = kTileX + 1, kElemY = kTileY + 1;
+
+template <typename T, typename TB, typename TY, int D>
+__global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB, TY> a) {
+  constexpr int K = 4 * D;
+  __shared__ Vec<T, D> sU[kHaloY][kHaloX];
+  __shared__ T sR[kElemY][kElemX];
+  __shared__ double red[kWaves];
+  const int t = xcd_remap(blockIdx.x, a.total);
+  const int row = t / a.tiles;
+  const int tt = t - row * a.tiles;
+  if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
+  const int by = tt / a.ntx, bx = tt - by * a.ntx;
+  const int x0 = bx * kTileX, y0 = by * kTileY;
+  const T* u = a.u + row * a.su;
+  const T* rho = a.rho + row * a.sr;
+  for (int idx = threadIdx.x; idx < kHaloY * kHaloX; idx += kBlock) {
+    const int i = idx / kHaloX, k = idx - (idx / kHaloX) * kHaloX;
+    const int gy = y0 - 1 + i, gx = x0 - 1 + k;
+    Vec<T, D> w;
+#pragma unroll
+    for (int c = 0; c < D; ++c) w.v[c] = T(0);
+    if (gy >= 0 && gy < a.nny && gx >= 0 && gx < a.nnx) {
+      const long long node = (long long)gy * a.nnx + gx;
+      w = load_node<T, D>(u, node);
+      if (a.proj_in) {
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+          if (!mask_free(a.mask, node * D + c)) w.v[c] = T(0);
+      }
+    }
+    sU[i][k] = w;
+  }
+  for (int idx = threadIdx.x; idx < kElemY * kElemX; idx += kBlock) {
+    const int i = idx / kElemX, k = idx - (idx / kElemX) * kElemX;
+    const int ey = y0 - 1 + i, ex = x0 - 1 + k;
+    T v = T(0);
+    if (ey >= 0 && ey < a.nely && ex >= 0 && ex < a.nelx) v = rho[(long long)ey * a.nelx + ex];
+    sR[i][k] = v;
+  }
+  __syncthreads();
+
+  const int tx = threadIdx.x & (kTileX - 1), ty = threadIdx.x / kTileX;
+  const int gx = x0 + tx;
+  const int i0 = ty * kRowsPerThread + 1;  // LDS row of this thread's first node
+  Vec<T, D> W[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    W[0][c] = sU[i0 - 1][tx + c];
+    W[1][c] = sU[i0][tx + c];
+  }
+  T rUL = sR[i0 - 1][tx], rUR = sR[i0 - 1][tx + 1];
+  double dot = 0.0;
+#pragma unroll
+  for (int j = 0; j < kRowsPerThread; ++j) {
+    const int i = i0 + j;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W[2][c] = sU[i + 1][tx + c];
+    const T rDL = sR[i][tx], rDR = sR[i][tx + 1];
+    const T dens[4] = {rUL, rUR, rDL, rDR};
+    T acc[D];
+#pragma unroll
+    for (int q = 0; q < D; ++q) acc[q] = T(0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      constexpr int locs[4] = {1, 0, 2, 3};
+      const int loc = locs[q];
+      T part[D];
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) part[ii] = T(0);
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {
+        const int ur = (q >> 1) + loc_dr(jn), uc = (q & 1) + loc_dc(jn);
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+          for (int kk = 0; kk < D; ++kk)
+            part[ii] += a.ke.v[(loc * D + ii) * K + jn * D + kk] * W[ur][uc].v[kk];
+        }
+      }
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) acc[ii] += dens[q] * part[ii];
+    }
+    const int gy = y0 + i - 1;
+    if (gy < a.nny && gx < a.nnx) {
+      const long long node = (long long)gy * a.nnx + gx;
+      if (a.rdiag != nullptr) {
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) acc[ii] += a.rdiag[row * a.srd + node * D + ii] * W[1][1].v[ii];
+      }
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) {
+        const long long dof = node * D + ii;
+        const bool fr = a.mode == kMatvec || mask_free(a.mask, dof);
+        T out = T(0);
+        TB bval = TB(0);
+        if (a.mode == kSmooth || a.mode == kResid) bval = a.b[row * a.sb + dof];
+        if (fr) {
+          if (a.mode == kMatvec || a.mode == kOperator) {
+            out = acc[ii];
+          } else if (a.mode == kResid) {
+            out = static_cast<T>(bval) - acc[ii];
+          } else {  // kSmooth
+            out = W[1][1].v[ii] + a.omega * a.invd[row * a.sd + dof] * (static_cast<T>(bval) - acc[ii]);
+          }
+        }
+        const TY yo = static_cast<TY>(out);
+        a.y[row * a.sy + dof] = yo;
+        if (a.dot == kDotU) dot += (double)W[1][1].v[ii] * (double)yo;
+        if (a.dot == kDotB) dot += (double)bval * (double)yo;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      W[0][c] = W[1][c];
+      W[1][c] = W[2][c];
+    }
+    rUL = rDL;
+    rUR = rDR;
+  }
+  if (a.dot != kNoDot) {
+    const double sum = block_sum(dot, red);
+    if (threadIdx.x == 0) a.partial[(long long)row * a.tiles + tt] = sum;
+  }
+}
+
 template <typename T, typename TB, typename TY>
 int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
             long long su, const TB* b, long long sb, const T* invd, long long sd,
@@ -232,8 +369,13 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   a.nely = op.nely;
   a.nnx = op.nnx();
   a.nny = op.nny();
+  const int src = op.ptab != nullptr ? kSrcParam : op.keff != nullptr ? kSrcKeff : kSrcDensity;
   a.ntx = op.ntx();
-  a.tiles = op.tiles();
+  a.tiles = src == kSrcDensity ? op.tiles() : op.tiles_small();
+  if (src != kSrcDensity && dot != kNoDot) {
+    set_error("stencil: reduction partials are only produced at level 0");
+    return STTOPT_EINVAL;
+  }
   const long long total = (long long)a.tiles * rows;
   if (total > 0x7fffffffLL) {
     set_error("stencil: grid too large");
@@ -268,7 +410,6 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   a.prho = static_cast<const T*>(op.prho);
   a.spr = op.spr;
   a.ke = make_ke<T>(op.ke, op.d);
-  const int src = op.ptab != nullptr ? kSrcParam : op.keff != nullptr ? kSrcKeff : kSrcDensity;
   if (src == kSrcDensity && op.rho == nullptr) {
     set_error("stencil: level has neither density nor keff");
     return STTOPT_EINVAL;
@@ -279,17 +420,23 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     return STTOPT_EINVAL;
   }
   dim3 grid((unsigned)a.total), block(kBlock);
+  const bool p22 = op.pkx == 2 && op.pky == 2;
 #define STT_STENCIL(D, SRC) \
   hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, SRC>), grid, block, 0, s, a)
+#define STT_STENCIL22(D) \
+  hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, kSrcParam, 2, 2>), grid, block, 0, s, a)
+#define STT_TILE(D) hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D>), grid, block, 0, s, a)
   if (op.d == 2) {
-    if (src == kSrcParam) STT_STENCIL(2, kSrcParam);
+    if (src == kSrcParam) { if (p22) STT_STENCIL22(2); else STT_STENCIL(2, kSrcParam); }
     else if (src == kSrcKeff) STT_STENCIL(2, kSrcKeff);
-    else STT_STENCIL(2, kSrcDensity);
+    else STT_TILE(2);
   } else if (op.d == 1) {
-    if (src == kSrcParam) STT_STENCIL(1, kSrcParam);
+    if (src == kSrcParam) { if (p22) STT_STENCIL22(1); else STT_STENCIL(1, kSrcParam); }
     else if (src == kSrcKeff) STT_STENCIL(1, kSrcKeff);
-    else STT_STENCIL(1, kSrcDensity);
+    else STT_TILE(1);
 #undef STT_STENCIL
+#undef STT_STENCIL22
+#undef STT_TILE
   } else {
     set_error("stencil: d must be 1 or 2");
     return STTOPT_EINVAL;

@@ -11,11 +11,16 @@
 namespace stt {
 
 constexpr int kWave = 64;
-// Node tiles of the stencil kernels: 64 nodes along x (one wavefront) by 4 node
-// rows (one wavefront each).  A tile's four waves share their halo rows in L1.
+// Level-0 node tiles: 64 nodes along x (one wavefront) by 16 node rows; each of the
+// block's four wavefronts owns 4 consecutive node rows (4 nodes per thread).  Every
+// level-0 kernel that writes per-tile reduction partials uses this tile.
 constexpr int kTileX = 64;
-constexpr int kTileY = 4;
-constexpr int kBlock = kTileX * kTileY;
+constexpr int kWaves = 4;
+constexpr int kRowsPerThread = 4;
+constexpr int kTileY = kWaves * kRowsPerThread;
+constexpr int kBlock = kTileX * kWaves;
+// Coarse-level stencils: one node per thread, 64 x 4 node tiles.
+constexpr int kSmallTileY = 4;
 
 void set_error(const std::string& msg);
 int hip_status(hipError_t e, const char* what);

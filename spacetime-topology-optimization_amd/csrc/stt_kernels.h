@@ -35,7 +35,9 @@ struct OpDesc {
   long long nel() const { return (long long)nelx * nely; }
   int ntx() const { return ceil_div(nnx(), kTileX); }
   int nty() const { return ceil_div(nny(), kTileY); }
-  int tiles() const { return ntx() * nty(); }  // reduction partials per row
+  int tiles() const { return ntx() * nty(); }  // level-0 tiles = reduction partials per row
+  int nty_small() const { return ceil_div(nny(), kSmallTileY); }
+  int tiles_small() const { return ntx() * nty_small(); }
 };
 
 // y = mode(u): see StencilMode.  Types: T is the stencil (storage and compute)

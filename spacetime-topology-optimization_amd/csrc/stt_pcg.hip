@@ -59,23 +59,23 @@ static Scratch carve(void* base, int rows, int tiles) {
 }
 
 // --------------------------------------------------------- tile kernels -----
-// Elementwise PCG kernels walk the same 64 x 4 node tiles as the stencil kernels,
-// so every fp64 partial-sum array has one slot per (row, tile).
+// Elementwise PCG kernels walk the same 64 x 16 node tiles as the level-0 stencil
+// (4 node rows per thread), so every fp64 partial-sum array has one slot per
+// (row, tile).
 
 struct TileGeom {
   int nnx, nny, ntx, tiles, total, d;
 };
 
-__device__ __forceinline__ bool tile_node(const TileGeom& g, int& row, int& tt,
-                                          long long& node) {
+// Row, tile and first node of this thread; nodes j = 0..3 follow at +j*nnx.
+__device__ __forceinline__ void tile_origin(const TileGeom& g, int& row, int& tt, int& c,
+                                           int& r0) {
   const int t = xcd_remap(blockIdx.x, g.total);
   row = t / g.tiles;
   tt = t - row * g.tiles;
   const int by = tt / g.ntx, bx = tt - by * g.ntx;
-  const int c = bx * kTileX + (threadIdx.x & (kTileX - 1));
-  const int r = by * kTileY + (threadIdx.x / kTileX);
-  node = (long long)r * g.nnx + c;
-  return c < g.nnx && r < g.nny;
+  c = bx * kTileX + (threadIdx.x & (kTileX - 1));
+  r0 = by * kTileY + (threadIdx.x / kTileX) * kRowsPerThread;
 }
 
 // x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).
@@ -87,20 +87,24 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(TileGeom g, long lon
                                                             const int* active,
                                                             double* prr, double* prz) {
   __shared__ double red[kBlock / kWave];
-  int row, tt;
-  long long node;
-  const bool in = tile_node(g, row, tt, node);
+  int row, tt, c, r0;
+  tile_origin(g, row, tt, c, r0);
   if (active[row] == 0) return;
   double rr = 0.0, rz = 0.0;
-  if (in) {
-    const double a = alpha[row];
-    for (int k = 0; k < g.d; ++k) {
-      const long long i = row * n + node * g.d + k;
-      x[i] += a * p[i];
-      const double rn = r[i] - a * Ap[i];
-      r[i] = rn;
-      rr += rn * rn;
-      if (z != nullptr) rz += rn * z[i];
+  const double a = alpha[row];
+  if (c < g.nnx) {
+#pragma unroll
+    for (int j = 0; j < kRowsPerThread; ++j) {
+      if (r0 + j >= g.nny) break;
+      const long long node = (long long)(r0 + j) * g.nnx + c;
+      for (int k = 0; k < g.d; ++k) {
+        const long long i = row * n + node * g.d + k;
+        x[i] += a * p[i];
+        const double rn = r[i] - a * Ap[i];
+        r[i] = rn;
+        rr += rn * rn;
+        if (z != nullptr) rz += rn * z[i];
+      }
     }
   }
   double s = block_sum(rr, red);
@@ -117,14 +121,18 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long lo
                                                              const double* beta, double* p,
                                                              const double* z,
                                                              const int* active) {
-  int row, tt;
-  long long node;
-  const bool in = tile_node(g, row, tt, node);
-  if (active[row] == 0 || !in) return;
+  int row, tt, c, r0;
+  tile_origin(g, row, tt, c, r0);
+  if (active[row] == 0 || c >= g.nnx) return;
   const double b = beta[row];
-  for (int k = 0; k < g.d; ++k) {
-    const long long i = row * n + node * g.d + k;
-    p[i] = z[i] + b * p[i];
+#pragma unroll
+  for (int j = 0; j < kRowsPerThread; ++j) {
+    if (r0 + j >= g.nny) break;
+    const long long node = (long long)(r0 + j) * g.nnx + c;
+    for (int k = 0; k < g.d; ++k) {
+      const long long i = row * n + node * g.d + k;
+      p[i] = z[i] + b * p[i];
+    }
   }
 }
 
@@ -135,17 +143,21 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, co
                                                      long long sb, const uint32_t* mask,
                                                      const int* active, double* part) {
   __shared__ double red[kBlock / kWave];
-  int row, tt;
-  long long node;
-  const bool in = tile_node(g, row, tt, node);
+  int row, tt, c, r0;
+  tile_origin(g, row, tt, c, r0);
   if (active != nullptr && active[row] == 0) return;
   double acc = 0.0;
-  if (in) {
-    for (int k = 0; k < g.d; ++k) {
-      const long long i = node * g.d + k;
-      if (!mask_free(mask, i)) continue;
-      const double av = a[row * sa + i];
-      acc += av * (b != nullptr ? b[row * sb + i] : av);
+  if (c < g.nnx) {
+#pragma unroll
+    for (int j = 0; j < kRowsPerThread; ++j) {
+      if (r0 + j >= g.nny) break;
+      const long long node = (long long)(r0 + j) * g.nnx + c;
+      for (int k = 0; k < g.d; ++k) {
+        const long long i = node * g.d + k;
+        if (!mask_free(mask, i)) continue;
+        const double av = a[row * sa + i];
+        acc += av * (b != nullptr ? b[row * sb + i] : av);
+      }
     }
   }
   const double s = block_sum(acc, red);

@@ -48,7 +48,7 @@ CHECK_EVERY = 1
 #: (keff_E = sum_g rho_g G_pos(g), exact by linearity of the Galerkin product) instead
 #: of reading an explicit (4d)^2 matrix per element: at 2x2 coarsening that is 4
 #: densities instead of 64 matrix entries per level-1 element.
-PARAM_MAX_CHILDREN = 16
+PARAM_MAX_CHILDREN = 4
 
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:

