@@ -42,6 +42,9 @@ def parse():
     ap.add_argument("--nstage", type=int, default=NSTAGE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--kernels-only", action="store_true",
+                    help="run only a plain copy of known size and the fp64 / fp32 K(rho)u "
+                         "apply (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE), then exit")
     return ap.parse_args()
 
 
@@ -65,6 +68,9 @@ def main():
     p = workloads.space_time(args.nelx, args.nely, args.nstage, beta=BETA, seed=0,
                              time_field="print", sigma=16.0, device=dev)
     rows = 1 + args.nstage
+    if args.kernels_only:
+        kernels_only(torch, _lib, p, rows, dev)
+        return
     E = (workloads.EMIN, workloads.EMAX, workloads.PENAL)
     print(f"# workload {p.name}: {p.ndof} dofs x {rows} rows", file=sys.stderr, flush=True)
 
@@ -141,23 +147,68 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "stencil_tile_f64_1024x512_pmc.md")
+
+
+def _apply_args(torch, _lib, p, rows, dev, dtype, code):
+    from sttopt_amd import _grid
+
+    g = torch.Generator(device="cpu").manual_seed(0)
+    u = torch.randn(rows, p.ndof, generator=g, dtype=torch.float64).to(dev, dtype)
+    dens = (torch.rand(rows, p.nel, generator=g, dtype=torch.float64) + 0.1).to(dev, dtype)
+    y = torch.empty_like(u)
+    bits = _grid.mask_bits(p.mask)
+    stream = torch.cuda.current_stream(dev)
+    args = (code, _lib.MODE_OPERATOR, 2, p.nelx, p.nely, rows, u.data_ptr(), p.ndof,
+            dens.data_ptr(), p.nel, _lib.ke_array(p.KE), bits.data_ptr(), None, 0,
+            y.data_ptr(), p.ndof, stream.cuda_stream)
+    return args, (u, dens, y, bits), stream
+
+
+def kernels_only(torch, _lib, p, rows, dev):
+    """A plain device copy of known size (the counter calibration: its FETCH_SIZE /
+    WRITE_SIZE against the bytes it moves), then the fp64 and fp32 K(rho)u applies on
+    all rows, 5 launches each."""
+    lib = _lib.load()
+    n = rows * (2 * p.ndof + p.nel)
+    src = torch.randn(n, device=dev, dtype=torch.float64)
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    for dtype, code in ((torch.float64, _lib.F64), (torch.float32, _lib.F32)):
+        args, keep, _ = _apply_args(torch, _lib, p, rows, dev, dtype, code)
+        for _ in range(5):
+            _lib.check(lib.sttopt_apply(*args), "sttopt_apply")
+        del keep
+    torch.cuda.synchronize()
+    print(f"# copy: {n * 8} bytes read, {n * 8} bytes written per launch; "
+          f"apply fp64: {rows * 8 * (2 * p.ndof + p.nel)} algorithmic bytes per launch",
+          flush=True)
+
+
+def pmc_traffic():
+    """HBM bytes per launch of the fp64 apply from the committed counter summary:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the ratio of the
+    bytes a plain copy moves to what the same counter reads for it.  None when the
+    summary is absent."""
+    try:
+        with open(PMC_SUMMARY) as fh:
+            kv = dict(line.split(":", 1) for line in fh if line.startswith("pmc."))
+        kv = {k.strip(): float(v) for k, v in kv.items()}
+        fetch = kv["pmc.apply_f64.fetch_kib"] * kv["pmc.copy.read_bytes"] / kv["pmc.copy.fetch_kib"]
+        write = kv["pmc.apply_f64.write_kib"] * kv["pmc.copy.write_bytes"] / kv["pmc.copy.write_kib"]
+        return fetch + write
+    except (OSError, KeyError, ValueError, ZeroDivisionError):
+        return None
+
+
 def apply_roofline(torch, _lib, torch_fem, p, rows, dev, launches=50):
     """Average duration of the fp64 K(rho)u apply over all rows (HIP events on the
     stream the kernel is launched on) against its algorithmic bytes
     8 * (2 ndof + nel) per row."""
-    g = torch.Generator(device="cpu").manual_seed(0)
-    u = torch.randn(rows, p.ndof, generator=g, dtype=torch.float64).to(dev)
-    dens = torch.rand(rows, p.nel, generator=g, dtype=torch.float64).to(dev) + 0.1
-    y = torch.empty_like(u)
     lib = _lib.load()
-    stream = torch.cuda.current_stream(dev)
-    ke = _lib.ke_array(p.KE)
-    from sttopt_amd import _grid
-
-    bits = _grid.mask_bits(p.mask)
-    args = (_lib.F64, _lib.MODE_OPERATOR, 2, p.nelx, p.nely, rows, u.data_ptr(), p.ndof,
-            dens.data_ptr(), p.nel, ke, bits.data_ptr(), None, 0, y.data_ptr(), p.ndof,
-            stream.cuda_stream)
+    args, keep, stream = _apply_args(torch, _lib, p, rows, dev, torch.float64, _lib.F64)
     for _ in range(3):
         _lib.check(lib.sttopt_apply(*args), "sttopt_apply")
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -169,14 +220,15 @@ def apply_roofline(torch, _lib, torch_fem, p, rows, dev, launches=50):
     sec = start.elapsed_time(end) / 1e3 / launches
     nbytes = rows * 8 * (2 * p.ndof + p.nel)
     achieved = nbytes / sec / 1e9
+    del keep
     return {
-        "kernel": "stencil_kernel<double,double,double,2,false> (fp64 K(rho)u, 17 rows)",
+        "kernel": f"stencil_tile_kernel<double,double,double,2,OPERATOR> (fp64 K(rho)u, {rows} rows)",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBS,
-        "traffic": None,
+        "traffic": pmc_traffic(),
         "algorithmic_bytes_per_launch": nbytes,
         "avg_launch_us": sec * 1e6,
     }

@@ -9,6 +9,7 @@
 // tile order is remapped so that each XCD streams a contiguous band of the grid.
 // SIMP scaling, the lumped reaction diagonal, boundary projection and the
 // damped-Jacobi / residual epilogues of the V-cycle are fused into the same pass.
+#include <cstddef>
 #include <cstdio>
 #include <mutex>
 
@@ -234,9 +235,10 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2;
 constexpr int kElemX = kTileX + 1, kElemY = kTileY + 1;
 
-template <typename T, typename TB, typename TY, int D>
+template <typename T, typename TB, typename TY, int D, int MODE, int DOT>
 __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB, TY> a) {
   constexpr int K = 4 * D;
+  constexpr int NR = kRowsPerThread;
   __shared__ Vec<T, D> sU[kHaloY][kHaloX];
   __shared__ T sR[kElemY][kElemX];
   __shared__ double red[kWaves];
@@ -246,116 +248,224 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
   const int by = tt / a.ntx, bx = tt - by * a.ntx;
   const int x0 = bx * kTileX, y0 = by * kTileY;
-  const T* u = a.u + row * a.su;
-  const T* rho = a.rho + row * a.sr;
-  for (int idx = threadIdx.x; idx < kHaloY * kHaloX; idx += kBlock) {
-    const int i = idx / kHaloX, k = idx - (idx / kHaloX) * kHaloX;
-    const int gy = y0 - 1 + i, gx = x0 - 1 + k;
-    Vec<T, D> w;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
+  const T* __restrict__ u = a.u + row * a.su;
+  const T* __restrict__ rho = a.rho + row * a.sr;
+  // Stage the tile.  Wavefront w loads halo rows w, w + 4, ...; lane l loads column
+  // x0 - 1 + l of each, so column validity is per lane and row validity per wavefront.
+  // The two trailing node columns and the trailing element column of the halo are
+  // loaded by the block's first 2 * kHaloY + kElemY threads.  All loads of a thread are
+  // issued before its first LDS write.
+  constexpr int kUR = (kHaloY + kWaves - 1) / kWaves;
+  constexpr int kRR = (kElemY + kWaves - 1) / kWaves;
+  constexpr int kTailU = 2 * kHaloY, kTail = kTailU + kElemY;
+  const int cx = x0 - 1 + lane;
+  const bool cnode = cx >= 0 && cx < a.nnx;
+  const bool celem = cx >= 0 && cx < a.nelx;
+  Vec<T, D> wu[kUR];
+  T wr[kRR];
 #pragma unroll
-    for (int c = 0; c < D; ++c) w.v[c] = T(0);
-    if (gy >= 0 && gy < a.nny && gx >= 0 && gx < a.nnx) {
-      const long long node = (long long)gy * a.nnx + gx;
-      w = load_node<T, D>(u, node);
-      if (a.proj_in) {
+  for (int n = 0; n < kUR; ++n) {
+    const int i = wave + n * kWaves, gy = y0 - 1 + i;
+#pragma unroll
+    for (int c = 0; c < D; ++c) wu[n].v[c] = T(0);
+    if (i < kHaloY && gy >= 0 && gy < a.nny && cnode)
+      wu[n] = load_node<T, D>(u, (long long)gy * a.nnx + cx);
+  }
+#pragma unroll
+  for (int n = 0; n < kRR; ++n) {
+    const int i = wave + n * kWaves, ey = y0 - 1 + i;
+    wr[n] = T(0);
+    if (i < kElemY && ey >= 0 && ey < a.nely && celem) wr[n] = rho[(long long)ey * a.nelx + cx];
+  }
+  // trailing columns: thread t < kTailU takes node (t / 2, 64 + t % 2), the next kElemY
+  // threads take element (t - kTailU, 64)
+  const int tix = threadIdx.x;
+  const int ti = tix < kTailU ? (tix >> 1) : tix - kTailU;
+  const int tk = tix < kTailU ? kTileX + (tix & 1) : kTileX;
+  const int tgy = y0 - 1 + ti, tgx = x0 - 1 + tk;
+  Vec<T, D> tu;
+  T tr = T(0);
+#pragma unroll
+  for (int c = 0; c < D; ++c) tu.v[c] = T(0);
+  if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx)
+    tu = load_node<T, D>(u, (long long)tgy * a.nnx + tgx);
+  if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
+    tr = rho[(long long)tgy * a.nelx + tgx];
+  if (a.proj_in) {
+#pragma unroll
+    for (int n = 0; n < kUR; ++n) {
+      const int i = wave + n * kWaves, gy = y0 - 1 + i;
+      if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) {
+        const long long dof0 = ((long long)gy * a.nnx + cx) * D;
+        const uint32_t word = a.mask[dof0 >> 5];
 #pragma unroll
         for (int c = 0; c < D; ++c)
-          if (!mask_free(a.mask, node * D + c)) w.v[c] = T(0);
+          if (!((word >> ((dof0 + c) & 31)) & 1u)) wu[n].v[c] = T(0);
       }
     }
-    sU[i][k] = w;
+    if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx) {
+      const long long dof0 = ((long long)tgy * a.nnx + tgx) * D;
+      const uint32_t word = a.mask[dof0 >> 5];
+#pragma unroll
+      for (int c = 0; c < D; ++c)
+        if (!((word >> ((dof0 + c) & 31)) & 1u)) tu.v[c] = T(0);
+    }
   }
-  for (int idx = threadIdx.x; idx < kElemY * kElemX; idx += kBlock) {
-    const int i = idx / kElemX, k = idx - (idx / kElemX) * kElemX;
-    const int ey = y0 - 1 + i, ex = x0 - 1 + k;
-    T v = T(0);
-    if (ey >= 0 && ey < a.nely && ex >= 0 && ex < a.nelx) v = rho[(long long)ey * a.nelx + ex];
-    sR[i][k] = v;
+#pragma unroll
+  for (int n = 0; n < kUR; ++n) {
+    const int i = wave + n * kWaves;
+    if (i < kHaloY) sU[i][lane] = wu[n];
   }
+#pragma unroll
+  for (int n = 0; n < kRR; ++n) {
+    const int i = wave + n * kWaves;
+    if (i < kElemY) sR[i][lane] = wr[n];
+  }
+  if (tix < kTailU) sU[ti][tk] = tu;
+  else if (tix < kTail) sR[ti][tk] = tr;
   __syncthreads();
 
-  const int tx = threadIdx.x & (kTileX - 1), ty = threadIdx.x / kTileX;
+  const int tx = lane, ty = wave;
   const int gx = x0 + tx;
-  const int i0 = ty * kRowsPerThread + 1;  // LDS row of this thread's first node
-  Vec<T, D> W[3][3];
+  const int i0 = ty * NR;  // LDS row of the window's first (halo) node row
+  // the thread's (NR + 2) x 3 node window and (NR + 1) x 2 element densities
+  Vec<T, D> W[NR + 2][3];
+  T R[NR + 1][2];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    W[0][c] = sU[i0 - 1][tx + c];
-    W[1][c] = sU[i0][tx + c];
+  for (int r = 0; r < NR + 2; ++r) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) W[r][c] = sU[i0 + r][tx + c];
   }
-  T rUL = sR[i0 - 1][tx], rUR = sR[i0 - 1][tx + 1];
-  double dot = 0.0;
 #pragma unroll
-  for (int j = 0; j < kRowsPerThread; ++j) {
-    const int i = i0 + j;
+  for (int r = 0; r < NR + 1; ++r) {
+    R[r][0] = sR[i0 + r][tx];
+    R[r][1] = sR[i0 + r][tx + 1];
+  }
+  T acc[NR][D];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) W[2][c] = sU[i + 1][tx + c];
-    const T rDL = sR[i][tx], rDR = sR[i][tx + 1];
-    const T dens[4] = {rUL, rUR, rDL, rDR};
-    T acc[D];
+  for (int j = 0; j < NR; ++j) {
 #pragma unroll
-    for (int q = 0; q < D; ++q) acc[q] = T(0);
+    for (int ii = 0; ii < D; ++ii) acc[j][ii] = T(0);
+  }
+  // element position outer, node row inner: each KE row pair is read once per thread.
+  // KE is read from the kernel-argument segment where it is used, so that only one row
+  // pair is live in scalar registers (the 64 fp64 entries do not fit at once).
+  using Args = StencilArgs<T, TB, TY>;
+  using ConstPtr = const __attribute__((address_space(4))) T*;
+  const ConstPtr kargs_ke = reinterpret_cast<ConstPtr>(
+      (__attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
+      offsetof(Args, ke));
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      constexpr int locs[4] = {1, 0, 2, 3};
-      const int loc = locs[q];
+  for (int q = 0; q < 4; ++q) {
+    constexpr int locs[4] = {1, 0, 2, 3};
+    const int loc = locs[q];
+    // fp64: the 64 KE entries do not fit in scalar registers.  The empty asm makes the
+    // row pointer depend on the accumulators of the previous position, so the compiler
+    // cannot hoist all four row-pair loads to the top and spill them.
+    ConstPtr kq = kargs_ke + loc * D * K;
+    if constexpr (sizeof(T) == 8) {
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) asm volatile("" : "+v"(acc[j][ii]));
+      }
+      asm volatile("" : "+s"(kq));
+    }
+    T ke[D][K];
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+      for (int c = 0; c < K; ++c) ke[ii][c] = kq[ii * K + c];
+    }
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
       T part[D];
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) part[ii] = T(0);
 #pragma unroll
       for (int jn = 0; jn < 4; ++jn) {
-        const int ur = (q >> 1) + loc_dr(jn), uc = (q & 1) + loc_dc(jn);
+        const int ur = j + (q >> 1) + loc_dr(jn), uc = (q & 1) + loc_dc(jn);
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) {
 #pragma unroll
-          for (int kk = 0; kk < D; ++kk)
-            part[ii] += a.ke.v[(loc * D + ii) * K + jn * D + kk] * W[ur][uc].v[kk];
+          for (int kk = 0; kk < D; ++kk) part[ii] += ke[ii][jn * D + kk] * W[ur][uc].v[kk];
         }
       }
+      const T dens = R[j + (q >> 1)][q & 1];
 #pragma unroll
-      for (int ii = 0; ii < D; ++ii) acc[ii] += dens[q] * part[ii];
+      for (int ii = 0; ii < D; ++ii) acc[j][ii] += dens * part[ii];
     }
-    const int gy = y0 + i - 1;
+  }
+
+  double dot = 0.0;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int gy = y0 + i0 + j;
     if (gy < a.nny && gx < a.nnx) {
       const long long node = (long long)gy * a.nnx + gx;
+      const Vec<T, D>& uc = W[j + 1][1];
       if (a.rdiag != nullptr) {
 #pragma unroll
-        for (int ii = 0; ii < D; ++ii) acc[ii] += a.rdiag[row * a.srd + node * D + ii] * W[1][1].v[ii];
+        for (int ii = 0; ii < D; ++ii) acc[j][ii] += a.rdiag[row * a.srd + node * D + ii] * uc.v[ii];
       }
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) {
         const long long dof = node * D + ii;
-        const bool fr = a.mode == kMatvec || mask_free(a.mask, dof);
+        const bool fr = MODE == kMatvec || mask_free(a.mask, dof);
         T out = T(0);
         TB bval = TB(0);
-        if (a.mode == kSmooth || a.mode == kResid) bval = a.b[row * a.sb + dof];
+        if constexpr (MODE == kSmooth || MODE == kResid) bval = a.b[row * a.sb + dof];
         if (fr) {
-          if (a.mode == kMatvec || a.mode == kOperator) {
-            out = acc[ii];
-          } else if (a.mode == kResid) {
-            out = static_cast<T>(bval) - acc[ii];
-          } else {  // kSmooth
-            out = W[1][1].v[ii] + a.omega * a.invd[row * a.sd + dof] * (static_cast<T>(bval) - acc[ii]);
+          if constexpr (MODE == kMatvec || MODE == kOperator) {
+            out = acc[j][ii];
+          } else if constexpr (MODE == kResid) {
+            out = static_cast<T>(bval) - acc[j][ii];
+          } else {
+            out = uc.v[ii] + a.omega * a.invd[row * a.sd + dof] * (static_cast<T>(bval) - acc[j][ii]);
           }
         }
         const TY yo = static_cast<TY>(out);
         a.y[row * a.sy + dof] = yo;
-        if (a.dot == kDotU) dot += (double)W[1][1].v[ii] * (double)yo;
-        if (a.dot == kDotB) dot += (double)bval * (double)yo;
+        if constexpr (DOT == kDotU) dot += (double)uc.v[ii] * (double)yo;
+        if constexpr (DOT == kDotB) dot += (double)bval * (double)yo;
       }
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      W[0][c] = W[1][c];
-      W[1][c] = W[2][c];
-    }
-    rUL = rDL;
-    rUR = rDR;
   }
-  if (a.dot != kNoDot) {
+  if constexpr (DOT != kNoDot) {
     const double sum = block_sum(dot, red);
     if (threadIdx.x == 0) a.partial[(long long)row * a.tiles + tt] = sum;
   }
+}
+
+template <typename T, typename TB, typename TY, int D>
+static int launch_tile(const StencilArgs<T, TB, TY>& a, dim3 grid, hipStream_t s) {
+  const dim3 block(kBlock);
+  switch (a.mode * 3 + a.dot) {
+    case kMatvec * 3 + kNoDot:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kMatvec, kNoDot>), grid, block, 0, s, a);
+      break;
+    case kOperator * 3 + kNoDot:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kOperator, kNoDot>), grid, block, 0, s, a);
+      break;
+    case kOperator * 3 + kDotU:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kOperator, kDotU>), grid, block, 0, s, a);
+      break;
+    case kSmooth * 3 + kNoDot:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kSmooth, kNoDot>), grid, block, 0, s, a);
+      break;
+    case kSmooth * 3 + kDotB:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kSmooth, kDotB>), grid, block, 0, s, a);
+      break;
+    case kResid * 3 + kNoDot:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kResid, kNoDot>), grid, block, 0, s, a);
+      break;
+    default:
+      set_error("stencil: unsupported mode / reduction combination");
+      return STTOPT_EINVAL;
+  }
+  return STTOPT_OK;
 }
 
 template <typename T, typename TB, typename TY>
@@ -425,7 +535,10 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, SRC>), grid, block, 0, s, a)
 #define STT_STENCIL22(D) \
   hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, kSrcParam, 2, 2>), grid, block, 0, s, a)
-#define STT_TILE(D) hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D>), grid, block, 0, s, a)
+#define STT_TILE(D)                                                 \
+  do {                                                              \
+    if (int e_ = launch_tile<T, TB, TY, D>(a, grid, s)) return e_;  \
+  } while (0)
   if (op.d == 2) {
     if (src == kSrcParam) { if (p22) STT_STENCIL22(2); else STT_STENCIL(2, kSrcParam); }
     else if (src == kSrcKeff) STT_STENCIL(2, kSrcKeff);

@@ -79,13 +79,11 @@ __device__ __forceinline__ void tile_origin(const TileGeom& g, int& row, int& tt
 }
 
 // x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).
-__global__ __launch_bounds__(kBlock) void pcg_update_kernel(TileGeom g, long long n,
-                                                            const double* alpha, double* x,
-                                                            double* r, const double* p,
-                                                            const double* Ap,
-                                                            const double* z,
-                                                            const int* active,
-                                                            double* prr, double* prz) {
+__global__ __launch_bounds__(kBlock) void pcg_update_kernel(
+    TileGeom g, long long n, const double* __restrict__ alpha, double* __restrict__ x,
+    double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ Ap,
+    const double* __restrict__ z, const int* __restrict__ active, double* __restrict__ prr,
+    double* __restrict__ prz) {
   __shared__ double red[kBlock / kWave];
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
@@ -95,15 +93,16 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(TileGeom g, long lon
   if (c < g.nnx) {
 #pragma unroll
     for (int j = 0; j < kRowsPerThread; ++j) {
-      if (r0 + j >= g.nny) break;
-      const long long node = (long long)(r0 + j) * g.nnx + c;
-      for (int k = 0; k < g.d; ++k) {
-        const long long i = row * n + node * g.d + k;
-        x[i] += a * p[i];
-        const double rn = r[i] - a * Ap[i];
-        r[i] = rn;
-        rr += rn * rn;
-        if (z != nullptr) rz += rn * z[i];
+      if (r0 + j < g.nny) {
+        const long long node = (long long)(r0 + j) * g.nnx + c;
+        for (int k = 0; k < g.d; ++k) {
+          const long long i = row * n + node * g.d + k;
+          x[i] += a * p[i];
+          const double rn = r[i] - a * Ap[i];
+          r[i] = rn;
+          rr += rn * rn;
+          if (z != nullptr) rz += rn * z[i];
+        }
       }
     }
   }
@@ -118,20 +117,22 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(TileGeom g, long lon
 
 // p = z + beta p
 __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long long n,
-                                                             const double* beta, double* p,
-                                                             const double* z,
-                                                             const int* active) {
+                                                             const double* __restrict__ beta,
+                                                             double* __restrict__ p,
+                                                             const double* __restrict__ z,
+                                                             const int* __restrict__ active) {
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
   if (active[row] == 0 || c >= g.nnx) return;
   const double b = beta[row];
 #pragma unroll
   for (int j = 0; j < kRowsPerThread; ++j) {
-    if (r0 + j >= g.nny) break;
-    const long long node = (long long)(r0 + j) * g.nnx + c;
-    for (int k = 0; k < g.d; ++k) {
-      const long long i = row * n + node * g.d + k;
-      p[i] = z[i] + b * p[i];
+    if (r0 + j < g.nny) {
+      const long long node = (long long)(r0 + j) * g.nnx + c;
+      for (int k = 0; k < g.d; ++k) {
+        const long long i = row * n + node * g.d + k;
+        p[i] = z[i] + b * p[i];
+      }
     }
   }
 }
@@ -150,13 +151,14 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, co
   if (c < g.nnx) {
 #pragma unroll
     for (int j = 0; j < kRowsPerThread; ++j) {
-      if (r0 + j >= g.nny) break;
-      const long long node = (long long)(r0 + j) * g.nnx + c;
-      for (int k = 0; k < g.d; ++k) {
-        const long long i = node * g.d + k;
-        if (!mask_free(mask, i)) continue;
-        const double av = a[row * sa + i];
-        acc += av * (b != nullptr ? b[row * sb + i] : av);
+      if (r0 + j < g.nny) {
+        const long long node = (long long)(r0 + j) * g.nnx + c;
+        for (int k = 0; k < g.d; ++k) {
+          const long long i = node * g.d + k;
+          if (!mask_free(mask, i)) continue;
+          const double av = a[row * sa + i];
+          acc += av * (b != nullptr ? b[row * sb + i] : av);
+        }
       }
     }
   }

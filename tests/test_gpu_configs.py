@@ -70,8 +70,9 @@ def test_c1_mbb_plain_fp64_solve_and_sensitivity(dev):
 
 def test_c2_space_time_self_weight_vs_oracle_and_fd(dev):
     """240x120, N_t = 10 stages (beta 16) as one batched fp64 solve (+ the stage-free
-    whole row), compliance and the hand-written adjoint vs the oracle; central FD
-    (h = 1e-6) on 20 seeded elements.
+    whole row), compliance and the hand-written adjoint vs the oracle; finite
+    differences of the full objective along dense directions, and of the
+    well-conditioned stages' objective on 10 seeded elements.
 
     The smoothed white-noise time field leaves islands of ~10 elements floating on
     Emin in the early stages; the reference's default hierarchy (coarsest 30x15) does
@@ -106,34 +107,66 @@ def test_c2_space_time_self_weight_vs_oracle_and_fd(dev):
         gt_ref = sum(s[2] for s in st_ref)
         assert rel_err(x.grad.cpu().numpy().ravel(), gx_ref) < 1e-5
         assert rel_err(t.grad.cpu().numpy().ravel(), gt_ref) < 1e-5
-        # central differences through the whole fused path; the differenced solves
-        # go to rtol 1e-12 (the floating early stages have cond ~ 1e9+)
+        # Finite differences through the whole fused path, differenced solves at rtol
+        # 1e-12.  Stages 0-3 carry floating islands (c_k ~ 1e3-4e4, cond ~ 1e9): each
+        # evaluation of the full objective carries ~1e-3 absolute noise.
         torch_solve.DEFAULT_CG_RTOL = 1e-12
-        # FD elements among those with a resolvable gradient (|dL/dx| above 1e-2 of its
-        # peak): with floating islands c ~ 1e4, and the cancellation error of the
-        # difference swamps smaller gradients
+
+        def objective(xx, tt, late_only=False):
+            cc, ss, _ = compliance.batched_whole_and_gravity_compliance(xx, tt, *args)
+            return (cc + sum(ss)) if not late_only else sum(ss[k] for k in late)
+
+        gx, gt = x.grad.reshape(-1), t.grad.reshape(-1)
+        gen = torch.Generator(device="cpu").manual_seed(7)
+        with torch.no_grad():
+            # (1) full objective along seeded dense directions (relative steps in x, so
+            # x stays positive): the signal sums all 28,800 elements; central
+            # differences at h and h/2, Richardson-extrapolated (the O(h^2) term
+            # cancels), at steps where the noise contributes < 1e-4
+            for k in range(3):
+                vx = torch.randn(p.nel, generator=gen, dtype=torch.float64).to(dev)
+                vx = vx * p.xPhys.reshape(-1)
+                vt = torch.randn(p.nel, generator=gen, dtype=torch.float64).to(dev)
+                xx, tt = p.xPhys.reshape(-1), p.tPhys.reshape(-1)
+                d = []
+                for h in (2e-3, 1e-3):
+                    fp = float(objective((xx + h * vx).view_as(p.xPhys), (tt + h * vt).view_as(p.tPhys)))
+                    fm = float(objective((xx - h * vx).view_as(p.xPhys), (tt - h * vt).view_as(p.tPhys)))
+                    d.append((fp - fm) / (2 * h))
+                fd = (4 * d[1] - d[0]) / 3
+                g = float(gx @ vx + gt @ vt)
+                err = abs(fd - g) / abs(g)
+                print(f"C2 directional {k}: D(2h) {d[0]:.9e} D(h) {d[1]:.9e} extrapolated "
+                      f"{fd:.9e} adjoint {g:.9e} rel {err:.1e}")
+                assert err < 1e-3
+        # (2) single elements, on the objective of the well-conditioned stages 4-9
+        # (c_k ~ 0.4-2.4): its own forward + adjoint, then central differences at
+        # h = 1e-6 on 10 seeded elements with a resolvable gradient
+        late = [k for k in range(len(stages)) if k >= 4]
+        x2 = p.xPhys.clone().requires_grad_(True)
+        t2 = p.tPhys.clone().requires_grad_(True)
+        objective(x2, t2, late_only=True).backward()
+        gx2, gt2 = x2.grad.reshape(-1), t2.grad.reshape(-1)
         rng = np.random.default_rng(20)
-        gx_abs = x.grad.abs().reshape(-1)
-        cand = torch.nonzero(gx_abs >= 1e-2 * gx_abs.max()).flatten().cpu().numpy()
-        elems = rng.choice(cand, 20, replace=False)
-        h = 1e-6
+        cand = torch.nonzero(gx2.abs() >= 1e-2 * gx2.abs().max()).flatten().cpu().numpy()
+        elems = rng.choice(cand, 10, replace=False)
         worst = 0.0
+        h = 1e-6
         with torch.no_grad():
             for e in elems:
-                for which, grad in (("x", x.grad), ("t", t.grad)):
+                for which, grad in (("x", gx2), ("t", gt2)):
                     vals = []
                     for sgn in (1, -1):
                         xx, tt = p.xPhys.clone(), p.tPhys.clone()
                         (xx if which == "x" else tt).view(-1)[e] += sgn * h
-                        cc, ss, _ = compliance.batched_whole_and_gravity_compliance(xx, tt, *args)
-                        vals.append(float(cc + sum(ss)))
+                        vals.append(float(objective(xx, tt, late_only=True)))
                     fd = (vals[0] - vals[1]) / (2 * h)
-                    g = float(grad.view(-1)[e])
+                    g = float(grad[e])
                     err = abs(fd - g) / max(abs(g), 1e-2 * float(grad.abs().max()))
-                    print(f"C2 FD {which} e={e}: fd {fd:.6e} adjoint {g:.6e} rel {err:.1e}")
+                    print(f"C2 FD stages 4-9 {which} e={e}: fd {fd:.6e} adjoint {g:.6e} rel {err:.1e}")
                     worst = max(worst, err)
-        print(f"C2: worst FD rel error {worst:.2e}")
-        assert worst < 1e-3
+        print(f"C2: worst per-element FD rel error {worst:.2e}")
+        assert worst < 1e-4
     finally:
         torch_mg.VCYCLE_DTYPE = old
         torch_solve.DEFAULT_CG_RTOL = 1e-8
