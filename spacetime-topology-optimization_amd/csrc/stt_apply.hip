@@ -74,6 +74,19 @@ __device__ __forceinline__ Vec<T, D> load_node(const T* base, long long node) {
   return out;
 }
 
+template <typename T, int D>
+__device__ __forceinline__ void store_node(T* base, long long node, const Vec<T, D>& val) {
+  if constexpr (D == 2) {
+    using V2 = HIP_vector_type<T, 2>;
+    V2 w;
+    w.x = val.v[0];
+    w.y = val.v[1];
+    *reinterpret_cast<V2*>(base + 2 * node) = w;
+  } else {
+    base[node] = val.v[0];
+  }
+}
+
 // SRC: 0 = density * KE (level 0), 1 = explicit entry-major keff, 2 = keff
 // parameterized by the fine densities under each coarse element.
 enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2 };
@@ -330,18 +343,23 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const int tx = lane, ty = wave;
   const int gx = x0 + tx;
   const int i0 = ty * NR;  // LDS row of the window's first (halo) node row
-  // the thread's (NR + 2) x 3 node window and (NR + 1) x 2 element densities
-  Vec<T, D> W[NR + 2][3];
-  T R[NR + 1][2];
+  // fp32: the thread's whole (NR + 2) x 3 node window and (NR + 1) x 2 densities live in
+  // registers.  fp64: only the (NR + 1) x 2 sub-window of one element position at a
+  // time, read from LDS per position, which keeps the kernel under 96 VGPRs.
+  constexpr bool kFullWindow = sizeof(T) == 4;
+  Vec<T, D> W[kFullWindow ? NR + 2 : 1][3];
+  T R[kFullWindow ? NR + 1 : 1][2];
+  if constexpr (kFullWindow) {
 #pragma unroll
-  for (int r = 0; r < NR + 2; ++r) {
+    for (int r = 0; r < NR + 2; ++r) {
 #pragma unroll
-    for (int c = 0; c < 3; ++c) W[r][c] = sU[i0 + r][tx + c];
-  }
+      for (int c = 0; c < 3; ++c) W[r][c] = sU[i0 + r][tx + c];
+    }
 #pragma unroll
-  for (int r = 0; r < NR + 1; ++r) {
-    R[r][0] = sR[i0 + r][tx];
-    R[r][1] = sR[i0 + r][tx + 1];
+    for (int r = 0; r < NR + 1; ++r) {
+      R[r][0] = sR[i0 + r][tx];
+      R[r][1] = sR[i0 + r][tx + 1];
+    }
   }
   T acc[NR][D];
 #pragma unroll
@@ -379,6 +397,18 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
 #pragma unroll
       for (int c = 0; c < K; ++c) ke[ii][c] = kq[ii * K + c];
     }
+    const int qr = q >> 1, qc = q & 1;
+    Vec<T, D> S[kFullWindow ? 1 : NR + 1][2];
+    T RQ[kFullWindow ? 1 : NR];
+    if constexpr (!kFullWindow) {
+#pragma unroll
+      for (int r = 0; r < NR + 1; ++r) {
+#pragma unroll
+        for (int c = 0; c < 2; ++c) S[r][c] = sU[i0 + qr + r][tx + qc + c];
+      }
+#pragma unroll
+      for (int j = 0; j < NR; ++j) RQ[j] = sR[i0 + qr + j][tx + qc];
+    }
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
       T part[D];
@@ -386,14 +416,15 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
       for (int ii = 0; ii < D; ++ii) part[ii] = T(0);
 #pragma unroll
       for (int jn = 0; jn < 4; ++jn) {
-        const int ur = j + (q >> 1) + loc_dr(jn), uc = (q & 1) + loc_dc(jn);
+        const Vec<T, D>& un = kFullWindow ? W[j + qr + loc_dr(jn)][qc + loc_dc(jn)]
+                                          : S[j + loc_dr(jn)][loc_dc(jn)];
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) {
 #pragma unroll
-          for (int kk = 0; kk < D; ++kk) part[ii] += ke[ii][jn * D + kk] * W[ur][uc].v[kk];
+          for (int kk = 0; kk < D; ++kk) part[ii] += ke[ii][jn * D + kk] * un.v[kk];
         }
       }
-      const T dens = R[j + (q >> 1)][q & 1];
+      const T dens = kFullWindow ? R[j + qr][qc] : RQ[j];
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) acc[j][ii] += dens * part[ii];
     }
@@ -405,32 +436,44 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
     const int gy = y0 + i0 + j;
     if (gy < a.nny && gx < a.nnx) {
       const long long node = (long long)gy * a.nnx + gx;
-      const Vec<T, D>& uc = W[j + 1][1];
+      const Vec<T, D> uc = kFullWindow ? W[j + 1][1] : sU[i0 + 1 + j][tx + 1];
+      // one vector access per node and array, one mask word per node (the D dofs of a
+      // node share it)
       if (a.rdiag != nullptr) {
+        const Vec<T, D> rd = load_node<T, D>(a.rdiag + row * a.srd, node);
 #pragma unroll
-        for (int ii = 0; ii < D; ++ii) acc[j][ii] += a.rdiag[row * a.srd + node * D + ii] * uc.v[ii];
+        for (int ii = 0; ii < D; ++ii) acc[j][ii] += rd.v[ii] * uc.v[ii];
       }
+      uint32_t word = ~0u;
+      if (MODE != kMatvec && a.mask != nullptr) word = a.mask[(node * D) >> 5];
+      Vec<TB, D> bv;
+      Vec<T, D> iv;
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) {
-        const long long dof = node * D + ii;
-        const bool fr = MODE == kMatvec || mask_free(a.mask, dof);
+        bv.v[ii] = TB(0);
+        iv.v[ii] = T(0);
+      }
+      if constexpr (MODE == kSmooth || MODE == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
+      if constexpr (MODE == kSmooth) iv = load_node<T, D>(a.invd + row * a.sd, node);
+      Vec<TY, D> yo;
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) {
+        const bool fr = (word >> ((node * D + ii) & 31)) & 1u;
         T out = T(0);
-        TB bval = TB(0);
-        if constexpr (MODE == kSmooth || MODE == kResid) bval = a.b[row * a.sb + dof];
         if (fr) {
           if constexpr (MODE == kMatvec || MODE == kOperator) {
             out = acc[j][ii];
           } else if constexpr (MODE == kResid) {
-            out = static_cast<T>(bval) - acc[j][ii];
+            out = static_cast<T>(bv.v[ii]) - acc[j][ii];
           } else {
-            out = uc.v[ii] + a.omega * a.invd[row * a.sd + dof] * (static_cast<T>(bval) - acc[j][ii]);
+            out = uc.v[ii] + a.omega * iv.v[ii] * (static_cast<T>(bv.v[ii]) - acc[j][ii]);
           }
         }
-        const TY yo = static_cast<TY>(out);
-        a.y[row * a.sy + dof] = yo;
-        if constexpr (DOT == kDotU) dot += (double)uc.v[ii] * (double)yo;
-        if constexpr (DOT == kDotB) dot += (double)bval * (double)yo;
+        yo.v[ii] = static_cast<TY>(out);
+        if constexpr (DOT == kDotU) dot += (double)uc.v[ii] * (double)yo.v[ii];
+        if constexpr (DOT == kDotB) dot += (double)bv.v[ii] * (double)yo.v[ii];
       }
+      store_node<TY, D>(a.y + row * a.sy, node, yo);
     }
   }
   if constexpr (DOT != kNoDot) {
@@ -485,6 +528,19 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   if (src != kSrcDensity && dot != kNoDot) {
     set_error("stencil: reduction partials are only produced at level 0");
     return STTOPT_EINVAL;
+  }
+  if (op.d == 2) {
+    // nodes are read and written as one 2-vector: pointers and row strides must keep
+    // every node's pair of dofs aligned to the pair
+    auto ok = [](const void* ptr, long long stride, size_t elem) {
+      return ptr == nullptr ||
+             (reinterpret_cast<uintptr_t>(ptr) % (2 * elem) == 0 && stride % 2 == 0);
+    };
+    if (!ok(u, su, sizeof(T)) || !ok(y, sy, sizeof(TY)) || !ok(b, sb, sizeof(TB)) ||
+        !ok(invd, sd, sizeof(T)) || !ok(op.rdiag, op.srd, sizeof(T))) {
+      set_error("stencil: d = 2 vectors need node-aligned pointers and even row strides");
+      return STTOPT_EINVAL;
+    }
   }
   const long long total = (long long)a.tiles * rows;
   if (total > 0x7fffffffLL) {

@@ -44,6 +44,16 @@ def test_library_reports_errors_without_touching_the_gpu():
     assert st == _lib.EINVAL and "invalid grid" in _lib.last_error()
     st = lib.sttopt_galerkin(2, 5, 4, 2, 2, 1, None, 0, None, 0, None, None, None, 0, None, None, 0, None)
     assert st == _lib.EINVAL
+    # d = 2 nodes are moved as 2-vectors: a pointer off the 16-byte node pair, or an odd
+    # row stride, is refused before any launch (the addresses are never dereferenced)
+    ke = _lib.ke_array(torch.eye(8, dtype=torch.float64))
+    ndof = 2 * 5 * 5
+    st = lib.sttopt_apply(_lib.F64, 0, 2, 4, 4, 1, 4096 + 8, ndof, 8192, 16, ke, None, None, 0,
+                          12288, ndof, None)
+    assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
+    st = lib.sttopt_apply(_lib.F64, 0, 2, 4, 4, 2, 4096, ndof + 1, 8192, 16, ke, None, None, 0,
+                          12288, ndof, None)
+    assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
 
 
 def test_coarsening_factors():
