@@ -14,7 +14,10 @@
  * with row 0 at the top, node (r, c) numbered r * (nelx + 1) + c, d dofs per node
  * interleaved (dof = d * node + component), element-local node order
  * bottom-left, bottom-right, top-right, top-left.  Free-dof masks are packed bits:
- * dof i is free when (mask[i >> 5] >> (i & 31)) & 1.
+ * dof i is free when (mask[i >> 5] >> (i & 31)) & 1.  With d = 2 the kernels move
+ * a node's two dofs as one vector: per-dof vectors (u, y, b, rdiag, PCG vectors and
+ * work) need pointers aligned to 2 * sizeof(element) and even row strides
+ * (STTOPT_EINVAL otherwise).
  *
  * Each declaration names the function of the reference (a pure PyTorch package)
  * whose work it replaces.  The reference has no FFI of its own; the Python modules

@@ -55,38 +55,6 @@ struct StencilArgs {
   KE<T> ke;
 };
 
-template <typename T, int D>
-struct Vec {
-  T v[D];
-};
-
-template <typename T, int D>
-__device__ __forceinline__ Vec<T, D> load_node(const T* base, long long node) {
-  Vec<T, D> out;
-  if constexpr (D == 2) {
-    using V2 = HIP_vector_type<T, 2>;
-    const V2 w = *reinterpret_cast<const V2*>(base + 2 * node);
-    out.v[0] = w.x;
-    out.v[1] = w.y;
-  } else {
-    out.v[0] = base[node];
-  }
-  return out;
-}
-
-template <typename T, int D>
-__device__ __forceinline__ void store_node(T* base, long long node, const Vec<T, D>& val) {
-  if constexpr (D == 2) {
-    using V2 = HIP_vector_type<T, 2>;
-    V2 w;
-    w.x = val.v[0];
-    w.y = val.v[1];
-    *reinterpret_cast<V2*>(base + 2 * node) = w;
-  } else {
-    base[node] = val.v[0];
-  }
-}
-
 // SRC: 0 = density * KE (level 0), 1 = explicit entry-major keff, 2 = keff
 // parameterized by the fine densities under each coarse element.
 enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2 };
@@ -125,12 +93,11 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 #pragma unroll
           for (int k = 0; k < D; ++k) w.v[k] = T(0);
         }
+        const uint32_t word =
+            (a.proj_in && ok && a.mask != nullptr) ? a.mask[(node * D) >> 5] : ~0u;
 #pragma unroll
-        for (int k = 0; k < D; ++k) {
-          T val = w.v[k];
-          if (a.proj_in && ok && !mask_free(a.mask, node * D + k)) val = T(0);
-          U[dr][dc][k] = val;
-        }
+        for (int k = 0; k < D; ++k)
+          U[dr][dc][k] = ((word >> ((node * D + k) & 31)) & 1u) ? w.v[k] : T(0);
       }
     }
 
@@ -209,30 +176,41 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 
     const long long node = (long long)r * a.nnx + c;
     if (a.rdiag != nullptr) {
+      const Vec<T, D> rd = load_node<T, D>(a.rdiag + row * a.srd, node);
 #pragma unroll
-      for (int i = 0; i < D; ++i) acc[i] += a.rdiag[row * a.srd + node * D + i] * U[1][1][i];
+      for (int i = 0; i < D; ++i) acc[i] += rd.v[i] * U[1][1][i];
     }
+    // one vector access per node and array, one mask word per node
+    uint32_t word = ~0u;
+    if (a.mode != kMatvec && a.mask != nullptr) word = a.mask[(node * D) >> 5];
+    Vec<TB, D> bv;
+    Vec<T, D> iv;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
-      const long long dof = node * D + i;
-      const bool fr = a.mode == kMatvec || mask_free(a.mask, dof);
+      bv.v[i] = TB(0);
+      iv.v[i] = T(0);
+    }
+    if (a.mode == kSmooth || a.mode == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
+    if (a.mode == kSmooth) iv = load_node<T, D>(a.invd + row * a.sd, node);
+    Vec<TY, D> yo;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+      const bool fr = (word >> ((node * D + i) & 31)) & 1u;
       T out = T(0);
-      TB bval = TB(0);
-      if (a.mode == kSmooth || a.mode == kResid) bval = a.b[row * a.sb + dof];
       if (fr) {
         if (a.mode == kMatvec || a.mode == kOperator) {
           out = acc[i];
         } else if (a.mode == kResid) {
-          out = static_cast<T>(bval) - acc[i];
+          out = static_cast<T>(bv.v[i]) - acc[i];
         } else {  // kSmooth
-          out = U[1][1][i] + a.omega * a.invd[row * a.sd + dof] * (static_cast<T>(bval) - acc[i]);
+          out = U[1][1][i] + a.omega * iv.v[i] * (static_cast<T>(bv.v[i]) - acc[i]);
         }
       }
-      const TY yo = static_cast<TY>(out);
-      a.y[row * a.sy + dof] = yo;
-      if (a.dot == kDotU) dot += (double)U[1][1][i] * (double)yo;
-      if (a.dot == kDotB) dot += (double)bval * (double)yo;
+      yo.v[i] = static_cast<TY>(out);
+      if (a.dot == kDotU) dot += (double)U[1][1][i] * (double)yo.v[i];
+      if (a.dot == kDotB) dot += (double)bv.v[i] * (double)yo.v[i];
     }
+    store_node<TY, D>(a.y + row * a.sy, node, yo);
   }
   if (a.dot != kNoDot) {
     const double s = block_sum(dot, red);
@@ -532,10 +510,7 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   if (op.d == 2) {
     // nodes are read and written as one 2-vector: pointers and row strides must keep
     // every node's pair of dofs aligned to the pair
-    auto ok = [](const void* ptr, long long stride, size_t elem) {
-      return ptr == nullptr ||
-             (reinterpret_cast<uintptr_t>(ptr) % (2 * elem) == 0 && stride % 2 == 0);
-    };
+    auto ok = pair_aligned;
     if (!ok(u, su, sizeof(T)) || !ok(y, sy, sizeof(TY)) || !ok(b, sb, sizeof(TB)) ||
         !ok(invd, sd, sizeof(T)) || !ok(op.rdiag, op.srd, sizeof(T))) {
       set_error("stencil: d = 2 vectors need node-aligned pointers and even row strides");
@@ -549,7 +524,7 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   }
   a.total = (int)total;
   a.mode = mode;
-  a.proj_in = proj_in ? 1 : 0;
+  a.proj_in = (proj_in && op.mask != nullptr) ? 1 : 0;  // projection needs the mask
   a.dot = dot;
   a.u = u;
   a.su = su;
@@ -726,20 +701,33 @@ __global__ __launch_bounds__(256) void lumped_kernel(int nelx, int nely, const T
 
 // ------------------------------------------------------------- elementwise --
 
-template <typename T, typename TB>
-__global__ __launch_bounds__(256) void jacobi_first_kernel(long long n, const TB* b,
-                                                           long long sb, const T* invd,
+// x = w D^-1 b at free dofs (0 at fixed), optionally with b copied into the V-cycle
+// precision; V consecutive dofs per thread (V = 2 when every array allows pairs).
+template <typename T, typename TB, int V>
+__global__ __launch_bounds__(256) void jacobi_first_kernel(long long n, const TB* __restrict__ b,
+                                                           long long sb,
+                                                           const T* __restrict__ invd,
                                                            long long sd, T omega,
-                                                           const uint32_t* mask, T* x,
-                                                           long long sx, T* bcopy,
-                                                           long long sbc, const int* active) {
+                                                           const uint32_t* __restrict__ mask,
+                                                           T* __restrict__ x, long long sx,
+                                                           T* __restrict__ bcopy, long long sbc,
+                                                           const int* __restrict__ active) {
   const int row = blockIdx.y;
   if (active != nullptr && active[row] == 0) return;
-  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+  const long long nv = n / V;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < nv;
        i += (long long)gridDim.x * blockDim.x) {
-    const T bv = static_cast<T>(b[row * sb + i]);
-    if (bcopy != nullptr) bcopy[row * sbc + i] = bv;
-    x[row * sx + i] = mask_free(mask, i) ? omega * invd[row * sd + i] * bv : T(0);
+    const uint32_t word = mask != nullptr ? mask[(i * V) >> 5] : ~0u;
+    const Vec<TB, V> bv = load_node<TB, V>(b + row * sb, i);
+    const Vec<T, V> dv = load_node<T, V>(invd + row * sd, i);
+    Vec<T, V> cv, xv;
+#pragma unroll
+    for (int k = 0; k < V; ++k) {
+      cv.v[k] = static_cast<T>(bv.v[k]);
+      xv.v[k] = ((word >> ((i * V + k) & 31)) & 1u) ? omega * dv.v[k] * cv.v[k] : T(0);
+    }
+    if (bcopy != nullptr) store_node<T, V>(bcopy + row * sbc, i, cv);
+    store_node<T, V>(x + row * sx, i, xv);
   }
 }
 
@@ -754,9 +742,17 @@ int jacobi_first(long long n, int rows, const TB* b, long long sb, const T* invd
                  long long sd, double omega, const uint32_t* mask, T* x, long long sx,
                  T* bcopy, long long sbc, const int* active, hipStream_t s) {
   if (rows <= 0 || n <= 0) return STTOPT_OK;
-  hipLaunchKernelGGL((jacobi_first_kernel<T, TB>), elem_grid(n, rows), dim3(256), 0, s, n,
-                     b, sb, invd, sd, static_cast<T>(omega), mask, x, sx, bcopy, sbc,
-                     active);
+  const bool pairs = n % 2 == 0 && pair_aligned(b, sb, sizeof(TB)) &&
+                     pair_aligned(invd, sd, sizeof(T)) && pair_aligned(x, sx, sizeof(T)) &&
+                     pair_aligned(bcopy, sbc, sizeof(T));
+  if (pairs)
+    hipLaunchKernelGGL((jacobi_first_kernel<T, TB, 2>), elem_grid(n / 2, rows), dim3(256), 0,
+                       s, n, b, sb, invd, sd, static_cast<T>(omega), mask, x, sx, bcopy, sbc,
+                       active);
+  else
+    hipLaunchKernelGGL((jacobi_first_kernel<T, TB, 1>), elem_grid(n, rows), dim3(256), 0, s,
+                       n, b, sb, invd, sd, static_cast<T>(omega), mask, x, sx, bcopy, sbc,
+                       active);
   STT_CHECK_LAUNCH("jacobi_first_kernel");
   return STTOPT_OK;
 }

@@ -59,6 +59,39 @@ __device__ __forceinline__ int xcd_remap(int L, int total) {
   return xcd * q + min(xcd, r) + local;
 }
 
+// The D dofs of a node, moved as one D-vector (D = 2: 8- or 16-byte aligned pairs).
+template <typename T, int D>
+struct Vec {
+  T v[D];
+};
+
+template <typename T, int D>
+__device__ __forceinline__ Vec<T, D> load_node(const T* base, long long node) {
+  Vec<T, D> out;
+  if constexpr (D == 2) {
+    using V2 = HIP_vector_type<T, 2>;
+    const V2 w = *reinterpret_cast<const V2*>(base + 2 * node);
+    out.v[0] = w.x;
+    out.v[1] = w.y;
+  } else {
+    out.v[0] = base[node];
+  }
+  return out;
+}
+
+template <typename T, int D>
+__device__ __forceinline__ void store_node(T* base, long long node, const Vec<T, D>& val) {
+  if constexpr (D == 2) {
+    using V2 = HIP_vector_type<T, 2>;
+    V2 w;
+    w.x = val.v[0];
+    w.y = val.v[1];
+    *reinterpret_cast<V2*>(base + 2 * node) = w;
+  } else {
+    base[node] = val.v[0];
+  }
+}
+
 // Element-local node order BL, BR, TR, TL as (row, col) offsets from the
 // element's top-left node.
 __device__ __forceinline__ int loc_dr(int j) { return (j == 0 || j == 1) ? 1 : 0; }
@@ -85,5 +118,12 @@ __device__ __forceinline__ double block_sum(double v, double* lds /* >= kBlock/k
 }
 
 inline int ceil_div(long long a, long long b) { return (int)((a + b - 1) / b); }
+
+// A batched vector whose rows can be moved as whole 2-vectors: null, or base pointer
+// aligned to the pair and an even row stride.
+inline bool pair_aligned(const void* p, long long stride, size_t elem) {
+  return p == nullptr ||
+         (reinterpret_cast<uintptr_t>(p) % (2 * elem) == 0 && stride % 2 == 0);
+}
 
 }  // namespace stt

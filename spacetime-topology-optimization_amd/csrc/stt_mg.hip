@@ -59,72 +59,94 @@ __global__ __launch_bounds__(64) void galerkin_kernel(int nelx, int nely, int kx
 
 // ------------------------------------------------------------ transfers -----
 
-template <typename T>
-__global__ __launch_bounds__(256) void restrict_kernel(int d, int cnelx, int cnely, int kx,
-                                                       int ky, const T* fine, long long sf,
-                                                       const uint32_t* cmask, T* coarse,
-                                                       long long sc, const int* active) {
+// Both transfers work on whole nodes (the D dofs as one vector).  KX = KY = 0 takes
+// the coarsening factors at run time; the common 2 x 2 case is compiled with them.
+template <typename T, int D, int KX, int KY>
+__global__ __launch_bounds__(256) void restrict_kernel(int cnelx, int cnely, int kx_, int ky_,
+                                                       const T* __restrict__ fine,
+                                                       long long sf,
+                                                       const uint32_t* __restrict__ cmask,
+                                                       T* __restrict__ coarse, long long sc,
+                                                       const int* __restrict__ active) {
   const int row = blockIdx.y;
   if (active != nullptr && active[row] == 0) return;
+  const int kx = KX > 0 ? KX : kx_, ky = KY > 0 ? KY : ky_;
   const int cnnx = cnelx + 1, cnny = cnely + 1;
   const int fnnx = cnelx * kx + 1, fnny = cnely * ky + 1;
   const long long cnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (cnode >= (long long)cnnx * cnny) return;
   const int I = (int)(cnode / cnnx), J = (int)(cnode - (long long)I * cnnx);
   const T* f = fine + row * sf;
-  for (int comp = 0; comp < d; ++comp) {
-    T acc = T(0);
-    for (int dy = -(ky - 1); dy <= ky - 1; ++dy) {
-      const int R = I * ky + dy;
-      if (R < 0 || R >= fnny) continue;
-      const T wy = T(ky - (dy < 0 ? -dy : dy)) / T(ky);
-      T racc = T(0);
-      for (int dx = -(kx - 1); dx <= kx - 1; ++dx) {
-        const int C = J * kx + dx;
-        if (C < 0 || C >= fnnx) continue;
-        const T wx = T(kx - (dx < 0 ? -dx : dx)) / T(kx);
-        racc += wx * f[((long long)R * fnnx + C) * d + comp];
-      }
-      acc += wy * racc;
+  T acc[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) acc[k] = T(0);
+#pragma unroll
+  for (int dy = -(ky - 1); dy <= ky - 1; ++dy) {
+    const int R = I * ky + dy;
+    if (R < 0 || R >= fnny) continue;
+    const T wy = T(ky - (dy < 0 ? -dy : dy)) / T(ky);
+    T racc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) racc[k] = T(0);
+#pragma unroll
+    for (int dx = -(kx - 1); dx <= kx - 1; ++dx) {
+      const int C = J * kx + dx;
+      if (C < 0 || C >= fnnx) continue;
+      const T wx = T(kx - (dx < 0 ? -dx : dx)) / T(kx);
+      const Vec<T, D> v = load_node<T, D>(f, (long long)R * fnnx + C);
+#pragma unroll
+      for (int k = 0; k < D; ++k) racc[k] += wx * v.v[k];
     }
-    const long long dof = cnode * d + comp;
-    coarse[row * sc + dof] = mask_free(cmask, dof) ? acc : T(0);
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[k] += wy * racc[k];
   }
+  const uint32_t word = cmask != nullptr ? cmask[(cnode * D) >> 5] : ~0u;
+  Vec<T, D> out;
+#pragma unroll
+  for (int k = 0; k < D; ++k) out.v[k] = ((word >> ((cnode * D + k) & 31)) & 1u) ? acc[k] : T(0);
+  store_node<T, D>(coarse + row * sc, cnode, out);
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void prolong_add_kernel(int d, int cnelx, int cnely,
-                                                          int kx, int ky, const T* coarse,
-                                                          long long sc, const uint32_t* fmask,
-                                                          T* fine, long long sf,
-                                                          const int* active) {
+template <typename T, int D, int KX, int KY>
+__global__ __launch_bounds__(256) void prolong_add_kernel(int cnelx, int cnely, int kx_,
+                                                          int ky_, const T* __restrict__ coarse,
+                                                          long long sc,
+                                                          const uint32_t* __restrict__ fmask,
+                                                          T* __restrict__ fine, long long sf,
+                                                          const int* __restrict__ active) {
   const int row = blockIdx.y;
   if (active != nullptr && active[row] == 0) return;
+  const int kx = KX > 0 ? KX : kx_, ky = KY > 0 ? KY : ky_;
   const int cnnx = cnelx + 1;
   const int fnnx = cnelx * kx + 1, fnny = cnely * ky + 1;
   const long long fnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (fnode >= (long long)fnnx * fnny) return;
+  const uint32_t word = fmask != nullptr ? fmask[(fnode * D) >> 5] : ~0u;
+  const uint32_t bits = (word >> ((fnode * D) & 31)) & ((1u << D) - 1u);
+  if (bits == 0) return;  // every dof of the node fixed: nothing to add
   const int R = (int)(fnode / fnnx), C = (int)(fnode - (long long)R * fnnx);
   const int I0 = R / ky, fy = R - I0 * ky;
   const int J0 = C / kx, fx = C - J0 * kx;
   const T wy1 = T(fy) / T(ky), wy0 = T(1) - wy1;
   const T wx1 = T(fx) / T(kx), wx0 = T(1) - wx1;
   const T* c = coarse + row * sc;
-  for (int comp = 0; comp < d; ++comp) {
-    const long long dof = fnode * d + comp;
-    if (!mask_free(fmask, dof)) continue;
-    const long long c00 = ((long long)I0 * cnnx + J0) * d + comp;
-    T top = wx0 * c[c00];
-    if (fx > 0) top += wx1 * c[c00 + d];
-    T v = wy0 * top;
-    if (fy > 0) {
-      const long long c10 = c00 + (long long)cnnx * d;
-      T bot = wx0 * c[c10];
-      if (fx > 0) bot += wx1 * c[c10 + d];
-      v += wy1 * bot;
-    }
-    fine[row * sf + dof] += v;
+  const long long c00 = (long long)I0 * cnnx + J0;
+  const Vec<T, D> a00 = load_node<T, D>(c, c00);
+  Vec<T, D> a01, a10, a11;
+#pragma unroll
+  for (int k = 0; k < D; ++k) a01.v[k] = a10.v[k] = a11.v[k] = T(0);
+  if (fx > 0) a01 = load_node<T, D>(c, c00 + 1);
+  if (fy > 0) a10 = load_node<T, D>(c, c00 + cnnx);
+  if (fx > 0 && fy > 0) a11 = load_node<T, D>(c, c00 + cnnx + 1);
+  T* fr = fine + row * sf;
+  Vec<T, D> v = load_node<T, D>(fr, fnode);
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    const T top = wx0 * a00.v[k] + wx1 * a01.v[k];
+    const T bot = wx0 * a10.v[k] + wx1 * a11.v[k];
+    if ((bits >> k) & 1u) v.v[k] += wy0 * top + wy1 * bot;
   }
+  store_node<T, D>(fr, fnode, v);
 }
 
 template <typename T>
@@ -132,9 +154,28 @@ int restrict_to(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* 
                 long long sf, const uint32_t* cmask, T* coarse, long long sc,
                 const int* active, hipStream_t s) {
   if (rows <= 0) return STTOPT_OK;
+  if (d == 2 && !(pair_aligned(fine, sf, sizeof(T)) && pair_aligned(coarse, sc, sizeof(T)))) {
+    set_error("restrict: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
   const long long n = (long long)(cnelx + 1) * (cnely + 1);
-  hipLaunchKernelGGL((restrict_kernel<T>), dim3(ceil_div(n, 256), rows), dim3(256), 0, s, d,
-                     cnelx, cnely, kx, ky, fine, sf, cmask, coarse, sc, active);
+  const dim3 grid(ceil_div(n, 256), rows), block(256);
+  const bool k22 = kx == 2 && ky == 2;
+  if (d == 2) {
+    if (k22)
+      hipLaunchKernelGGL((restrict_kernel<T, 2, 2, 2>), grid, block, 0, s, cnelx, cnely, kx,
+                         ky, fine, sf, cmask, coarse, sc, active);
+    else
+      hipLaunchKernelGGL((restrict_kernel<T, 2, 0, 0>), grid, block, 0, s, cnelx, cnely, kx,
+                         ky, fine, sf, cmask, coarse, sc, active);
+  } else {
+    if (k22)
+      hipLaunchKernelGGL((restrict_kernel<T, 1, 2, 2>), grid, block, 0, s, cnelx, cnely, kx,
+                         ky, fine, sf, cmask, coarse, sc, active);
+    else
+      hipLaunchKernelGGL((restrict_kernel<T, 1, 0, 0>), grid, block, 0, s, cnelx, cnely, kx,
+                         ky, fine, sf, cmask, coarse, sc, active);
+  }
   STT_CHECK_LAUNCH("restrict_kernel");
   return STTOPT_OK;
 }
@@ -144,9 +185,28 @@ int prolong_add(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* 
                 long long sc, const uint32_t* fmask, T* fine, long long sf,
                 const int* active, hipStream_t s) {
   if (rows <= 0) return STTOPT_OK;
+  if (d == 2 && !(pair_aligned(fine, sf, sizeof(T)) && pair_aligned(coarse, sc, sizeof(T)))) {
+    set_error("prolong: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
   const long long n = (long long)(cnelx * kx + 1) * (cnely * ky + 1);
-  hipLaunchKernelGGL((prolong_add_kernel<T>), dim3(ceil_div(n, 256), rows), dim3(256), 0, s,
-                     d, cnelx, cnely, kx, ky, coarse, sc, fmask, fine, sf, active);
+  const dim3 grid(ceil_div(n, 256), rows), block(256);
+  const bool k22 = kx == 2 && ky == 2;
+  if (d == 2) {
+    if (k22)
+      hipLaunchKernelGGL((prolong_add_kernel<T, 2, 2, 2>), grid, block, 0, s, cnelx, cnely,
+                         kx, ky, coarse, sc, fmask, fine, sf, active);
+    else
+      hipLaunchKernelGGL((prolong_add_kernel<T, 2, 0, 0>), grid, block, 0, s, cnelx, cnely,
+                         kx, ky, coarse, sc, fmask, fine, sf, active);
+  } else {
+    if (k22)
+      hipLaunchKernelGGL((prolong_add_kernel<T, 1, 2, 2>), grid, block, 0, s, cnelx, cnely,
+                         kx, ky, coarse, sc, fmask, fine, sf, active);
+    else
+      hipLaunchKernelGGL((prolong_add_kernel<T, 1, 0, 0>), grid, block, 0, s, cnelx, cnely,
+                         kx, ky, coarse, sc, fmask, fine, sf, active);
+  }
   STT_CHECK_LAUNCH("prolong_add_kernel");
   return STTOPT_OK;
 }

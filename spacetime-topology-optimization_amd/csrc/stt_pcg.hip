@@ -78,7 +78,9 @@ __device__ __forceinline__ void tile_origin(const TileGeom& g, int& row, int& tt
   r0 = by * kTileY + (threadIdx.x / kTileX) * kRowsPerThread;
 }
 
-// x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).
+// x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).  Each
+// thread moves its nodes' D dofs as one vector.
+template <int D>
 __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
     TileGeom g, long long n, const double* __restrict__ alpha, double* __restrict__ x,
     double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ Ap,
@@ -90,19 +92,33 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
   if (active[row] == 0) return;
   double rr = 0.0, rz = 0.0;
   const double a = alpha[row];
+  double* xr = x + row * n;
+  double* rrow = r + row * n;
+  const double* pr = p + row * n;
+  const double* apr = Ap + row * n;
+  const double* zr = z != nullptr ? z + row * n : nullptr;
   if (c < g.nnx) {
 #pragma unroll
     for (int j = 0; j < kRowsPerThread; ++j) {
       if (r0 + j < g.nny) {
         const long long node = (long long)(r0 + j) * g.nnx + c;
-        for (int k = 0; k < g.d; ++k) {
-          const long long i = row * n + node * g.d + k;
-          x[i] += a * p[i];
-          const double rn = r[i] - a * Ap[i];
-          r[i] = rn;
-          rr += rn * rn;
-          if (z != nullptr) rz += rn * z[i];
+        Vec<double, D> xv = load_node<double, D>(xr, node);
+        Vec<double, D> rv = load_node<double, D>(rrow, node);
+        const Vec<double, D> pv = load_node<double, D>(pr, node);
+        const Vec<double, D> av = load_node<double, D>(apr, node);
+        Vec<double, D> zv;
+#pragma unroll
+        for (int k = 0; k < D; ++k) zv.v[k] = 0.0;
+        if (zr != nullptr) zv = load_node<double, D>(zr, node);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          xv.v[k] += a * pv.v[k];
+          rv.v[k] -= a * av.v[k];
+          rr += rv.v[k] * rv.v[k];
+          rz += rv.v[k] * zv.v[k];
         }
+        store_node<double, D>(xr, node, xv);
+        store_node<double, D>(rrow, node, rv);
       }
     }
   }
@@ -116,6 +132,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
 }
 
 // p = z + beta p
+template <int D>
 __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long long n,
                                                              const double* __restrict__ beta,
                                                              double* __restrict__ p,
@@ -125,20 +142,24 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long lo
   tile_origin(g, row, tt, c, r0);
   if (active[row] == 0 || c >= g.nnx) return;
   const double b = beta[row];
+  double* pr = p + row * n;
+  const double* zr = z + row * n;
 #pragma unroll
   for (int j = 0; j < kRowsPerThread; ++j) {
     if (r0 + j < g.nny) {
       const long long node = (long long)(r0 + j) * g.nnx + c;
-      for (int k = 0; k < g.d; ++k) {
-        const long long i = row * n + node * g.d + k;
-        p[i] = z[i] + b * p[i];
-      }
+      Vec<double, D> pv = load_node<double, D>(pr, node);
+      const Vec<double, D> zv = load_node<double, D>(zr, node);
+#pragma unroll
+      for (int k = 0; k < D; ++k) pv.v[k] = zv.v[k] + b * pv.v[k];
+      store_node<double, D>(pr, node, pv);
     }
   }
 }
 
 // partial sums of a.b over free dofs (b == nullptr: a.a), optionally only of
 // active rows.
+template <int D>
 __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, const double* a,
                                                      long long sa, const double* b,
                                                      long long sb, const uint32_t* mask,
@@ -153,18 +174,27 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, co
     for (int j = 0; j < kRowsPerThread; ++j) {
       if (r0 + j < g.nny) {
         const long long node = (long long)(r0 + j) * g.nnx + c;
-        for (int k = 0; k < g.d; ++k) {
-          const long long i = node * g.d + k;
-          if (!mask_free(mask, i)) continue;
-          const double av = a[row * sa + i];
-          acc += av * (b != nullptr ? b[row * sb + i] : av);
-        }
+        const uint32_t word = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
+        const Vec<double, D> av = load_node<double, D>(a + row * sa, node);
+        const Vec<double, D> bv = b != nullptr ? load_node<double, D>(b + row * sb, node) : av;
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          if ((word >> ((node * D + k) & 31)) & 1u) acc += av.v[k] * bv.v[k];
       }
     }
   }
   const double s = block_sum(acc, red);
   if (threadIdx.x == 0) part[(long long)row * g.tiles + tt] = s;
 }
+
+// Launch a TileGeom kernel for the geometry's d (1 or 2; validated by the callers).
+#define STT_TILE_LAUNCH(kernel, g, s, ...)                                            \
+  do {                                                                                \
+    if ((g).d == 2)                                                                   \
+      hipLaunchKernelGGL(kernel<2>, dim3((g).total), dim3(kBlock), 0, s, g, __VA_ARGS__); \
+    else                                                                              \
+      hipLaunchKernelGGL(kernel<1>, dim3((g).total), dim3(kBlock), 0, s, g, __VA_ARGS__); \
+  } while (0)
 
 // ---------------------------------------------------- per-row finalizers ----
 
@@ -438,7 +468,7 @@ struct VCycle {
   int dot_rz(const double* r, long long rs, const double* z, long long zs, double* part) {
     TileGeom g{op[0].nnx(), op[0].nny(), op[0].ntx(), op[0].tiles(),
                op[0].tiles() * rows, mg->d};
-    hipLaunchKernelGGL(dot_kernel, dim3(g.total), dim3(kBlock), 0, s, g, n[0], r, rs, z, zs,
+    STT_TILE_LAUNCH(dot_kernel, g, s, n[0], r, rs, z, zs,
                        op[0].mask, active, part);
     STT_CHECK_LAUNCH("dot_kernel");
     return STTOPT_OK;
@@ -549,7 +579,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   double* p = work + 2 * rows * n;
   double* Ap = work + 3 * rows * n;
   TileGeom g{op.nnx(), op.nny(), op.ntx(), tiles, tiles * rows, mg->d};
-  const dim3 tgrid(g.total), tblock(kBlock), rgrid(rows);
+  const dim3 tblock(kBlock), rgrid(rows);
   *n_iter = 0;
 
   // x <- P x0;  r = P b - A x;  relative residual; active flags
@@ -557,9 +587,9 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (int e = stencil<double, double, double>(op, rows, kResid, false, kNoDot, x, xs, b, bs,
                                               nullptr, 0, r, n, 0.0, nullptr, nullptr, s))
     return e;
-  hipLaunchKernelGGL(dot_kernel, tgrid, tblock, 0, s, g, n, b, bs, nullptr, 0, op.mask,
+  STT_TILE_LAUNCH(dot_kernel, g, s, n, b, bs, nullptr, 0, op.mask,
                      nullptr, sc.part[4]);
-  hipLaunchKernelGGL(dot_kernel, tgrid, tblock, 0, s, g, n, r, n, nullptr, 0, op.mask,
+  STT_TILE_LAUNCH(dot_kernel, g, s, n, r, n, nullptr, 0, op.mask,
                      nullptr, sc.part[1]);
   if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
   hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4], sc.part[1],
@@ -584,7 +614,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       return e;
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
                        sc.rz, sc.alpha);
-    hipLaunchKernelGGL(pcg_update_kernel, tgrid, tblock, 0, s, g, n, sc.alpha, x, r, p, Ap,
+    STT_TILE_LAUNCH(pcg_update_kernel, g, s, n, sc.alpha, x, r, p, Ap,
                        flexible ? z : nullptr, sc.active, sc.part[1], sc.part[3]);
     if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
     hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, tiles, sc.part[1], rtol, k,
@@ -601,7 +631,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     if (int e = vc.run(r, n, z, n, sc.part[2])) return e;
     hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.part[3],
                        flexible, sc.active, sc.rz, sc.beta);
-    hipLaunchKernelGGL(pcg_pupdate_kernel, tgrid, tblock, 0, s, g, n, sc.beta, p, z,
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, g, s, n, sc.beta, p, z,
                        sc.active);
     STT_CHECK_LAUNCH("pcg direction");
   }
@@ -689,6 +719,12 @@ int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double*
       resid == nullptr || n_iter == nullptr || mg->scratch == nullptr ||
       mg->density64 == nullptr || max_iter < 0) {
     set_error("sttopt_mgpcg: null pointer or negative max_iter");
+    return STTOPT_EINVAL;
+  }
+  if (mg->d == 2 && ((reinterpret_cast<uintptr_t>(b) | reinterpret_cast<uintptr_t>(x) |
+                       reinterpret_cast<uintptr_t>(work)) % 16 != 0 ||
+                      b_stride % 2 != 0 || x_stride % 2 != 0)) {
+    set_error("sttopt_mgpcg: d = 2 vectors need node-aligned pointers and even row strides");
     return STTOPT_EINVAL;
   }
   if (mg->rows == 0) {
