@@ -37,6 +37,7 @@ extern "C" {
 #define STTOPT_EINVAL (-1)     /* bad sizes, null pointers, unsupported mode */
 #define STTOPT_EHIP (-2)       /* a HIP runtime call failed */
 #define STTOPT_ENOCONV (-3)    /* PCG reached max_iter with rows above rtol */
+#define STTOPT_ENOTSPD (-4)    /* a dense matrix is not positive definite */
 
 #define STTOPT_MAX_LEVELS 16
 
@@ -194,6 +195,17 @@ int sttopt_dense_assemble(int32_t d, int32_t nelx, int32_t nely, int32_t rows,
                           const double* rdiag, int64_t rdiag_stride,
                           const double* ke, const uint32_t* mask, double* out,
                           int64_t out_stride, void* stream);
+
+/* In-place inverse of a batch of dense SPD matrices (the coarsest operator) by the
+ * block symmetric sweep.  a: rows matrices of n x n fp64, matrix b at a + b * a_stride;
+ * work: sttopt_spd_inverse_work_count(n, rows) doubles.  Synchronizes the stream once
+ * (to read the definiteness flag).  STTOPT_ENOTSPD when a pivot is not positive (a is
+ * then partially overwritten).
+ * Replaces torch.linalg.cholesky of the coarsest level (torch_mg.py:269-290) with the
+ * factor's use by the coarse solve (torch_mg.py:448-451) folded into the inverse. */
+int64_t sttopt_spd_inverse_work_count(int32_t n, int32_t rows);
+int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, double* work,
+                       void* stream);
 
 /* Grid transfers on the node grid (torch_mg._on_node_grid with _restrict_axis /
  * _interp_axis, torch_mg.py:75-109).  restrict: coarse = P_c (R fine);

@@ -16,7 +16,7 @@ from . import _build
 
 LIB_PATH = _build.LIB_PATH
 
-OK, EINVAL, EHIP, ENOCONV = 0, -1, -2, -3
+OK, EINVAL, EHIP, ENOCONV, ENOTSPD = 0, -1, -2, -3, -4
 F64, F32 = 0, 1
 MODE_MATVEC, MODE_OPERATOR = 0, 1
 MAX_LEVELS = 16
@@ -130,6 +130,8 @@ _SIGNATURES = {
     "sttopt_invert_diagonal": (
         c_int32, [c_int32, c_int64, c_int32, _P, c_int64, _P, _P, c_int64, _P]
     ),
+    "sttopt_spd_inverse_work_count": (c_int64, [c_int32, c_int32]),
+    "sttopt_spd_inverse": (c_int32, [c_int32, c_int32, _P, c_int64, _P, _P]),
     "sttopt_dense_assemble": (
         c_int32,
         [c_int32, c_int32, c_int32, c_int32, _P, c_int64, _P, c_int64, _P, c_int64, _P, _P,

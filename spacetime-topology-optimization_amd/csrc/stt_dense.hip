@@ -1,0 +1,327 @@
+// Batched in-place inverse of dense SPD matrices (the coarsest multigrid operator).
+//
+// The reference factors the coarsest operator with torch.linalg.cholesky
+// (torch_mg.py:269-290) and applies it with two triangular solves per V-cycle
+// (torch_mg.py:448-451).  Here the coarse solve is a dense matvec with the inverse, and
+// the inverse is formed by the block symmetric sweep (Gauss-Jordan elimination without
+// pivoting, stable for SPD input): for pivot block K with D = A_KK^-1,
+//
+//   A_IJ <- A_IJ - A_IK D A_KJ   (I, J != K)      A_KJ <- D A_KJ,  A_IK <- A_IK D
+//   A_KK <- -D
+//
+// After every block has been swept the matrix holds -A^-1; the last step writes with
+// the sign flipped.  The matrix stays symmetric throughout, so the update is computed
+// on the lower block triangle only and each tile is written to both halves.
+//
+// Per pivot block (64 wide): sweep_pivot_kernel inverts A_KK in LDS (one workgroup per
+// matrix), sweep_panel_kernel copies the row panel R = A_K: and forms P = D R, and
+// sweep_update_kernel writes every tile from A, R, P and D -- it reads no tile that
+// another workgroup of the same launch writes.
+#include "stt_kernels.h"
+
+namespace stt {
+
+constexpr int kSB = 64;  // sweep block
+
+struct SweepWork {
+  double* D;  // rows x kSB x kSB
+  double* R;  // rows x kSB x n
+  double* P;  // rows x kSB x n
+  int* bad;   // one flag
+};
+
+static SweepWork carve_sweep(double* work, int n, int rows) {
+  SweepWork w;
+  w.D = work;
+  w.R = w.D + (long long)rows * kSB * kSB;
+  w.P = w.R + (long long)rows * kSB * n;
+  w.bad = reinterpret_cast<int*>(w.P + (long long)rows * kSB * n);
+  return w;
+}
+
+// D = A_KK^-1 by the scalar sweep of the bk x bk pivot block.  The block lives in
+// registers: thread t holds column j = t % 64 of rows t / 64 + 4 m (m < 16), padded with
+// the identity past bk; each pivot step publishes the pivot column through LDS (double
+// buffered, one barrier per step).  A pivot that is not positive (or not finite) marks
+// the matrix as not positive definite.
+__global__ __launch_bounds__(256) void sweep_pivot_kernel(int n, int k0, int bk,
+                                                          const double* __restrict__ a,
+                                                          long long sa, double* __restrict__ D,
+                                                          int* __restrict__ bad) {
+  __shared__ double col[2][kSB];
+  const int row = blockIdx.x;
+  const double* A = a + row * sa;
+  const int j = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
+  double m[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = g + 4 * q;
+    m[q] = (i < bk && j < bk) ? A[(long long)(k0 + i) * n + (k0 + j)] : (i == j ? 1.0 : 0.0);
+  }
+  bool bad_pivot = false;
+  for (int p = 0; p < bk; ++p) {
+    double* c = col[p & 1];
+    if (j == p) {
+#pragma unroll
+      for (int q = 0; q < 16; ++q) c[g + 4 * q] = m[q];
+    }
+    __syncthreads();
+    const double piv = c[p];
+    const bool ok = piv > 0.0 && isfinite(piv);
+    bad_pivot |= !ok;
+    const double d = ok ? 1.0 / piv : 0.0;
+    const double cj = c[j];
+    const double dcj = d * cj;
+    const bool colp = j == p;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = g + 4 * q;
+      const double ci = c[i];
+      // off the pivot row and column: m - ci d cj; pivot column: ci d; pivot row: cj d;
+      // pivot: -d (selects, no divergent branches)
+      double v = colp ? ci * d : m[q] - ci * dcj;
+      if (i == p) v = colp ? -d : dcj;  // i == p is uniform across the wavefront
+      m[q] = v;
+    }
+  }
+  if (bad_pivot && threadIdx.x == 0) *bad = 1;  // every thread saw the same pivots
+  double* Dr = D + (long long)row * kSB * kSB;
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = g + 4 * q;
+    Dr[i * kSB + j] = (i < bk && j < bk) ? -m[q] : 0.0;
+  }
+}
+
+constexpr int kKC = 16;  // k rows per LDS chunk
+
+// out[x][y] += sum_{k < bk} X(k, 4 ti + x) Y(k, 4 tj + y) for the 64 x 64 tile of a
+// 256-thread block (ti = t / 16, tj = t % 16).  X and Y are k-major (row k at
+// X + k ldx) with xcols / ycols valid columns; they pass through LDS 16 rows at a time
+// (zero past bk and past the valid columns), the next chunk's loads in flight while the
+// current one is multiplied.  Returns after a barrier-free last chunk: callers that reuse
+// sX / sY synchronize first.
+__device__ __forceinline__ void gemm_tile_kmajor(const double* __restrict__ X, long long ldx,
+                                                 int xcols, const double* __restrict__ Y,
+                                                 long long ldy, int ycols, int bk,
+                                                 double (&out)[4][4], double* sX, double* sY) {
+  constexpr int kPer = kKC * kSB / 256;  // elements of each chunk per thread
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  double nx[kPer], ny[kPer];
+  auto fetch = [&](int kc) {
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      const int e = threadIdx.x + r * 256;
+      const int k = e / kSB, c = e & (kSB - 1);
+      const bool kin = kc + k < bk;
+      nx[r] = (kin && c < xcols) ? X[(long long)(kc + k) * ldx + c] : 0.0;
+      ny[r] = (kin && c < ycols) ? Y[(long long)(kc + k) * ldy + c] : 0.0;
+    }
+  };
+  fetch(0);
+  for (int kc = 0; kc < bk; kc += kKC) {
+    __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+      sX[threadIdx.x + r * 256] = nx[r];
+      sY[threadIdx.x + r * 256] = ny[r];
+    }
+    __syncthreads();
+    if (kc + kKC < bk) fetch(kc + kKC);
+#pragma unroll 4
+    for (int k = 0; k < kKC; ++k) {
+      const double2 a01 = *reinterpret_cast<const double2*>(sX + k * kSB + 4 * ti);
+      const double2 a23 = *reinterpret_cast<const double2*>(sX + k * kSB + 4 * ti + 2);
+      const double2 b01 = *reinterpret_cast<const double2*>(sY + k * kSB + 4 * tj);
+      const double2 b23 = *reinterpret_cast<const double2*>(sY + k * kSB + 4 * tj + 2);
+      const double ra[4] = {a01.x, a01.y, a23.x, a23.y};
+      const double pb[4] = {b01.x, b01.y, b23.x, b23.y};
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+#pragma unroll
+        for (int y = 0; y < 4; ++y) out[x][y] += ra[x] * pb[y];
+      }
+    }
+  }
+}
+
+// R = A[K rows, column block J] (copy) and P = D R for column block J = blockIdx.x.
+__global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
+                                                          const double* __restrict__ a,
+                                                          long long sa,
+                                                          const double* __restrict__ D,
+                                                          double* __restrict__ R,
+                                                          double* __restrict__ P) {
+  __shared__ __align__(16) double sX[kKC * kSB];
+  __shared__ __align__(16) double sY[kKC * kSB];
+  const int row = blockIdx.y;
+  const int j0 = blockIdx.x * kSB;
+  const int bj = min(kSB, n - j0);
+  const double* Ak = a + row * sa + (long long)k0 * n + j0;  // row k of the panel: Ak + k n
+  double* Rr = R + (long long)row * kSB * n + j0;
+  double* Pr = P + (long long)row * kSB * n + j0;
+  for (int e = threadIdx.x; e < kSB * kSB; e += 256) {
+    const int k = e / kSB, c = e & (kSB - 1);
+    if (k < bk && c < bj) Rr[(long long)k * n + c] = Ak[(long long)k * n + c];
+  }
+  double out[4][4] = {};
+  // P(i, j) = sum_k D(i, k) R(k, j); D is symmetric, so its rows serve as X(k, i)
+  gemm_tile_kmajor(D + (long long)row * kSB * kSB, kSB, bk, Ak, n, bj, bk, out, sX, sY);
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int i = 4 * ti + x;
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int j = 4 * tj + y;
+      if (i < bk && j < bj) Pr[(long long)i * n + j] = out[x][y];
+    }
+  }
+}
+
+// Lower-triangle tile t = (I, J), I >= J, of matrix blockIdx.y: every tile of the
+// matrix is written once per sweep step (the tile and, off the diagonal, its mirror
+// through an LDS transpose).  A workgroup reads only its own tile of A, and no other
+// workgroup writes that tile or reads the mirror.
+__global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0, int bk,
+                                                           double* __restrict__ a, long long sa,
+                                                           const double* __restrict__ D,
+                                                           const double* __restrict__ R,
+                                                           const double* __restrict__ P,
+                                                           double sign) {
+  // the k-chunks of the product and, afterwards, the transpose of the mirror tile
+  __shared__ __align__(16) double lds[kSB * (kSB + 1)];
+  double* sX = lds;
+  double* sY = lds + kKC * kSB;
+  double (*sT)[kSB + 1] = reinterpret_cast<double (*)[kSB + 1]>(lds);
+  const int row = blockIdx.y;
+  const int t = blockIdx.x;
+  int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
+  while ((I + 1) * (I + 2) / 2 <= t) ++I;
+  while (I * (I + 1) / 2 > t) --I;
+  const int J = t - I * (I + 1) / 2;
+  const int i0 = I * kSB, j0 = J * kSB;
+  const int bi = min(kSB, n - i0), bj = min(kSB, n - j0);
+  double* A = a + row * sa;
+  const double* Rr = R + (long long)row * kSB * n;
+  const double* Pr = P + (long long)row * kSB * n;
+  const double* Dr = D + (long long)row * kSB * kSB;
+  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  double out[4][4] = {};
+  if (I != kb && J != kb) {
+    // A_IJ - R_KI^T P_KJ  (R_KI^T = A_IK before this step)
+    gemm_tile_kmajor(Rr + i0, n, bi, Pr + j0, n, bj, bk, out, sX, sY);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int i = 4 * ti + x;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int j = 4 * tj + y;
+        out[x][y] = (i < bi && j < bj) ? A[(long long)(i0 + i) * n + (j0 + j)] - out[x][y] : 0.0;
+      }
+    }
+  } else if (I == kb && J == kb) {
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+#pragma unroll
+      for (int y = 0; y < 4; ++y) out[x][y] = -Dr[(4 * ti + x) * kSB + 4 * tj + y];
+    }
+  } else if (I == kb) {
+    // (K, J) = P_KJ
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int i = 4 * ti + x;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int j = 4 * tj + y;
+        out[x][y] = (i < bi && j < bj) ? Pr[(long long)i * n + j0 + j] : 0.0;
+      }
+    }
+  } else {
+    // (I, K) = A_IK D = P_KI^T
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int i = 4 * ti + x;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int j = 4 * tj + y;
+        out[x][y] = (i < bi && j < bj) ? Pr[(long long)j * n + i0 + i] : 0.0;
+      }
+    }
+  }
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {
+    const int i = 4 * ti + x;
+#pragma unroll
+    for (int y = 0; y < 4; ++y) {
+      const int j = 4 * tj + y;
+      out[x][y] *= sign;
+      if (i < bi && j < bj) A[(long long)(i0 + i) * n + (j0 + j)] = out[x][y];
+    }
+  }
+  if (I != J) {
+    // mirror tile (J, I): element (r, c) = out(c, r), written row by row
+    __syncthreads();  // the product's last chunk is consumed before sT overwrites it
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+#pragma unroll
+      for (int y = 0; y < 4; ++y) sT[4 * ti + x][4 * tj + y] = out[x][y];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+      const int r = 4 * ti + x;
+#pragma unroll
+      for (int y = 0; y < 4; ++y) {
+        const int c = 4 * tj + y;
+        if (r < bj && c < bi) A[(long long)(j0 + r) * n + (i0 + c)] = sT[c][r];
+      }
+    }
+  }
+}
+
+}  // namespace stt
+
+using namespace stt;
+
+extern "C" {
+
+int64_t sttopt_spd_inverse_work_count(int32_t n, int32_t rows) {
+  if (n <= 0 || rows <= 0) return 0;
+  // D, R, P in doubles plus one double slot for the flag
+  return (int64_t)rows * kSB * kSB + 2LL * rows * kSB * n + 1;
+}
+
+int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, double* work,
+                       void* stream) {
+  if (n <= 0 || rows < 0 || a == nullptr || work == nullptr ||
+      (rows > 1 && a_stride < (int64_t)n * n)) {
+    set_error("sttopt_spd_inverse: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  hipStream_t s = (hipStream_t)stream;
+  SweepWork w = carve_sweep(work, n, rows);
+  if (int e = fill<int>(1, 1, 0, w.bad, 1, nullptr, s)) return e;
+  const int nb = (n + kSB - 1) / kSB;
+  const int ntile = nb * (nb + 1) / 2;
+  for (int kb = 0; kb < nb; ++kb) {
+    const int k0 = kb * kSB, bk = n - k0 < kSB ? n - k0 : kSB;
+    hipLaunchKernelGGL(sweep_pivot_kernel, dim3(rows), dim3(256), 0, s, n, k0, bk, a,
+                       a_stride, w.D, w.bad);
+    hipLaunchKernelGGL(sweep_panel_kernel, dim3(nb, rows), dim3(256), 0, s, n, k0, bk, a,
+                       a_stride, w.D, w.R, w.P);
+    hipLaunchKernelGGL(sweep_update_kernel, dim3(ntile, rows), dim3(256), 0, s, n, kb, k0,
+                       bk, a, a_stride, w.D, w.R, w.P, kb == nb - 1 ? -1.0 : 1.0);
+    STT_CHECK_LAUNCH("sweep");
+  }
+  int bad = 0;
+  if (int e = read_device_int(w.bad, s, &bad)) return e;
+  if (bad) {
+    set_error("sttopt_spd_inverse: a matrix is not positive definite");
+    return STTOPT_ENOTSPD;
+  }
+  return STTOPT_OK;
+}
+
+}  // extern "C"

@@ -89,4 +89,7 @@ int coarse_solve(long long n, int rows, const double* cinv, long long scinv,
                  const T* b, long long sb, const uint32_t* mask, T* x, long long sx,
                  const int* active, hipStream_t s);
 
+// Copy one int from device memory on stream s and wait for it (pinned readback word).
+int read_device_int(const int* dev, hipStream_t s, int* out);
+
 }  // namespace stt

@@ -542,14 +542,14 @@ static int* pinned_counter() {
   return p;
 }
 
-static int read_counter(const int* dev, hipStream_t s, int* out) {
+int read_device_int(const int* dev, hipStream_t s, int* out) {
   int* h = pinned_counter();
   if (h == nullptr) {
-    set_error("sttopt_mgpcg: hipHostMalloc of the readback word failed");
+    set_error("hipHostMalloc of the readback word failed");
     return STTOPT_EHIP;
   }
   hipError_t e = hipMemcpyAsync(h, dev, sizeof(int), hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(counter)");
+  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(readback word)");
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
   *out = *h;
@@ -596,7 +596,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
                      rtol, sc.bsafe, resid, sc.active, iters, sc.counter);
   STT_CHECK_LAUNCH("pcg init");
   int remaining = 0;
-  if (int e = read_counter(sc.counter, s, &remaining)) return e;
+  if (int e = read_device_int(sc.counter, s, &remaining)) return e;
   if (remaining == 0) return STTOPT_OK;
 
   VCycle<T> vc(mg, s, sc.active);
@@ -621,7 +621,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
                        sc.bsafe, resid, sc.active, iters, sc.counter);
     STT_CHECK_LAUNCH("pcg update");
     if (k % check_every == 0 || k == max_iter) {
-      if (int e = read_counter(sc.counter, s, &remaining)) return e;
+      if (int e = read_device_int(sc.counter, s, &remaining)) return e;
       if (remaining == 0) {
         done = true;
         break;

@@ -192,7 +192,10 @@ class _Level:
     reaction_diag: Tensor | None = None
     kx: int = 1
     ky: int = 1
-    chol: Tensor | None = None
+    # coarsest level: the assembled dense operator (*batch, n, n); its Cholesky factor
+    # `chol` is formed from it on first access (the V-cycle uses `_cinv`)
+    _dense: Tensor | None = field(default=None, repr=False)
+    _chol: Tensor | None = field(default=None, repr=False)
     # device storage used by the kernels
     _soa: Tensor | None = field(default=None, repr=False)  # (rows, k*k, nel) fp64
     _cinv: Tensor | None = field(default=None, repr=False)  # (rows, n, n) fp64
@@ -210,6 +213,14 @@ class _Level:
     @property
     def rows(self) -> int:
         return _grid.numel(self._dbatch)
+
+    @property
+    def chol(self) -> Tensor | None:
+        """Lower Cholesky factor of the coarsest level's dense operator (the reference's
+        `_Level.chol`, torch_mg.py:397), computed on first access."""
+        if self._chol is None and self._dense is not None:
+            self._chol = torch.linalg.cholesky(self._dense)
+        return self._chol
 
     def bits(self) -> Tensor:
         return _grid.mask_bits(self.mask)
@@ -254,7 +265,8 @@ class _Level:
             density=_select_rows(self.density, rows, 1),
             reaction_diag=_select_rows(self.reaction_diag, rows, 1),
             keff=None if soa is None else _soa_view(soa, k, dbatch),
-            chol=_select_rows(self.chol, rows, 2),
+            _dense=_select_rows(self._dense, rows, 2),
+            _chol=_select_rows(self._chol, rows, 2),
             _soa=soa,
             _cinv=cinv,
             _dbatch=dbatch,
@@ -292,8 +304,13 @@ class _Level:
 
 
 def _dense_factor(level: _Level, rows: int) -> None:
-    """Dense projected operator of `level`, its Cholesky factor and inverse
-    (torch_mg._dense_cholesky, torch_mg.py:269-290)."""
+    """Dense projected operator of `level` and its inverse (torch_mg._dense_cholesky,
+    torch_mg.py:269-290; the inverse replaces the factor's two triangular solves per
+    coarse solve, torch_mg.py:448-451).
+
+    :raises torch.linalg.LinAlgError: if the operator is not positive definite (the
+        reference's `torch.linalg.cholesky` raises the same)
+    """
     dev = level.mask.device
     n = level.ndof
     dense = torch.empty(rows, n, n, dtype=torch.float64, device=dev)
@@ -307,17 +324,29 @@ def _dense_factor(level: _Level, rows: int) -> None:
         level._copies["_dense_keep"] = (den, rd, ke_dev)
         args = (None, 0, den.data_ptr(), den.stride(0), _lib.ptr(rd),
                 0 if rd is None else rd.stride(0), ke_dev.data_ptr())
+    lib = _lib.load()
+    stream = _lib.stream_handle(dev)
     _lib.check(
-        _lib.load().sttopt_dense_assemble(
+        lib.sttopt_dense_assemble(
             level.d, level.nelx, level.nely, rows, *args, level.bits().data_ptr(),
-            dense.data_ptr(), n * n, _lib.stream_handle(dev),
+            dense.data_ptr(), n * n, stream,
         ),
         "sttopt_dense_assemble",
     )
-    chol = torch.linalg.cholesky(dense)
-    level._cinv = torch.cholesky_inverse(chol).contiguous()
     level._copies.pop("_dense_keep", None)
-    level.chol = chol.reshape(*level._dbatch, n, n)
+    inv = dense.clone()
+    work = torch.empty(lib.sttopt_spd_inverse_work_count(n, rows), dtype=torch.float64,
+                       device=dev)
+    status = lib.sttopt_spd_inverse(n, rows, inv.data_ptr(), n * n, work.data_ptr(), stream)
+    if status == _lib.ENOTSPD:
+        raise torch.linalg.LinAlgError(
+            "linalg.cholesky: The factorization could not be completed because the input is "
+            "not positive-definite (the coarsest multigrid operator)."
+        )
+    _lib.check(status, "sttopt_spd_inverse")
+    level._cinv = inv
+    level._dense = dense.reshape(*level._dbatch, n, n)
+    level._chol = None
 
 
 def build_hierarchy(
