@@ -156,13 +156,17 @@ typedef struct sttopt_mg {
  * scalars, reduction partials, flags).  levels[*].nelx/nely and rows must be set. */
 int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg);
 
-/* Galerkin coarse element matrices of level (l+1) from level l, fp64, entry-major.
- * Level 0 is given implicitly by density * KE (+ reaction * I/4): pass
- * fine_keff = NULL and the child products G = S^T KE S (g) and GM = S^T (I/4) S
- * (gm, may be NULL) as device arrays of (kx*ky) x (4d)^2 doubles, with
- * density/reaction per fine element.  For coarser levels pass fine_keff and the
- * interpolation matrices S (device, (kx*ky) x (4d)^2, S[m][p][a]: child m's local
- * dof p from the coarse element's local dof a).
+/* Galerkin coarse element matrices, fp64, entry-major.
+ * Density form (fine_keff = NULL): each coarse element covers a kx x ky block of
+ * fine elements of an nelx x nely grid given by density (+ reaction) per element;
+ * keff_E = sum over the block of density_g G[pos g] (+ reaction_g GM[pos g]), with
+ * G (and GM, may be NULL) device arrays of (kx*ky) x (4d)^2 doubles in row-major block
+ * order.  With G = S^T KE S (GM = S^T (I/4) S) over one coarsening step this is
+ * level 1; with G the per-position products of KE through several steps it is any
+ * deeper level directly (blocks up to 4096 fine elements).
+ * Explicit form: pass fine_keff (level l) and the interpolation matrices S (device,
+ * (kx*ky) x (4d)^2, S[m][p][a]: child m's local dof p from the coarse element's local
+ * dof a), kx, ky <= 3.
  * Replaces the einsums of torch_mg.build_hierarchy (torch_mg.py:355-370). */
 int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t ky,
                     int32_t rows, const double* density, int64_t density_stride,

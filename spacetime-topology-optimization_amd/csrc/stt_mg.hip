@@ -6,17 +6,13 @@
 namespace stt {
 
 // ------------------------------------------------------------- Galerkin -----
-// One wavefront per (64 coarse elements, one entry (a, b) of the (4d)^2 element
-// matrix, one row): the entry index is uniform across the wave, so the
-// interpolation weights and child products are scalar loads, and the 64 lanes read
-// 64 neighbouring fine elements.
-//   level 1:   keff_E[a][b] = sum_m rho_{child m} G_m[a][b] (+ react GM_m[a][b])
-//   deeper:    keff_E[a][b] = sum_m sum_{p,q} S_m[p][a] keff_{child m}[p][q] S_m[q][b]
-template <int D, bool FROM_DENSITY>
+// Explicit form (used with a reaction term, where no per-position table exists):
+//   keff_E[a][b] = sum_m sum_{p,q} S_m[p][a] keff_{child m}[p][q] S_m[q][b]
+// One wavefront per (64 coarse elements, one entry (a, b), one row): the entry is
+// uniform across the wave, so the interpolation weights are scalar loads, and the 64
+// lanes read 64 neighbouring fine elements.
+template <int D>
 __global__ __launch_bounds__(64) void galerkin_kernel(int nelx, int nely, int kx, int ky,
-                                                      const double* rho, long long sr,
-                                                      const double* react, long long sre,
-                                                      const double* g, const double* gm,
                                                       const double* fk, long long sfk,
                                                       const double* S, double* ck,
                                                       long long sck) {
@@ -35,26 +31,64 @@ __global__ __launch_bounds__(64) void galerkin_kernel(int nelx, int nely, int kx
     for (int mx = 0; mx < kx; ++mx) {
       const int m = my * kx + mx;
       const long long f = (long long)(cr * ky + my) * nelx + (cc * kx + mx);
-      if constexpr (FROM_DENSITY) {
-        acc += rho[row * sr + f] * g[m * KK + entry];
-        if (react != nullptr) acc += react[row * sre + f] * gm[m * KK + entry];
-      } else {
-        const double* Sm = S + m * KK;
-        for (int p = 0; p < K; ++p) {
-          const double spa = Sm[p * K + A];
-          if (spa == 0.0) continue;
-          double inner = 0.0;
-          for (int q = 0; q < K; ++q) {
-            const double sqb = Sm[q * K + B];
-            if (sqb == 0.0) continue;
-            inner += fk[row * sfk + (long long)(p * K + q) * fnel + f] * sqb;
-          }
-          acc += spa * inner;
+      const double* Sm = S + m * KK;
+      for (int p = 0; p < K; ++p) {
+        const double spa = Sm[p * K + A];
+        if (spa == 0.0) continue;
+        double inner = 0.0;
+        for (int q = 0; q < K; ++q) {
+          const double sqb = Sm[q * K + B];
+          if (sqb == 0.0) continue;
+          inner += fk[row * sfk + (long long)(p * K + q) * fnel + f] * sqb;
         }
+        acc += spa * inner;
       }
     }
   }
   ck[row * sck + (long long)entry * cnel + E] = acc;
+}
+
+// Density form, any block size: keff_E[a][b] = sum over the pkx x pky fine elements g
+// under E of rho_g G_pos(g)[a][b] (+ react_g GM_pos(g)[a][b]).  By linearity this equals
+// the level-by-level Galerkin product when G holds the per-position products of KE
+// through every intermediate level.  One thread per coarse element and EG consecutive
+// entries; the table rows are uniform across the wavefront (scalar loads).
+template <int D, int EG>
+__global__ __launch_bounds__(256) void galerkin_param_kernel(
+    int fnelx, int pkx, int pky, int cnelx, long long cnel, const double* __restrict__ rho,
+    long long sr, const double* __restrict__ react, long long sre,
+    const double* __restrict__ g, const double* __restrict__ gm, double* __restrict__ ck,
+    long long sck) {
+  constexpr int KK = 16 * D * D;
+  const long long E = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int e0 = blockIdx.y * EG;
+  const int row = blockIdx.z;
+  if (E >= cnel) return;
+  const int cr = (int)(E / cnelx), cc = (int)(E - (long long)cr * cnelx);
+  const double* rr = rho + row * sr;
+  const double* re = react != nullptr ? react + row * sre : nullptr;
+  double acc[EG];
+#pragma unroll
+  for (int q = 0; q < EG; ++q) acc[q] = 0.0;
+  for (int gy = 0; gy < pky; ++gy) {
+    const long long frow = (long long)(cr * pky + gy) * fnelx + (long long)cc * pkx;
+    for (int gx = 0; gx < pkx; ++gx) {
+      const int pos = gy * pkx + gx;
+      const double r = rr[frow + gx];
+      const double* G = g + (long long)pos * KK + e0;
+#pragma unroll
+      for (int q = 0; q < EG; ++q) acc[q] += r * G[q];
+      if (re != nullptr) {
+        const double t = re[frow + gx];
+        const double* M = gm + (long long)pos * KK + e0;
+#pragma unroll
+        for (int q = 0; q < EG; ++q) acc[q] += t * M[q];
+      }
+    }
+  }
+  double* out = ck + row * sck + E;
+#pragma unroll
+  for (int q = 0; q < EG; ++q) out[(long long)(e0 + q) * cnel] = acc[q];
 }
 
 // ------------------------------------------------------------ transfers -----
@@ -340,13 +374,15 @@ int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t k
                     const double* gm, const double* fine_keff, int64_t fine_keff_stride,
                     const double* s, double* coarse_keff, int64_t coarse_keff_stride,
                     void* stream) {
-  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || kx < 1 || ky < 1 || kx > 3 ||
-      ky > 3 || nelx % kx || nely % ky || coarse_keff == nullptr || rows < 0) {
+  const bool from_density = fine_keff == nullptr;
+  const int kmax = from_density ? 4096 : 3;  // block (density form) or coarsening factor
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || kx < 1 || ky < 1 || kx > kmax ||
+      ky > kmax || kx * ky > 4096 || nelx % kx || nely % ky || coarse_keff == nullptr ||
+      rows < 0) {
     set_error("sttopt_galerkin: invalid arguments");
     return STTOPT_EINVAL;
   }
   if (rows == 0) return STTOPT_OK;
-  const bool from_density = fine_keff == nullptr;
   if (from_density && (density == nullptr || g == nullptr || (reaction && !gm))) {
     set_error("sttopt_galerkin: level-1 build needs density and G (and GM with reaction)");
     return STTOPT_EINVAL;
@@ -357,17 +393,29 @@ int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t k
   }
   const long long cnel = (long long)(nelx / kx) * (nely / ky);
   const int K = 4 * d;
-  dim3 grid(ceil_div(cnel, 64), K * K, rows), block(64);
   hipStream_t st = (hipStream_t)stream;
-#define STT_GAL(D, FD)                                                                   \
-  hipLaunchKernelGGL((galerkin_kernel<D, FD>), grid, block, 0, st, nelx, nely, kx, ky,    \
-                     density, density_stride, reaction, reaction_stride, g, gm, fine_keff, \
-                     fine_keff_stride, s, coarse_keff, coarse_keff_stride)
-  if (d == 2) {
-    if (from_density) STT_GAL(2, true); else STT_GAL(2, false);
-  } else {
-    if (from_density) STT_GAL(1, true); else STT_GAL(1, false);
+  if (from_density) {
+    // few coarse elements: narrower entry groups for more parallelism
+    const bool wide = cnel * rows >= 65536;
+    const int eg = d == 1 ? 16 : (wide ? 32 : 8);
+    const dim3 pgrid(ceil_div(cnel, 256), (K * K) / eg, rows), pblock(256);
+#define STT_GALP(D, EG)                                                                 \
+  hipLaunchKernelGGL((galerkin_param_kernel<D, EG>), pgrid, pblock, 0, st, nelx, kx, ky, \
+                     nelx / kx, cnel, density, density_stride, reaction, reaction_stride, g, gm, \
+                     coarse_keff, coarse_keff_stride)
+    if (d == 1) STT_GALP(1, 16);
+    else if (wide) STT_GALP(2, 32);
+    else STT_GALP(2, 8);
+#undef STT_GALP
+    STT_CHECK_LAUNCH("galerkin_param_kernel");
+    return STTOPT_OK;
   }
+  dim3 grid(ceil_div(cnel, 64), K * K, rows), block(64);
+#define STT_GAL(D)                                                                \
+  hipLaunchKernelGGL((galerkin_kernel<D>), grid, block, 0, st, nelx, nely, kx, ky,  \
+                     fine_keff, fine_keff_stride, s, coarse_keff, coarse_keff_stride)
+  if (d == 2) STT_GAL(2);
+  else STT_GAL(1);
 #undef STT_GAL
   STT_CHECK_LAUNCH("galerkin_kernel");
   return STTOPT_OK;

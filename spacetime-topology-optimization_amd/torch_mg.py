@@ -49,6 +49,10 @@ CHECK_EVERY = 1
 #: of reading an explicit (4d)^2 matrix per element: at 2x2 coarsening that is 4
 #: densities instead of 64 matrix entries per level-1 element.
 PARAM_MAX_CHILDREN = 4
+#: Coarse levels whose elements cover at most this many level-0 elements are built
+#: straight from the level-0 densities (density form of sttopt_galerkin); deeper ones
+#: from the level above.
+GALERKIN_MAX_BLOCK = 4096
 
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
@@ -399,22 +403,34 @@ def build_hierarchy(
         S = child_interpolation_matrices(kx, ky, dev, torch.float64, d)
         cnelx, cnely = top.nelx // kx, top.nely // ky
         csoa = torch.empty(rows, k * k, cnelx * cnely, dtype=torch.float64, device=dev)
+        # per-position matrices of the new level: level-0 element (gy, gx) of the block
+        # under one coarse element contributes density * ptab[gy, gx]
         if soa is None:
-            G = torch.einsum("mpi,pq,mqj->mij", S, KE64, S).contiguous()
+            ptab = torch.einsum("mpi,pq,mqj->mij", S, KE64, S).reshape(ky, kx, k, k)
+        elif ptab is not None:
+            Ky, Kx = ptab.shape[:2]
+            nxt = torch.empty(ky, Ky, kx, Kx, k, k, dtype=torch.float64, device=dev)
+            for my in range(ky):
+                for mx in range(kx):
+                    Sm = S[my * kx + mx]
+                    nxt[my, :, mx] = torch.einsum("pi,yxpq,qj->yxij", Sm, ptab, Sm)
+            ptab = nxt.reshape(ky * Ky, kx * Kx, k, k)
+        if soa is None or (react64 is None and ptab is not None
+                           and ptab.shape[0] * ptab.shape[1] <= GALERKIN_MAX_BLOCK):
+            # density form: straight from the level-0 densities (any depth)
             GM = None
-            if react64 is not None:
+            if soa is None and react64 is not None:
                 ME = torch.eye(k, dtype=torch.float64, device=dev) / 4
                 GM = torch.einsum("mpi,pq,mqj->mij", S, ME, S).contiguous()
+            table = ptab.reshape(-1, k * k).contiguous()
             _lib.check(
                 lib.sttopt_galerkin(
-                    d, top.nelx, top.nely, kx, ky, rows, den64.data_ptr(), nel,
-                    _lib.ptr(react64), nel, G.data_ptr(), _lib.ptr(GM), None, 0, None,
+                    d, nelx, nely, ptab.shape[1], ptab.shape[0], rows, den64.data_ptr(), nel,
+                    _lib.ptr(react64), nel, table.data_ptr(), _lib.ptr(GM), None, 0, None,
                     csoa.data_ptr(), csoa.stride(0), stream,
                 ),
                 "sttopt_galerkin",
             )
-            if react64 is None:
-                ptab = G.reshape(ky, kx, k, k)
         else:
             _lib.check(
                 lib.sttopt_galerkin(
@@ -424,17 +440,8 @@ def build_hierarchy(
                 ),
                 "sttopt_galerkin",
             )
-        else_ptab = None
-        if soa is not None and ptab is not None:
-            Ky, Kx = ptab.shape[:2]
-            nxt = torch.empty(ky, Ky, kx, Kx, k, k, dtype=torch.float64, device=dev)
-            for my in range(ky):
-                for mx in range(kx):
-                    Sm = S[my * kx + mx]
-                    nxt[my, :, mx] = torch.einsum("pi,yxpq,qj->yxij", Sm, ptab, Sm)
-            else_ptab = nxt.reshape(ky * Ky, kx * Kx, k, k)
-        if soa is not None:
-            ptab = else_ptab
+        if react64 is not None:
+            ptab = None  # the reaction term has no per-position form below level 1
         soa = csoa
         top.kx, top.ky = kx, ky
         cndof = d * (cnelx + 1) * (cnely + 1)
