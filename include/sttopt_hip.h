@@ -45,6 +45,12 @@ extern "C" {
 #define STTOPT_F64 0
 #define STTOPT_F32 1
 
+/* layouts of coarse element matrices (entry-major, [rows][entries][nel]):
+ * FULL: the (4d)^2 entries a * 4d + b; UPPER: the (4d)(4d+1)/2 entries a <= b of the
+ * symmetric matrix, row-major over a (entry index a * 4d - a (a - 1) / 2 + b - a) */
+#define STTOPT_KEFF_FULL 0
+#define STTOPT_KEFF_UPPER 1
+
 /* operator modes for sttopt_apply */
 #define STTOPT_MODE_MATVEC 0   /* y = K u (no projection)                 */
 #define STTOPT_MODE_OPERATOR 1 /* y = P (K P u + rdiag * P u)              */
@@ -115,6 +121,8 @@ typedef struct sttopt_level {
   int64_t rdiag_stride;
   const void* keff;
   int64_t keff_stride;
+  int32_t keff_layout;    /* STTOPT_KEFF_FULL or STTOPT_KEFF_UPPER                 */
+  int32_t reserved;       /* 0                                                    */
   /* coarser levels, alternative to keff: keff_E = sum over the pkx x pky level-0
    * elements g under E of density0_g * ptable[g], ptable: (pkx*pky) x (4d)^2 in the
    * V-cycle precision (device); the V-cycle then reads only level 0's density      */
@@ -151,6 +159,12 @@ typedef struct sttopt_mg {
   /* device scratch sized by sttopt_mg_scratch_bytes                              */
   void* scratch;
 } sttopt_mg;
+
+/* Upper-triangle copy of fp64 FULL element matrices, (A + A^T) / 2, in dtype (V-cycle
+ * storage of the coarse levels; 36 of 64 entries at d = 2).  No counterpart in the
+ * reference (whose coarse matrices stay full, torch_mg.py:355-370). */
+int sttopt_pack_keff(int32_t dtype, int32_t d, int64_t nel, int32_t rows, const double* keff,
+                     int64_t keff_stride, void* out, int64_t out_stride, void* stream);
 
 /* Bytes of device scratch the PCG / V-cycle need for this hierarchy (per-row
  * scalars, reduction partials, flags).  levels[*].nelx/nely and rows must be set. */

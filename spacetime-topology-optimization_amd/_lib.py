@@ -17,6 +17,7 @@ from . import _build
 LIB_PATH = _build.LIB_PATH
 
 OK, EINVAL, EHIP, ENOCONV, ENOTSPD = 0, -1, -2, -3, -4
+KEFF_FULL, KEFF_UPPER = 0, 1
 F64, F32 = 0, 1
 MODE_MATVEC, MODE_OPERATOR = 0, 1
 MAX_LEVELS = 16
@@ -53,6 +54,8 @@ class Level(ctypes.Structure):
         ("rdiag_stride", c_int64),
         ("keff", c_void_p),
         ("keff_stride", c_int64),
+        ("keff_layout", c_int32),
+        ("reserved", c_int32),
         ("ptable", c_void_p),
         ("pkx", c_int32),
         ("pky", c_int32),
@@ -131,6 +134,7 @@ _SIGNATURES = {
         c_int32, [c_int32, c_int64, c_int32, _P, c_int64, _P, _P, c_int64, _P]
     ),
     "sttopt_spd_inverse_work_count": (c_int64, [c_int32, c_int32]),
+    "sttopt_pack_keff": (c_int32, [c_int32, c_int32, c_int64, c_int32, _P, c_int64, _P, c_int64, _P]),
     "sttopt_spd_inverse": (c_int32, [c_int32, c_int32, _P, c_int64, _P, _P]),
     "sttopt_dense_assemble": (
         c_int32,

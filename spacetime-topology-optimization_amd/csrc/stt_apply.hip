@@ -57,7 +57,13 @@ struct StencilArgs {
 
 // SRC: 0 = density * KE (level 0), 1 = explicit entry-major keff, 2 = keff
 // parameterized by the fine densities under each coarse element.
-enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2 };
+// 3 = explicit keff, packed upper triangle.
+enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2, kSrcKeffUpper = 3 };
+
+// Entry (r, c) of a K x K symmetric matrix in the packed upper-triangle order.
+__host__ __device__ constexpr int upper_index(int r, int c, int K) {
+  return r <= c ? r * K - r * (r - 1) / 2 + (c - r) : c * K - c * (c - 1) / 2 + (r - c);
+}
 
 // Coarse-level stencil (SRC 1 or 2): one node per thread on 64 x 4 node tiles.
 // PKX, PKY > 0 fix the parameterized block at compile time (fully unrolled).
@@ -167,8 +173,11 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 #pragma unroll
           for (int i = 0; i < D; ++i) {
 #pragma unroll
-            for (int k = 0; k < D; ++k)
-              acc[i] += ke_e[(long long)((loc * D + i) * K + j * D + k) * nel] * U[ur][uc][k];
+            for (int k = 0; k < D; ++k) {
+              const int r = loc * D + i, cl = j * D + k;
+              const int idx = SRC == kSrcKeffUpper ? upper_index(r, cl, K) : r * K + cl;
+              acc[i] += ke_e[(long long)idx * nel] * U[ur][uc][k];
+            }
           }
         }
       }
@@ -500,7 +509,9 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   a.nely = op.nely;
   a.nnx = op.nnx();
   a.nny = op.nny();
-  const int src = op.ptab != nullptr ? kSrcParam : op.keff != nullptr ? kSrcKeff : kSrcDensity;
+  const int src = op.ptab != nullptr ? kSrcParam
+                  : op.keff != nullptr ? (op.keff_upper ? kSrcKeffUpper : kSrcKeff)
+                                       : kSrcDensity;
   a.ntx = op.ntx();
   a.tiles = src == kSrcDensity ? op.tiles() : op.tiles_small();
   if (src != kSrcDensity && dot != kNoDot) {
@@ -573,10 +584,12 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   if (op.d == 2) {
     if (src == kSrcParam) { if (p22) STT_STENCIL22(2); else STT_STENCIL(2, kSrcParam); }
     else if (src == kSrcKeff) STT_STENCIL(2, kSrcKeff);
+    else if (src == kSrcKeffUpper) STT_STENCIL(2, kSrcKeffUpper);
     else STT_TILE(2);
   } else if (op.d == 1) {
     if (src == kSrcParam) { if (p22) STT_STENCIL22(1); else STT_STENCIL(1, kSrcParam); }
     else if (src == kSrcKeff) STT_STENCIL(1, kSrcKeff);
+    else if (src == kSrcKeffUpper) STT_STENCIL(1, kSrcKeffUpper);
     else STT_TILE(1);
 #undef STT_STENCIL
 #undef STT_STENCIL22

@@ -81,25 +81,49 @@ struct StageParams {
   double beta, emin, emax, penal, fe;
 };
 
+// Per-stage constants of H_beta: a = tanh(beta t_k), den = a + tanh(beta (1 - t_k)).
+struct MaskRow {
+  double tk, a, den;
+};
+
+__device__ __forceinline__ MaskRow mask_row(double tk, double beta) {
+  MaskRow r;
+  r.tk = tk;
+  r.a = tanh(beta * tk);
+  r.den = r.a + tanh(beta * (1.0 - tk));
+  return r;
+}
+
 // H_beta(t_k - t) (compliance.time_mask) and its t-derivative
 // (tests/reference/compliance.time_mask_derivative), zero where the clamp acts.
-__device__ __forceinline__ void stage_mask(double t, double tk, double beta, double& m,
+__device__ __forceinline__ void stage_mask(double t, const MaskRow& k, double beta, double& m,
                                            double& dm) {
-  const double a = tanh(beta * tk);
-  const double th = tanh(beta * (t - tk));
-  const double den = a + tanh(beta * (1.0 - tk));
-  const double v = 1.0 - (a + th) / den;
+  const double th = tanh(beta * (t - k.tk));
+  const double v = 1.0 - (k.a + th) / k.den;
   if (v >= 0.0) {
     m = v;
-    dm = -beta * (1.0 - th * th) / den;
+    dm = -beta * (1.0 - th * th) / k.den;
   } else {
     m = 0.0;
     dm = 0.0;
   }
 }
 
+// x^p, by multiplication for the small integer exponents SIMP uses (p = 3 in practice).
+__device__ __forceinline__ double pow_p(double x, double p) {
+  if (p == 3.0) return x * x * x;
+  if (p == 2.0) return x * x;
+  if (p == 1.0) return x;
+  if (p == 4.0) {
+    const double y = x * x;
+    return y * y;
+  }
+  if (p == 0.0) return 1.0;
+  return pow(x, p);
+}
+
 __device__ __forceinline__ double simp(double x, const StageParams& p) {
-  return p.emin + pow(x, p.penal) * (p.emax - p.emin);
+  return p.emin + pow_p(x, p.penal) * (p.emax - p.emin);
 }
 
 __global__ __launch_bounds__(256) void stage_density_kernel(StageParams p, const double* x,
@@ -112,43 +136,55 @@ __global__ __launch_bounds__(256) void stage_density_kernel(StageParams p, const
   if (e >= nel) return;
   double xe = x[e];
   if (row > 0) {
+    const MaskRow k = mask_row(times[row - 1], p.beta);
     double m, dm;
-    stage_mask(t[e], times[row - 1], p.beta, m, dm);
+    stage_mask(t[e], k, p.beta, m, dm);
     xe *= m;
   }
   density[row * nel + e] = simp(xe, p);
 }
 
 // Self-weight load of stage rows (row 0 copies the external load):
-// F_y(node) = -fe/4 * sum of x*m over the node's elements, F_x = 0.
+// F_y(node) = -fe/4 * sum of x*m over the node's elements, F_x = 0.  A 64 x 4 node tile
+// per block: the 65 x 5 elements around it get x*m once, in LDS.
 __global__ __launch_bounds__(256) void stage_load_kernel(StageParams p, const double* x,
                                                          const double* t, const double* times,
                                                          const double* fext, double* load) {
+  __shared__ double sxm[5][65];
   const int nnx = p.nelx + 1, nny = p.nely + 1;
-  const long long nnodes = (long long)nnx * nny;
-  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long ndof = 2LL * nnx * nny;
+  const int tnx = (nnx + 63) / 64;
+  const int bx = blockIdx.x % tnx, by = blockIdx.x / tnx;
+  const int c0 = bx * 64, r0 = by * 4;
+  const int c = c0 + (threadIdx.x & 63), r = r0 + (threadIdx.x >> 6);
   const int row = blockIdx.y;
-  if (node >= nnodes) return;
-  const long long ndof = 2 * nnodes;
+  const bool own = r < nny && c < nnx;
+  const long long node = (long long)r * nnx + c;
   if (row == 0) {
-    load[2 * node] = fext[2 * node];
-    load[2 * node + 1] = fext[2 * node + 1];
-    return;
+    if (own) {
+      load[2 * node] = fext[2 * node];
+      load[2 * node + 1] = fext[2 * node + 1];
+    }
+    return;  // uniform: the whole block is row 0
   }
-  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
-  const double tk = times[row - 1];
-  double acc = 0.0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
-    if (er < 0 || er >= p.nely || ec < 0 || ec >= p.nelx) continue;
-    const long long e = (long long)er * p.nelx + ec;
-    double m, dm;
-    stage_mask(t[e], tk, p.beta, m, dm);
-    acc += (p.fe / 4.0) * (x[e] * m);
+  const MaskRow k = mask_row(times[row - 1], p.beta);
+  for (int i = threadIdx.x; i < 5 * 65; i += 256) {
+    const int er = r0 - 1 + i / 65, ec = c0 - 1 + i % 65;
+    double v = 0.0;
+    if (er >= 0 && er < p.nely && ec >= 0 && ec < p.nelx) {
+      const long long e = (long long)er * p.nelx + ec;
+      double m, dm;
+      stage_mask(t[e], k, p.beta, m, dm);
+      v = x[e] * m;
+    }
+    sxm[i / 65][i % 65] = v;
   }
+  __syncthreads();
+  if (!own) return;
+  const int lr = threadIdx.x >> 6, lc = threadIdx.x & 63;
+  const double acc = sxm[lr][lc] + sxm[lr][lc + 1] + sxm[lr + 1][lc] + sxm[lr + 1][lc + 1];
   load[row * ndof + 2 * node] = 0.0;
-  load[row * ndof + 2 * node + 1] = -acc;
+  load[row * ndof + 2 * node + 1] = -(p.fe / 4.0) * acc;
 }
 
 // Per-row compliance partials: sum_e SIMP(xt_row)_e * (U_e^T KE U_e).
@@ -168,8 +204,9 @@ __global__ __launch_bounds__(256) void stage_comp_kernel(StageParams p, const do
     gather_elem<2>(u + row * ndof, p.nelx + 1, er, ec, ue);
     double xe = x[e];
     if (row > 0) {
+      const MaskRow k = mask_row(times[row - 1], p.beta);
       double m, dm;
-      stage_mask(t[e], times[row - 1], p.beta, m, dm);
+      stage_mask(t[e], k, p.beta, m, dm);
       xe *= m;
     }
     v = simp(xe, p) * ke_form<2>(ke, ue, ue);
@@ -198,32 +235,44 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(StageParams p, const
                                                             KE<double> ke, const double* u,
                                                             const double* g, double* dx,
                                                             double* dt) {
+  constexpr int kChunk = 64;  // stage constants staged in LDS per chunk of stages
+  __shared__ MaskRow rows[kChunk];
   const long long nel = (long long)p.nelx * p.nely;
   const long long ndof = 2LL * (p.nelx + 1) * (p.nely + 1);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nel) return;
-  const int er = (int)(e / p.nelx), ec = (int)(e - (long long)er * p.nelx);
+  const bool own = e < nel;
+  const long long ee = own ? e : 0;
+  const int er = (int)(ee / p.nelx), ec = (int)(ee - (long long)er * p.nelx);
   const double de = p.emax - p.emin;
-  const double xe = x[e], te = p.nstage > 0 ? t[e] : 0.0;
+  const double xe = x[ee], te = p.nstage > 0 ? t[ee] : 0.0;
   double ue[8];
   gather_elem<2>(u, p.nelx + 1, er, ec, ue);
-  double gx = g[0] * (-p.penal * de * pow(xe, p.penal - 1.0) * ke_form<2>(ke, ue, ue));
+  double gx = g[0] * (-p.penal * de * pow_p(xe, p.penal - 1.0) * ke_form<2>(ke, ue, ue));
   double gt = 0.0;
-  for (int k = 1; k <= p.nstage; ++k) {
-    const double gk = g[k];
-    gather_elem<2>(u + k * ndof, p.nelx + 1, er, ec, ue);
-    double m, dm;
-    stage_mask(te, times[k - 1], p.beta, m, dm);
-    const double xt = xe * m;
-    const double A = -p.penal * de * pow(xt, p.penal - 1.0) * ke_form<2>(ke, ue, ue);
-    // (C^T U_y)_e = fe/4 * sum of the element's four nodal y-displacements
-    const double cu = (p.fe / 4.0) * (ue[1] + ue[3] + ue[5] + ue[7]);
-    const double s = A - 2.0 * cu;
-    gx += gk * s * m;
-    gt += gk * s * xe * dm;
+  for (int k0 = 1; k0 <= p.nstage; k0 += kChunk) {
+    const int nk = min(kChunk, p.nstage - k0 + 1);
+    __syncthreads();  // the previous chunk's constants are consumed
+    if (threadIdx.x < nk) rows[threadIdx.x] = mask_row(times[k0 - 1 + threadIdx.x], p.beta);
+    __syncthreads();
+    for (int j = 0; j < nk; ++j) {
+      const int k = k0 + j;
+      const double gk = g[k];
+      gather_elem<2>(u + k * ndof, p.nelx + 1, er, ec, ue);
+      double m, dm;
+      stage_mask(te, rows[j], p.beta, m, dm);
+      const double xt = xe * m;
+      const double A = -p.penal * de * pow_p(xt, p.penal - 1.0) * ke_form<2>(ke, ue, ue);
+      // (C^T U_y)_e = fe/4 * sum of the element's four nodal y-displacements
+      const double cu = (p.fe / 4.0) * (ue[1] + ue[3] + ue[5] + ue[7]);
+      const double s = A - 2.0 * cu;
+      gx += gk * s * m;
+      gt += gk * s * xe * dm;
+    }
   }
-  dx[e] = gx;
-  dt[e] = gt;
+  if (own) {
+    dx[e] = gx;
+    dt[e] = gt;
+  }
 }
 
 }  // namespace stt
@@ -290,12 +339,12 @@ int sttopt_stage_prologue(int32_t nelx, int32_t nely, int32_t nstage, const doub
   }
   StageParams p{nelx, nely, nstage, beta, emin, emax, penal, fe};
   const long long nel = (long long)nelx * nely;
-  const long long nnodes = (long long)(nelx + 1) * (nely + 1);
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(stage_density_kernel, dim3(ceil_div(nel, 256), 1 + nstage), dim3(256), 0,
                      s, p, x, t, stage_times, density);
-  hipLaunchKernelGGL(stage_load_kernel, dim3(ceil_div(nnodes, 256), 1 + nstage), dim3(256), 0,
-                     s, p, x, t, stage_times, f_ext, load);
+  const int load_tiles = ceil_div(nelx + 1, 64) * ceil_div(nely + 1, 4);
+  hipLaunchKernelGGL(stage_load_kernel, dim3(load_tiles, 1 + nstage), dim3(256), 0, s, p, x,
+                     t, stage_times, f_ext, load);
   STT_CHECK_LAUNCH("stage prologue");
   return STTOPT_OK;
 }

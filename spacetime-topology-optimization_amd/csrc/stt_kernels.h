@@ -16,6 +16,7 @@ struct OpDesc {
   long long sr = 0;
   const void* keff = nullptr;
   long long sk = 0;
+  int keff_upper = 0;  // keff holds the packed upper triangle (STTOPT_KEFF_UPPER)
   const void* rdiag = nullptr;
   long long srd = 0;
   const uint32_t* mask = nullptr;

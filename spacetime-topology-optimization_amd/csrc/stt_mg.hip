@@ -91,6 +91,29 @@ __global__ __launch_bounds__(256) void galerkin_param_kernel(
   for (int q = 0; q < EG; ++q) out[(long long)(e0 + q) * cnel] = acc[q];
 }
 
+// Packed upper triangle of symmetric entry-major element matrices, (A + A^T) / 2.
+template <typename TO, int K>
+__global__ __launch_bounds__(256) void pack_keff_kernel(long long nel,
+                                                        const double* __restrict__ keff,
+                                                        long long sk, TO* __restrict__ out,
+                                                        long long so) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (e >= nel) return;
+  const double* in = keff + row * sk + e;
+  TO* o = out + row * so + e;
+  int idx = 0;
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+#pragma unroll
+    for (int c = r; c < K; ++c) {
+      o[(long long)idx * nel] =
+          static_cast<TO>(0.5 * (in[(long long)(r * K + c) * nel] + in[(long long)(c * K + r) * nel]));
+      ++idx;
+    }
+  }
+}
+
 // ------------------------------------------------------------ transfers -----
 
 // Both transfers work on whole nodes (the D dofs as one vector).  KX = KY = 0 takes
@@ -418,6 +441,33 @@ int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t k
   else STT_GAL(1);
 #undef STT_GAL
   STT_CHECK_LAUNCH("galerkin_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_pack_keff(int32_t dtype, int32_t d, int64_t nel, int32_t rows, const double* keff,
+                     int64_t keff_stride, void* out, int64_t out_stride, void* stream) {
+  if ((d != 1 && d != 2) || nel <= 0 || rows < 0 || keff == nullptr || out == nullptr ||
+      (dtype != STTOPT_F64 && dtype != STTOPT_F32)) {
+    set_error("sttopt_pack_keff: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  const dim3 grid(ceil_div(nel, 256), rows), block(256);
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == STTOPT_F32) {
+    float* o = static_cast<float*>(out);
+    if (d == 2)
+      hipLaunchKernelGGL((pack_keff_kernel<float, 8>), grid, block, 0, st, nel, keff, keff_stride, o, out_stride);
+    else
+      hipLaunchKernelGGL((pack_keff_kernel<float, 4>), grid, block, 0, st, nel, keff, keff_stride, o, out_stride);
+  } else {
+    double* o = static_cast<double*>(out);
+    if (d == 2)
+      hipLaunchKernelGGL((pack_keff_kernel<double, 8>), grid, block, 0, st, nel, keff, keff_stride, o, out_stride);
+    else
+      hipLaunchKernelGGL((pack_keff_kernel<double, 4>), grid, block, 0, st, nel, keff, keff_stride, o, out_stride);
+  }
+  STT_CHECK_LAUNCH("pack_keff_kernel");
   return STTOPT_OK;
 }
 

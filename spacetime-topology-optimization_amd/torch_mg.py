@@ -50,9 +50,10 @@ CHECK_EVERY = 1
 #: densities instead of 64 matrix entries per level-1 element.
 PARAM_MAX_CHILDREN = 4
 #: Coarse levels whose elements cover at most this many level-0 elements are built
-#: straight from the level-0 densities (density form of sttopt_galerkin); deeper ones
-#: from the level above.
-GALERKIN_MAX_BLOCK = 4096
+#: straight from the level-0 densities (density form of sttopt_galerkin: one thread per
+#: coarse element runs over its block, so very deep levels lack parallelism); deeper
+#: ones from the level above.
+GALERKIN_MAX_BLOCK = 256
 
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
@@ -302,7 +303,24 @@ class _Level:
         elif self._ptab is not None and self._pblock[0] * self._pblock[1] <= PARAM_MAX_CHILDREN:
             out["ptable"] = self._ptab.to(dtype).contiguous()
         else:
-            out["keff"] = self._soa if dtype == torch.float64 else self._soa.to(dtype)
+            if dtype == torch.float64:
+                out["keff"] = self._soa
+            else:
+                # the fp32 V-cycle reads the symmetric matrices' upper triangle only
+                k = 4 * self.d
+                nel = self.nelx * self.nely
+                nrow = self._soa.shape[0]
+                packed = torch.empty(nrow, k * (k + 1) // 2, nel, dtype=dtype, device=dev)
+                _lib.check(
+                    _lib.load().sttopt_pack_keff(
+                        _lib.dtype_code(dtype), self.d, nel, nrow,
+                        self._soa.data_ptr(), self._soa.stride(0), packed.data_ptr(),
+                        packed.stride(0), _lib.stream_handle(dev),
+                    ),
+                    "sttopt_pack_keff",
+                )
+                out["keff"] = packed
+                out["keff_layout"] = _lib.KEFF_UPPER
         self._copies[dtype] = out
         return out
 
@@ -544,6 +562,7 @@ class VCycle:
             if "keff" in data:
                 L.keff = data["keff"].data_ptr()
                 L.keff_stride = 0 if shared else data["keff"].stride(0)
+                L.keff_layout = data.get("keff_layout", _lib.KEFF_FULL)
             if "ptable" in data:
                 L.ptable = data["ptable"].data_ptr()
                 L.pkx, L.pky = lv._pblock

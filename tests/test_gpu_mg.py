@@ -320,6 +320,54 @@ def test_density_parameterized_coarse_levels_equal_explicit_keff(dev, vdtype):
         torch_mg.PARAM_MAX_CHILDREN = old
 
 
+def test_packed_upper_triangle_keff(dev):
+    """sttopt_pack_keff writes (A + A^T) / 2 in the documented packed order, in fp64 and
+    fp32, for d = 1 and 2."""
+    from sttopt_amd import _lib
+
+    lib = _lib.load()
+    rng = np.random.default_rng(4)
+    for d in (1, 2):
+        k, nel, rows = 4 * d, 300, 2
+        full = torch.tensor(rng.standard_normal((rows, k * k, nel)), device=dev)
+        A = full.reshape(rows, k, k, nel)
+        sym = 0.5 * (A + A.transpose(1, 2))
+        order = [(r, c) for r in range(k) for c in range(r, k)]
+        want = torch.stack([sym[:, r, c] for r, c in order], dim=1)
+        for dtype in (torch.float64, torch.float32):
+            out = torch.empty(rows, len(order), nel, dtype=dtype, device=dev)
+            _lib.check(lib.sttopt_pack_keff(_lib.dtype_code(dtype), d, nel, rows, full.data_ptr(),
+                                            full.stride(0), out.data_ptr(), out.stride(0),
+                                            _lib.stream_handle(dev)), "pack")
+            tol = 0.0 if dtype == torch.float64 else 1e-7
+            assert float((out.double() - want).abs().max()) <= tol * float(want.abs().max())
+
+
+def test_fp32_vcycle_packed_coarse_matrices_equal_full(dev):
+    """The fp32 V-cycle reading the packed upper triangles of the explicit coarse levels
+    equals the one reading the full matrices (the Galerkin matrices are symmetric)."""
+    from sttopt_amd import _lib, torch_fem, torch_mg
+
+    nelx, nely = 64, 48
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(11)
+    r = torch_fem.project(s["t"](rng.standard_normal((3, s["ndof"]))), s["mask_t"])
+    x = rng.uniform(0.05, 1, (3, nelx * nely))
+    levels = _hip_hierarchy(s, nelx, nely, x, max_coarse_elements=12)
+    z_packed = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    explicit = [lv for lv in levels[1:] if lv.keff is not None and "keff" in lv.vdata(torch.float32)]
+    assert explicit and all(lv.vdata(torch.float32).get("keff_layout") == _lib.KEFF_UPPER
+                            for lv in explicit)
+    for lv in explicit:
+        data = lv.vdata(torch.float32)
+        data["keff"] = lv._soa.to(torch.float32)
+        data.pop("keff_layout")
+    z_full = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    err = float((z_packed - z_full).norm() / z_full.norm())
+    print(f"packed vs full fp32 V-cycle: {err:.2e}")
+    assert err < 1e-5
+
+
 # ------------------------------------------------------- dense SPD inverse ----
 
 
