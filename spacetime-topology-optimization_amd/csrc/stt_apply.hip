@@ -52,6 +52,8 @@ struct StencilArgs {
   int pkx, pky, fnelx;
   const T* prho;
   long long spr;
+  T* bcopy;
+  long long sbc;
   KE<T> ke;
 };
 
@@ -252,6 +254,20 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
   const T* __restrict__ u = a.u + row * a.su;
   const T* __restrict__ rho = a.rho + row * a.sr;
+  // node values of the tile: u, or for kSmoothFirst the first sweep x0 = w D^-1 b (0 at
+  // fixed dofs, where D^-1 is 0), in the same operation order as jacobi_first
+  auto stage_node = [&](long long node) -> Vec<T, D> {
+    if constexpr (MODE == kSmoothFirst) {
+      const Vec<TB, D> bv = load_node<TB, D>(a.b + row * a.sb, node);
+      const Vec<T, D> dv = load_node<T, D>(a.invd + row * a.sd, node);
+      Vec<T, D> o;
+#pragma unroll
+      for (int c = 0; c < D; ++c) o.v[c] = a.omega * dv.v[c] * static_cast<T>(bv.v[c]);
+      return o;
+    } else {
+      return load_node<T, D>(u, node);
+    }
+  };
   // Stage the tile.  Wavefront w loads halo rows w, w + 4, ...; lane l loads column
   // x0 - 1 + l of each, so column validity is per lane and row validity per wavefront.
   // The two trailing node columns and the trailing element column of the halo are
@@ -270,8 +286,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
     const int i = wave + n * kWaves, gy = y0 - 1 + i;
 #pragma unroll
     for (int c = 0; c < D; ++c) wu[n].v[c] = T(0);
-    if (i < kHaloY && gy >= 0 && gy < a.nny && cnode)
-      wu[n] = load_node<T, D>(u, (long long)gy * a.nnx + cx);
+    if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) wu[n] = stage_node((long long)gy * a.nnx + cx);
   }
 #pragma unroll
   for (int n = 0; n < kRR; ++n) {
@@ -290,7 +305,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
 #pragma unroll
   for (int c = 0; c < D; ++c) tu.v[c] = T(0);
   if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx)
-    tu = load_node<T, D>(u, (long long)tgy * a.nnx + tgx);
+    tu = stage_node((long long)tgy * a.nnx + tgx);
   if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
     tr = rho[(long long)tgy * a.nelx + tgx];
   if (a.proj_in) {
@@ -440,8 +455,17 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
         bv.v[ii] = TB(0);
         iv.v[ii] = T(0);
       }
-      if constexpr (MODE == kSmooth || MODE == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
-      if constexpr (MODE == kSmooth) iv = load_node<T, D>(a.invd + row * a.sd, node);
+      constexpr bool kSweep = MODE == kSmooth || MODE == kSmoothFirst;
+      if constexpr (kSweep || MODE == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
+      if constexpr (kSweep) iv = load_node<T, D>(a.invd + row * a.sd, node);
+      if constexpr (MODE == kSmoothFirst) {
+        if (a.bcopy != nullptr) {
+          Vec<T, D> bc;
+#pragma unroll
+          for (int ii = 0; ii < D; ++ii) bc.v[ii] = static_cast<T>(bv.v[ii]);
+          store_node<T, D>(a.bcopy + row * a.sbc, node, bc);
+        }
+      }
       Vec<TY, D> yo;
 #pragma unroll
       for (int ii = 0; ii < D; ++ii) {
@@ -491,6 +515,9 @@ static int launch_tile(const StencilArgs<T, TB, TY>& a, dim3 grid, hipStream_t s
     case kResid * 3 + kNoDot:
       hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kResid, kNoDot>), grid, block, 0, s, a);
       break;
+    case kSmoothFirst * 3 + kNoDot:
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kSmoothFirst, kNoDot>), grid, block, 0, s, a);
+      break;
     default:
       set_error("stencil: unsupported mode / reduction combination");
       return STTOPT_EINVAL;
@@ -502,7 +529,7 @@ template <typename T, typename TB, typename TY>
 int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
             long long su, const TB* b, long long sb, const T* invd, long long sd,
             TY* y, long long sy, double omega, const int* active, double* partial,
-            hipStream_t s) {
+            hipStream_t s, T* bcopy, long long sbc) {
   if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
   StencilArgs<T, TB, TY> a;
   a.nelx = op.nelx;
@@ -514,8 +541,12 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
                                        : kSrcDensity;
   a.ntx = op.ntx();
   a.tiles = src == kSrcDensity ? op.tiles() : op.tiles_small();
-  if (src != kSrcDensity && dot != kNoDot) {
-    set_error("stencil: reduction partials are only produced at level 0");
+  if (src != kSrcDensity && (dot != kNoDot || mode == kSmoothFirst)) {
+    set_error("stencil: reduction partials and the fused first sweeps are level-0 only");
+    return STTOPT_EINVAL;
+  }
+  if (mode == kSmoothFirst && (b == nullptr || invd == nullptr)) {
+    set_error("stencil: the fused first sweeps need b and the inverse diagonal");
     return STTOPT_EINVAL;
   }
   if (op.d == 2) {
@@ -523,7 +554,8 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     // every node's pair of dofs aligned to the pair
     auto ok = pair_aligned;
     if (!ok(u, su, sizeof(T)) || !ok(y, sy, sizeof(TY)) || !ok(b, sb, sizeof(TB)) ||
-        !ok(invd, sd, sizeof(T)) || !ok(op.rdiag, op.srd, sizeof(T))) {
+        !ok(invd, sd, sizeof(T)) || !ok(op.rdiag, op.srd, sizeof(T)) ||
+        !ok(bcopy, sbc, sizeof(T))) {
       set_error("stencil: d = 2 vectors need node-aligned pointers and even row strides");
       return STTOPT_EINVAL;
     }
@@ -561,6 +593,8 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   a.fnelx = op.fnelx;
   a.prho = static_cast<const T*>(op.prho);
   a.spr = op.spr;
+  a.bcopy = bcopy;
+  a.sbc = sbc;
   a.ke = make_ke<T>(op.ke, op.d);
   if (src == kSrcDensity && op.rho == nullptr) {
     set_error("stencil: level has neither density nor keff");
@@ -602,21 +636,15 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   return STTOPT_OK;
 }
 
-template int stencil<double, double, double>(const OpDesc&, int, int, bool, int,
-                                             const double*, long long, const double*,
-                                             long long, const double*, long long,
-                                             double*, long long, double, const int*,
-                                             double*, hipStream_t);
-template int stencil<float, float, float>(const OpDesc&, int, int, bool, int,
-                                          const float*, long long, const float*,
-                                          long long, const float*, long long, float*,
-                                          long long, double, const int*, double*,
-                                          hipStream_t);
-template int stencil<float, double, double>(const OpDesc&, int, int, bool, int,
-                                            const float*, long long, const double*,
-                                            long long, const float*, long long,
-                                            double*, long long, double, const int*,
-                                            double*, hipStream_t);
+#define STT_STENCIL_INST(T, TB, TY)                                                      \
+  template int stencil<T, TB, TY>(const OpDesc&, int, int, bool, int, const T*, long long,  \
+                                  const TB*, long long, const T*, long long, TY*, long long,  \
+                                  double, const int*, double*, hipStream_t, T*, long long)
+STT_STENCIL_INST(double, double, double);
+STT_STENCIL_INST(float, float, float);
+STT_STENCIL_INST(float, double, double);
+STT_STENCIL_INST(float, double, float);
+#undef STT_STENCIL_INST
 
 // ------------------------------------------------------------------ diagonal --
 

@@ -5,7 +5,10 @@
 
 namespace stt {
 
-enum StencilMode { kMatvec = 0, kOperator = 1, kSmooth = 2, kResid = 3 };
+// kSmoothFirst (level 0 only): two damped-Jacobi sweeps from x = 0 in one pass, the
+// first one (x0 = w D^-1 b) formed while staging the tile; also copies b into bcopy in
+// the stencil precision when given.
+enum StencilMode { kMatvec = 0, kOperator = 1, kSmooth = 2, kResid = 3, kSmoothFirst = 4 };
 enum DotKind { kNoDot = 0, kDotU = 1, kDotB = 2 };
 
 // One level's operator, type-erased.  Exactly one of rho (level 0: density * KE)
@@ -49,7 +52,7 @@ template <typename T, typename TB, typename TY>
 int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
             long long su, const TB* b, long long sb, const T* invd, long long sd,
             TY* y, long long sy, double omega, const int* active, double* partial,
-            hipStream_t s);
+            hipStream_t s, T* bcopy = nullptr, long long sbc = 0);
 
 // Node-centric diagonal of an operator (+ rdiag).  out_mode: 0 raw diagonal,
 // 1 diagonal with 1 at fixed dofs, 2 inverse at free dofs and 0 at fixed.

@@ -433,26 +433,33 @@ struct VCycle {
     if constexpr (sizeof(T) == sizeof(double)) {
       b = reinterpret_cast<const T*>(r);
       sb = rs;
-      if (ns >= 1) {
-        if (int e = jacobi_first<T, T>(nl, rows, b, sb, invd(0), sd(0), mg->omega, L.mask, x,
-                                       nl, nullptr, 0, active, s))
-          return e;
-      }
     } else {
       b = buf(L.vb);
       sb = nl;
-      if (ns >= 1) {
-        if (int e = jacobi_first<T, double>(nl, rows, r, rs, invd(0), sd(0), mg->omega,
-                                            L.mask, x, nl, buf(L.vb), nl, active, s))
+    }
+    if (ns >= 2) {
+      // the first two sweeps from x = 0 in one pass (x0 = w D^-1 r staged in the tile);
+      // the fp32 cycle gets its copy of r from the same pass
+      T* bcopy = sizeof(T) == sizeof(double) ? nullptr : buf(L.vb);
+      if (int e = stencil<T, double, T>(op[0], rows, kSmoothFirst, false, kNoDot,
+                                        static_cast<const T*>(nullptr), 0, r, rs, invd(0),
+                                        sd(0), x, nl, mg->omega, active, nullptr, s, bcopy, nl))
+        return e;
+      if (int e = sweeps(0, ns - 2, x, alt, b, sb)) return e;
+    } else if (ns == 1) {
+      if constexpr (sizeof(T) == sizeof(double)) {
+        if (int e = jacobi_first<T, T>(nl, rows, b, sb, invd(0), sd(0), mg->omega, L.mask, x,
+                                       nl, nullptr, 0, active, s))
           return e;
-      } else if (int e = convert<double, T>(nl, rows, r, rs, buf(L.vb), nl, active, s)) {
+      } else if (int e = jacobi_first<T, double>(nl, rows, r, rs, invd(0), sd(0), mg->omega,
+                                                 L.mask, x, nl, buf(L.vb), nl, active, s)) {
         return e;
       }
-    }
-    if (ns >= 1) {
-      if (int e = sweeps(0, ns - 1, x, alt, b, sb)) return e;
-    } else if (int e = fill<T>(nl, rows, T(0), x, nl, active, s)) {
-      return e;
+    } else {
+      if constexpr (sizeof(T) != sizeof(double)) {
+        if (int e = convert<double, T>(nl, rows, r, rs, buf(L.vb), nl, active, s)) return e;
+      }
+      if (int e = fill<T>(nl, rows, T(0), x, nl, active, s)) return e;
     }
     if (int e = coarse_correct(0, x, b, sb)) return e;
     if (ns >= 1) {
