@@ -13,10 +13,12 @@
 // the sign flipped.  The matrix stays symmetric throughout, so the update is computed
 // on the lower block triangle only and each tile is written to both halves.
 //
-// Per pivot block (64 wide): sweep_pivot_kernel inverts A_KK in LDS (one workgroup per
-// matrix), sweep_panel_kernel copies the row panel R = A_K: and forms P = D R, and
-// sweep_update_kernel writes every tile from A, R, P and D -- it reads no tile that
-// another workgroup of the same launch writes.
+// Per pivot block (64 wide): sweep_panel_kernel forms P = D R from the row panel
+// R = A_K:, and sweep_update_kernel writes every tile from A, R, P and D -- it reads no
+// tile that another workgroup of the same launch writes.  D of the first block comes from
+// sweep_pivot_kernel; the update launch of step K inverts the updated diagonal block
+// K + 1 in the workgroup that writes it (dispatched first), so the next pivot block is
+// ready when step K ends.
 #include "stt_kernels.h"
 
 namespace stt {
@@ -24,40 +26,29 @@ namespace stt {
 constexpr int kSB = 64;  // sweep block
 
 struct SweepWork {
-  double* D;  // rows x kSB x kSB
-  double* R;  // rows x kSB x n
-  double* P;  // rows x kSB x n
-  int* bad;   // one flag
+  double* D[2];  // rows x kSB x kSB each: step K uses D[K & 1]
+  double* R;     // rows x kSB x n
+  double* P;     // rows x kSB x n
+  int* bad;      // one flag
 };
 
 static SweepWork carve_sweep(double* work, int n, int rows) {
   SweepWork w;
-  w.D = work;
-  w.R = w.D + (long long)rows * kSB * kSB;
+  w.D[0] = work;
+  w.D[1] = w.D[0] + (long long)rows * kSB * kSB;
+  w.R = w.D[1] + (long long)rows * kSB * kSB;
   w.P = w.R + (long long)rows * kSB * n;
   w.bad = reinterpret_cast<int*>(w.P + (long long)rows * kSB * n);
   return w;
 }
 
-// D = A_KK^-1 by the scalar sweep of the bk x bk pivot block.  The block lives in
-// registers: thread t holds column j = t % 64 of rows t / 64 + 4 m (m < 16), padded with
-// the identity past bk; each pivot step publishes the pivot column through LDS (double
-// buffered, one barrier per step).  A pivot that is not positive (or not finite) marks
-// the matrix as not positive definite.
-__global__ __launch_bounds__(256) void sweep_pivot_kernel(int n, int k0, int bk,
-                                                          const double* __restrict__ a,
-                                                          long long sa, double* __restrict__ D,
-                                                          int* __restrict__ bad) {
-  __shared__ double col[2][kSB];
-  const int row = blockIdx.x;
-  const double* A = a + row * sa;
+// Scalar sweep of a bk x bk block held in registers by a 256-thread block: thread t
+// holds column j = t % 64 of rows t / 64 + 4 q (q < 16), padded with the identity past
+// bk.  Each pivot step publishes the pivot column through LDS (col: two buffers, one
+// barrier per step).  On return m holds -(block)^-1; the result is false if a pivot was
+// not positive (or not finite).  Every thread sees the same pivots.
+__device__ bool sweep_block_registers(double (&m)[16], int bk, double (*col)[kSB]) {
   const int j = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
-  double m[16];
-#pragma unroll
-  for (int q = 0; q < 16; ++q) {
-    const int i = g + 4 * q;
-    m[q] = (i < bk && j < bk) ? A[(long long)(k0 + i) * n + (k0 + j)] : (i == j ? 1.0 : 0.0);
-  }
   bool bad_pivot = false;
   for (int p = 0; p < bk; ++p) {
     double* c = col[p & 1];
@@ -84,7 +75,25 @@ __global__ __launch_bounds__(256) void sweep_pivot_kernel(int n, int k0, int bk,
       m[q] = v;
     }
   }
-  if (bad_pivot && threadIdx.x == 0) *bad = 1;  // every thread saw the same pivots
+  return !bad_pivot;
+}
+
+// D = A_KK^-1 of the first pivot block (later ones come from the update launch).
+__global__ __launch_bounds__(256) void sweep_pivot_kernel(int n, int k0, int bk,
+                                                          const double* __restrict__ a,
+                                                          long long sa, double* __restrict__ D,
+                                                          int* __restrict__ bad) {
+  __shared__ double col[2][kSB];
+  const int row = blockIdx.x;
+  const double* A = a + row * sa;
+  const int j = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
+  double m[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    const int i = g + 4 * q;
+    m[q] = (i < bk && j < bk) ? A[(long long)(k0 + i) * n + (k0 + j)] : (i == j ? 1.0 : 0.0);
+  }
+  if (!sweep_block_registers(m, bk, col) && threadIdx.x == 0) *bad = 1;
   double* Dr = D + (long long)row * kSB * kSB;
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
@@ -179,7 +188,7 @@ __global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
   }
 }
 
-// Lower-triangle tile t = (I, J), I >= J, of matrix blockIdx.y: every tile of the
+// Lower-triangle tile t = (I, J), I >= J, of matrix blockIdx.x: every tile of the
 // matrix is written once per sweep step (the tile and, off the diagonal, its mirror
 // through an LDS transpose).  A workgroup reads only its own tile of A, and no other
 // workgroup writes that tile or reads the mirror.
@@ -188,14 +197,23 @@ __global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0
                                                            const double* __restrict__ D,
                                                            const double* __restrict__ R,
                                                            const double* __restrict__ P,
-                                                           double sign) {
+                                                           double sign, int ntile,
+                                                           double* __restrict__ Dnext,
+                                                           int* __restrict__ bad) {
   // the k-chunks of the product and, afterwards, the transpose of the mirror tile
   __shared__ __align__(16) double lds[kSB * (kSB + 1)];
   double* sX = lds;
   double* sY = lds + kKC * kSB;
   double (*sT)[kSB + 1] = reinterpret_cast<double (*)[kSB + 1]>(lds);
-  const int row = blockIdx.y;
-  const int t = blockIdx.x;
+  __shared__ double col[2][kSB];
+  // grid (rows, tiles): the matrices are the fast dimension, so tile slot 0 of every
+  // matrix is dispatched first; that slot is the next pivot block's diagonal tile,
+  // whose workgroup also inverts the block
+  const int row = blockIdx.x;
+  const int kn = kb + 1;
+  const int tnext = kn * (kn + 1) / 2 + kn;
+  const bool ahead = Dnext != nullptr;
+  const int t = ahead ? (int)((blockIdx.y + tnext) % ntile) : (int)blockIdx.y;
   int I = (int)((sqrt(8.0 * t + 1.0) - 1.0) * 0.5);
   while ((I + 1) * (I + 2) / 2 <= t) ++I;
   while (I * (I + 1) / 2 > t) --I;
@@ -259,6 +277,31 @@ __global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0
       if (i < bi && j < bj) A[(long long)(i0 + i) * n + (j0 + j)] = out[x][y];
     }
   }
+  if (ahead && I == kn && J == kn) {
+    // look-ahead: D_{K+1} = (this updated tile)^-1, re-laid out through LDS into the
+    // pivot layout (column t % 64, rows t / 64 + 4 q)
+    __syncthreads();  // the product's last chunk is consumed before sT overwrites it
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {
+#pragma unroll
+      for (int y = 0; y < 4; ++y) sT[4 * ti + x][4 * tj + y] = out[x][y];
+    }
+    __syncthreads();
+    const int jj = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
+    double m[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = g + 4 * q;
+      m[q] = (i < bi && jj < bj) ? sT[i][jj] : (i == jj ? 1.0 : 0.0);
+    }
+    if (!sweep_block_registers(m, bi, col) && threadIdx.x == 0) *bad = 1;
+    double* Dr2 = Dnext + (long long)row * kSB * kSB;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int i = g + 4 * q;
+      Dr2[i * kSB + jj] = (i < bi && jj < bj) ? -m[q] : 0.0;
+    }
+  }
   if (I != J) {
     // mirror tile (J, I): element (r, c) = out(c, r), written row by row
     __syncthreads();  // the product's last chunk is consumed before sT overwrites it
@@ -288,8 +331,8 @@ extern "C" {
 
 int64_t sttopt_spd_inverse_work_count(int32_t n, int32_t rows) {
   if (n <= 0 || rows <= 0) return 0;
-  // D, R, P in doubles plus one double slot for the flag
-  return (int64_t)rows * kSB * kSB + 2LL * rows * kSB * n + 1;
+  // two D, R, P in doubles plus one double slot for the flag
+  return 2LL * rows * kSB * kSB + 2LL * rows * kSB * n + 1;
 }
 
 int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, double* work,
@@ -305,14 +348,17 @@ int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, dou
   if (int e = fill<int>(1, 1, 0, w.bad, 1, nullptr, s)) return e;
   const int nb = (n + kSB - 1) / kSB;
   const int ntile = nb * (nb + 1) / 2;
+  hipLaunchKernelGGL(sweep_pivot_kernel, dim3(rows), dim3(256), 0, s, n, 0,
+                     n < kSB ? n : kSB, a, a_stride, w.D[0], w.bad);
   for (int kb = 0; kb < nb; ++kb) {
     const int k0 = kb * kSB, bk = n - k0 < kSB ? n - k0 : kSB;
-    hipLaunchKernelGGL(sweep_pivot_kernel, dim3(rows), dim3(256), 0, s, n, k0, bk, a,
-                       a_stride, w.D, w.bad);
+    double* Dk = w.D[kb & 1];
+    double* Dn = kb + 1 < nb ? w.D[(kb + 1) & 1] : nullptr;
     hipLaunchKernelGGL(sweep_panel_kernel, dim3(nb, rows), dim3(256), 0, s, n, k0, bk, a,
-                       a_stride, w.D, w.R, w.P);
-    hipLaunchKernelGGL(sweep_update_kernel, dim3(ntile, rows), dim3(256), 0, s, n, kb, k0,
-                       bk, a, a_stride, w.D, w.R, w.P, kb == nb - 1 ? -1.0 : 1.0);
+                       a_stride, Dk, w.R, w.P);
+    hipLaunchKernelGGL(sweep_update_kernel, dim3(rows, ntile), dim3(256), 0, s, n, kb, k0,
+                       bk, a, a_stride, Dk, w.R, w.P, kb == nb - 1 ? -1.0 : 1.0, ntile, Dn,
+                       w.bad);
     STT_CHECK_LAUNCH("sweep");
   }
   int bad = 0;

@@ -72,6 +72,7 @@ __global__ __launch_bounds__(256) void galerkin_param_kernel(
   for (int q = 0; q < EG; ++q) acc[q] = 0.0;
   for (int gy = 0; gy < pky; ++gy) {
     const long long frow = (long long)(cr * pky + gy) * fnelx + (long long)cc * pkx;
+#pragma unroll 4
     for (int gx = 0; gx < pkx; ++gx) {
       const int pos = gy * pkx + gx;
       const double r = rr[frow + gx];

@@ -53,7 +53,7 @@ PARAM_MAX_CHILDREN = 4
 #: straight from the level-0 densities (density form of sttopt_galerkin: one thread per
 #: coarse element runs over its block, so very deep levels lack parallelism); deeper
 #: ones from the level above.
-GALERKIN_MAX_BLOCK = 256
+GALERKIN_MAX_BLOCK = 64
 
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
