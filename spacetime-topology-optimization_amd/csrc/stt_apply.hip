@@ -281,12 +281,18 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const bool celem = cx >= 0 && cx < a.nelx;
   Vec<T, D> wu[kUR];
   T wr[kRR];
+  uint32_t wm[kUR];  // free-dof words of the staged nodes (input projection)
 #pragma unroll
   for (int n = 0; n < kUR; ++n) {
     const int i = wave + n * kWaves, gy = y0 - 1 + i;
 #pragma unroll
     for (int c = 0; c < D; ++c) wu[n].v[c] = T(0);
-    if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) wu[n] = stage_node((long long)gy * a.nnx + cx);
+    wm[n] = ~0u;
+    if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) {
+      const long long node = (long long)gy * a.nnx + cx;
+      wu[n] = stage_node(node);
+      if (a.proj_in) wm[n] = a.mask[(node * D) >> 5];
+    }
   }
 #pragma unroll
   for (int n = 0; n < kRR; ++n) {
@@ -308,16 +314,42 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
     tu = stage_node((long long)tgy * a.nnx + tgx);
   if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
     tr = rho[(long long)tgy * a.nelx + tgx];
+  // Epilogue operands of this thread's output nodes (mask words, b, D^-1) are loaded
+  // now, before the barrier, so that the epilogue does not wait for them.
+  constexpr bool kSweep = MODE == kSmooth || MODE == kSmoothFirst;
+  constexpr bool kNeedB = kSweep || MODE == kResid;
+  uint32_t ow[NR];
+  Vec<TB, D> ob[kNeedB ? NR : 1];
+  Vec<T, D> oi[kSweep ? NR : 1];
+  {
+    const int gxo = x0 + lane, gyo = y0 + wave * NR;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const bool in = gyo + j < a.nny && gxo < a.nnx;
+      const long long node = (long long)(gyo + j) * a.nnx + gxo;
+      ow[j] = ~0u;
+      if (MODE != kMatvec && a.mask != nullptr && in) ow[j] = a.mask[(node * D) >> 5];
+      if constexpr (kNeedB) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) ob[j].v[c] = TB(0);
+        if (in) ob[j] = load_node<TB, D>(a.b + row * a.sb, node);
+      }
+      if constexpr (kSweep) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) oi[j].v[c] = T(0);
+        if (in) oi[j] = load_node<T, D>(a.invd + row * a.sd, node);
+      }
+    }
+  }
   if (a.proj_in) {
 #pragma unroll
     for (int n = 0; n < kUR; ++n) {
       const int i = wave + n * kWaves, gy = y0 - 1 + i;
       if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) {
         const long long dof0 = ((long long)gy * a.nnx + cx) * D;
-        const uint32_t word = a.mask[dof0 >> 5];
 #pragma unroll
         for (int c = 0; c < D; ++c)
-          if (!((word >> ((dof0 + c) & 31)) & 1u)) wu[n].v[c] = T(0);
+          if (!((wm[n] >> ((dof0 + c) & 31)) & 1u)) wu[n].v[c] = T(0);
       }
     }
     if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx) {
@@ -446,8 +478,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) acc[j][ii] += rd.v[ii] * uc.v[ii];
       }
-      uint32_t word = ~0u;
-      if (MODE != kMatvec && a.mask != nullptr) word = a.mask[(node * D) >> 5];
+      const uint32_t word = ow[j];
       Vec<TB, D> bv;
       Vec<T, D> iv;
 #pragma unroll
@@ -455,9 +486,8 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
         bv.v[ii] = TB(0);
         iv.v[ii] = T(0);
       }
-      constexpr bool kSweep = MODE == kSmooth || MODE == kSmoothFirst;
-      if constexpr (kSweep || MODE == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
-      if constexpr (kSweep) iv = load_node<T, D>(a.invd + row * a.sd, node);
+      if constexpr (kNeedB) bv = ob[j];
+      if constexpr (kSweep) iv = oi[j];
       if constexpr (MODE == kSmoothFirst) {
         if (a.bcopy != nullptr) {
           Vec<T, D> bc;
