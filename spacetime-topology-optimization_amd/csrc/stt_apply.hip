@@ -244,6 +244,11 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   __shared__ Vec<T, D> sU[kHaloY][kHaloX];
   __shared__ T sR[kElemY][kElemX];
   __shared__ double red[kWaves];
+  // kSmoothFirst: b (in the stencil precision) and D^-1 of the interior nodes, kept from
+  // the staging loads for the epilogue
+  constexpr bool kFirst = MODE == kSmoothFirst;
+  __shared__ Vec<T, D> sBc[kFirst ? kTileY : 1][kFirst ? kTileX : 1];
+  __shared__ Vec<T, D> sIc[kFirst ? kTileY : 1][kFirst ? kTileX : 1];
   const int t = xcd_remap(blockIdx.x, a.total);
   const int row = t / a.tiles;
   const int tt = t - row * a.tiles;
@@ -254,20 +259,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
   const T* __restrict__ u = a.u + row * a.su;
   const T* __restrict__ rho = a.rho + row * a.sr;
-  // node values of the tile: u, or for kSmoothFirst the first sweep x0 = w D^-1 b (0 at
-  // fixed dofs, where D^-1 is 0), in the same operation order as jacobi_first
-  auto stage_node = [&](long long node) -> Vec<T, D> {
-    if constexpr (MODE == kSmoothFirst) {
-      const Vec<TB, D> bv = load_node<TB, D>(a.b + row * a.sb, node);
-      const Vec<T, D> dv = load_node<T, D>(a.invd + row * a.sd, node);
-      Vec<T, D> o;
-#pragma unroll
-      for (int c = 0; c < D; ++c) o.v[c] = a.omega * dv.v[c] * static_cast<T>(bv.v[c]);
-      return o;
-    } else {
-      return load_node<T, D>(u, node);
-    }
-  };
+  auto stage_node = [&](long long node) -> Vec<T, D> { return load_node<T, D>(u, node); };
   // Stage the tile.  Wavefront w loads halo rows w, w + 4, ...; lane l loads column
   // x0 - 1 + l of each, so column validity is per lane and row validity per wavefront.
   // The two trailing node columns and the trailing element column of the halo are
@@ -282,15 +274,29 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   Vec<T, D> wu[kUR];
   T wr[kRR];
   uint32_t wm[kUR];  // free-dof words of the staged nodes (input projection)
+  Vec<TB, D> wb[kFirst ? kUR : 1];
+  Vec<T, D> wd[kFirst ? kUR : 1];
 #pragma unroll
   for (int n = 0; n < kUR; ++n) {
     const int i = wave + n * kWaves, gy = y0 - 1 + i;
 #pragma unroll
     for (int c = 0; c < D; ++c) wu[n].v[c] = T(0);
+    if constexpr (kFirst) {
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        wb[n].v[c] = TB(0);
+        wd[n].v[c] = T(0);
+      }
+    }
     wm[n] = ~0u;
     if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) {
       const long long node = (long long)gy * a.nnx + cx;
-      wu[n] = stage_node(node);
+      if constexpr (kFirst) {
+        wb[n] = load_node<TB, D>(a.b + row * a.sb, node);
+        wd[n] = load_node<T, D>(a.invd + row * a.sd, node);
+      } else {
+        wu[n] = stage_node(node);
+      }
       if (a.proj_in) wm[n] = a.mask[(node * D) >> 5];
     }
   }
@@ -308,15 +314,53 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const int tgy = y0 - 1 + ti, tgx = x0 - 1 + tk;
   Vec<T, D> tu;
   T tr = T(0);
+  Vec<TB, D> tb;
+  Vec<T, D> td;
 #pragma unroll
-  for (int c = 0; c < D; ++c) tu.v[c] = T(0);
-  if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx)
-    tu = stage_node((long long)tgy * a.nnx + tgx);
+  for (int c = 0; c < D; ++c) {
+    tu.v[c] = T(0);
+    tb.v[c] = TB(0);
+    td.v[c] = T(0);
+  }
+  if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx) {
+    const long long node = (long long)tgy * a.nnx + tgx;
+    if constexpr (kFirst) {
+      tb = load_node<TB, D>(a.b + row * a.sb, node);
+      td = load_node<T, D>(a.invd + row * a.sd, node);
+    } else {
+      tu = stage_node(node);
+    }
+  }
   if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
     tr = rho[(long long)tgy * a.nelx + tgx];
+  if constexpr (kFirst) {
+    // x0 = w D^-1 b (the first sweep); interior nodes also keep b and D^-1 in LDS and
+    // write the copy of b.  The last interior column (x0 + 63) is a trailing-column node.
+    auto finish = [&](const Vec<TB, D>& bv, const Vec<T, D>& dv, int li, int lk, int gyy,
+                      int gxx) -> Vec<T, D> {
+      Vec<T, D> bc, o;
+#pragma unroll
+      for (int c = 0; c < D; ++c) {
+        bc.v[c] = static_cast<T>(bv.v[c]);
+        o.v[c] = a.omega * dv.v[c] * bc.v[c];
+      }
+      if (li >= 1 && li <= kTileY && lk >= 1 && lk <= kTileX && gyy < a.nny && gxx < a.nnx) {
+        sBc[li - 1][lk - 1] = bc;
+        sIc[li - 1][lk - 1] = dv;
+        if (a.bcopy != nullptr) store_node<T, D>(a.bcopy + row * a.sbc, (long long)gyy * a.nnx + gxx, bc);
+      }
+      return o;
+    };
+#pragma unroll
+    for (int n = 0; n < kUR; ++n) {
+      const int i = wave + n * kWaves;
+      if (i < kHaloY) wu[n] = finish(wb[n], wd[n], i, lane, y0 - 1 + i, cx);
+    }
+    if (tix < kTailU) tu = finish(tb, td, ti, tk, tgy, tgx);
+  }
   // Epilogue operands of this thread's output nodes (mask words, b, D^-1) are loaded
   // now, before the barrier, so that the epilogue does not wait for them.
-  constexpr bool kSweep = MODE == kSmooth || MODE == kSmoothFirst;
+  constexpr bool kSweep = MODE == kSmooth;  // kSmoothFirst reads them from LDS
   constexpr bool kNeedB = kSweep || MODE == kResid;
   uint32_t ow[NR];
   Vec<TB, D> ob[kNeedB ? NR : 1];
@@ -488,13 +532,10 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
       }
       if constexpr (kNeedB) bv = ob[j];
       if constexpr (kSweep) iv = oi[j];
-      if constexpr (MODE == kSmoothFirst) {
-        if (a.bcopy != nullptr) {
-          Vec<T, D> bc;
+      if constexpr (kFirst) {
 #pragma unroll
-          for (int ii = 0; ii < D; ++ii) bc.v[ii] = static_cast<T>(bv.v[ii]);
-          store_node<T, D>(a.bcopy + row * a.sbc, node, bc);
-        }
+        for (int ii = 0; ii < D; ++ii) bv.v[ii] = static_cast<TB>(sBc[i0 + j][tx].v[ii]);
+        iv = sIc[i0 + j][tx];
       }
       Vec<TY, D> yo;
 #pragma unroll
