@@ -237,12 +237,18 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2;
 constexpr int kElemX = kTileX + 1, kElemY = kTileY + 1;
 
-template <typename T, typename TB, typename TY, int D, int MODE, int DOT>
+template <typename T, typename TB, typename TY, int D, int SRC, int MODE, int DOT>
 __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB, TY> a) {
   constexpr int K = 4 * D;
   constexpr int NR = kRowsPerThread;
+  // SRC kSrcDensity: level 0 (density * KE).  kSrcParam: level 1 at 2 x 2 coarsening,
+  // keff_E = sum over the 4 level-0 elements g under E of rho_g G_pos(g); the tile
+  // stages, per coarse element and level-0 row, the pair of level-0 densities under it.
+  constexpr bool kParam = SRC == kSrcParam;
+  constexpr int kRows = kParam ? 2 * kElemY : kElemY;  // density rows staged
   __shared__ Vec<T, D> sU[kHaloY][kHaloX];
-  __shared__ T sR[kElemY][kElemX];
+  __shared__ T sR[kParam ? 1 : kElemY][kParam ? 1 : kElemX];
+  __shared__ Vec<T, 2> sRf[kParam ? kRows : 1][kParam ? kElemX : 1];
   __shared__ double red[kWaves];
   // kSmoothFirst: b (in the stencil precision) and D^-1 of the interior nodes, kept from
   // the staging loads for the epilogue
@@ -258,7 +264,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
   const T* __restrict__ u = a.u + row * a.su;
-  const T* __restrict__ rho = a.rho + row * a.sr;
+  const T* __restrict__ rho = kParam ? a.prho + row * a.spr : a.rho + row * a.sr;
   auto stage_node = [&](long long node) -> Vec<T, D> { return load_node<T, D>(u, node); };
   // Stage the tile.  Wavefront w loads halo rows w, w + 4, ...; lane l loads column
   // x0 - 1 + l of each, so column validity is per lane and row validity per wavefront.
@@ -266,8 +272,8 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   // loaded by the block's first 2 * kHaloY + kElemY threads.  All loads of a thread are
   // issued before its first LDS write.
   constexpr int kUR = (kHaloY + kWaves - 1) / kWaves;
-  constexpr int kRR = (kElemY + kWaves - 1) / kWaves;
-  constexpr int kTailU = 2 * kHaloY, kTail = kTailU + kElemY;
+  constexpr int kRR = (kRows + kWaves - 1) / kWaves;
+  constexpr int kTailU = 2 * kHaloY, kTail = kTailU + kRows;
   const int cx = x0 - 1 + lane;
   const bool cnode = cx >= 0 && cx < a.nnx;
   const bool celem = cx >= 0 && cx < a.nelx;
@@ -300,11 +306,21 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
       if (a.proj_in) wm[n] = a.mask[(node * D) >> 5];
     }
   }
+  Vec<T, 2> wr2[kParam ? kRR : 1];
 #pragma unroll
   for (int n = 0; n < kRR; ++n) {
-    const int i = wave + n * kWaves, ey = y0 - 1 + i;
-    wr[n] = T(0);
-    if (i < kElemY && ey >= 0 && ey < a.nely && celem) wr[n] = rho[(long long)ey * a.nelx + cx];
+    const int i = wave + n * kWaves;
+    if constexpr (kParam) {
+      // level-0 row 2 ey + (i % 2) under coarse element row ey
+      const int ey = y0 - 1 + (i >> 1);
+      wr2[n].v[0] = wr2[n].v[1] = T(0);
+      if (i < kRows && ey >= 0 && ey < a.nely && celem)
+        wr2[n] = load_node<T, 2>(rho, (long long)(2 * ey + (i & 1)) * a.fnelx / 2 + cx);
+    } else {
+      const int ey = y0 - 1 + i;
+      wr[n] = T(0);
+      if (i < kElemY && ey >= 0 && ey < a.nely && celem) wr[n] = rho[(long long)ey * a.nelx + cx];
+    }
   }
   // trailing columns: thread t < kTailU takes node (t / 2, 64 + t % 2), the next kElemY
   // threads take element (t - kTailU, 64)
@@ -331,8 +347,15 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
       tu = stage_node(node);
     }
   }
-  if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
+  Vec<T, 2> tr2;
+  tr2.v[0] = tr2.v[1] = T(0);
+  if constexpr (kParam) {
+    const int fr = tix - kTailU, ey = y0 - 1 + (fr >> 1);
+    if (tix >= kTailU && tix < kTail && ey >= 0 && ey < a.nely && tgx < a.nelx)
+      tr2 = load_node<T, 2>(rho, (long long)(2 * ey + (fr & 1)) * a.fnelx / 2 + tgx);
+  } else if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx) {
     tr = rho[(long long)tgy * a.nelx + tgx];
+  }
   if constexpr (kFirst) {
     // x0 = w D^-1 b (the first sweep); interior nodes also keep b and D^-1 in LDS and
     // write the copy of b.  The last interior column (x0 + 63) is a trailing-column node.
@@ -412,10 +435,18 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
 #pragma unroll
   for (int n = 0; n < kRR; ++n) {
     const int i = wave + n * kWaves;
-    if (i < kElemY) sR[i][lane] = wr[n];
+    if constexpr (kParam) {
+      if (i < kRows) sRf[i][lane] = wr2[n];
+    } else if (i < kElemY) {
+      sR[i][lane] = wr[n];
+    }
   }
-  if (tix < kTailU) sU[ti][tk] = tu;
-  else if (tix < kTail) sR[ti][tk] = tr;
+  if (tix < kTailU) {
+    sU[ti][tk] = tu;
+  } else if (tix < kTail) {
+    if constexpr (kParam) sRf[ti][tk] = tr2;
+    else sR[ti][tk] = tr;
+  }
   __syncthreads();
 
   const int tx = lane, ty = wave;
@@ -433,10 +464,12 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
 #pragma unroll
       for (int c = 0; c < 3; ++c) W[r][c] = sU[i0 + r][tx + c];
     }
+    if constexpr (!kParam) {
 #pragma unroll
-    for (int r = 0; r < NR + 1; ++r) {
-      R[r][0] = sR[i0 + r][tx];
-      R[r][1] = sR[i0 + r][tx + 1];
+      for (int r = 0; r < NR + 1; ++r) {
+        R[r][0] = sR[i0 + r][tx];
+        R[r][1] = sR[i0 + r][tx + 1];
+      }
     }
   }
   T acc[NR][D];
@@ -453,10 +486,58 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   const ConstPtr kargs_ke = reinterpret_cast<ConstPtr>(
       (__attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
       offsetof(Args, ke));
+  // level 1: the per-position tables, read the same way (constant address space, one
+  // element position's 4 x (D x 4D) rows live at a time)
+  const ConstPtr ptab_c =
+      reinterpret_cast<ConstPtr>(reinterpret_cast<uintptr_t>(kParam ? a.ptab : nullptr));
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     constexpr int locs[4] = {1, 0, 2, 3};
     const int loc = locs[q];
+    if constexpr (kParam) {
+      ConstPtr tq = ptab_c + loc * D * K;
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) asm volatile("" : "+v"(acc[j][ii]));
+      }
+      asm volatile("" : "+s"(tq));
+      T tab[4][D][K];
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+          for (int c = 0; c < K; ++c) tab[g][ii][c] = tq[g * K * K + ii * K + c];
+        }
+      }
+      const int qr = q >> 1, qc = q & 1;
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        // the four level-0 densities under this element: rows 2 er, 2 er + 1 of the pairs
+        const Vec<T, 2> r0 = sRf[2 * (i0 + j + qr)][tx + qc];
+        const Vec<T, 2> r1 = sRf[2 * (i0 + j + qr) + 1][tx + qc];
+        const T rg[4] = {r0.v[0], r0.v[1], r1.v[0], r1.v[1]};
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          T part[D];
+#pragma unroll
+          for (int ii = 0; ii < D; ++ii) part[ii] = T(0);
+#pragma unroll
+          for (int jn = 0; jn < 4; ++jn) {
+            const Vec<T, D>& un = W[j + qr + loc_dr(jn)][qc + loc_dc(jn)];
+#pragma unroll
+            for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+              for (int kk = 0; kk < D; ++kk) part[ii] += tab[g][ii][jn * D + kk] * un.v[kk];
+            }
+          }
+#pragma unroll
+          for (int ii = 0; ii < D; ++ii) acc[j][ii] += rg[g] * part[ii];
+        }
+      }
+      continue;
+    }
     // fp64: the 64 KE entries do not fit in scalar registers.  The empty asm makes the
     // row pointer depend on the accumulators of the previous position, so the compiler
     // cannot hoist all four row-pair loads to the top and spill them.
@@ -564,30 +645,30 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   }
 }
 
-template <typename T, typename TB, typename TY, int D>
+template <typename T, typename TB, typename TY, int D, int SRC>
 static int launch_tile(const StencilArgs<T, TB, TY>& a, dim3 grid, hipStream_t s) {
   const dim3 block(kBlock);
   switch (a.mode * 3 + a.dot) {
     case kMatvec * 3 + kNoDot:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kMatvec, kNoDot>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kMatvec, kNoDot>), grid, block, 0, s, a);
       break;
     case kOperator * 3 + kNoDot:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kOperator, kNoDot>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kOperator, kNoDot>), grid, block, 0, s, a);
       break;
     case kOperator * 3 + kDotU:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kOperator, kDotU>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kOperator, kDotU>), grid, block, 0, s, a);
       break;
     case kSmooth * 3 + kNoDot:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kSmooth, kNoDot>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kSmooth, kNoDot>), grid, block, 0, s, a);
       break;
     case kSmooth * 3 + kDotB:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kSmooth, kDotB>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kSmooth, kDotB>), grid, block, 0, s, a);
       break;
     case kResid * 3 + kNoDot:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kResid, kNoDot>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kResid, kNoDot>), grid, block, 0, s, a);
       break;
     case kSmoothFirst * 3 + kNoDot:
-      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, kSmoothFirst, kNoDot>), grid, block, 0, s, a);
+      hipLaunchKernelGGL((stencil_tile_kernel<T, TB, TY, D, SRC, kSmoothFirst, kNoDot>), grid, block, 0, s, a);
       break;
     default:
       set_error("stencil: unsupported mode / reduction combination");
@@ -611,7 +692,10 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
                   : op.keff != nullptr ? (op.keff_upper ? kSrcKeffUpper : kSrcKeff)
                                        : kSrcDensity;
   a.ntx = op.ntx();
-  a.tiles = src == kSrcDensity ? op.tiles() : op.tiles_small();
+  // level 0, and in fp32 the parameterized level 1 at 2 x 2, run on 64 x 16 LDS tiles
+  const bool p22 = op.pkx == 2 && op.pky == 2;
+  const bool param_tile = src == kSrcParam && p22 && sizeof(T) == sizeof(float);
+  a.tiles = (src == kSrcDensity || param_tile) ? op.tiles() : op.tiles_small();
   if (src != kSrcDensity && (dot != kNoDot || mode == kSmoothFirst)) {
     set_error("stencil: reduction partials and the fused first sweeps are level-0 only");
     return STTOPT_EINVAL;
@@ -677,28 +761,39 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     return STTOPT_EINVAL;
   }
   dim3 grid((unsigned)a.total), block(kBlock);
-  const bool p22 = op.pkx == 2 && op.pky == 2;
 #define STT_STENCIL(D, SRC) \
   hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, SRC>), grid, block, 0, s, a)
 #define STT_STENCIL22(D) \
   hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, kSrcParam, 2, 2>), grid, block, 0, s, a)
-#define STT_TILE(D)                                                 \
-  do {                                                              \
-    if (int e_ = launch_tile<T, TB, TY, D>(a, grid, s)) return e_;  \
+#define STT_TILE(D, SRC)                                                 \
+  do {                                                                   \
+    if (int e_ = launch_tile<T, TB, TY, D, SRC>(a, grid, s)) return e_;  \
+  } while (0)
+#define STT_PARAM(D)                                 \
+  do {                                               \
+    if constexpr (sizeof(T) == sizeof(float)) {      \
+      if (param_tile) {                              \
+        STT_TILE(D, kSrcParam);                      \
+        break;                                       \
+      }                                              \
+    }                                                \
+    if (p22) STT_STENCIL22(D);                       \
+    else STT_STENCIL(D, kSrcParam);                  \
   } while (0)
   if (op.d == 2) {
-    if (src == kSrcParam) { if (p22) STT_STENCIL22(2); else STT_STENCIL(2, kSrcParam); }
+    if (src == kSrcParam) STT_PARAM(2);
     else if (src == kSrcKeff) STT_STENCIL(2, kSrcKeff);
     else if (src == kSrcKeffUpper) STT_STENCIL(2, kSrcKeffUpper);
-    else STT_TILE(2);
+    else STT_TILE(2, kSrcDensity);
   } else if (op.d == 1) {
-    if (src == kSrcParam) { if (p22) STT_STENCIL22(1); else STT_STENCIL(1, kSrcParam); }
+    if (src == kSrcParam) STT_PARAM(1);
     else if (src == kSrcKeff) STT_STENCIL(1, kSrcKeff);
     else if (src == kSrcKeffUpper) STT_STENCIL(1, kSrcKeffUpper);
-    else STT_TILE(1);
+    else STT_TILE(1, kSrcDensity);
 #undef STT_STENCIL
 #undef STT_STENCIL22
 #undef STT_TILE
+#undef STT_PARAM
   } else {
     set_error("stencil: d must be 1 or 2");
     return STTOPT_EINVAL;
