@@ -80,11 +80,11 @@ __device__ __forceinline__ void tile_origin(const TileGeom& g, int& row, int& tt
 
 // x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).  Each
 // thread moves its nodes' D dofs as one vector.
-template <int D>
+template <int D, typename ZT>
 __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
     TileGeom g, long long n, const double* __restrict__ alpha, double* __restrict__ x,
     double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ Ap,
-    const double* __restrict__ z, const int* __restrict__ active, double* __restrict__ prr,
+    const ZT* __restrict__ z, const int* __restrict__ active, double* __restrict__ prr,
     double* __restrict__ prz) {
   __shared__ double red[kBlock / kWave];
   int row, tt, c, r0;
@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
   double* rrow = r + row * n;
   const double* pr = p + row * n;
   const double* apr = Ap + row * n;
-  const double* zr = z != nullptr ? z + row * n : nullptr;
+  const ZT* zr = z != nullptr ? z + row * n : nullptr;
   if (c < g.nnx) {
 #pragma unroll
     for (int j = 0; j < kRowsPerThread; ++j) {
@@ -106,16 +106,16 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
         Vec<double, D> rv = load_node<double, D>(rrow, node);
         const Vec<double, D> pv = load_node<double, D>(pr, node);
         const Vec<double, D> av = load_node<double, D>(apr, node);
-        Vec<double, D> zv;
+        Vec<ZT, D> zv;
 #pragma unroll
-        for (int k = 0; k < D; ++k) zv.v[k] = 0.0;
-        if (zr != nullptr) zv = load_node<double, D>(zr, node);
+        for (int k = 0; k < D; ++k) zv.v[k] = ZT(0);
+        if (zr != nullptr) zv = load_node<ZT, D>(zr, node);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           xv.v[k] += a * pv.v[k];
           rv.v[k] -= a * av.v[k];
           rr += rv.v[k] * rv.v[k];
-          rz += rv.v[k] * zv.v[k];
+          rz += rv.v[k] * (double)zv.v[k];
         }
         store_node<double, D>(xr, node, xv);
         store_node<double, D>(rrow, node, rv);
@@ -132,26 +132,26 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
 }
 
 // p = z + beta p
-template <int D>
+template <int D, typename ZT>
 __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long long n,
                                                              const double* __restrict__ beta,
                                                              double* __restrict__ p,
-                                                             const double* __restrict__ z,
+                                                             const ZT* __restrict__ z,
                                                              const int* __restrict__ active) {
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
   if (active[row] == 0 || c >= g.nnx) return;
   const double b = beta[row];
   double* pr = p + row * n;
-  const double* zr = z + row * n;
+  const ZT* zr = z + row * n;
 #pragma unroll
   for (int j = 0; j < kRowsPerThread; ++j) {
     if (r0 + j < g.nny) {
       const long long node = (long long)(r0 + j) * g.nnx + c;
       Vec<double, D> pv = load_node<double, D>(pr, node);
-      const Vec<double, D> zv = load_node<double, D>(zr, node);
+      const Vec<ZT, D> zv = load_node<ZT, D>(zr, node);
 #pragma unroll
-      for (int k = 0; k < D; ++k) pv.v[k] = zv.v[k] + b * pv.v[k];
+      for (int k = 0; k < D; ++k) pv.v[k] = (double)zv.v[k] + b * pv.v[k];
       store_node<double, D>(pr, node, pv);
     }
   }
@@ -159,9 +159,9 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long lo
 
 // partial sums of a.b over free dofs (b == nullptr: a.a), optionally only of
 // active rows.
-template <int D>
+template <int D, typename BT>
 __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, const double* a,
-                                                     long long sa, const double* b,
+                                                     long long sa, const BT* b,
                                                      long long sb, const uint32_t* mask,
                                                      const int* active, double* part) {
   __shared__ double red[kBlock / kWave];
@@ -176,7 +176,12 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, co
         const long long node = (long long)(r0 + j) * g.nnx + c;
         const uint32_t word = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
         const Vec<double, D> av = load_node<double, D>(a + row * sa, node);
-        const Vec<double, D> bv = b != nullptr ? load_node<double, D>(b + row * sb, node) : av;
+        Vec<double, D> bv = av;
+        if (b != nullptr) {
+          const Vec<BT, D> bt = load_node<BT, D>(b + row * sb, node);
+#pragma unroll
+          for (int k = 0; k < D; ++k) bv.v[k] = (double)bt.v[k];
+        }
 #pragma unroll
         for (int k = 0; k < D; ++k)
           if ((word >> ((node * D + k) & 31)) & 1u) acc += av.v[k] * bv.v[k];
@@ -188,12 +193,12 @@ __global__ __launch_bounds__(kBlock) void dot_kernel(TileGeom g, long long n, co
 }
 
 // Launch a TileGeom kernel for the geometry's d (1 or 2; validated by the callers).
-#define STT_TILE_LAUNCH(kernel, g, s, ...)                                            \
-  do {                                                                                \
-    if ((g).d == 2)                                                                   \
-      hipLaunchKernelGGL(kernel<2>, dim3((g).total), dim3(kBlock), 0, s, g, __VA_ARGS__); \
-    else                                                                              \
-      hipLaunchKernelGGL(kernel<1>, dim3((g).total), dim3(kBlock), 0, s, g, __VA_ARGS__); \
+#define STT_TILE_LAUNCH(kernel, TA, g, s, ...)                                           \
+  do {                                                                                   \
+    if ((g).d == 2)                                                                      \
+      hipLaunchKernelGGL((kernel<2, TA>), dim3((g).total), dim3(kBlock), 0, s, g, __VA_ARGS__); \
+    else                                                                                 \
+      hipLaunchKernelGGL((kernel<1, TA>), dim3((g).total), dim3(kBlock), 0, s, g, __VA_ARGS__); \
   } while (0)
 
 // ---------------------------------------------------- per-row finalizers ----
@@ -413,16 +418,31 @@ struct VCycle {
                           active, s);
   }
 
-  // z = M(r) on level 0.  r, z fp64.  When part != nullptr, writes the tile
-  // partials of r.z (free dofs) for the PCG.
-  int run(const double* r, long long rs, double* z, long long zs, double* part) {
+  // z = M(r) on level 0.  r fp64; z fp64 or in the V-cycle precision (ZT = T: the
+  // cycle's last sweep computes in T, so storing z in T loses nothing).  When part !=
+  // nullptr, writes the tile partials of r.z (free dofs) for the PCG.  tmp64: rows x
+  // ndof doubles of scratch, needed when ZT is not double and there is one level.
+  template <typename ZT>
+  int run(const double* r, long long rs, ZT* z, long long zs, double* part,
+          double* tmp64 = nullptr) {
     const sttopt_level& L = mg->levels[0];
     const long long nl = n[0];
     const int ns = mg->n_smooth;
     if (mg->nlevels == 1) {
-      if (int e = coarse_solve<double>(nl, rows, L.cinv, L.cinv_stride, r, rs, L.mask, z, zs,
-                                       active, s))
-        return e;
+      if constexpr (sizeof(ZT) == sizeof(double)) {
+        if (int e = coarse_solve<double>(nl, rows, L.cinv, L.cinv_stride, r, rs, L.mask, z, zs,
+                                         active, s))
+          return e;
+      } else {
+        if (tmp64 == nullptr) {
+          set_error("vcycle: a one-level cycle with a reduced-precision z needs scratch");
+          return STTOPT_EINVAL;
+        }
+        if (int e = coarse_solve<double>(nl, rows, L.cinv, L.cinv_stride, r, rs, L.mask,
+                                         tmp64, nl, active, s))
+          return e;
+        if (int e = convert<double, ZT>(nl, rows, tmp64, nl, z, zs, active, s)) return e;
+      }
       return part ? dot_rz(r, rs, z, zs, part) : STTOPT_OK;
     }
     // fine-level right-hand side in the V-cycle precision
@@ -464,19 +484,20 @@ struct VCycle {
     if (int e = coarse_correct(0, x, b, sb)) return e;
     if (ns >= 1) {
       if (int e = sweeps(0, ns - 1, x, alt, b, sb)) return e;
-      // last sweep writes z in fp64 and the r.z partials in the same pass
-      return stencil<T, double, double>(op[0], rows, kSmooth, false,
-                                        part ? kDotB : kNoDot, x, nl, r, rs, invd(0), sd(0),
-                                        z, zs, mg->omega, active, part, s);
+      // last sweep writes z and the r.z partials in the same pass
+      return stencil<T, double, ZT>(op[0], rows, kSmooth, false, part ? kDotB : kNoDot, x,
+                                    nl, r, rs, invd(0), sd(0), z, zs, mg->omega, active, part,
+                                    s);
     }
-    if (int e = convert<T, double>(nl, rows, x, nl, z, zs, active, s)) return e;
+    if (int e = convert<T, ZT>(nl, rows, x, nl, z, zs, active, s)) return e;
     return part ? dot_rz(r, rs, z, zs, part) : STTOPT_OK;
   }
 
-  int dot_rz(const double* r, long long rs, const double* z, long long zs, double* part) {
+  template <typename ZT>
+  int dot_rz(const double* r, long long rs, const ZT* z, long long zs, double* part) {
     TileGeom g{op[0].nnx(), op[0].nny(), op[0].ntx(), op[0].tiles(),
                op[0].tiles() * rows, mg->d};
-    STT_TILE_LAUNCH(dot_kernel, g, s, n[0], r, rs, z, zs,
+    STT_TILE_LAUNCH(dot_kernel, ZT, g, s, n[0], r, rs, z, zs,
                        op[0].mask, active, part);
     STT_CHECK_LAUNCH("dot_kernel");
     return STTOPT_OK;
@@ -583,7 +604,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   const int tiles = op.tiles();
   Scratch sc = carve(mg->scratch, rows, tiles);
   double* r = work;
-  double* z = work + rows * n;
+  T* z = reinterpret_cast<T*>(work + rows * n);  // the preconditioned residual, in T
   double* p = work + 2 * rows * n;
   double* Ap = work + 3 * rows * n;
   TileGeom g{op.nnx(), op.nny(), op.ntx(), tiles, tiles * rows, mg->d};
@@ -595,9 +616,9 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (int e = stencil<double, double, double>(op, rows, kResid, false, kNoDot, x, xs, b, bs,
                                               nullptr, 0, r, n, 0.0, nullptr, nullptr, s))
     return e;
-  STT_TILE_LAUNCH(dot_kernel, g, s, n, b, bs, nullptr, 0, op.mask,
+  STT_TILE_LAUNCH(dot_kernel, double, g, s, n, b, bs, static_cast<const double*>(nullptr), 0, op.mask,
                      nullptr, sc.part[4]);
-  STT_TILE_LAUNCH(dot_kernel, g, s, n, r, n, nullptr, 0, op.mask,
+  STT_TILE_LAUNCH(dot_kernel, double, g, s, n, r, n, static_cast<const double*>(nullptr), 0, op.mask,
                      nullptr, sc.part[1]);
   if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
   hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4], sc.part[1],
@@ -608,9 +629,9 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (remaining == 0) return STTOPT_OK;
 
   VCycle<T> vc(mg, s, sc.active);
-  if (int e = vc.run(r, n, z, n, sc.part[2])) return e;
+  if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
   hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.active, sc.rz);
-  if (int e = convert<double, double>(n, rows, z, n, p, n, sc.active, s)) return e;
+  if (int e = convert<T, double>(n, rows, z, n, p, n, sc.active, s)) return e;
   STT_CHECK_LAUNCH("pcg first preconditioner");
 
   if (check_every < 1) check_every = 1;
@@ -622,8 +643,9 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       return e;
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
                        sc.rz, sc.alpha);
-    STT_TILE_LAUNCH(pcg_update_kernel, g, s, n, sc.alpha, x, r, p, Ap,
-                       flexible ? z : nullptr, sc.active, sc.part[1], sc.part[3]);
+    STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, x, r, p, Ap,
+                    flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
+                    sc.part[3]);
     if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
     hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, tiles, sc.part[1], rtol, k,
                        sc.bsafe, resid, sc.active, iters, sc.counter);
@@ -636,10 +658,10 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       }
     }
     if (k == max_iter) break;
-    if (int e = vc.run(r, n, z, n, sc.part[2])) return e;
+    if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
     hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.part[3],
                        flexible, sc.active, sc.rz, sc.beta);
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, g, s, n, sc.beta, p, z,
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.beta, p, z,
                        sc.active);
     STT_CHECK_LAUNCH("pcg direction");
   }
