@@ -78,23 +78,21 @@ __device__ __forceinline__ void tile_origin(const TileGeom& g, int& row, int& tt
   r0 = by * kTileY + (threadIdx.x / kTileX) * kRowsPerThread;
 }
 
-// x += alpha p; r -= alpha Ap; partial r.r (and r.z for the flexible beta).  Each
-// thread moves its nodes' D dofs as one vector.
+// r -= alpha Ap; partial r.r (and r.z for the flexible beta).  x += alpha p is left to
+// the direction update, which reads p anyway.  Each thread moves its nodes' D dofs as
+// one vector.
 template <int D, typename ZT>
 __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
-    TileGeom g, long long n, const double* __restrict__ alpha, double* __restrict__ x,
-    double* __restrict__ r, const double* __restrict__ p, const double* __restrict__ Ap,
-    const ZT* __restrict__ z, const int* __restrict__ active, double* __restrict__ prr,
-    double* __restrict__ prz) {
+    TileGeom g, long long n, const double* __restrict__ alpha, double* __restrict__ r,
+    const double* __restrict__ Ap, const ZT* __restrict__ z, const int* __restrict__ active,
+    double* __restrict__ prr, double* __restrict__ prz) {
   __shared__ double red[kBlock / kWave];
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
   if (active[row] == 0) return;
   double rr = 0.0, rz = 0.0;
   const double a = alpha[row];
-  double* xr = x + row * n;
   double* rrow = r + row * n;
-  const double* pr = p + row * n;
   const double* apr = Ap + row * n;
   const ZT* zr = z != nullptr ? z + row * n : nullptr;
   if (c < g.nnx) {
@@ -102,9 +100,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
     for (int j = 0; j < kRowsPerThread; ++j) {
       if (r0 + j < g.nny) {
         const long long node = (long long)(r0 + j) * g.nnx + c;
-        Vec<double, D> xv = load_node<double, D>(xr, node);
         Vec<double, D> rv = load_node<double, D>(rrow, node);
-        const Vec<double, D> pv = load_node<double, D>(pr, node);
         const Vec<double, D> av = load_node<double, D>(apr, node);
         Vec<ZT, D> zv;
 #pragma unroll
@@ -112,12 +108,10 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
         if (zr != nullptr) zv = load_node<ZT, D>(zr, node);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          xv.v[k] += a * pv.v[k];
           rv.v[k] -= a * av.v[k];
           rr += rv.v[k] * rv.v[k];
           rz += rv.v[k] * (double)zv.v[k];
         }
-        store_node<double, D>(xr, node, xv);
         store_node<double, D>(rrow, node, rv);
       }
     }
@@ -131,17 +125,20 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
   }
 }
 
-// p = z + beta p
+// For the rows that took iteration k (iters == k): x += alpha p, then, for those still
+// active and when update_p, p = z + beta p.
 template <int D, typename ZT>
-__global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long long n,
-                                                             const double* __restrict__ beta,
-                                                             double* __restrict__ p,
-                                                             const ZT* __restrict__ z,
-                                                             const int* __restrict__ active) {
+__global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
+    TileGeom g, long long n, const double* __restrict__ alpha, const double* __restrict__ beta,
+    double* __restrict__ x, double* __restrict__ p, const ZT* __restrict__ z,
+    const int* __restrict__ active, const int* __restrict__ iters, int k, int update_p) {
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
-  if (active[row] == 0 || c >= g.nnx) return;
-  const double b = beta[row];
+  if (iters[row] != k || c >= g.nnx) return;
+  const bool dir = update_p && active[row] != 0;
+  const double a = alpha[row];
+  const double b = dir ? beta[row] : 0.0;
+  double* xr = x + row * n;
   double* pr = p + row * n;
   const ZT* zr = z + row * n;
 #pragma unroll
@@ -149,10 +146,16 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(TileGeom g, long lo
     if (r0 + j < g.nny) {
       const long long node = (long long)(r0 + j) * g.nnx + c;
       Vec<double, D> pv = load_node<double, D>(pr, node);
-      const Vec<ZT, D> zv = load_node<ZT, D>(zr, node);
+      Vec<double, D> xv = load_node<double, D>(xr, node);
 #pragma unroll
-      for (int k = 0; k < D; ++k) pv.v[k] = (double)zv.v[k] + b * pv.v[k];
-      store_node<double, D>(pr, node, pv);
+      for (int q = 0; q < D; ++q) xv.v[q] += a * pv.v[q];
+      store_node<double, D>(xr, node, xv);
+      if (dir) {
+        const Vec<ZT, D> zv = load_node<ZT, D>(zr, node);
+#pragma unroll
+        for (int q = 0; q < D; ++q) pv.v[q] = (double)zv.v[q] + b * pv.v[q];
+        store_node<double, D>(pr, node, pv);
+      }
     }
   }
 }
@@ -636,6 +639,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
 
   if (check_every < 1) check_every = 1;
   bool done = false;
+  int k_last = 0;  // iteration whose x += alpha p is still pending
   for (int k = 1; k <= max_iter; ++k) {
     if (int e = stencil<double, double, double>(op, rows, kOperator, false, kDotU, p, n,
                                                 (const double*)nullptr, 0, nullptr, 0, Ap, n,
@@ -643,13 +647,14 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       return e;
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
                        sc.rz, sc.alpha);
-    STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, x, r, p, Ap,
+    STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, r, Ap,
                     flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
                     sc.part[3]);
     if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
     hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, tiles, sc.part[1], rtol, k,
                        sc.bsafe, resid, sc.active, iters, sc.counter);
     STT_CHECK_LAUNCH("pcg update");
+    k_last = k;
     if (k % check_every == 0 || k == max_iter) {
       if (int e = read_device_int(sc.counter, s, &remaining)) return e;
       if (remaining == 0) {
@@ -661,9 +666,16 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
     hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.part[3],
                        flexible, sc.active, sc.rz, sc.beta);
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.beta, p, z,
-                       sc.active);
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
+                    iters, k, 1);
     STT_CHECK_LAUNCH("pcg direction");
+    k_last = 0;
+  }
+  if (k_last > 0) {
+    // the last iteration's x += alpha p (no direction update follows it)
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
+                    iters, k_last, 0);
+    STT_CHECK_LAUNCH("pcg last x update");
   }
   // n_iter: the slowest row's count
   std::vector<int> h(rows > 0 ? rows : 1);
