@@ -51,6 +51,10 @@ extern "C" {
 #define STTOPT_KEFF_FULL 0
 #define STTOPT_KEFF_UPPER 1
 
+/* sttopt_level.flags (level 0 only) */
+#define STTOPT_SEPARATE_SWEEPS 1 /* fp32 V-cycle: one pass per smoothing sweep instead of
+                                    the two-sweep passes (same results; for testing) */
+
 /* operator modes for sttopt_apply */
 #define STTOPT_MODE_MATVEC 0   /* y = K u (no projection)                 */
 #define STTOPT_MODE_OPERATOR 1 /* y = P (K P u + rdiag * P u)              */
@@ -122,7 +126,7 @@ typedef struct sttopt_level {
   const void* keff;
   int64_t keff_stride;
   int32_t keff_layout;    /* STTOPT_KEFF_FULL or STTOPT_KEFF_UPPER                 */
-  int32_t reserved;       /* 0                                                    */
+  int32_t flags;          /* level 0: STTOPT_SEPARATE_SWEEPS or 0; others: 0      */
   /* coarser levels, alternative to keff: keff_E = sum over the pkx x pky level-0
    * elements g under E of density0_g * ptable[g], ptable: (pkx*pky) x (4d)^2 in the
    * V-cycle precision (device); the V-cycle then reads only level 0's density      */

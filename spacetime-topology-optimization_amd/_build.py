@@ -16,7 +16,8 @@ LIB_NAME = "libsttopt_hip.so"
 LIB_PATH = PKG_DIR / LIB_NAME
 ARCH = os.environ.get("STTOPT_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["stt_apply.hip", "stt_mg.hip", "stt_pcg.hip", "stt_grad.hip", "stt_dense.hip"]
+SOURCES = ["stt_apply.hip", "stt_mg.hip", "stt_pcg.hip", "stt_grad.hip", "stt_dense.hip",
+           "stt_smooth.hip"]
 
 HIPCC_FLAGS = [
     f"--offload-arch={ARCH}",

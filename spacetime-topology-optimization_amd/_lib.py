@@ -18,6 +18,7 @@ LIB_PATH = _build.LIB_PATH
 
 OK, EINVAL, EHIP, ENOCONV, ENOTSPD = 0, -1, -2, -3, -4
 KEFF_FULL, KEFF_UPPER = 0, 1
+SEPARATE_SWEEPS = 1  # sttopt_level.flags of level 0
 F64, F32 = 0, 1
 MODE_MATVEC, MODE_OPERATOR = 0, 1
 MAX_LEVELS = 16
@@ -55,7 +56,7 @@ class Level(ctypes.Structure):
         ("keff", c_void_p),
         ("keff_stride", c_int64),
         ("keff_layout", c_int32),
-        ("reserved", c_int32),
+        ("flags", c_int32),
         ("ptable", c_void_p),
         ("pkx", c_int32),
         ("pky", c_int32),

@@ -93,6 +93,21 @@ int coarse_solve(long long n, int rows, const double* cinv, long long scinv,
                  const T* b, long long sb, const uint32_t* mask, T* x, long long sx,
                  const int* active, hipStream_t s);
 
+// Level-0 smoothing of the fp32 V-cycle with two sweeps per pass (n_smooth = 2), on
+// 62 x 14 node output tiles (pair_tiles per row).  pre: x = two sweeps from zero on
+// b (fp64, rounded to fp32 as the cycle's copy of b would be) and r = b - A x.  post:
+// z = two sweeps from x + P ec (ec on the 2 x 2 coarser grid with cnnx nodes per row);
+// partial, when given, receives the per-tile r.z sums (free dofs).
+int pair_tiles(const OpDesc& op);
+int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
+                    const float* invd, long long sd, float* x, long long sx, float* r,
+                    long long sr, double omega, const int* active, hipStream_t s);
+template <typename ZT>
+int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
+                     const float* ec, long long sec, int cnnx, const double* b, long long sb,
+                     const float* invd, long long sd, ZT* z, long long sz, double omega,
+                     const int* active, double* partial, hipStream_t s);
+
 // Copy one int from device memory on stream s and wait for it (pinned readback word).
 int read_device_int(const int* dev, hipStream_t s, int* out);
 

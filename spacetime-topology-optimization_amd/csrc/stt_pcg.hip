@@ -37,6 +37,13 @@ static long long scratch_bytes(int rows, int tiles) {
   return b;
 }
 
+// Partial-sum slots per row: the level-0 tiles, or the two-sweep passes' output tiles
+// when there are more of them.
+static int partial_slots(const OpDesc& op) {
+  const int t = op.tiles(), p = pair_tiles(op);
+  return t > p ? t : p;
+}
+
 static Scratch carve(void* base, int rows, int tiles) {
   Scratch s;
   char* p = static_cast<char*>(base);
@@ -281,7 +288,8 @@ __global__ __launch_bounds__(kBlock) void fin_resid_kernel(int tiles, const doub
 }
 
 __global__ __launch_bounds__(kBlock) void fin_beta_kernel(int tiles, const double* prz,
-                                                          const double* przo, int flexible,
+                                                          int tiles_o, const double* przo,
+                                                          int flexible,
                                                           const int* active, double* rz,
                                                           double* beta) {
   __shared__ double red[kBlock / kWave];
@@ -291,7 +299,7 @@ __global__ __launch_bounds__(kBlock) void fin_beta_kernel(int tiles, const doubl
   double rzo = 0.0;
   if (flexible) {
     __syncthreads();
-    rzo = row_sum(przo, tiles, row, red);
+    rzo = row_sum(przo, tiles_o, row, red);
   }
   if (threadIdx.x == 0) {
     beta[row] = safe_div(rzn - rzo, rz[row]);
@@ -340,6 +348,17 @@ struct VCycle {
       n[l] = o.ndof();
     }
   }
+
+  // Level-0 smoothing in two-sweep passes: the fp32 cycle with two sweeps and one
+  // coarse visit over a 2 x 2 coarsening of the density operator.
+  bool pair() const {
+    const sttopt_level& L = mg->levels[0];
+    return sizeof(T) == sizeof(float) && mg->n_smooth == 2 && mg->gamma == 1 &&
+           mg->nlevels > 1 && L.kx == 2 && L.ky == 2 && op[0].rho != nullptr &&
+           (L.flags & STTOPT_SEPARATE_SWEEPS) == 0;
+  }
+  // reduction partials per row of the r.z sums written by run()
+  int rz_tiles() const { return pair() ? pair_tiles(op[0]) : op[0].tiles(); }
 
   const T* invd(int l) const { return static_cast<const T*>(mg->levels[l].invdiag); }
   long long sd(int l) const { return mg->levels[l].invdiag_stride; }
@@ -447,6 +466,23 @@ struct VCycle {
         if (int e = convert<double, ZT>(nl, rows, tmp64, nl, z, zs, active, s)) return e;
       }
       return part ? dot_rz(r, rs, z, zs, part) : STTOPT_OK;
+    }
+    if constexpr (sizeof(T) == sizeof(float)) {
+      if (pair()) {
+        const sttopt_level& C = mg->levels[1];
+        const long long nc = n[1];
+        T* x = buf(L.vx);
+        if (int e = smooth_pair_pre(op[0], rows, r, rs, invd(0), sd(0), x, nl, buf(L.vr), nl,
+                                    mg->omega, active, s))
+          return e;
+        if (int e = restrict_to<T>(mg->d, C.nelx, C.nely, 2, 2, rows, buf(L.vr), nl, C.mask,
+                                   buf(C.vb), nc, active, s))
+          return e;
+        T* ec = nullptr;
+        if (int e = cycle(1, buf(C.vb), &ec)) return e;
+        return smooth_pair_post<ZT>(op[0], rows, x, nl, ec, nc, C.nelx + 1, r, rs, invd(0),
+                                    sd(0), z, zs, mg->omega, active, part, s);
+      }
     }
     // fine-level right-hand side in the V-cycle precision
     const T* b;
@@ -605,7 +641,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   for (int i = 0; i < 64; ++i) op.ke[i] = mg->ke[i];
   const long long n = op.ndof();
   const int tiles = op.tiles();
-  Scratch sc = carve(mg->scratch, rows, tiles);
+  Scratch sc = carve(mg->scratch, rows, partial_slots(op));
   double* r = work;
   T* z = reinterpret_cast<T*>(work + rows * n);  // the preconditioned residual, in T
   double* p = work + 2 * rows * n;
@@ -632,8 +668,10 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (remaining == 0) return STTOPT_OK;
 
   VCycle<T> vc(mg, s, sc.active);
+  const int rz_tiles = vc.rz_tiles();
   if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
-  hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.active, sc.rz);
+  hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2], sc.active,
+                     sc.rz);
   if (int e = convert<T, double>(n, rows, z, n, p, n, sc.active, s)) return e;
   STT_CHECK_LAUNCH("pcg first preconditioner");
 
@@ -664,8 +702,8 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     }
     if (k == max_iter) break;
     if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
-    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, tiles, sc.part[2], sc.part[3],
-                       flexible, sc.active, sc.rz, sc.beta);
+    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2], tiles,
+                       sc.part[3], flexible, sc.active, sc.rz, sc.beta);
     STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
                     iters, k, 1);
     STT_CHECK_LAUNCH("pcg direction");
@@ -705,7 +743,7 @@ int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg) {
   op.d = mg->d;
   op.nelx = mg->levels[0].nelx;
   op.nely = mg->levels[0].nely;
-  return scratch_bytes(mg->rows, op.tiles());
+  return scratch_bytes(mg->rows, partial_slots(op));
 }
 
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {

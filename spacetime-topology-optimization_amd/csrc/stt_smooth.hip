@@ -1,0 +1,510 @@
+// Level-0 passes of the fp32 V-cycle that evaluate the stencil twice per tile
+// (temporal blocking), for two damped-Jacobi pre- and post-smoothing sweeps
+// (torch_mg.py:440-455 with n_smooth = 2):
+//
+//   pre:  x1 = w D^-1 b,  x2 = x1 + w D^-1 (b - A x1),  r = b - A x2
+//   post: xp = x2 + P e,  x3 = xp + w D^-1 (b - A xp),  z = x3 + w D^-1 (b - A x3)
+//
+// A workgroup owns a 62 x 14 node output tile.  The first stencil runs on the
+// surrounding 64 x 16 nodes (the output tile plus a one-node ring, one wavefront
+// column per lane, 4 node rows per thread as in the single-pass tile kernel), its
+// result goes to LDS, and the second stencil reads it from there.  So x1, xp and
+// x3 never reach HBM: the pre pass reads b, D^-1 and the densities and writes x2
+// and r; the post pass reads x2, the coarse correction e, b, D^-1 and the
+// densities and writes z.  The arithmetic of each sweep is the one of the
+// single-pass kernels (same operands, same order), so results are identical.
+#include <cstddef>
+
+#include "stt_kernels.h"
+
+namespace stt {
+
+namespace {
+
+constexpr int kOutX = kTileX - 2, kOutY = kTileY - 2;  // output tile
+constexpr int kHX = kTileX + 2, kHY = kTileY + 2;      // staged halo of the first stencil
+constexpr int kEX = kTileX + 1, kEY = kTileY + 1;      // element densities
+
+template <typename ZT>
+struct PairArgs {
+  int nelx, nely, nnx, nny, ntx, tiles, total;
+  int cnnx;               // post: nodes per row of the coarse grid (2 x 2 coarsening)
+  const float* x;         // post: x2, the result of the pre-smoothing
+  long long sx;
+  const float* ec;        // post: coarse-grid correction e
+  long long sec;
+  const double* b;        // right-hand side (the fp64 PCG residual)
+  long long sb;
+  const float* invd;
+  long long sd;
+  const float* rho;
+  long long sr;
+  const float* rdiag;
+  long long srd;
+  const uint32_t* mask;
+  ZT* out;                // pre: x2; post: z
+  long long so;
+  float* res;             // pre: r = b - A x2
+  long long sres;
+  float omega;
+  const int* active;
+  double* partial;        // post: r.z per tile (may be null)
+  KE<float> ke;
+};
+
+using KePtr = const __attribute__((address_space(4))) float*;
+
+// acc[j] = sum over the 4 elements around node row j of rho_e (KE u_e) for the NR node
+// rows of one thread: W is the thread's (NR + 2) x 3 node window, R the (NR + 1) x 2
+// element densities around it.  Element position outer, node row inner.
+template <int D, int NR>
+__device__ __forceinline__ void apply_window(const Vec<float, D> (&W)[NR + 2][3],
+                                             const float (&R)[NR + 1][2], KePtr ke_args,
+                                             float (&acc)[NR][D]) {
+  constexpr int K = 4 * D;
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) acc[j][ii] = 0.0f;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    constexpr int locs[4] = {1, 0, 2, 3};
+    const int loc = locs[q];
+    KePtr kq = ke_args + loc * D * K;
+    float ke[D][K];
+#pragma unroll
+    for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+      for (int c = 0; c < K; ++c) ke[ii][c] = kq[ii * K + c];
+    }
+    const int qr = q >> 1, qc = q & 1;
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      float part[D];
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) part[ii] = 0.0f;
+#pragma unroll
+      for (int jn = 0; jn < 4; ++jn) {
+        const Vec<float, D>& un = W[j + qr + loc_dr(jn)][qc + loc_dc(jn)];
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) {
+#pragma unroll
+          for (int kk = 0; kk < D; ++kk) part[ii] += ke[ii][jn * D + kk] * un.v[kk];
+        }
+      }
+      const float dens = R[j + qr][qc];
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) acc[j][ii] += dens * part[ii];
+    }
+  }
+}
+
+template <typename ZT, int D, bool POST>
+__global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
+  constexpr int NR = kRowsPerThread;
+  __shared__ Vec<float, D> sU[kHY][kHX];          // input of the first stencil
+  __shared__ float sR[kEY][kEX];
+  __shared__ Vec<float, D> sX[kTileY + 2][kTileX + 2];  // its output, one-node border
+  // pre: b and D^-1 of the first stencil's nodes, from the staging loads
+  __shared__ Vec<float, D> sB[POST ? 1 : kTileY][POST ? 1 : kTileX];
+  __shared__ Vec<float, D> sI[POST ? 1 : kTileY][POST ? 1 : kTileX];
+  __shared__ double red[kWaves];
+  const int t = xcd_remap(blockIdx.x, a.total);
+  const int row = t / a.tiles;
+  const int tt = t - row * a.tiles;
+  if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
+  const int by = tt / a.ntx, bx = tt - by * a.ntx;
+  // origin of the first stencil's 64 x 16 nodes; the staged halo starts one node
+  // further up and left
+  const int x0 = bx * kOutX - 1, y0 = by * kOutY - 1;
+  const int lane = threadIdx.x & (kWave - 1);
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const uint32_t* __restrict__ mask = a.mask;
+  const double* __restrict__ b = a.b + row * a.sb;
+  const float* __restrict__ invd = a.invd + row * a.sd;
+  const float* __restrict__ rho = a.rho + row * a.sr;
+
+  // ---- staging: wavefront w takes halo rows w, w + 4, ...; lane l column x0 - 1 + l;
+  // the two trailing node columns and the trailing element column are taken by the
+  // block's first 2 * kHY + kEY threads.
+  constexpr int kUR = (kHY + kWaves - 1) / kWaves;
+  constexpr int kRR = (kEY + kWaves - 1) / kWaves;
+  constexpr int kTailU = 2 * kHY, kTail = kTailU + kEY;
+  const int cx = x0 - 1 + lane;
+  const bool cnode = cx >= 0 && cx < a.nnx;
+  const bool celem = cx >= 0 && cx < a.nelx;
+  const int tix = threadIdx.x;
+  const int ti = tix < kTailU ? (tix >> 1) : tix - kTailU;
+  const int tk = tix < kTailU ? kTileX + (tix & 1) : kTileX;
+  const int tgy = y0 - 1 + ti, tgx = x0 - 1 + tk;
+  const bool tnode = tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx;
+
+  // staged value of node (gy, gx) of the grid: pre x1 = w D^-1 b (b and D^-1 also
+  // returned), post xp = x2 + P e at free dofs.
+  struct Staged {
+    Vec<float, D> u, bc, dv;
+  };
+  auto stage = [&](int gy, int gx) -> Staged {
+    Staged s;
+    const long long node = (long long)gy * a.nnx + gx;
+    if constexpr (POST) {
+      const float* ec = a.ec + row * a.sec;
+      s.u = load_node<float, D>(a.x + row * a.sx, node);
+      const uint32_t word = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
+      const uint32_t bits = (word >> ((node * D) & 31)) & ((1u << D) - 1u);
+      const int I0 = gy >> 1, fy = gy & 1, J0 = gx >> 1, fx = gx & 1;
+      const float wy1 = float(fy) / 2.0f, wy0 = 1.0f - wy1;
+      const float wx1 = float(fx) / 2.0f, wx0 = 1.0f - wx1;
+      const long long c00 = (long long)I0 * a.cnnx + J0;
+      const Vec<float, D> a00 = load_node<float, D>(ec, c00);
+      Vec<float, D> a01, a10, a11;
+#pragma unroll
+      for (int k = 0; k < D; ++k) a01.v[k] = a10.v[k] = a11.v[k] = 0.0f;
+      if (fx > 0) a01 = load_node<float, D>(ec, c00 + 1);
+      if (fy > 0) a10 = load_node<float, D>(ec, c00 + a.cnnx);
+      if (fx > 0 && fy > 0) a11 = load_node<float, D>(ec, c00 + a.cnnx + 1);
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const float top = wx0 * a00.v[k] + wx1 * a01.v[k];
+        const float bot = wx0 * a10.v[k] + wx1 * a11.v[k];
+        if ((bits >> k) & 1u) s.u.v[k] += wy0 * top + wy1 * bot;
+      }
+    } else {
+      const Vec<double, D> bv = load_node<double, D>(b, node);
+      s.dv = load_node<float, D>(invd, node);
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        s.bc.v[k] = static_cast<float>(bv.v[k]);
+        s.u.v[k] = a.omega * s.dv.v[k] * s.bc.v[k];
+      }
+    }
+    return s;
+  };
+  auto zero = [&]() -> Staged {
+    Staged s;
+#pragma unroll
+    for (int k = 0; k < D; ++k) s.u.v[k] = s.bc.v[k] = s.dv.v[k] = 0.0f;
+    return s;
+  };
+  Staged st[kUR];
+#pragma unroll
+  for (int n = 0; n < kUR; ++n) {
+    const int i = wave + n * kWaves, gy = y0 - 1 + i;
+    st[n] = (i < kHY && gy >= 0 && gy < a.nny && cnode) ? stage(gy, cx) : zero();
+  }
+  const Staged stt = tnode ? stage(tgy, tgx) : zero();
+  float wr[kRR];
+#pragma unroll
+  for (int n = 0; n < kRR; ++n) {
+    const int i = wave + n * kWaves, ey = y0 - 1 + i;
+    wr[n] = 0.0f;
+    if (i < kEY && ey >= 0 && ey < a.nely && celem) wr[n] = rho[(long long)ey * a.nelx + cx];
+  }
+  float tr = 0.0f;
+  if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
+    tr = rho[(long long)tgy * a.nelx + tgx];
+
+  // operands of this thread's first-stencil nodes (y0 + 4 wave + j, x0 + lane): mask
+  // words, reaction diagonal, and for post b and D^-1 (pre has them in LDS)
+  const int gxo = x0 + lane, gyo = y0 + wave * NR;
+  uint32_t ow[NR];
+  Vec<double, D> ob[POST ? NR : 1];
+  Vec<float, D> oi[POST ? NR : 1];
+  Vec<float, D> ord[NR];
+#pragma unroll
+  for (int j = 0; j < NR; ++j) {
+    const int gy = gyo + j;
+    const bool in = gy >= 0 && gy < a.nny && gxo >= 0 && gxo < a.nnx;
+    const long long node = (long long)gy * a.nnx + gxo;
+    ow[j] = 0u;  // nodes outside the grid stay zero
+    if (in) ow[j] = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
+#pragma unroll
+    for (int k = 0; k < D; ++k) ord[j].v[k] = 0.0f;
+    if (a.rdiag != nullptr && in) ord[j] = load_node<float, D>(a.rdiag + row * a.srd, node);
+    if constexpr (POST) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        ob[j].v[k] = 0.0;
+        oi[j].v[k] = 0.0f;
+      }
+      if (in) {
+        ob[j] = load_node<double, D>(b, node);
+        oi[j] = load_node<float, D>(invd, node);
+      }
+    }
+  }
+
+  // ---- LDS writes
+#pragma unroll
+  for (int n = 0; n < kUR; ++n) {
+    const int i = wave + n * kWaves;
+    if (i < kHY) {
+      sU[i][lane] = st[n].u;
+      if constexpr (!POST) {
+        // halo node (i, lane) is node (i - 1, lane - 1) of the first stencil
+        if (i >= 1 && i <= kTileY && lane >= 1) {
+          sB[i - 1][lane - 1] = st[n].bc;
+          sI[i - 1][lane - 1] = st[n].dv;
+        }
+      }
+    }
+  }
+  if (tix < kTailU) {
+    sU[ti][tk] = stt.u;
+    if constexpr (!POST) {
+      // halo column 64 is the first stencil's last node column (63); 65 is outside it
+      if (tk == kTileX && ti >= 1 && ti <= kTileY) {
+        sB[ti - 1][kTileX - 1] = stt.bc;
+        sI[ti - 1][kTileX - 1] = stt.dv;
+      }
+    }
+  } else if (tix < kTail) {
+    sR[ti][tk] = tr;
+  }
+#pragma unroll
+  for (int n = 0; n < kRR; ++n) {
+    const int i = wave + n * kWaves;
+    if (i < kEY) sR[i][lane] = wr[n];
+  }
+  // border of sX: read only by nodes without output; kept at zero
+  {
+    Vec<float, D> zv;
+#pragma unroll
+    for (int k = 0; k < D; ++k) zv.v[k] = 0.0f;
+    if (tix < kTileX + 2) {
+      sX[0][tix] = zv;
+      sX[kTileY + 1][tix] = zv;
+    } else if (tix < kTileX + 2 + kTileY) {
+      sX[tix - (kTileX + 2) + 1][0] = zv;
+      sX[tix - (kTileX + 2) + 1][kTileX + 1] = zv;
+    }
+  }
+  __syncthreads();
+
+  const int i0 = wave * NR;
+  Vec<float, D> bc[NR], iv[NR];  // b (in fp32) and D^-1 of the first stencil's nodes
+  if constexpr (!POST) {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      bc[j] = sB[i0 + j][lane];
+      iv[j] = sI[i0 + j][lane];
+    }
+  } else {
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        bc[j].v[k] = static_cast<float>(ob[j].v[k]);
+        iv[j].v[k] = oi[j].v[k];
+      }
+    }
+  }
+
+  using Args = PairArgs<ZT>;
+  const KePtr ke_args = reinterpret_cast<KePtr>(
+      (__attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
+      offsetof(Args, ke));
+  float R[NR + 1][2];
+#pragma unroll
+  for (int r = 0; r < NR + 1; ++r) {
+    R[r][0] = sR[i0 + r][lane];
+    R[r][1] = sR[i0 + r][lane + 1];
+  }
+
+  // ---- first stencil on the 64 x 16 nodes: one damped-Jacobi sweep
+  {
+    Vec<float, D> W[NR + 2][3];
+#pragma unroll
+    for (int r = 0; r < NR + 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) W[r][c] = sU[i0 + r][lane + c];
+    }
+    float acc[NR][D];
+    apply_window<D, NR>(W, R, ke_args, acc);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const Vec<float, D>& uc = W[j + 1][1];
+      const long long node = (long long)(gyo + j) * a.nnx + gxo;
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) acc[j][ii] += ord[j].v[ii] * uc.v[ii];
+      Vec<float, D> o;
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) {
+        const bool fr = (ow[j] >> ((node * D + ii) & 31)) & 1u;
+        o.v[ii] = fr ? uc.v[ii] + a.omega * iv[j].v[ii] * (bc[j].v[ii] - acc[j][ii]) : 0.0f;
+      }
+      sX[i0 + j + 1][lane + 1] = o;
+      if constexpr (!POST) {
+        // x2 of the output nodes
+        const int oy = i0 + j;
+        if (oy >= 1 && oy <= kOutY && lane >= 1 && lane <= kOutX && gyo + j < a.nny &&
+            gxo < a.nnx)
+          store_node<float, D>(reinterpret_cast<float*>(a.out) + row * a.so, node, o);
+      }
+    }
+  }
+  __syncthreads();
+
+  // ---- second stencil on the output nodes
+  double dot = 0.0;
+  {
+    Vec<float, D> W[NR + 2][3];
+#pragma unroll
+    for (int r = 0; r < NR + 2; ++r) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) W[r][c] = sX[i0 + r][lane + c];
+    }
+    float acc[NR][D];
+    apply_window<D, NR>(W, R, ke_args, acc);
+#pragma unroll
+    for (int j = 0; j < NR; ++j) {
+      const int oy = i0 + j;
+      const int gy = gyo + j;
+      if (!(oy >= 1 && oy <= kOutY && lane >= 1 && lane <= kOutX && gy < a.nny && gxo < a.nnx))
+        continue;
+      const long long node = (long long)gy * a.nnx + gxo;
+      const Vec<float, D>& uc = W[j + 1][1];
+#pragma unroll
+      for (int ii = 0; ii < D; ++ii) acc[j][ii] += ord[j].v[ii] * uc.v[ii];
+      if constexpr (POST) {
+        Vec<ZT, D> zo;
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) {
+          const bool fr = (ow[j] >> ((node * D + ii) & 31)) & 1u;
+          const float o =
+              fr ? uc.v[ii] + a.omega * iv[j].v[ii] * (bc[j].v[ii] - acc[j][ii]) : 0.0f;
+          zo.v[ii] = static_cast<ZT>(o);
+          dot += ob[j].v[ii] * (double)zo.v[ii];
+        }
+        store_node<ZT, D>(a.out + row * a.so, node, zo);
+      } else {
+        Vec<float, D> ro;
+#pragma unroll
+        for (int ii = 0; ii < D; ++ii) {
+          const bool fr = (ow[j] >> ((node * D + ii) & 31)) & 1u;
+          ro.v[ii] = fr ? bc[j].v[ii] - acc[j][ii] : 0.0f;
+        }
+        store_node<float, D>(a.res + row * a.sres, node, ro);
+      }
+    }
+  }
+  if constexpr (POST) {
+    if (a.partial != nullptr) {
+      const double sum = block_sum(dot, red);
+      if (threadIdx.x == 0) a.partial[(long long)row * a.tiles + tt] = sum;
+    }
+  }
+}
+
+template <typename ZT>
+PairArgs<ZT> pair_args(const OpDesc& op, int rows) {
+  PairArgs<ZT> a{};
+  a.nelx = op.nelx;
+  a.nely = op.nely;
+  a.nnx = op.nnx();
+  a.nny = op.nny();
+  a.ntx = ceil_div(op.nnx(), kOutX);
+  a.tiles = pair_tiles(op);
+  a.total = a.tiles * rows;
+  a.rho = static_cast<const float*>(op.rho);
+  a.sr = op.sr;
+  a.rdiag = static_cast<const float*>(op.rdiag);
+  a.srd = op.srd;
+  a.mask = op.mask;
+  a.ke = make_ke<float>(op.ke, op.d);
+  return a;
+}
+
+}  // namespace
+
+int pair_tiles(const OpDesc& op) {
+  return ceil_div(op.nnx(), kOutX) * ceil_div(op.nny(), kOutY);
+}
+
+int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
+                    const float* invd, long long sd, float* x, long long sx, float* r,
+                    long long sr, double omega, const int* active, hipStream_t s) {
+  if (rows <= 0) return STTOPT_OK;
+  if (op.rho == nullptr || (op.d != 1 && op.d != 2)) {
+    set_error("smooth_pair_pre: level-0 density operator with d = 1 or 2 required");
+    return STTOPT_EINVAL;
+  }
+  if (op.d == 2 && !(pair_aligned(b, sb, sizeof(double)) && pair_aligned(invd, sd, sizeof(float)) &&
+                     pair_aligned(x, sx, sizeof(float)) && pair_aligned(r, sr, sizeof(float)) &&
+                     pair_aligned(op.rdiag, op.srd, sizeof(float)))) {
+    set_error("smooth_pair_pre: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  PairArgs<float> a = pair_args<float>(op, rows);
+  a.b = b;
+  a.sb = sb;
+  a.invd = invd;
+  a.sd = sd;
+  a.out = x;
+  a.so = sx;
+  a.res = r;
+  a.sres = sr;
+  a.omega = static_cast<float>(omega);
+  a.active = active;
+  const dim3 grid(a.total), block(kBlock);
+  if (op.d == 2)
+    hipLaunchKernelGGL((smooth_pair_kernel<float, 2, false>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((smooth_pair_kernel<float, 1, false>), grid, block, 0, s, a);
+  STT_CHECK_LAUNCH("smooth_pair_kernel(pre)");
+  return STTOPT_OK;
+}
+
+template <typename ZT>
+int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
+                     const float* ec, long long sec, int cnnx, const double* b, long long sb,
+                     const float* invd, long long sd, ZT* z, long long sz, double omega,
+                     const int* active, double* partial, hipStream_t s) {
+  if (rows <= 0) return STTOPT_OK;
+  if (op.rho == nullptr || (op.d != 1 && op.d != 2) || op.nelx % 2 != 0 || op.nely % 2 != 0 ||
+      cnnx != op.nelx / 2 + 1) {
+    set_error("smooth_pair_post: level-0 density operator over a 2 x 2 coarsening required");
+    return STTOPT_EINVAL;
+  }
+  if (op.d == 2 && !(pair_aligned(b, sb, sizeof(double)) && pair_aligned(invd, sd, sizeof(float)) &&
+                     pair_aligned(x, sx, sizeof(float)) && pair_aligned(ec, sec, sizeof(float)) &&
+                     pair_aligned(z, sz, sizeof(ZT)) &&
+                     pair_aligned(op.rdiag, op.srd, sizeof(float)))) {
+    set_error("smooth_pair_post: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  PairArgs<ZT> a = pair_args<ZT>(op, rows);
+  a.cnnx = cnnx;
+  a.x = x;
+  a.sx = sx;
+  a.ec = ec;
+  a.sec = sec;
+  a.b = b;
+  a.sb = sb;
+  a.invd = invd;
+  a.sd = sd;
+  a.out = z;
+  a.so = sz;
+  a.omega = static_cast<float>(omega);
+  a.active = active;
+  a.partial = partial;
+  const dim3 grid(a.total), block(kBlock);
+  if (op.d == 2)
+    hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, true>), grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, true>), grid, block, 0, s, a);
+  STT_CHECK_LAUNCH("smooth_pair_kernel(post)");
+  return STTOPT_OK;
+}
+
+template int smooth_pair_post<float>(const OpDesc&, int, const float*, long long, const float*,
+                                     long long, int, const double*, long long, const float*,
+                                     long long, float*, long long, double, const int*, double*,
+                                     hipStream_t);
+template int smooth_pair_post<double>(const OpDesc&, int, const float*, long long,
+                                      const float*, long long, int, const double*, long long,
+                                      const float*, long long, double*, long long, double,
+                                      const int*, double*, hipStream_t);
+
+}  // namespace stt

@@ -494,11 +494,14 @@ class VCycle:
     :param n_smooth: pre- and post-smoothing sweeps per level
     :param gamma: coarse visits per level (1: V-cycle, 2: W-cycle)
     :param dtype: V-cycle precision (default `VCYCLE_DTYPE`)
+    :param separate_sweeps: run every level-0 smoothing sweep of the fp32 cycle as its
+        own pass instead of the two-sweep passes (same results; for testing)
     """
 
     def __init__(self, levels, *, omega: float = 0.6, n_smooth: int = 2, gamma: int = 1,
-                 dtype: torch.dtype | None = None) -> None:
+                 dtype: torch.dtype | None = None, separate_sweeps: bool = False) -> None:
         self.levels = levels
+        self.separate_sweeps = separate_sweeps
         self.omega = omega
         self.n_smooth = n_smooth
         self.gamma = gamma
@@ -507,7 +510,8 @@ class VCycle:
 
     def select(self, rows: Tensor) -> "VCycle":
         return VCycle([lv.select(rows) for lv in self.levels], omega=self.omega,
-                      n_smooth=self.n_smooth, gamma=self.gamma, dtype=self.dtype)
+                      n_smooth=self.n_smooth, gamma=self.gamma, dtype=self.dtype,
+                      separate_sweeps=self.separate_sweeps)
 
     # -- native descriptor --------------------------------------------------
 
@@ -546,6 +550,8 @@ class VCycle:
         for l, lv in enumerate(self.levels):
             L = mg.levels[l]
             L.nelx, L.nely, L.kx, L.ky = lv.nelx, lv.nely, lv.kx, lv.ky
+            if l == 0 and self.separate_sweeps:
+                L.flags = _lib.SEPARATE_SWEEPS
             bits = lv.bits()
             keep.append(bits)
             L.mask = bits.data_ptr()

@@ -368,6 +368,63 @@ def test_fp32_vcycle_packed_coarse_matrices_equal_full(dev):
     assert err < 1e-5
 
 
+@pytest.mark.parametrize("nelx, nely, reaction", [(64, 48, False), (64, 48, True), (124, 28, False),
+                                                  (122, 26, False), (16, 12, True), (4, 2, False)])
+def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction):
+    """The fp32 V-cycle's level-0 two-sweep passes (pre: two sweeps and the residual;
+    post: prolongation and two sweeps) equal one pass per sweep bit for bit, on grids
+    whose node counts fall on and next to the 62 x 14 output tiles (125 = 2 * 62 + 1,
+    29 = 2 * 14 + 1, 123, 27), with and without a lumped reaction diagonal."""
+    from sttopt_amd import torch_fem, torch_mg
+
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(nelx * 1000 + nely)
+    x = rng.uniform(0.05, 1, (3, nelx * nely))
+    kw = {"max_coarse_elements": 2} if nelx * nely < 64 else {}
+    if reaction:
+        kw["reaction"] = s["t"](rng.uniform(0.0, 1e-2, (3, nelx * nely)))
+    levels = _hip_hierarchy(s, nelx, nely, x, **kw)
+    assert (levels[0].kx, levels[0].ky) == (2, 2)
+    if reaction:
+        assert levels[0].reaction_diag is not None
+    r = torch_fem.project(s["t"](rng.standard_normal((3, s["ndof"]))), s["mask_t"])
+    z_pair = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    z_sep = torch_mg.VCycle(levels, dtype=torch.float32, separate_sweeps=True)(r)
+    assert float(z_sep.abs().max()) > 0
+    diff = float((z_pair - z_sep).abs().max())
+    print(f"{nelx}x{nely} reaction={reaction}: max |z_pair - z_sep| = {diff:.3e}")
+    assert torch.equal(z_pair, z_sep)
+
+
+def test_mgpcg_two_sweep_passes_equal_separate_sweeps(dev):
+    """MG-PCG with the two-sweep passes: the same per-row iteration counts and the same
+    solution as with one pass per sweep.  The r.z sums are taken over other tiles (62 x 14
+    instead of 64 x 16 nodes), so the two agree to rounding, not bit for bit; the
+    designs are grey, where rounding-level changes of beta do not move the iteration
+    count (on a 0/1 design with Emin = 1e-9 the count at rtol 1e-10 does move with
+    them, in either path)."""
+    from sttopt_amd import torch_mg
+
+    nelx, nely = 126, 54  # 127 x 55 nodes: 12 two-sweep output tiles, 8 level-0 tiles
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(5)
+    xs = np.stack([rng.uniform(0.05, 1, nelx * nely), np.full(nelx * nely, 0.4),
+                   rng.uniform(0.3, 1, nelx * nely)])
+    levels = _hip_hierarchy(s, nelx, nely, xs)
+    F = s["F_t"].expand(3, -1).clone()
+    out = []
+    for sep in (False, True):
+        info = {}
+        vc = torch_mg.VCycle(levels, dtype=torch.float32, separate_sweeps=sep)
+        U, n_iter = torch_mg.native_mgpcg(vc, F, rtol=1e-10, max_iter=300, info=info)
+        out.append((U, n_iter, info["row_iter_list"]))
+    (U1, n1, it1), (U2, n2, it2) = out
+    print(f"iterations: pair {it1}, separate {it2}")
+    assert it1 == it2 and n1 == n2
+    err = float((U1 - U2).norm() / U2.norm())
+    assert err < 1e-9, err
+
+
 # ------------------------------------------------------- dense SPD inverse ----
 
 
