@@ -97,7 +97,8 @@ int coarse_solve(long long n, int rows, const double* cinv, long long scinv,
 // 62 x 14 node output tiles (pair_tiles per row).  pre: x = two sweeps from zero on
 // b (fp64, rounded to fp32 as the cycle's copy of b would be) and r = b - A x.  post:
 // z = two sweeps from x + P ec (ec on the 2 x 2 coarser grid with cnnx nodes per row);
-// partial, when given, receives the per-tile r.z sums (free dofs).
+// partial, when given, receives the per-tile r.z sums (free dofs).  Both need the
+// level-0 free-dof mask and rows below 4 GiB (32-bit offsets).
 int pair_tiles(const OpDesc& op);
 int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
                     const float* invd, long long sd, float* x, long long sx, float* r,

@@ -355,6 +355,8 @@ struct VCycle {
     const sttopt_level& L = mg->levels[0];
     return sizeof(T) == sizeof(float) && mg->n_smooth == 2 && mg->gamma == 1 &&
            mg->nlevels > 1 && L.kx == 2 && L.ky == 2 && op[0].rho != nullptr &&
+           L.mask != nullptr &&
+           op[0].ndof() * (long long)sizeof(double) < (1LL << 32) &&
            (L.flags & STTOPT_SEPARATE_SWEEPS) == 0;
   }
   // reduction partials per row of the r.z sums written by run()

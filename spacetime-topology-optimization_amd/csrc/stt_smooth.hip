@@ -100,7 +100,31 @@ __device__ __forceinline__ void apply_window(const Vec<float, D> (&W)[NR + 2][3]
   }
 }
 
-template <typename ZT, int D, bool POST>
+// Node access with a 32-bit byte offset from a wave-uniform row base (one row of a
+// batched vector stays below 4 GiB), so that loads take the scalar-base form.
+template <typename T, int D>
+__device__ __forceinline__ Vec<T, D> ld(const T* base, uint32_t node) {
+  return load_node<T, D>(reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) +
+                                                    node * (uint32_t)(D * sizeof(T))),
+                         0);
+}
+template <typename T, int D>
+__device__ __forceinline__ void st(T* base, uint32_t node, const Vec<T, D>& v) {
+  store_node<T, D>(reinterpret_cast<T*>(reinterpret_cast<char*>(base) +
+                                        node * (uint32_t)(D * sizeof(T))),
+                   0, v);
+}
+
+template <typename T, int D>
+__device__ __forceinline__ Vec<T, D> zvec() {
+  Vec<T, D> z;
+#pragma unroll
+  for (int k = 0; k < D; ++k) z.v[k] = T(0);
+  return z;
+}
+
+// RD: the operator carries a lumped reaction diagonal.
+template <typename ZT, int D, bool POST, bool RD>
 __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   constexpr int NR = kRowsPerThread;
   __shared__ Vec<float, D> sU[kHY][kHX];          // input of the first stencil
@@ -120,165 +144,174 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   const int x0 = bx * kOutX - 1, y0 = by * kOutY - 1;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int nnx = a.nnx, nny = a.nny;
   const uint32_t* __restrict__ mask = a.mask;
   const double* __restrict__ b = a.b + row * a.sb;
   const float* __restrict__ invd = a.invd + row * a.sd;
   const float* __restrict__ rho = a.rho + row * a.sr;
+  const float* __restrict__ xin = POST ? a.x + row * a.sx : nullptr;
+  const float* __restrict__ ec = POST ? a.ec + row * a.sec : nullptr;
+  const float* __restrict__ rdg = RD ? a.rdiag + row * a.srd : nullptr;
 
-  // ---- staging: wavefront w takes halo rows w, w + 4, ...; lane l column x0 - 1 + l;
-  // the two trailing node columns and the trailing element column are taken by the
-  // block's first 2 * kHY + kEY threads.
+  // ---- staging: wavefront w takes halo rows w, w + 4, ...; lane l column x0 - 1 + l.
+  // The two trailing node columns and the trailing element column are taken by the
+  // block's first 2 * kHY + kEY threads.  Every staging load is issued first, from rows
+  // and columns clamped into the grid (so none is conditional); the staged values are
+  // then formed, zeroed outside the grid, and written to LDS.
   constexpr int kUR = (kHY + kWaves - 1) / kWaves;
   constexpr int kRR = (kEY + kWaves - 1) / kWaves;
   constexpr int kTailU = 2 * kHY, kTail = kTailU + kEY;
   const int cx = x0 - 1 + lane;
-  const bool cnode = cx >= 0 && cx < a.nnx;
+  const bool cnode = cx >= 0 && cx < nnx;
   const bool celem = cx >= 0 && cx < a.nelx;
+  const uint32_t cxn = (uint32_t)min(max(cx, 0), nnx - 1);
+  const uint32_t cxe = (uint32_t)min(max(cx, 0), a.nelx - 1);
   const int tix = threadIdx.x;
   const int ti = tix < kTailU ? (tix >> 1) : tix - kTailU;
   const int tk = tix < kTailU ? kTileX + (tix & 1) : kTileX;
   const int tgy = y0 - 1 + ti, tgx = x0 - 1 + tk;
-  const bool tnode = tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx;
+  auto clampy = [&](int gy) -> uint32_t { return (uint32_t)min(max(gy, 0), nny - 1); };
 
-  // staged value of node (gy, gx) of the grid: pre x1 = w D^-1 b (b and D^-1 also
-  // returned), post xp = x2 + P e at free dofs.
+  // loads of one staged node (gy, gx), both inside the grid
+  struct Raw {
+    Vec<float, D> x, c00, c01, c10, c11;  // post: x2 and the coarse values around
+    uint32_t word;
+    Vec<double, D> b;                      // pre: b and D^-1
+    Vec<float, D> dv;
+  };
+  auto load_raw = [&](uint32_t gy, uint32_t gx) -> Raw {
+    Raw r;
+    const uint32_t node = gy * (uint32_t)nnx + gx;
+    if constexpr (POST) {
+      r.x = ld<float, D>(xin, node);
+      r.word = mask[(node * D) >> 5];
+      // the 2 x 2 coarse nodes around; at an even fine position the second one of a
+      // pair is the first one again (its weight is 0)
+      const uint32_t c00 = (gy >> 1) * (uint32_t)a.cnnx + (gx >> 1);
+      const uint32_t c10 = c00 + (gy & 1u) * (uint32_t)a.cnnx;
+      r.c00 = ld<float, D>(ec, c00);
+      r.c01 = ld<float, D>(ec, c00 + (gx & 1u));
+      r.c10 = ld<float, D>(ec, c10);
+      r.c11 = ld<float, D>(ec, c10 + (gx & 1u));
+    } else {
+      r.b = ld<double, D>(b, node);
+      r.dv = ld<float, D>(invd, node);
+    }
+    return r;
+  };
+  // staged value (ok: the node is inside the grid, else zero): pre x1 = w D^-1 b, with
+  // b and D^-1 also returned; post xp = x2 + P e at free dofs (bilinear weights 1, 0 at
+  // even and 1/2, 1/2 at odd fine positions)
   struct Staged {
     Vec<float, D> u, bc, dv;
   };
-  auto stage = [&](int gy, int gx) -> Staged {
+  auto finish = [&](const Raw& r, uint32_t gy, uint32_t gx, bool ok) -> Staged {
     Staged s;
-    const long long node = (long long)gy * a.nnx + gx;
     if constexpr (POST) {
-      const float* ec = a.ec + row * a.sec;
-      s.u = load_node<float, D>(a.x + row * a.sx, node);
-      const uint32_t word = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
-      const uint32_t bits = (word >> ((node * D) & 31)) & ((1u << D) - 1u);
-      const int I0 = gy >> 1, fy = gy & 1, J0 = gx >> 1, fx = gx & 1;
-      const float wy1 = float(fy) / 2.0f, wy0 = 1.0f - wy1;
-      const float wx1 = float(fx) / 2.0f, wx0 = 1.0f - wx1;
-      const long long c00 = (long long)I0 * a.cnnx + J0;
-      const Vec<float, D> a00 = load_node<float, D>(ec, c00);
-      Vec<float, D> a01, a10, a11;
-#pragma unroll
-      for (int k = 0; k < D; ++k) a01.v[k] = a10.v[k] = a11.v[k] = 0.0f;
-      if (fx > 0) a01 = load_node<float, D>(ec, c00 + 1);
-      if (fy > 0) a10 = load_node<float, D>(ec, c00 + a.cnnx);
-      if (fx > 0 && fy > 0) a11 = load_node<float, D>(ec, c00 + a.cnnx + 1);
+      const uint32_t node = gy * (uint32_t)nnx + gx;
+      const uint32_t bits = (r.word >> ((node * D) & 31)) & ((1u << D) - 1u);
+      const float wy1 = (gy & 1u) ? 0.5f : 0.0f, wy0 = 1.0f - wy1;
+      const float wx1 = (gx & 1u) ? 0.5f : 0.0f, wx0 = 1.0f - wx1;
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        const float top = wx0 * a00.v[k] + wx1 * a01.v[k];
-        const float bot = wx0 * a10.v[k] + wx1 * a11.v[k];
-        if ((bits >> k) & 1u) s.u.v[k] += wy0 * top + wy1 * bot;
+        const float top = wx0 * r.c00.v[k] + wx1 * r.c01.v[k];
+        const float bot = wx0 * r.c10.v[k] + wx1 * r.c11.v[k];
+        float u = r.x.v[k];
+        if ((bits >> k) & 1u) u += wy0 * top + wy1 * bot;
+        s.u.v[k] = ok ? u : 0.0f;
       }
     } else {
-      const Vec<double, D> bv = load_node<double, D>(b, node);
-      s.dv = load_node<float, D>(invd, node);
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        s.bc.v[k] = static_cast<float>(bv.v[k]);
+        s.bc.v[k] = ok ? static_cast<float>(r.b.v[k]) : 0.0f;
+        s.dv.v[k] = ok ? r.dv.v[k] : 0.0f;
         s.u.v[k] = a.omega * s.dv.v[k] * s.bc.v[k];
       }
     }
     return s;
   };
-  auto zero = [&]() -> Staged {
-    Staged s;
+  Raw rw[kUR];
 #pragma unroll
-    for (int k = 0; k < D; ++k) s.u.v[k] = s.bc.v[k] = s.dv.v[k] = 0.0f;
-    return s;
-  };
-  Staged st[kUR];
-#pragma unroll
-  for (int n = 0; n < kUR; ++n) {
-    const int i = wave + n * kWaves, gy = y0 - 1 + i;
-    st[n] = (i < kHY && gy >= 0 && gy < a.nny && cnode) ? stage(gy, cx) : zero();
-  }
-  const Staged stt = tnode ? stage(tgy, tgx) : zero();
+  for (int n = 0; n < kUR; ++n) rw[n] = load_raw(clampy(y0 - 1 + wave + n * kWaves), cxn);
+  const uint32_t tgyc = clampy(tgy), tgxc = (uint32_t)min(tgx, nnx - 1);
+  Raw rt;
+  if (tix < kTailU) rt = load_raw(tgyc, tgxc);
   float wr[kRR];
 #pragma unroll
   for (int n = 0; n < kRR; ++n) {
-    const int i = wave + n * kWaves, ey = y0 - 1 + i;
-    wr[n] = 0.0f;
-    if (i < kEY && ey >= 0 && ey < a.nely && celem) wr[n] = rho[(long long)ey * a.nelx + cx];
+    const int ey = y0 - 1 + wave + n * kWaves;
+    wr[n] = rho[(uint32_t)min(max(ey, 0), a.nely - 1) * (uint32_t)a.nelx + cxe];
   }
   float tr = 0.0f;
-  if (tix >= kTailU && tix < kTail && tgy >= 0 && tgy < a.nely && tgx < a.nelx)
-    tr = rho[(long long)tgy * a.nelx + tgx];
+  if (tix >= kTailU && tix < kTail)
+    tr = rho[(uint32_t)min(max(tgy, 0), a.nely - 1) * (uint32_t)a.nelx +
+             (uint32_t)min(tgx, a.nelx - 1)];
 
-  // operands of this thread's first-stencil nodes (y0 + 4 wave + j, x0 + lane): mask
-  // words, reaction diagonal, and for post b and D^-1 (pre has them in LDS)
-  const int gxo = x0 + lane, gyo = y0 + wave * NR;
-  uint32_t ow[NR];
-  Vec<double, D> ob[POST ? NR : 1];
-  Vec<float, D> oi[POST ? NR : 1];
-  Vec<float, D> ord[NR];
-#pragma unroll
-  for (int j = 0; j < NR; ++j) {
-    const int gy = gyo + j;
-    const bool in = gy >= 0 && gy < a.nny && gxo >= 0 && gxo < a.nnx;
-    const long long node = (long long)gy * a.nnx + gxo;
-    ow[j] = 0u;  // nodes outside the grid stay zero
-    if (in) ow[j] = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
-#pragma unroll
-    for (int k = 0; k < D; ++k) ord[j].v[k] = 0.0f;
-    if (a.rdiag != nullptr && in) ord[j] = load_node<float, D>(a.rdiag + row * a.srd, node);
-    if constexpr (POST) {
-#pragma unroll
-      for (int k = 0; k < D; ++k) {
-        ob[j].v[k] = 0.0;
-        oi[j].v[k] = 0.0f;
-      }
-      if (in) {
-        ob[j] = load_node<double, D>(b, node);
-        oi[j] = load_node<float, D>(invd, node);
-      }
-    }
-  }
-
-  // ---- LDS writes
+  // ---- staged values to LDS
 #pragma unroll
   for (int n = 0; n < kUR; ++n) {
-    const int i = wave + n * kWaves;
+    const int i = wave + n * kWaves, gy = y0 - 1 + i;
     if (i < kHY) {
-      sU[i][lane] = st[n].u;
+      const Staged sg = finish(rw[n], clampy(gy), cxn, gy >= 0 && gy < nny && cnode);
+      sU[i][lane] = sg.u;
       if constexpr (!POST) {
         // halo node (i, lane) is node (i - 1, lane - 1) of the first stencil
         if (i >= 1 && i <= kTileY && lane >= 1) {
-          sB[i - 1][lane - 1] = st[n].bc;
-          sI[i - 1][lane - 1] = st[n].dv;
+          sB[i - 1][lane - 1] = sg.bc;
+          sI[i - 1][lane - 1] = sg.dv;
         }
       }
     }
   }
   if (tix < kTailU) {
-    sU[ti][tk] = stt.u;
+    const Staged sg = finish(rt, tgyc, tgxc, tgy >= 0 && tgy < nny && tgx < nnx);
+    sU[ti][tk] = sg.u;
     if constexpr (!POST) {
       // halo column 64 is the first stencil's last node column (63); 65 is outside it
       if (tk == kTileX && ti >= 1 && ti <= kTileY) {
-        sB[ti - 1][kTileX - 1] = stt.bc;
-        sI[ti - 1][kTileX - 1] = stt.dv;
+        sB[ti - 1][kTileX - 1] = sg.bc;
+        sI[ti - 1][kTileX - 1] = sg.dv;
       }
     }
   } else if (tix < kTail) {
-    sR[ti][tk] = tr;
+    sR[ti][tk] = (tgy >= 0 && tgy < a.nely && tgx < a.nelx) ? tr : 0.0f;
   }
 #pragma unroll
   for (int n = 0; n < kRR; ++n) {
-    const int i = wave + n * kWaves;
-    if (i < kEY) sR[i][lane] = wr[n];
+    const int i = wave + n * kWaves, ey = y0 - 1 + i;
+    if (i < kEY) sR[i][lane] = (ey >= 0 && ey < a.nely && celem) ? wr[n] : 0.0f;
   }
-  // border of sX: read only by nodes without output; kept at zero
-  {
-    Vec<float, D> zv;
+
+  // operands of this thread's first-stencil nodes (y0 + 4 wave + j, x0 + lane), loaded
+  // now for the epilogues: mask words (0 outside the grid: those nodes stay zero),
+  // reaction diagonal, and for post b and D^-1 (pre has them in LDS)
+  const int gxo = x0 + lane, gyo = y0 + wave * NR;
+  const bool oin = gxo >= 0 && gxo < nnx;
+  const uint32_t gxc = (uint32_t)min(max(gxo, 0), nnx - 1);
+  uint32_t ow[NR];
+  Vec<double, D> ob[POST ? NR : 1];
+  Vec<float, D> oi[POST ? NR : 1];
+  Vec<float, D> ord[RD ? NR : 1];
 #pragma unroll
-    for (int k = 0; k < D; ++k) zv.v[k] = 0.0f;
-    if (tix < kTileX + 2) {
-      sX[0][tix] = zv;
-      sX[kTileY + 1][tix] = zv;
-    } else if (tix < kTileX + 2 + kTileY) {
-      sX[tix - (kTileX + 2) + 1][0] = zv;
-      sX[tix - (kTileX + 2) + 1][kTileX + 1] = zv;
+  for (int j = 0; j < NR; ++j) {
+    const int gy = gyo + j;
+    const uint32_t node = clampy(gy) * (uint32_t)nnx + gxc;
+    const uint32_t w = mask[(node * D) >> 5];
+    ow[j] = (oin && gy >= 0 && gy < nny) ? w : 0u;
+    if constexpr (RD) ord[j] = ld<float, D>(rdg, node);
+    if constexpr (POST) {
+      ob[j] = ld<double, D>(b, node);
+      oi[j] = ld<float, D>(invd, node);
     }
+  }
+
+  // border of sX: read only by nodes without output; kept at zero
+  if (tix < kTileX + 2) {
+    sX[0][tix] = zvec<float, D>();
+    sX[kTileY + 1][tix] = zvec<float, D>();
+  } else if (tix < kTileX + 2 + kTileY) {
+    sX[tix - (kTileX + 2) + 1][0] = zvec<float, D>();
+    sX[tix - (kTileX + 2) + 1][kTileX + 1] = zvec<float, D>();
   }
   __syncthreads();
 
@@ -311,6 +344,11 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
     R[r][0] = sR[i0 + r][lane];
     R[r][1] = sR[i0 + r][lane + 1];
   }
+  // free-dof bit of dof ii of node row j
+  auto freebit = [&](int j, int ii) -> bool {
+    const uint32_t node = (uint32_t)(gyo + j) * (uint32_t)nnx + (uint32_t)gxo;
+    return (ow[j] >> ((node * D + ii) & 31)) & 1u;
+  };
 
   // ---- first stencil on the 64 x 16 nodes: one damped-Jacobi sweep
   {
@@ -325,22 +363,22 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
 #pragma unroll
     for (int j = 0; j < NR; ++j) {
       const Vec<float, D>& uc = W[j + 1][1];
-      const long long node = (long long)(gyo + j) * a.nnx + gxo;
+      if constexpr (RD) {
 #pragma unroll
-      for (int ii = 0; ii < D; ++ii) acc[j][ii] += ord[j].v[ii] * uc.v[ii];
+        for (int ii = 0; ii < D; ++ii) acc[j][ii] += ord[j].v[ii] * uc.v[ii];
+      }
       Vec<float, D> o;
 #pragma unroll
-      for (int ii = 0; ii < D; ++ii) {
-        const bool fr = (ow[j] >> ((node * D + ii) & 31)) & 1u;
-        o.v[ii] = fr ? uc.v[ii] + a.omega * iv[j].v[ii] * (bc[j].v[ii] - acc[j][ii]) : 0.0f;
-      }
+      for (int ii = 0; ii < D; ++ii)
+        o.v[ii] = freebit(j, ii) ? uc.v[ii] + a.omega * iv[j].v[ii] * (bc[j].v[ii] - acc[j][ii])
+                                 : 0.0f;
       sX[i0 + j + 1][lane + 1] = o;
       if constexpr (!POST) {
         // x2 of the output nodes
         const int oy = i0 + j;
-        if (oy >= 1 && oy <= kOutY && lane >= 1 && lane <= kOutX && gyo + j < a.nny &&
-            gxo < a.nnx)
-          store_node<float, D>(reinterpret_cast<float*>(a.out) + row * a.so, node, o);
+        if (oy >= 1 && oy <= kOutY && lane >= 1 && lane <= kOutX && gyo + j < nny && gxo < nnx)
+          st<float, D>(reinterpret_cast<float*>(a.out) + row * a.so,
+                       (uint32_t)(gyo + j) * (uint32_t)nnx + (uint32_t)gxo, o);
       }
     }
   }
@@ -361,31 +399,30 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
     for (int j = 0; j < NR; ++j) {
       const int oy = i0 + j;
       const int gy = gyo + j;
-      if (!(oy >= 1 && oy <= kOutY && lane >= 1 && lane <= kOutX && gy < a.nny && gxo < a.nnx))
+      if (!(oy >= 1 && oy <= kOutY && lane >= 1 && lane <= kOutX && gy < nny && gxo < nnx))
         continue;
-      const long long node = (long long)gy * a.nnx + gxo;
+      const uint32_t node = (uint32_t)gy * (uint32_t)nnx + (uint32_t)gxo;
       const Vec<float, D>& uc = W[j + 1][1];
+      if constexpr (RD) {
 #pragma unroll
-      for (int ii = 0; ii < D; ++ii) acc[j][ii] += ord[j].v[ii] * uc.v[ii];
+        for (int ii = 0; ii < D; ++ii) acc[j][ii] += ord[j].v[ii] * uc.v[ii];
+      }
       if constexpr (POST) {
         Vec<ZT, D> zo;
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) {
-          const bool fr = (ow[j] >> ((node * D + ii) & 31)) & 1u;
-          const float o =
-              fr ? uc.v[ii] + a.omega * iv[j].v[ii] * (bc[j].v[ii] - acc[j][ii]) : 0.0f;
+          const float o = freebit(j, ii)
+                              ? uc.v[ii] + a.omega * iv[j].v[ii] * (bc[j].v[ii] - acc[j][ii])
+                              : 0.0f;
           zo.v[ii] = static_cast<ZT>(o);
           dot += ob[j].v[ii] * (double)zo.v[ii];
         }
-        store_node<ZT, D>(a.out + row * a.so, node, zo);
+        st<ZT, D>(a.out + row * a.so, node, zo);
       } else {
         Vec<float, D> ro;
 #pragma unroll
-        for (int ii = 0; ii < D; ++ii) {
-          const bool fr = (ow[j] >> ((node * D + ii) & 31)) & 1u;
-          ro.v[ii] = fr ? bc[j].v[ii] - acc[j][ii] : 0.0f;
-        }
-        store_node<float, D>(a.res + row * a.sres, node, ro);
+        for (int ii = 0; ii < D; ++ii) ro.v[ii] = freebit(j, ii) ? bc[j].v[ii] - acc[j][ii] : 0.0f;
+        st<float, D>(a.res + row * a.sres, node, ro);
       }
     }
   }
@@ -416,6 +453,22 @@ PairArgs<ZT> pair_args(const OpDesc& op, int rows) {
   return a;
 }
 
+template <typename ZT, bool POST>
+void launch_pair(const PairArgs<ZT>& a, int d, hipStream_t s) {
+  const dim3 grid(a.total), block(kBlock);
+  const bool rd = a.rdiag != nullptr;
+  if (d == 2) {
+    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, false>), grid, block, 0, s, a);
+  } else {
+    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, false>), grid, block, 0, s, a);
+  }
+}
+
+// One row of a batched level-0 vector must stay below 4 GiB (32-bit offsets).
+bool fits_u32(const OpDesc& op) { return op.ndof() * (long long)sizeof(double) < (1LL << 32); }
+
 }  // namespace
 
 int pair_tiles(const OpDesc& op) {
@@ -426,8 +479,9 @@ int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
                     const float* invd, long long sd, float* x, long long sx, float* r,
                     long long sr, double omega, const int* active, hipStream_t s) {
   if (rows <= 0) return STTOPT_OK;
-  if (op.rho == nullptr || (op.d != 1 && op.d != 2)) {
-    set_error("smooth_pair_pre: level-0 density operator with d = 1 or 2 required");
+  if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) || !fits_u32(op)) {
+    set_error("smooth_pair_pre: level-0 density operator with a free-dof mask, d = 1 or 2, "
+              "rows below 4 GiB required");
     return STTOPT_EINVAL;
   }
   if (op.d == 2 && !(pair_aligned(b, sb, sizeof(double)) && pair_aligned(invd, sd, sizeof(float)) &&
@@ -447,11 +501,7 @@ int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
   a.sres = sr;
   a.omega = static_cast<float>(omega);
   a.active = active;
-  const dim3 grid(a.total), block(kBlock);
-  if (op.d == 2)
-    hipLaunchKernelGGL((smooth_pair_kernel<float, 2, false>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((smooth_pair_kernel<float, 1, false>), grid, block, 0, s, a);
+  launch_pair<float, false>(a, op.d, s);
   STT_CHECK_LAUNCH("smooth_pair_kernel(pre)");
   return STTOPT_OK;
 }
@@ -462,8 +512,8 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
                      const float* invd, long long sd, ZT* z, long long sz, double omega,
                      const int* active, double* partial, hipStream_t s) {
   if (rows <= 0) return STTOPT_OK;
-  if (op.rho == nullptr || (op.d != 1 && op.d != 2) || op.nelx % 2 != 0 || op.nely % 2 != 0 ||
-      cnnx != op.nelx / 2 + 1) {
+  if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) ||
+      op.nelx % 2 != 0 || op.nely % 2 != 0 || cnnx != op.nelx / 2 + 1 || !fits_u32(op)) {
     set_error("smooth_pair_post: level-0 density operator over a 2 x 2 coarsening required");
     return STTOPT_EINVAL;
   }
@@ -489,11 +539,7 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
   a.omega = static_cast<float>(omega);
   a.active = active;
   a.partial = partial;
-  const dim3 grid(a.total), block(kBlock);
-  if (op.d == 2)
-    hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, true>), grid, block, 0, s, a);
-  else
-    hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, true>), grid, block, 0, s, a);
+  launch_pair<ZT, true>(a, op.d, s);
   STT_CHECK_LAUNCH("smooth_pair_kernel(post)");
   return STTOPT_OK;
 }
