@@ -306,6 +306,45 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage,
                           double* scratch, void* stream);
 int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage);
 
+/* ---- Gauss-point gradient fields of the scalar (d = 1) mesh ------------------------
+ * Element fields X are [rows][nel][4][2] doubles (Gauss point g = 2 * eta_index +
+ * xi_index, then x, y), 16-byte aligned with even row strides.  B: host array of
+ * 4 x 2 x 4 doubles, the shape-function gradients at the Gauss points in the local node
+ * order BL, BR, TR, TL (ScalarMesh.B). */
+
+/* out[row][e][g][d] = scale * escale[row][e] * sum_i B[g][d][i] u[row][node_i(e)]
+ * (escale NULL: 1).  normalize != 0: the direction field G / sqrt(|G|^2 + DBL_MIN)
+ * per Gauss point instead (scale and escale unused).
+ * Replaces ScalarMesh.gauss_gradients and direction_field (virtual_heat.py:290-307). */
+int sttopt_gauss_gradients(int32_t nelx, int32_t nely, int32_t rows, const double* B,
+                           const double* u, int64_t u_stride, const double* escale,
+                           int64_t escale_stride, double scale, int32_t normalize,
+                           double* out, int64_t out_stride, void* stream);
+
+/* F[row][n] = scale * sum over the elements e at node n of
+ * w[row][e] sum_{g,d} X[row][e][g][d] B[g][d][loc_n(e)] (w NULL: 1), gathered per node
+ * in a fixed order (no atomics).  With scale 1/4: ScalarMesh.divergence_load
+ * (virtual_heat.py:309-315); with w NULL, scale 1: the adjoint of gauss_gradients. */
+int sttopt_divergence_load(int32_t nelx, int32_t nely, int32_t rows, const double* B,
+                           const double* w, int64_t w_stride, const double* X,
+                           int64_t X_stride, double scale, double* F, int64_t F_stride,
+                           void* stream);
+
+/* out[row][e] = scale * sum_{g,d} X[row][e][g][d] sum_i B[g][d][i] v[row][node_i(e)]:
+ * the weight gradient of divergence_load.  No counterpart as a function in the
+ * reference (autograd of virtual_heat.py:309-315). */
+int sttopt_gauss_dot(int32_t nelx, int32_t nely, int32_t rows, const double* B,
+                     const double* X, int64_t X_stride, const double* v, int64_t v_stride,
+                     double scale, double* out, int64_t out_stride, void* stream);
+
+/* Adjoint of the direction field at the Gauss points: gG = (gY - Y (Y . gY)) / n with
+ * G the gradients of s, n = sqrt(|G|^2 + DBL_MIN), Y = G / n.  No counterpart as a
+ * function in the reference (autograd of virtual_heat.py:294-307). */
+int sttopt_direction_field_backward(int32_t nelx, int32_t nely, int32_t rows,
+                                    const double* B, const double* s, int64_t s_stride,
+                                    const double* gY, int64_t gY_stride, double* gG,
+                                    int64_t gG_stride, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -172,6 +172,23 @@ _SIGNATURES = {
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,
          c_double, _P, _P, _P, _P],
     ),
+    "sttopt_gauss_gradients": (
+        c_int32,
+        [c_int32, c_int32, c_int32, _P, _P, c_int64, _P, c_int64, c_double, c_int32, _P,
+         c_int64, _P],
+    ),
+    "sttopt_divergence_load": (
+        c_int32,
+        [c_int32, c_int32, c_int32, _P, _P, c_int64, _P, c_int64, c_double, _P, c_int64, _P],
+    ),
+    "sttopt_gauss_dot": (
+        c_int32,
+        [c_int32, c_int32, c_int32, _P, _P, c_int64, _P, c_int64, c_double, _P, c_int64, _P],
+    ),
+    "sttopt_direction_field_backward": (
+        c_int32,
+        [c_int32, c_int32, c_int32, _P, _P, c_int64, _P, c_int64, _P, c_int64, _P],
+    ),
     "sttopt_stage_epilogue": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,

@@ -150,12 +150,13 @@ def test_lifted_scalar_solve_matches_direct(dev):
 
     nelx, nely = 90, 30
     rng = np.random.default_rng(4)
-    mesh = virtual_heat.ScalarMesh.build(nelx, nely, dev)
+    mesh = virtual_heat.ScalarMesh.build(nelx, nely, "left_edge", dev)
     chi = 10 ** rng.uniform(-3, 0, nelx * nely)
     react = np.full(nelx * nely, 0.05)
     nn = mesh.ndof
     cols = np.arange(nn) % (nelx + 1)
     fixed = cols == 0
+    assert np.array_equal(mesh.plate.cpu().numpy(), fixed)
     F = rng.uniform(-1, 1, nn)
     g = np.where(fixed, rng.uniform(0, 1, nn), 0.0)
     t = lambda a: torch.tensor(a, dtype=torch.float64, device=dev)  # noqa: E731

@@ -150,3 +150,42 @@ def test_bench_refuses_more_than_one_gpu():
     out = subprocess.run([sys.executable, str(ROOT / "bench.py"), "--gpus", "2"],
                          capture_output=True, text=True, timeout=120)
     assert out.returncode == 2 and "one GPU" in out.stderr
+
+
+def test_plate_nodes_and_wall_arc_on_a_rectangle_and_an_l_shape():
+    """Host-side mesh data of the scalar mesh (virtual_heat.py:79-172): the plate is the
+    print-base edge; the wall arc runs from the plate's end around the domain back to
+    its start, clockwise, each off-plate boundary node once."""
+    import numpy as np
+
+    from sttopt_amd import virtual_heat as vh
+
+    nelx, nely = 4, 3
+    nn = (nelx + 1) * (nely + 1)
+    node = lambda r, c: r * (nelx + 1) + c  # noqa: E731
+    plate = vh.plate_nodes(nelx, nely, "bottom_edge")
+    assert plate.sum() == nelx + 1 and all(plate[node(nely, c)] for c in range(nelx + 1))
+    arc = vh.wall_arc(nelx, nely, "bottom_edge")
+    want = ([node(r, 0) for r in range(nely - 1, -1, -1)] + [node(0, c) for c in range(1, nelx + 1)]
+            + [node(r, nelx) for r in range(1, nely)])
+    assert arc.tolist() == want
+    assert vh.plate_nodes(nelx, nely, "left_edge").reshape(nely + 1, nelx + 1)[:, 0].all()
+    with pytest.raises(ValueError):
+        vh.plate_nodes(nelx, nely, "top_edge")
+    # L-shape: the top-right 2 x 2 elements removed
+    active = np.ones((nely, nelx), bool)
+    active[:2, 2:] = False
+    dom = vh.domain_nodes(active)
+    assert not dom[0, 3] and dom[0, 2] and dom[2, 4]
+    loop = vh._boundary_loop(active)
+    assert len(set(loop.tolist())) == len(loop) == 14
+    arc = vh.wall_arc(nelx, nely, "left_edge", active)
+    lp = vh.plate_nodes(nelx, nely, "left_edge", active)
+    assert not lp[arc].any() and len(arc) == len(loop) - lp.sum()
+    assert arc[0] == node(0, 1) and arc[-1] == node(nely, 1)
+    # a domain with two pieces is rejected
+    two = np.zeros((nely, nelx), bool)
+    two[0, 0] = two[2, 3] = True
+    with pytest.raises(ValueError):
+        vh._boundary_loop(two)
+    assert vh.ScalarMesh.build(nelx, nely, "bottom_edge", "cpu").arc.numel() == nn - (nelx + 1) - (nelx - 1) * (nely - 1)

@@ -297,3 +297,91 @@ def test_c1_mbb_oracle_solution(nelx, nely):
     assert p.ndof == 9922 and c > 0 and np.all(dcx < 0)
     # compliance equals F . U (work of the external load)
     assert abs(c - float(p.F.numpy() @ U)) <= 1e-9 * c
+
+
+# ------------------------------------------------ scalar-mesh gradient fields ----
+
+
+def test_heat_shape_gradients_are_the_shape_function_derivatives():
+    """B against central differences of the four shape functions at the Gauss points,
+    partition of unity (rows sum to 0), and the eta-major Gauss-point order."""
+    from oracle import heat_fields
+
+    B = heat_fields.shape_gradients()
+    h = 1e-6
+    for g, (xi, eta) in enumerate(heat_fields.gauss_points()):
+        dx = (heat_fields.shape_functions(xi + h, eta) - heat_fields.shape_functions(xi - h, eta)) / (2 * h)
+        dy = (heat_fields.shape_functions(xi, eta + h) - heat_fields.shape_functions(xi, eta - h)) / (2 * h)
+        np.testing.assert_allclose(B[g, 0], dx, atol=1e-9)
+        np.testing.assert_allclose(B[g, 1], dy, atol=1e-9)
+    np.testing.assert_allclose(B.sum(-1), 0.0, atol=1e-15)
+    pts = heat_fields.gauss_points()
+    assert pts[0][0] < 0.5 and pts[0][1] < 0.5 and pts[1][0] > 0.5 and pts[2][1] > 0.5
+
+
+@pytest.mark.parametrize("nelx, nely", [(3, 2), (7, 5)])
+def test_heat_gauss_gradients_of_coordinate_and_bilinear_fields(nelx, nely):
+    """x (column) and y (up: nely - row) have gradients (1, 0) and (0, 1); x y has
+    (y_g, x_g) at each Gauss point's global coordinates (exact for bilinear fields)."""
+    from oracle import heat_fields
+
+    r, c = np.divmod(np.arange((nelx + 1) * (nely + 1)), nelx + 1)
+    x, y = c.astype(float), (nely - r).astype(float)
+    np.testing.assert_allclose(heat_fields.gauss_gradients(x, nelx, nely), np.broadcast_to([1.0, 0.0], (nelx * nely, 4, 2)), atol=1e-14)
+    np.testing.assert_allclose(heat_fields.gauss_gradients(y, nelx, nely), np.broadcast_to([0.0, 1.0], (nelx * nely, 4, 2)), atol=1e-14)
+    G = heat_fields.gauss_gradients(x * y, nelx, nely)
+    er, ec = np.divmod(np.arange(nelx * nely), nelx)
+    for g, (xi, eta) in enumerate(heat_fields.gauss_points()):
+        xg = ec + xi
+        yg = (nely - er - 1) + eta  # element row er spans y in [nely - er - 1, nely - er]
+        np.testing.assert_allclose(G[:, g, 0], yg, atol=1e-13)
+        np.testing.assert_allclose(G[:, g, 1], xg, atol=1e-13)
+
+
+def test_heat_divergence_load_of_a_constant_field_is_the_boundary_flux():
+    """w = 1, X = (1, 0): F = integral of dN_i/dx = the outward x-flux through the
+    boundary: -1/2 per vertical boundary edge at the left nodes, +1/2 at the right,
+    0 inside; X = (0, 1) the same through the bottom (-) and top (+) edges."""
+    from oracle import heat_fields
+
+    nelx, nely = 6, 4
+    nel = nelx * nely
+    w = np.ones(nel)
+    F = heat_fields.divergence_load(w, np.broadcast_to([1.0, 0.0], (nel, 4, 2)), nelx, nely)
+    want = np.zeros((nely + 1, nelx + 1))
+    edges = np.full(nely + 1, 2.0)
+    edges[[0, -1]] = 1.0
+    want[:, 0] = -0.5 * edges
+    want[:, -1] = 0.5 * edges
+    np.testing.assert_allclose(F.reshape(nely + 1, nelx + 1), want, atol=1e-15)
+    F = heat_fields.divergence_load(w, np.broadcast_to([0.0, 1.0], (nel, 4, 2)), nelx, nely)
+    want = np.zeros((nely + 1, nelx + 1))
+    edges = np.full(nelx + 1, 2.0)
+    edges[[0, -1]] = 1.0
+    want[0, :] = 0.5 * edges  # top row (y up)
+    want[-1, :] = -0.5 * edges
+    np.testing.assert_allclose(F.reshape(nely + 1, nelx + 1), want, atol=1e-15)
+
+
+def test_heat_divergence_load_is_the_adjoint_of_the_gradients():
+    """F(w, X) . phi = 1/4 sum_e w_e X_e : G(phi)_e for random fields: the element scatter
+    and the element gather use the same node map; and the direction field is a unit
+    field parallel to the gradient (0 at a flat element)."""
+    from oracle import heat_fields
+
+    rng = np.random.default_rng(0)
+    nelx, nely = 5, 3
+    nel, nn = nelx * nely, (nelx + 1) * (nely + 1)
+    w = rng.uniform(0.1, 1, nel)
+    X = rng.standard_normal((nel, 4, 2))
+    phi = rng.standard_normal(nn)
+    lhs = heat_fields.divergence_load(w, X, nelx, nely) @ phi
+    rhs = 0.25 * np.einsum("e,egd,egd->", w, X, heat_fields.gauss_gradients(phi, nelx, nely))
+    assert abs(lhs - rhs) < 1e-12 * abs(rhs)
+    s = rng.standard_normal(nn)
+    Y = heat_fields.direction_field(s, nelx, nely)
+    G = heat_fields.gauss_gradients(s, nelx, nely)
+    np.testing.assert_allclose(np.linalg.norm(Y, axis=-1), 1.0, atol=1e-14)
+    np.testing.assert_allclose(Y[..., 0] * G[..., 1] - Y[..., 1] * G[..., 0], 0.0, atol=1e-14)
+    assert np.all((Y * G).sum(-1) > 0)
+    assert np.all(heat_fields.direction_field(np.ones(nn), nelx, nely) == 0.0)
