@@ -345,6 +345,34 @@ int sttopt_direction_field_backward(int32_t nelx, int32_t nely, int32_t rows,
                                     const double* gY, int64_t gY_stride, double* gG,
                                     int64_t gG_stride, void* stream);
 
+/* ---- density filter and projection ------------------------------------------------
+ * scale_mode: 0 y = H x; 1 y = (H x) / scale (per output entry); 2 y = H (x / scale)
+ * (per input entry: the adjoint of mode 1 for symmetric H).  scale: device array. */
+
+/* The density filter H[e1, e2] = max(0, rmin - |e1 - e2|) over the offsets
+ * |di|, |dj| <= ceil(rmin) - 1 inside the nelx x nely element grid, applied matrix-free
+ * to rows fields x ([rows][nel], row-major elements).  With scale = Hs and mode 1 this is
+ * filters.apply_density_filter (filters.py:67-76) for the H, Hs of
+ * filters.density_filter (filters.py:38-64). */
+int sttopt_density_filter(int32_t nelx, int32_t nely, int32_t rows, double rmin,
+                          const double* x, int64_t x_stride, const double* scale,
+                          int32_t scale_mode, double* y, int64_t y_stride, void* stream);
+
+/* y = A x for an n x n CSR matrix (int64 row pointers and column indices, fp64 values,
+ * all device arrays), rows right-hand sides.  For a symmetric A this is
+ * torch_util.symmetric_matmul (torch_util.py:157-188), forward and backward. */
+int sttopt_csr_matvec(int64_t n, const int64_t* row_ptr, const int64_t* col_idx,
+                      const double* values, int32_t rows, const double* x, int64_t x_stride,
+                      const double* scale, int32_t scale_mode, double* y, int64_t y_stride,
+                      void* stream);
+
+/* y = H_beta,eta(x) elementwise (filters.heaviside_projection, filters.py:108-125; beta
+ * = +inf: the step with 1/2 at eta) or, derivative != 0, its derivative
+ * (heaviside_projection_derivative, filters.py:128-131; finite beta only), times g when
+ * g != NULL. */
+int sttopt_heaviside(int64_t n, double beta, double eta, int32_t derivative, const double* x,
+                     const double* g, double* y, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -189,6 +189,15 @@ _SIGNATURES = {
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, c_int64, _P, c_int64, _P, c_int64, _P],
     ),
+    "sttopt_density_filter": (
+        c_int32,
+        [c_int32, c_int32, c_int32, c_double, _P, c_int64, _P, c_int32, _P, c_int64, _P],
+    ),
+    "sttopt_csr_matvec": (
+        c_int32,
+        [c_int64, _P, _P, _P, c_int32, _P, c_int64, _P, c_int32, _P, c_int64, _P],
+    ),
+    "sttopt_heaviside": (c_int32, [c_int64, c_double, c_double, c_int32, _P, _P, _P, _P]),
     "sttopt_stage_epilogue": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,

@@ -189,3 +189,36 @@ def test_plate_nodes_and_wall_arc_on_a_rectangle_and_an_l_shape():
     with pytest.raises(ValueError):
         vh._boundary_loop(two)
     assert vh.ScalarMesh.build(nelx, nely, "bottom_edge", "cpu").arc.numel() == nn - (nelx + 1) - (nelx - 1) * (nely - 1)
+
+
+def test_density_filter_matrix_equals_oracle_and_continuity_filter_rows():
+    """Host-side filter matrices (filters.py:38-106) against the oracle's pairwise
+    construction; the continuity operator has unit diagonal and zero row sums."""
+    import numpy as np
+    import scipy.sparse as sp
+
+    from oracle import filters_ref
+    from sttopt_amd import filters, torch_util
+
+    for nelx, nely, rmin in ((7, 5, 1.5), (9, 4, 2.3), (6, 6, 3.0), (5, 3, 1.0)):
+        H, Hs = filters.density_filter(nelx, nely, rmin)
+        Ho, Hso = filters_ref.filter_matrix(nelx, nely, rmin)
+        np.testing.assert_allclose(H.toarray(), Ho, atol=1e-15)
+        np.testing.assert_allclose(Hs, Hso, atol=1e-14)
+        assert H._sttopt_stencil == (nelx, nely, float(rmin))
+    L = filters.continuity_filter(8, 5, 2.0)
+    np.testing.assert_allclose(L.diagonal(), 1.0)
+    np.testing.assert_allclose(np.asarray(L.sum(axis=1)).ravel(), 0.0, atol=1e-14)
+    with pytest.raises(ValueError):
+        filters.continuity_filter(4, 3, 1.0)
+    with pytest.raises(ValueError):
+        torch_util.symmetric_csr_to_tensor(sp.csr_matrix(np.triu(np.ones((3, 3)))), "cpu", torch.float64)
+    H, _ = filters.density_filter(5, 4, 1.5)
+    t = torch_util.symmetric_csr_to_tensor(H, "cpu", torch.float64)
+    assert torch_util.stencil_of(t) == (5, 4, 1.5)
+    np.testing.assert_allclose(torch_util.csr_to_scipy(t).toarray(), H.toarray())
+    # no CPU path: the product refuses host tensors
+    from sttopt_amd import _lib
+
+    with pytest.raises(_lib.DeviceError):
+        filters.apply_density_filter(torch.ones(4, 5, dtype=torch.float64), t, torch.ones(20, dtype=torch.float64))

@@ -2,6 +2,8 @@
 independent derivations and central finite differences, chosen so that a dropped
 term, a wrong sign or index, or a transposed operand fails one of them."""
 
+import math
+
 import numpy as np
 import pytest
 import scipy.sparse as sp
@@ -385,3 +387,42 @@ def test_heat_divergence_load_is_the_adjoint_of_the_gradients():
     np.testing.assert_allclose(Y[..., 0] * G[..., 1] - Y[..., 1] * G[..., 0], 0.0, atol=1e-14)
     assert np.all((Y * G).sum(-1) > 0)
     assert np.all(heat_fields.direction_field(np.ones(nn), nelx, nely) == 0.0)
+
+
+# ------------------------------------------------ density filter, projection ----
+
+
+def test_filter_oracle_closed_forms():
+    from oracle import filters_ref
+
+    nelx, nely = 7, 5
+    H, Hs = filters_ref.filter_matrix(nelx, nely, 1.5)
+    assert np.array_equal(H, H.T)
+    np.testing.assert_allclose(Hs, H.sum(1))
+    e = 2 * nelx + 3  # interior element (3, 2)
+    row = H[e].reshape(nely, nelx)
+    want = np.zeros((nely, nelx))
+    want[1:4, 2:5] = 1.5 - np.sqrt(2)
+    want[2, 2] = want[2, 4] = want[1, 3] = want[3, 3] = 0.5
+    want[2, 3] = 1.5
+    np.testing.assert_allclose(row, want, atol=1e-15)
+    # corner (0, 0): itself, two edge neighbours, one diagonal
+    assert abs(Hs[0] - (1.5 + 2 * 0.5 + (1.5 - np.sqrt(2)))) < 1e-15
+    x = np.random.default_rng(0).uniform(size=(nely, nelx))
+    np.testing.assert_allclose(filters_ref.apply(np.full((nely, nelx), 0.3), H, Hs), 0.3, atol=1e-15)
+    for rmin in (0.5, 1.0):
+        H1, Hs1 = filters_ref.filter_matrix(nelx, nely, rmin)
+        np.testing.assert_allclose(filters_ref.apply(x, H1, Hs1), x, atol=1e-15)
+
+
+def test_heaviside_oracle_closed_forms():
+    from oracle import filters_ref
+
+    x = np.linspace(-0.2, 1.2, 57)
+    for beta, eta in ((1.0, 0.5), (8.0, 0.3), (32.0, 0.6)):
+        np.testing.assert_allclose(filters_ref.heaviside(np.array([0.0, 1.0]), beta, eta), [0.0, 1.0], atol=1e-15)
+        h = 1e-6
+        fd = (filters_ref.heaviside(x + h, beta, eta) - filters_ref.heaviside(x - h, beta, eta)) / (2 * h)
+        np.testing.assert_allclose(filters_ref.heaviside_derivative(x, beta, eta), fd, rtol=1e-6, atol=1e-8)
+    np.testing.assert_allclose(filters_ref.heaviside(x, 1e-4, 0.5), x, atol=1e-8)
+    assert filters_ref.heaviside(np.array([0.29, 0.3, 0.31]), math.inf, 0.3).tolist() == [0.0, 0.5, 1.0]
