@@ -45,6 +45,10 @@ def parse():
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only a plain copy of known size and the fp64 / fp32 K(rho)u "
                          "apply (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE), then exit")
+    ap.add_argument("--next-rows", action="store_true",
+                    help="time the kernels around the solve (density filter, Heaviside "
+                         "projection, heat-path gradient fields and load) on the same grid "
+                         "and rows, print one JSON line, then exit")
     return ap.parse_args()
 
 
@@ -70,6 +74,9 @@ def main():
     rows = 1 + args.nstage
     if args.kernels_only:
         kernels_only(torch, _lib, p, rows, dev)
+        return
+    if args.next_rows:
+        next_rows(torch, _lib, args.nelx, args.nely, rows, dev)
         return
     E = (workloads.EMIN, workloads.EMAX, workloads.PENAL)
     print(f"# workload {p.name}: {p.ndof} dofs x {rows} rows", file=sys.stderr, flush=True)
@@ -193,6 +200,65 @@ def kernels_only(torch, _lib, p, rows, dev):
         del keep
     torch.cuda.synchronize()
     print(f"# copy: {n * 8} bytes read, {n * 8} bytes written per launch", flush=True)
+
+
+def next_rows(torch, _lib, nelx, nely, rows, dev, launches=50, rmin=3.0):
+    """HIP-event timings of the kernels around the solve on `rows` fields of the
+    nelx x nely grid: the density filter (fp64, radius rmin, scaled by Hs), the
+    Heaviside projection, and the scalar mesh's direction field and divergence load.
+    Algorithmic bytes per launch: filter 8 (x + Hs + y) per element, projection
+    8 (x + y), direction field 8 per node + 64 per element, load 8 (w) + 64 (X) per
+    element + 8 per node."""
+    from sttopt_amd import virtual_heat
+
+    lib = _lib.load()
+    nel, nn = nelx * nely, (nelx + 1) * (nely + 1)
+    g = torch.Generator(device="cpu").manual_seed(0)
+    rnd = lambda *shape: torch.rand(*shape, generator=g, dtype=torch.float64).to(dev)  # noqa: E731
+    x, hs = rnd(rows, nel), rnd(nel) + 1.0
+    y = torch.empty_like(x)
+    mesh = virtual_heat.ScalarMesh.build(nelx, nely, "bottom_edge", dev)
+    sfield, w = rnd(rows, nn), rnd(rows, nel)
+    X = torch.empty(rows, nel * 8, dtype=torch.float64, device=dev)
+    F = torch.empty(rows, nn, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    h = stream.cuda_stream
+    calls = {
+        "density_filter_kernel": (
+            lambda: lib.sttopt_density_filter(nelx, nely, rows, rmin, x.data_ptr(), nel,
+                                              hs.data_ptr(), 1, y.data_ptr(), nel, h),
+            rows * nel * 8 * 3),
+        "heaviside_kernel": (
+            lambda: lib.sttopt_heaviside(rows * nel, 16.0, 0.5, 0, x.data_ptr(), None,
+                                         y.data_ptr(), h),
+            rows * nel * 8 * 2),
+        "gauss_gradients_kernel<true> (direction field)": (
+            lambda: lib.sttopt_gauss_gradients(nelx, nely, rows, mesh._B_host,
+                                               sfield.data_ptr(), nn, None, 0, 1.0, 1,
+                                               X.data_ptr(), nel * 8, h),
+            rows * (nn * 8 + nel * 64)),
+        "divergence_load_kernel": (
+            lambda: lib.sttopt_divergence_load(nelx, nely, rows, mesh._B_host, w.data_ptr(),
+                                               nel, X.data_ptr(), nel * 8, 0.25,
+                                               F.data_ptr(), nn, h),
+            rows * (nel * 72 + nn * 8)),
+    }
+    out = {}
+    for name, (fn, nbytes) in calls.items():
+        for _ in range(3):
+            _lib.check(fn(), name)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        for _ in range(launches):
+            fn()
+        end.record(stream)
+        end.synchronize()
+        us = start.elapsed_time(end) * 1e3 / launches
+        out[name] = {"avg_launch_us": us, "algorithmic_bytes_per_launch": nbytes,
+                     "achieved_GBs": nbytes / us / 1e3,
+                     "frac": nbytes / us / 1e3 / HBM_PEAK_GBS}
+    print(json.dumps({"grid": f"{nelx}x{nely}", "rows": rows, "rmin": rmin,
+                      "kernels": out}), flush=True)
 
 
 def pmc_traffic():

@@ -40,19 +40,24 @@ __global__ __launch_bounds__(256) void density_filter_kernel(
     const int dj = i / K - R, di = i % K - R;
     sW[i] = fmax(0.0, rmin - sqrt((double)(di * di + dj * dj)));
   }
-  for (int i = threadIdx.x; i < W * H; i += blockDim.x) {
-    const int ly = i / W, lx = i - ly * W;
-    const int gy = y0 - R + ly, gx = x0 - R + lx;
-    double v = 0.0;
-    if (gy >= 0 && gy < nely && gx >= 0 && gx < nelx) {
-      const long long e = (long long)gy * nelx + gx;
-      v = xr[e];
-      if (mode == kScalePre) v /= sc[e];
+  // staging: wavefront w takes halo rows w, w + 4, ...; lane l columns l and 64 + l
+  // (the latter for l < 2R)
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int ly = w; ly < H; ly += 4) {
+    const int gy = y0 - R + ly;
+    const bool rok = gy >= 0 && gy < nely;
+    for (int lx = lane; lx < W; lx += 64) {
+      const int gx = x0 - R + lx;
+      double v = 0.0;
+      if (rok && gx >= 0 && gx < nelx) {
+        const long long e = (long long)gy * nelx + gx;
+        v = xr[e];
+        if (mode == kScalePre) v /= sc[e];
+      }
+      sX[ly][lx] = v;
     }
-    sX[ly][lx] = v;
   }
   __syncthreads();
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int gx = x0 + lane;
   if (gx >= nelx) return;
 #pragma unroll
@@ -69,6 +74,81 @@ __global__ __launch_bounds__(256) void density_filter_kernel(
     if (mode == kScalePost) acc /= sc[e];
     y[row * sy + e] = acc;
   }
+}
+
+// Radius R known at compile time: each thread holds its (4 + 2R) x (2R + 1) input window
+// in registers (read once from LDS) and the (2R + 1)^2 weights come from the kernel
+// arguments (uniform, scalar registers), so a tap costs one FMA and no LDS read.
+template <int R>
+struct FilterWeights {
+  double v[(2 * R + 1) * (2 * R + 1)];
+};
+
+template <int R>
+__global__ __launch_bounds__(256) void density_filter_r_kernel(
+    int nelx, int nely, FilterWeights<R> fw, const double* __restrict__ x, long long sx,
+    const double* __restrict__ sc, int mode, double* __restrict__ y, long long sy, int ntx,
+    int tiles) {
+  constexpr int W = kFX + 2 * R, H = kFY + 2 * R, K = 2 * R + 1;
+  __shared__ double sX[H][W];
+  const int row = blockIdx.x / tiles;
+  const int t = blockIdx.x - row * tiles;
+  const int by = t / ntx, bx = t - by * ntx;
+  const int x0 = bx * kFX, y0 = by * kFY;
+  const double* xr = x + row * sx;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int ly = w; ly < H; ly += 4) {
+    const int gy = y0 - R + ly;
+    const bool rok = gy >= 0 && gy < nely;
+#pragma unroll
+    for (int lx = lane; lx < W; lx += 64) {
+      const int gx = x0 - R + lx;
+      double v = 0.0;
+      if (rok && gx >= 0 && gx < nelx) {
+        const long long e = (long long)gy * nelx + gx;
+        v = xr[e];
+        if (mode == kScalePre) v /= sc[e];
+      }
+      sX[ly][lx] = v;
+    }
+  }
+  __syncthreads();
+  const int gx = x0 + lane;
+  const int i0 = w * kFRows;
+  double win[kFRows + 2 * R][K];
+#pragma unroll
+  for (int r = 0; r < kFRows + 2 * R; ++r) {
+#pragma unroll
+    for (int c = 0; c < K; ++c) win[r][c] = sX[i0 + r][lane + c];
+  }
+  if (gx >= nelx) return;
+#pragma unroll
+  for (int j = 0; j < kFRows; ++j) {
+    const int gy = y0 + i0 + j;
+    if (gy >= nely) break;
+    double acc = 0.0;
+#pragma unroll
+    for (int dj = 0; dj < K; ++dj) {
+#pragma unroll
+      for (int di = 0; di < K; ++di) acc += fw.v[dj * K + di] * win[j + dj][di];
+    }
+    const long long e = (long long)gy * nelx + gx;
+    if (mode == kScalePost) acc /= sc[e];
+    y[row * sy + e] = acc;
+  }
+}
+
+template <int R>
+void launch_filter_r(int nelx, int nely, int rows, double rmin, const double* x, long long sx,
+                     const double* sc, int mode, double* y, long long sy, hipStream_t s) {
+  FilterWeights<R> fw;
+  constexpr int K = 2 * R + 1;
+  for (int dj = -R; dj <= R; ++dj)
+    for (int di = -R; di <= R; ++di)
+      fw.v[(dj + R) * K + di + R] = std::fmax(0.0, rmin - std::sqrt((double)(di * di + dj * dj)));
+  const int ntx = (nelx + kFX - 1) / kFX, nty = (nely + kFY - 1) / kFY;
+  hipLaunchKernelGGL(density_filter_r_kernel<R>, dim3(ntx * nty * rows), dim3(256), 0, s, nelx,
+                     nely, fw, x, sx, sc, mode, y, sy, ntx, ntx * nty);
 }
 
 // Radius beyond the LDS tile: every tap gathered from global memory (cache served).
@@ -121,7 +201,9 @@ __global__ __launch_bounds__(256) void csr_matvec_kernel(
 // Smoothed threshold (tanh(b eta) + tanh(b (x - eta))) / (tanh(b eta) + tanh(b (1 - eta)))
 // or its derivative b (1 - tanh^2(b (x - eta))) / denominator; beta = inf: the step
 // with value 1/2 at eta (the derivative is not defined there and is not offered).
+// te = tanh(beta eta) and denom = te + tanh(beta (1 - eta)) come from the host.
 __global__ __launch_bounds__(256) void heaviside_kernel(long long n, double beta, double eta,
+                                                        double te, double denom,
                                                         int derivative,
                                                         const double* __restrict__ x,
                                                         const double* __restrict__ g,
@@ -133,8 +215,6 @@ __global__ __launch_bounds__(256) void heaviside_kernel(long long n, double beta
   if (isinf(beta)) {
     out = v > eta ? 1.0 : (v < eta ? 0.0 : 0.5);
   } else {
-    const double te = tanh(beta * eta);
-    const double denom = te + tanh(beta * (1.0 - eta));
     const double th = tanh(beta * (v - eta));
     out = derivative ? beta * (1.0 - th * th) / denom : (te + th) / denom;
   }
@@ -161,7 +241,14 @@ int sttopt_density_filter(int32_t nelx, int32_t nely, int32_t rows, double rmin,
   if (rows == 0) return STTOPT_OK;
   const int R = (int)std::ceil(rmin) - 1;
   hipStream_t s = (hipStream_t)stream;
-  if (R <= kMaxR) {
+  if (R >= 1 && R <= 4) {
+    switch (R) {
+      case 1: launch_filter_r<1>(nelx, nely, rows, rmin, x, x_stride, scale, scale_mode, y, y_stride, s); break;
+      case 2: launch_filter_r<2>(nelx, nely, rows, rmin, x, x_stride, scale, scale_mode, y, y_stride, s); break;
+      case 3: launch_filter_r<3>(nelx, nely, rows, rmin, x, x_stride, scale, scale_mode, y, y_stride, s); break;
+      default: launch_filter_r<4>(nelx, nely, rows, rmin, x, x_stride, scale, scale_mode, y, y_stride, s); break;
+    }
+  } else if (R <= kMaxR) {
     const int ntx = (nelx + kFX - 1) / kFX, nty = (nely + kFY - 1) / kFY;
     hipLaunchKernelGGL(density_filter_kernel, dim3(ntx * nty * rows), dim3(256), 0, s, nelx,
                        nely, R, rmin, x, x_stride, scale, scale_mode, y, y_stride, ntx,
@@ -206,8 +293,11 @@ int sttopt_heaviside(int64_t n, double beta, double eta, int32_t derivative, con
     return STTOPT_EINVAL;
   }
   if (n == 0) return STTOPT_OK;
+  const double te = std::isinf(beta) ? 0.0 : std::tanh(beta * eta);
+  const double denom = std::isinf(beta) ? 1.0 : te + std::tanh(beta * (1.0 - eta));
   hipLaunchKernelGGL(heaviside_kernel, dim3(ceil_div(n, 256)), dim3(256), 0,
-                     (hipStream_t)stream, (long long)n, beta, eta, derivative, x, g, y);
+                     (hipStream_t)stream, (long long)n, beta, eta, te, denom, derivative, x, g,
+                     y);
   STT_CHECK_LAUNCH("heaviside_kernel");
   return STTOPT_OK;
 }

@@ -1,6 +1,7 @@
 """Density filter, symmetric sparse products and the Heaviside projection on the device
-against the oracle (oracle/filters_ref.py) and SciPy: the matrix-free stencil (LDS tile
-and the direct kernel for radii beyond it) and the CSR product, forward and adjoint
+against the oracle (oracle/filters_ref.py) and SciPy: the matrix-free stencil (register
+windows for radii 1-4, the LDS tile up to 8, the direct kernel beyond) and the CSR
+product, forward and adjoint
 (sttopt/filters.py:67-76,108-131, torch_util.py:157-188)."""
 
 import math
@@ -20,7 +21,8 @@ def _t(a, dev):
 
 
 @pytest.mark.parametrize("nelx, nely, rmin", [(7, 5, 1.0), (7, 5, 1.5), (23, 17, 2.3), (70, 33, 4.5),
-                                              (31, 29, 9.7), (130, 70, 3.0)])
+                                              (45, 21, 3.5), (40, 26, 6.2), (31, 29, 9.7),
+                                              (130, 70, 3.0)])
 def test_apply_density_filter_matches_oracle(dev, nelx, nely, rmin):
     from sttopt_amd import filters, torch_util
 
