@@ -373,6 +373,44 @@ int sttopt_csr_matvec(int64_t n, const int64_t* row_ptr, const int64_t* col_idx,
 int sttopt_heaviside(int64_t n, double beta, double eta, int32_t derivative, const double* x,
                      const double* g, double* y, void* stream);
 
+/* ---- MMA subproblem ------------------------------------------------------------------
+ * All arrays fp64 device memory; P, Q (and dfdx) [m][n] row-major, 1 <= m <= 126.
+ *
+ * sttopt_mma_prepare: the asymptotes (re-initialized when iteration < 2, else adapted
+ * from xold1, xold2 and clamped), the move-limited bounds alfa, beta, and the
+ * subproblem's p0, q0, P, Q and b = P / (upp - x) + Q / (x - low) - fval
+ * (mma.mmasub, mma.py:150-276).  asyinit, asymin, asymax, move: n-vectors or scalars
+ * (stride 0).  work: sttopt_mma_prepare_work_count doubles.
+ *
+ * sttopt_mma_subsolv: one step of the primal-dual interior-point method of mma.subsolv
+ * (mma.py:346-557) on a device state kept in work (sttopt_mma_work_count doubles):
+ * op 0 the initial point; op 1 the residual (its 2-norm and max-abs written to the
+ * state's output pair: the last two doubles of the state block returned by
+ * sttopt_mma_state_offsets) at the current point (jj < 0) or at the line-search
+ * candidate steg / 2^jj; op 2 the Newton direction and step bound steg at barrier epsi
+ * (m < n: the (m+1) system in lam and z; m >= n: the (n+1) system in x and z, needing
+ * n <= 126); op 3 the move to candidate jj.  The host drives the barrier and Newton
+ * loops (the reference's while loops), reading the residual pair after each op 1. */
+int64_t sttopt_mma_prepare_work_count(int32_t m, int32_t n);
+int sttopt_mma_prepare(int32_t m, int32_t n, int32_t iteration, const double* xval,
+                       const double* xmin, const double* xmax, const double* xold1,
+                       const double* xold2, const double* df0dx, const double* fval,
+                       const double* dfdx, double* low, double* upp, const double* asyinit,
+                       int64_t asyinit_stride, const double* asymin, int64_t asymin_stride,
+                       const double* asymax, int64_t asymax_stride, const double* move,
+                       int64_t move_stride, double raa0, double albefa, double asyincr,
+                       double asydecr, double* alfa, double* beta, double* p0, double* q0,
+                       double* P, double* Q, double* b, double* work, void* stream);
+int64_t sttopt_mma_work_count(int32_t m, int32_t n);
+/* Offsets (in doubles, into work) of x, xsi, eta (n each), y, lam, mu, s (m each), the
+ * scalars z, zet, and the residual pair (norm, max): offsets[0..9]. */
+int sttopt_mma_state_offsets(int32_t m, int32_t n, int64_t* offsets);
+int sttopt_mma_subsolv(int32_t m, int32_t n, const double* low, const double* upp,
+                       const double* alfa, const double* beta, const double* p0,
+                       const double* q0, const double* P, const double* Q, double a0,
+                       const double* a, const double* b, const double* c, const double* d,
+                       double* work, int32_t op, double epsi, int32_t jj, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

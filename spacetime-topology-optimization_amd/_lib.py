@@ -198,6 +198,20 @@ _SIGNATURES = {
         [c_int64, _P, _P, _P, c_int32, _P, c_int64, _P, c_int32, _P, c_int64, _P],
     ),
     "sttopt_heaviside": (c_int32, [c_int64, c_double, c_double, c_int32, _P, _P, _P, _P]),
+    "sttopt_mma_prepare_work_count": (c_int64, [c_int32, c_int32]),
+    "sttopt_mma_prepare": (
+        c_int32,
+        [c_int32, c_int32, c_int32, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, _P, c_int64, _P,
+         c_int64, _P, c_int64, _P, c_int64, c_double, c_double, c_double, c_double, _P, _P,
+         _P, _P, _P, _P, _P, _P, _P],
+    ),
+    "sttopt_mma_work_count": (c_int64, [c_int32, c_int32]),
+    "sttopt_mma_state_offsets": (c_int32, [c_int32, c_int32, _P]),
+    "sttopt_mma_subsolv": (
+        c_int32,
+        [c_int32, c_int32, _P, _P, _P, _P, _P, _P, _P, _P, c_double, _P, _P, _P, _P, _P,
+         c_int32, c_double, c_int32, _P],
+    ),
     "sttopt_stage_epilogue": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,

@@ -426,3 +426,84 @@ def test_heaviside_oracle_closed_forms():
         np.testing.assert_allclose(filters_ref.heaviside_derivative(x, beta, eta), fd, rtol=1e-6, atol=1e-8)
     np.testing.assert_allclose(filters_ref.heaviside(x, 1e-4, 0.5), x, atol=1e-8)
     assert filters_ref.heaviside(np.array([0.29, 0.3, 0.31]), math.inf, 0.3).tolist() == [0.0, 0.5, 1.0]
+
+
+# ------------------------------------------------------------------- MMA ----
+
+
+def _mma_problem(m, n, seed):
+    rng = np.random.default_rng(seed)
+    return dict(low=np.full(n, -1.0), upp=np.full(n, 1.0), alfa=np.full(n, -0.9),
+                beta=np.full(n, 0.9), p0=rng.uniform(0.5, 2.0, n), q0=rng.uniform(0.5, 2.0, n),
+                P=rng.uniform(0.1, 1.0, (m, n)), Q=rng.uniform(0.1, 1.0, (m, n)), a0=1.0,
+                a=rng.uniform(0.1, 0.3, m), c=np.full(m, 10.0), d=rng.uniform(0.25, 1.0, m))
+
+
+@pytest.mark.parametrize("m, n", [(3, 8), (3, 2), (5, 40)])
+def test_mma_subsolv_oracle_satisfies_kkt(m, n):
+    """The barrier subproblem's KKT conditions at the returned point (epsi = 1e-6, the
+    last barrier level), with binding constraints (b below each row's value at x = 0)."""
+    from oracle import mma_ref
+
+    pr = _mma_problem(m, n, m * 10 + n)
+    pr["b"] = 0.8 * (pr["P"].sum(1) + pr["Q"].sum(1))
+    sol = mma_ref.subsolv(m, n, 1e-7, pr["low"], pr["upp"], pr["alfa"], pr["beta"], pr["p0"],
+                          pr["q0"], pr["P"], pr["Q"], pr["a0"], pr["a"], pr["b"], pr["c"], pr["d"])
+    stat, comp = mma_ref.kkt_residuals(sol, **{k: pr[k] for k in ("low", "upp", "alfa", "beta", "p0", "q0",
+                                                                  "P", "Q", "a0", "a", "b", "c", "d")})
+    assert stat < 1e-5
+    for prod in comp:
+        assert np.all(np.abs(prod - 1e-6) < 1e-5)
+    x, lam = sol[0], sol[3]
+    assert np.all(x > pr["alfa"]) and np.all(x < pr["beta"]) and lam.max() > 1e-3
+
+
+def test_mma_oracle_asymptote_update_golden_case():
+    """The reference's hand-picked asymptote case (tests/test_mma.py:62-117): oscillating,
+    monotone and stationary variables and each of the four clamps."""
+    from oracle import mma_ref
+
+    n, m = 6, 1
+    xval = np.full(n, 5.0)
+    res = mma_ref.mmasub(m, n, 3, xval, np.zeros(n), np.full(n, 10.0),
+                         np.array([4.0, 4.0, 5.0, 4.0, 4.0, 4.0]), np.array([3.0, 5.0, 2.0, 3.0, 3.0, 4.0]),
+                         np.zeros(n), np.zeros(m), np.zeros((m, n)),
+                         np.array([2.0, 2.0, 2.0, -1000.0, 2.0, 4.8]),
+                         np.array([8.0, 8.0, 8.0, 8.0, 1000.0, 4.0]), 1.0, np.zeros(m),
+                         np.full(m, 2500.0), np.zeros(m))
+    np.testing.assert_allclose(res[-2], [2.6, 3.6, 2.0, -95.0, 2.6, 4.9], atol=1e-12)
+    np.testing.assert_allclose(res[-1], [9.8, 7.8, 8.0, 9.8, 105.0, 5.1], atol=1e-12)
+
+
+def _mma_toy(mmasub, beam: bool):
+    n, m = (5, 1) if beam else (2, 1)
+    coef = np.array([61.0, 37.0, 19.0, 7.0, 1.0])
+    if beam:
+        obj = lambda x: (0.0624 * x.sum(), np.full(n, 0.0624))  # noqa: E731
+        con = lambda x: (np.array([np.sum(coef / x**3) - 1.0]), (-3 * coef / x**4)[None, :])  # noqa: E731
+        xmin, xmax, x, iters = np.full(n, 0.1), np.full(n, 10.0), np.full(n, 5.0), 20
+    else:
+        obj = lambda x: (np.sum((x - 4) ** 2), 2 * (x - 4))  # noqa: E731
+        con = lambda x: (np.array([x.sum() - 2.0]), np.ones((1, n)))  # noqa: E731
+        xmin, xmax, x, iters = np.full(n, -10.0), np.full(n, 10.0), np.zeros(n), 10
+    xold1, xold2 = x.copy(), x.copy()
+    low, upp = np.zeros(n), np.zeros(n)
+    for it in range(iters):
+        _, g0 = obj(x)
+        fv, g = con(x)
+        res = mmasub(m, n, it, x, xmin, xmax, xold1, xold2, g0, fv, g, low, upp, 1.0,
+                     np.zeros(m), np.array([1000.0]), np.zeros(m))
+        xold2, xold1 = xold1, x
+        x, low, upp = np.asarray(res[0]), np.asarray(res[-2]), np.asarray(res[-1])
+    return x
+
+
+def test_mma_oracle_converges_to_closed_form_optima():
+    """Svanberg's cantilever beam (x* proportional to coef^(1/4), the constraint active)
+    and min |x - (4, 4)|^2 s.t. x1 + x2 <= 2 (x* = (1, 1))."""
+    from oracle import mma_ref
+
+    coef = np.array([61.0, 37.0, 19.0, 7.0, 1.0])
+    root4 = coef**0.25
+    np.testing.assert_allclose(_mma_toy(mma_ref.mmasub, True), np.sum(root4) ** (1 / 3) * root4, rtol=1e-4)
+    np.testing.assert_allclose(_mma_toy(mma_ref.mmasub, False), [1.0, 1.0], atol=1e-6)
