@@ -205,7 +205,9 @@ def kernels_only(torch, _lib, p, rows, dev):
 def next_rows(torch, _lib, nelx, nely, rows, dev, launches=50, rmin=3.0):
     """HIP-event timings of the kernels around the solve on `rows` fields of the
     nelx x nely grid: the density filter (fp64, radius rmin, scaled by Hs), the
-    Heaviside projection, and the scalar mesh's direction field and divergence load.
+    Heaviside projection, and the scalar mesh's direction field and divergence load;
+    and the wall clock of one MMA step (mmasub) with rows constraints over the
+    2 nel space-time design variables.
     Algorithmic bytes per launch: filter 8 (x + Hs + y) per element, projection
     8 (x + y), direction field 8 per node + 64 per element, load 8 (w) + 64 (X) per
     element + 8 per node."""
@@ -244,6 +246,31 @@ def next_rows(torch, _lib, nelx, nely, rows, dev, launches=50, rmin=3.0):
             rows * (nel * 72 + nn * 8)),
     }
     out = {}
+    # one MMA step on the space-time design (x and t: n = 2 nel variables, m = rows
+    # constraints); wall clock of mmasub, whose Newton and line-search loops read two
+    # numbers back per candidate
+    from sttopt_amd import mma
+
+    n, m = 2 * nel, rows
+    xv = rnd(n) * 0.8 + 0.1
+    g0, gv, G = rnd(n) - 0.5, rnd(m) - 0.5, (rnd(m, n) - 0.5) * 0.01
+    move = torch.full_like(xv, 0.05)
+
+    def mma_step(stats):
+        return mma.unit_box_step(mma.History.initial(xv), 0, xv, move, 0.0, g0, gv, G,
+                                 a0=1.0, c=1000.0, raa0=1e-5, stats=stats)
+
+    mma_step({})
+    torch.cuda.synchronize()
+    stats = {}
+    t0 = time.perf_counter()
+    for _ in range(3):
+        mma_step(stats)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / 3
+    out["mma_unit_box_step"] = {"m": m, "n": n, "wall_ms": ms,
+                                "newton_steps": stats.get("subsolv_newton_steps", 0) / 3,
+                                "cap_hits": stats.get("subsolv_cap_hits", 0)}
     for name, (fn, nbytes) in calls.items():
         for _ in range(3):
             _lib.check(fn(), name)

@@ -113,6 +113,7 @@ __global__ __launch_bounds__(kCol) void mma_col_kernel(MmaDev D, double epsi, in
     } else if constexpr (MODE == kColPass1) {
       const double x = D.x[j], xsi = D.xsi[j], eta = D.eta[j];
       double pl = 0.0, ql = 0.0;
+#pragma unroll 8
       for (int i = 0; i < m; ++i) {
         pl += D.P[(long long)i * n + j] * sv[i];
         ql += D.Q[(long long)i * n + j] * sv[i];
@@ -140,6 +141,7 @@ __global__ __launch_bounds__(kCol) void mma_col_kernel(MmaDev D, double epsi, in
     } else if constexpr (MODE == kColStep || MODE == kColStepGivenDx) {
       const double x = D.x[j], xsi = D.xsi[j], eta = D.eta[j];
       double pd = 0.0, qd = 0.0;
+#pragma unroll 8
       for (int i = 0; i < m; ++i) {
         pd += D.P[(long long)i * n + j] * sv[i];
         qd += D.Q[(long long)i * n + j] * sv[i];
@@ -238,28 +240,58 @@ __global__ __launch_bounds__(256) void mma_gemv_kernel(int m, int n, const doubl
   }
 }
 
+// out_v[i] = sum over the chunks of part[v][i][*] (one workgroup per (i, v))
+__global__ __launch_bounds__(256) void mma_rows_reduce_kernel(int m, int nchunk,
+                                                              const double* __restrict__ part,
+                                                              double* __restrict__ out0,
+                                                              double* __restrict__ out1) {
+  __shared__ double red[256 / kWave];
+  const int i = blockIdx.x, v = blockIdx.y;
+  double s = 0.0;
+  for (int ch = threadIdx.x; ch < nchunk; ch += blockDim.x)
+    s += part[((long long)v * m + i) * nchunk + ch];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) (v == 0 ? out0 : out1)[i] = s;
+}
+
 // ------------------------------------------------------------------- Gram ----
 
-constexpr int kGJ = 32;  // columns per LDS stage
+constexpr int kGJ = 64;  // columns per LDS stage
+
+// Dynamic LDS of the Gram kernel (doubles): the two stage arrays [m][kGJ + 1], reused
+// afterwards for the column groups' partial blocks [256][17].
+__host__ __device__ inline int gram_lds_doubles(int m) {
+  const int stage = 2 * m * (kGJ + 1), red = 256 * 17;
+  return stage > red ? stage : red;
+}
 
 // part[chunk][i][k] (i >= k, 4 x 4 blocks) = sum over the chunk's columns of
-// GG_ij * (GG_kj / diagx_j), GG = P diag(uxinv2) - Q diag(xlinv2)
+// GG_ij * (GG_kj / diagx_j), GG = P diag(uxinv2) - Q diag(xlinv2).  With nblk <= 256
+// lower-triangle blocks, the block's threads form 256 / nblk column groups (thread t:
+// block t % nblk, group t / nblk) whose partial blocks are summed through LDS at the
+// end; with more blocks each thread takes up to 3 of them over all columns.
 __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
-  __shared__ double sA[kMmaMaxM + 2][kGJ + 1];
-  __shared__ double sB[kMmaMaxM + 2][kGJ + 1];
+  extern __shared__ double gsm[];
   const int m = D.m, n = D.n;
+  double* sA = gsm;                   // [m][kGJ + 1]
+  double* sB = gsm + m * (kGJ + 1);   // [m][kGJ + 1]
+  double* sR = gsm;                   // [256][17], after the last stage
   const int ch = blockIdx.x;
   const long long per = (n + D.gchunk - 1) / D.gchunk;
   const long long j0 = ch * per, j1 = j0 + per < n ? j0 + per : n;
   const int nb = (m + 3) / 4, nblk = nb * (nb + 1) / 2;
+  const int groups = nblk <= 256 ? 256 / nblk : 1;
+  const int grp = nblk <= 256 ? (int)threadIdx.x / nblk : 0;
   constexpr int kMaxBlk = 3;  // blocks per thread: 528 for m = 126 over 256 threads
   double acc[kMaxBlk][4][4];
   int bi[kMaxBlk], bk[kMaxBlk];
 #pragma unroll
   for (int q = 0; q < kMaxBlk; ++q) {
-    int blk = threadIdx.x + q * 256, I = 0;
+    const int blk = nblk <= 256 ? (q == 0 && grp < groups ? (int)threadIdx.x % nblk : -1)
+                                : (int)threadIdx.x + q * 256;
+    int I = 0;
     bi[q] = bk[q] = -1;
-    if (blk < nblk) {
+    if (blk >= 0 && blk < nblk) {
       while ((I + 1) * (I + 2) / 2 <= blk) ++I;
       bi[q] = I;
       bk[q] = blk - I * (I + 1) / 2;
@@ -269,6 +301,7 @@ __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
 #pragma unroll
       for (int c = 0; c < 4; ++c) acc[q][r][c] = 0.0;
   }
+  constexpr int kLdS = kGJ + 1;
   for (long long jb = j0; jb < j1; jb += kGJ) {
     __syncthreads();
     for (int e = threadIdx.x; e < m * kGJ; e += blockDim.x) {
@@ -279,20 +312,20 @@ __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
         g = D.P[(long long)i * n + j] * D.uxinv2[j] - D.Q[(long long)i * n + j] * D.xlinv2[j];
         gs = g * (1.0 / D.diagx[j]);
       }
-      sA[i][jj] = g;
-      sB[i][jj] = gs;
+      sA[i * kLdS + jj] = g;
+      sB[i * kLdS + jj] = gs;
     }
     __syncthreads();
 #pragma unroll
     for (int q = 0; q < kMaxBlk; ++q) {
       if (bi[q] < 0) continue;
       const int i0 = 4 * bi[q], k0 = 4 * bk[q];
-      for (int jj = 0; jj < kGJ; ++jj) {
+      for (int jj = grp; jj < kGJ; jj += groups) {
         double ga[4], gb[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          ga[r] = i0 + r < m ? sA[i0 + r][jj] : 0.0;
-          gb[r] = k0 + r < m ? sB[k0 + r][jj] : 0.0;
+          ga[r] = i0 + r < m ? sA[(i0 + r) * kLdS + jj] : 0.0;
+          gb[r] = k0 + r < m ? sB[(k0 + r) * kLdS + jj] : 0.0;
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r)
@@ -302,6 +335,29 @@ __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
     }
   }
   double* out = D.grampart + (long long)ch * m * m;
+  if (groups > 1) {
+    // sum the column groups' partial blocks: group g's thread for block b is g nblk + b
+    __syncthreads();
+    if (bi[0] >= 0) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) sR[threadIdx.x * 17 + r * 4 + c] = acc[0][r][c];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < nblk) {
+      const int i0 = 4 * bi[0], k0 = 4 * bk[0];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          double v = 0.0;
+          for (int g = 0; g < groups; ++g) v += sR[(g * nblk + threadIdx.x) * 17 + r * 4 + c];
+          if (i0 + r < m && k0 + c < m) out[(i0 + r) * m + (k0 + c)] = v;
+        }
+    }
+    return;
+  }
 #pragma unroll
   for (int q = 0; q < kMaxBlk; ++q) {
     if (bi[q] < 0) continue;
@@ -314,28 +370,24 @@ __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
   }
 }
 
-// alam[i][k] = sum over chunks (every entry with i >= k - 3 is written by the Gram
-// kernel's blocks; only i >= k is used)
+// alam[i][k] = sum over chunks, one workgroup per entry with i >= k (every such entry is
+// written by the Gram kernel's blocks)
 __global__ __launch_bounds__(256) void mma_gram_reduce_kernel(MmaDev D) {
+  __shared__ double red[256 / kWave];
   const int m = D.m;
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= m * m) return;
+  const int e = blockIdx.x;
   const int i = e / m, k = e - i * m;
-  if (i < k) return;
+  if (i < k) return;  // uniform per block
   double s = 0.0;
-  for (int ch = 0; ch < D.gchunk; ++ch) s += D.grampart[(long long)ch * m * m + e];
-  D.alam[e] = s;
+  for (int ch = threadIdx.x; ch < D.gchunk; ch += blockDim.x)
+    s += D.grampart[(long long)ch * m * m + e];
+  s = block_sum(s, red);
+  if (threadIdx.x == 0) D.alam[e] = s;
 }
 
 // ----------------------------------------------------------- small (1 WG) ----
 
 enum SmallMode { kSmInit = 0, kSmResid = 1, kSmSolve = 2, kSmSteg = 3, kSmApply = 4 };
-
-__device__ double reduce_rows(const MmaDev& D, int v, int i) {
-  double s = 0.0;
-  for (int ch = 0; ch < D.nchunk; ++ch) s += D.rowpart[((long long)v * D.m + i) * D.nchunk + ch];
-  return s;
-}
 
 // In-place Gaussian elimination with partial pivoting of the N x N system in LDS
 // (row stride ld), right-hand side rhs; the solution replaces rhs.
@@ -435,8 +487,7 @@ __global__ __launch_bounds__(256) void mma_small_kernel(MmaDev D, double epsi, i
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
       const double y = D.y[i] + t * D.dy[i], lam = D.lam[i] + t * D.dlam[i];
       const double mu = D.mu[i] + t * D.dmu[i], s = D.s[i] + t * D.ds[i];
-      const double g = reduce_rows(D, 0, i);
-      D.gvec[i] = g;
+      const double g = D.gvec[i];
       const double rey = D.c[i] + D.d[i] * y - mu - lam;
       const double relam = g - D.a[i] * z - y + s - D.b[i];
       const double remu = mu * y - epsi;
@@ -474,7 +525,7 @@ __global__ __launch_bounds__(256) void mma_small_kernel(MmaDev D, double epsi, i
     double alam = 0.0;
     for (int i = threadIdx.x; i < m; i += blockDim.x) {
       const double y = D.y[i], lam = D.lam[i];
-      const double g = reduce_rows(D, 0, i);
+      const double g = D.gvec[i];
       const double dely = D.c[i] + D.d[i] * y - lam - epsi / y;
       const double dellam = g - D.a[i] * z - y - D.b[i] + epsi / lam;
       const double diagy = D.d[i] + D.mu[i] / y;
@@ -499,7 +550,7 @@ __global__ __launch_bounds__(256) void mma_small_kernel(MmaDev D, double epsi, i
       for (int i = threadIdx.x; i < m; i += blockDim.x) {
         sAA[i * kLd + m] = D.a[i];
         sAA[m * kLd + i] = D.a[i];
-        sb[i] = sdellam[i] + sdely[i] / sdiagy[i] - reduce_rows(D, 1, i);
+        sb[i] = sdellam[i] + sdely[i] / sdiagy[i] - D.ggd[i];
       }
       if (threadIdx.x == 0) {
         sAA[m * kLd + m] = -zet / z;
@@ -532,7 +583,7 @@ __global__ __launch_bounds__(256) void mma_small_kernel(MmaDev D, double epsi, i
           const double diagy = D.d[i] + D.mu[i] / y;
           const double diaglamyi = D.s[i] / lam + 1.0 / diagy;
           const double dely = D.c[i] + D.d[i] * y - lam - epsi / y;
-          const double dellam = reduce_rows(D, 0, i) - D.a[i] * z - y - D.b[i] + epsi / lam;
+          const double dellam = D.gvec[i] - D.a[i] * z - y - D.b[i] + epsi / lam;
           const double dellamyi = dellam + dely / diagy;
           if (r < n) {
             const double g = D.P[(long long)i * n + r] * D.uxinv2[r] - D.Q[(long long)i * n + r] * D.xlinv2[r];
@@ -561,7 +612,7 @@ __global__ __launch_bounds__(256) void mma_small_kernel(MmaDev D, double epsi, i
         const double diagy = D.d[i] + D.mu[i] / y;
         const double diaglamyi = D.s[i] / lam + 1.0 / diagy;
         const double dely = D.c[i] + D.d[i] * y - lam - epsi / y;
-        const double dellam = reduce_rows(D, 0, i) - D.a[i] * z - y - D.b[i] + epsi / lam;
+        const double dellam = D.gvec[i] - D.a[i] * z - y - D.b[i] + epsi / lam;
         const double dellamyi = dellam + dely / diagy;
         double ggdx = 0.0;
         for (int j = 0; j < n; ++j) {
@@ -710,8 +761,8 @@ using namespace stt;
 // The state and scratch layout of the subsolv workspace.
 static void carve_mma(MmaDev& D, double* work) {
   const int m = D.m, n = D.n;
-  D.nchunk = n < 4096 ? 1 : (n / 4096 < 64 ? n / 4096 : 64);
-  D.gchunk = (n + 2047) / 2048 < 512 ? (n + 2047) / 2048 : 512;
+  D.nchunk = n < 4096 ? 1 : (n / 4096 < 256 ? n / 4096 : 256);
+  D.gchunk = (n + 1023) / 1024 < 1024 ? (n + 1023) / 1024 : 1024;
   D.colblk = col_blocks(n);
   double* w = work;
   auto take = [&](long long k) {
@@ -825,6 +876,8 @@ int sttopt_mma_subsolv(int32_t m, int32_t n, const double* low, const double* up
                          jj < 0 ? -1 : jj);
       hipLaunchKernelGGL(mma_gemv_kernel, gv, dim3(256), 0, s, m, n, P, Q, D.uxc, D.xlc,
                          nullptr, nullptr, 1, D.rowpart, D.nchunk);
+      hipLaunchKernelGGL(mma_rows_reduce_kernel, dim3(m, 1), dim3(256), 0, s, m, D.nchunk,
+                         D.rowpart, D.gvec, D.ggd);
       hipLaunchKernelGGL(mma_small_kernel<kSmResid>, dim3(1), dim3(256), 0, s, D, epsi,
                          jj < 0 ? -1 : jj, mlt);
       break;
@@ -832,9 +885,17 @@ int sttopt_mma_subsolv(int32_t m, int32_t n, const double* low, const double* up
       hipLaunchKernelGGL(mma_col_kernel<kColPass1>, dim3(cb), dim3(kCol), 0, s, D, epsi, 0);
       hipLaunchKernelGGL(mma_gemv_kernel, gv, dim3(256), 0, s, m, n, P, Q, D.uxinv1, D.xlinv1,
                          D.wA, D.wB, 2, D.rowpart, D.nchunk);
+      hipLaunchKernelGGL(mma_rows_reduce_kernel, dim3(m, 2), dim3(256), 0, s, m, D.nchunk,
+                         D.rowpart, D.gvec, D.ggd);
       if (mlt) {
-        hipLaunchKernelGGL(mma_gram_kernel, dim3(D.gchunk), dim3(256), 0, s, D);
-        hipLaunchKernelGGL(mma_gram_reduce_kernel, dim3((m * m + 255) / 256), dim3(256), 0, s, D);
+        const size_t lds = sizeof(double) * gram_lds_doubles(m);
+        if (lds > 65536) {
+          hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(mma_gram_kernel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+          if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(mma_gram_kernel)");
+        }
+        hipLaunchKernelGGL(mma_gram_kernel, dim3(D.gchunk), dim3(256), lds, s, D);
+        hipLaunchKernelGGL(mma_gram_reduce_kernel, dim3(m * m), dim3(256), 0, s, D);
       }
       hipLaunchKernelGGL(mma_small_kernel<kSmSolve>, dim3(1), dim3(256), 0, s, D, epsi, 0, mlt);
       if (mlt)
@@ -874,7 +935,7 @@ int sttopt_mma_prepare(int32_t m, int32_t n, int32_t iteration, const double* xv
              work, work + n};
   hipStream_t s = (hipStream_t)stream;
   hipLaunchKernelGGL(mma_prep_kernel, dim3(col_blocks(n)), dim3(256), 0, s, A);
-  const int nchunk = n < 4096 ? 1 : (n / 4096 < 64 ? n / 4096 : 64);
+  const int nchunk = n < 4096 ? 1 : (n / 4096 < 256 ? n / 4096 : 256);
   double* part = work + 2LL * n;
   hipLaunchKernelGGL(mma_gemv_kernel, dim3(nchunk, m), dim3(256), 0, s, m, n, P, Q, work,
                      work + n, nullptr, nullptr, 1, part, nchunk);
@@ -885,7 +946,7 @@ int sttopt_mma_prepare(int32_t m, int32_t n, int32_t iteration, const double* xv
 
 int64_t sttopt_mma_prepare_work_count(int32_t m, int32_t n) {
   if (m <= 0 || n <= 0) return 0;
-  const long long nchunk = n < 4096 ? 1 : (n / 4096 < 64 ? n / 4096 : 64);
+  const long long nchunk = n < 4096 ? 1 : (n / 4096 < 256 ? n / 4096 : 256);
   return 2LL * n + (long long)m * nchunk;
 }
 

@@ -182,6 +182,8 @@ def subsolv(m: int, n: int, epsimin: float, low: Tensor, upp: Tensor, alfa: Tens
                     break
             op(3, epsi, jj)
             residunorm, residumax = resinew, resimax
+        if stats is not None:
+            stats["subsolv_newton_steps"] = stats.get("subsolv_newton_steps", 0) + ittt
         if ittt > 198:
             if stats is not None:
                 stats["subsolv_cap_hits"] = stats.get("subsolv_cap_hits", 0) + 1

@@ -32,7 +32,8 @@ def _t(a, dev):
     return torch.as_tensor(np.asarray(a), dtype=torch.float64, device=dev)
 
 
-@pytest.mark.parametrize("m, n", [(3, 8), (3, 2), (5, 40), (17, 5000), (40, 70000)])
+@pytest.mark.parametrize("m, n", [(3, 8), (3, 2), (5, 40), (17, 5000), (40, 70000), (100, 3000),
+                                  (20, 12)])
 def test_subsolv_matches_oracle_and_satisfies_kkt(dev, m, n):
     from sttopt_amd import mma
 
