@@ -58,6 +58,12 @@ extern "C" {
 /* operator modes for sttopt_apply */
 #define STTOPT_MODE_MATVEC 0   /* y = K u (no projection)                 */
 #define STTOPT_MODE_OPERATOR 1 /* y = P (K P u + rdiag * P u)              */
+#define STTOPT_MODE_OPERATOR_PROJECTED 2 /* y = P (K u + rdiag * u) for a u that is
+                                            already 0 at fixed dofs (a PCG direction):
+                                            the outer iteration's apply.  No
+                                            counterpart in the reference, whose pcg calls
+                                            operator() (the input projection is then a
+                                            no-op). */
 
 const char* sttopt_last_error(void);
 int sttopt_version(void);
@@ -305,6 +311,14 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage,
                           const double* g, double* comp, double* dx, double* dt,
                           double* scratch, void* stream);
 int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage);
+
+/* HIP-event timing of the outer PCG's K(rho)p launches inside sttopt_mgpcg (off by
+ * default; process-wide, not thread-safe).  enable resets the totals; read returns the
+ * summed launch durations, the algorithmic bytes (8 (2 ndof + nel) per row still active
+ * in each launch's iteration) and the launch count.  Measurement only: no counterpart
+ * in the reference. */
+int sttopt_profile_outer_apply(int32_t enable);
+int sttopt_profile_outer_apply_read(double* seconds, double* bytes, int64_t* launches);
 
 /* ---- Gauss-point gradient fields of the scalar (d = 1) mesh ------------------------
  * Element fields X are [rows][nel][4][2] doubles (Gauss point g = 2 * eta_index +

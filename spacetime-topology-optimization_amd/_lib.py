@@ -20,7 +20,7 @@ OK, EINVAL, EHIP, ENOCONV, ENOTSPD = 0, -1, -2, -3, -4
 KEFF_FULL, KEFF_UPPER = 0, 1
 SEPARATE_SWEEPS = 1  # sttopt_level.flags of level 0
 F64, F32 = 0, 1
-MODE_MATVEC, MODE_OPERATOR = 0, 1
+MODE_MATVEC, MODE_OPERATOR, MODE_OPERATOR_PROJECTED = 0, 1, 2
 MAX_LEVELS = 16
 
 
@@ -172,6 +172,8 @@ _SIGNATURES = {
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,
          c_double, _P, _P, _P, _P],
     ),
+    "sttopt_profile_outer_apply": (c_int32, [c_int32]),
+    "sttopt_profile_outer_apply_read": (c_int32, [_P, _P, _P]),
     "sttopt_gauss_gradients": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, c_int64, _P, c_int64, c_double, c_int32, _P,

@@ -74,8 +74,8 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
   constexpr int K = 4 * D;
   __shared__ double red[kBlock / kWave];
   const int t = xcd_remap(blockIdx.x, a.total);
-  const int row = t / a.tiles;
-  const int tt = t - row * a.tiles;
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
   const int by = tt / a.ntx, bx = tt - by * a.ntx;
   const int c = bx * kTileX + (threadIdx.x & (kTileX - 1));
@@ -256,8 +256,8 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   __shared__ Vec<T, D> sBc[kFirst ? kTileY : 1][kFirst ? kTileX : 1];
   __shared__ Vec<T, D> sIc[kFirst ? kTileY : 1][kFirst ? kTileX : 1];
   const int t = xcd_remap(blockIdx.x, a.total);
-  const int row = t / a.tiles;
-  const int tt = t - row * a.tiles;
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
   const int by = tt / a.ntx, bx = tt - by * a.ntx;
   const int x0 = bx * kTileX, y0 = by * kTileY;
@@ -1167,21 +1167,24 @@ int sttopt_apply(int32_t dtype, int32_t mode, int32_t d, int32_t nelx, int32_t n
                  void* stream) {
   if (int st = check_grid(d, nelx, nely, rows, "sttopt_apply")) return st;
   if (u == nullptr || density == nullptr || ke == nullptr || y == nullptr ||
-      (mode != STTOPT_MODE_MATVEC && mode != STTOPT_MODE_OPERATOR)) {
+      (mode != STTOPT_MODE_MATVEC && mode != STTOPT_MODE_OPERATOR &&
+       mode != STTOPT_MODE_OPERATOR_PROJECTED)) {
     set_error("sttopt_apply: null pointer or bad mode");
     return STTOPT_EINVAL;
   }
+  const bool opmode = mode != STTOPT_MODE_MATVEC;
   OpDesc op;
   op.d = d;
   op.nelx = nelx;
   op.nely = nely;
   op.rho = density;
   op.sr = density_stride;
-  op.rdiag = mode == STTOPT_MODE_OPERATOR ? rdiag : nullptr;
+  op.rdiag = opmode ? rdiag : nullptr;
   op.srd = rdiag_stride;
-  op.mask = mode == STTOPT_MODE_OPERATOR ? mask : nullptr;
+  op.mask = opmode ? mask : nullptr;
   for (int i = 0; i < 16 * d * d; ++i) op.ke[i] = ke[i];
-  const int m = mode == STTOPT_MODE_MATVEC ? kMatvec : kOperator;
+  const int m = opmode ? kOperator : kMatvec;
+  // the input projection is skipped when the caller guarantees a projected u
   const bool proj = mode == STTOPT_MODE_OPERATOR && mask != nullptr;
   hipStream_t s = (hipStream_t)stream;
   if (dtype == STTOPT_F64)

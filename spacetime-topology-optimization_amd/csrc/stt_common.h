@@ -59,6 +59,15 @@ __device__ __forceinline__ int xcd_remap(int L, int total) {
   return xcd * q + min(xcd, r) + local;
 }
 
+// Row and tile of (remapped) launch index t for launches over rows x tiles, tile-major:
+// the rows of one tile are consecutive, so each XCD's band covers every row of a range
+// of tiles, and rows that stop early (converged) leave the XCDs evenly loaded.
+__device__ __forceinline__ void row_tile(int t, int tiles, int total, int& row, int& tt) {
+  const int rows = total / tiles;
+  tt = t / rows;
+  row = t - tt * rows;
+}
+
 // The D dofs of a node, moved as one D-vector (D = 2: 8- or 16-byte aligned pairs).
 template <typename T, int D>
 struct Vec {

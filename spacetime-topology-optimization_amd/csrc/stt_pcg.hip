@@ -78,8 +78,7 @@ struct TileGeom {
 __device__ __forceinline__ void tile_origin(const TileGeom& g, int& row, int& tt, int& c,
                                            int& r0) {
   const int t = xcd_remap(blockIdx.x, g.total);
-  row = t / g.tiles;
-  tt = t - row * g.tiles;
+  row_tile(t, g.tiles, g.total, row, tt);
   const int by = tt / g.ntx, bx = tt - by * g.ntx;
   c = bx * kTileX + (threadIdx.x & (kTileX - 1));
   r0 = by * kTileY + (threadIdx.x / kTileX) * kRowsPerThread;
@@ -626,6 +625,28 @@ int read_device_int(const int* dev, hipStream_t s, int* out) {
   return STTOPT_OK;
 }
 
+// Optional HIP-event timing of the outer iteration's K(rho)p launches (process-wide, off
+// by default): one event pair per launch, resolved after the solve's final readback.
+// Bytes per launch: 8 (2 ndof + nel) per row still active in that iteration.
+struct ApplyProfile {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> iter_of;  // iteration of event pair i (events 2i, 2i + 1)
+  double seconds = 0.0, bytes = 0.0;
+  long long launches = 0;
+};
+static ApplyProfile g_apply_prof;
+
+static int prof_events(size_t pairs) {
+  while (g_apply_prof.pool.size() < 2 * pairs) {
+    hipEvent_t e;
+    hipError_t st = hipEventCreate(&e);
+    if (st != hipSuccess) return hip_status(st, "hipEventCreate");
+    g_apply_prof.pool.push_back(e);
+  }
+  return STTOPT_OK;
+}
+
 template <typename T>
 static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double* x,
                       long long xs, double rtol, int max_iter, int flexible, int check_every,
@@ -681,10 +702,19 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   bool done = false;
   int k_last = 0;  // iteration whose x += alpha p is still pending
   for (int k = 1; k <= max_iter; ++k) {
+    const size_t pi = g_apply_prof.iter_of.size();
+    if (g_apply_prof.on) {
+      if (int e = prof_events(pi + 1)) return e;
+      hipEventRecord(g_apply_prof.pool[2 * pi], s);
+    }
     if (int e = stencil<double, double, double>(op, rows, kOperator, false, kDotU, p, n,
                                                 (const double*)nullptr, 0, nullptr, 0, Ap, n,
                                                 0.0, sc.active, sc.part[0], s))
       return e;
+    if (g_apply_prof.on) {
+      hipEventRecord(g_apply_prof.pool[2 * pi + 1], s);
+      g_apply_prof.iter_of.push_back(k);
+    }
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
                        sc.rz, sc.alpha);
     STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, r, Ap,
@@ -725,6 +755,20 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
   int mx = 0;
   for (int i = 0; i < rows; ++i) mx = h[i] > mx ? h[i] : mx;
+  if (g_apply_prof.on && !g_apply_prof.iter_of.empty()) {
+    const double per_row = 8.0 * (2.0 * (double)n + (double)op.nel());
+    for (size_t i = 0; i < g_apply_prof.iter_of.size(); ++i) {
+      float ms = 0.0f;
+      e = hipEventElapsedTime(&ms, g_apply_prof.pool[2 * i], g_apply_prof.pool[2 * i + 1]);
+      if (e != hipSuccess) return hip_status(e, "hipEventElapsedTime");
+      int act = 0;
+      for (int r = 0; r < rows; ++r) act += h[r] >= g_apply_prof.iter_of[i] ? 1 : 0;
+      g_apply_prof.seconds += ms * 1e-3;
+      g_apply_prof.bytes += per_row * act;
+      g_apply_prof.launches += 1;
+    }
+    g_apply_prof.iter_of.clear();
+  }
   *n_iter = done ? mx : max_iter;
   if (!done) {
     set_error("sttopt_mgpcg: rows above rtol after max_iter iterations");
@@ -752,6 +796,25 @@ int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {
   if (mg == nullptr || mg->nlevels < 1) return 0;
   const long long n = (long long)(mg->levels[0].nelx + 1) * (mg->levels[0].nely + 1) * mg->d;
   return 4LL * mg->rows * n;
+}
+
+int sttopt_profile_outer_apply(int32_t enable) {
+  g_apply_prof.on = enable != 0;
+  g_apply_prof.iter_of.clear();
+  g_apply_prof.seconds = g_apply_prof.bytes = 0.0;
+  g_apply_prof.launches = 0;
+  return STTOPT_OK;
+}
+
+int sttopt_profile_outer_apply_read(double* seconds, double* bytes, int64_t* launches) {
+  if (seconds == nullptr || bytes == nullptr || launches == nullptr) {
+    set_error("sttopt_profile_outer_apply_read: null pointer");
+    return STTOPT_EINVAL;
+  }
+  *seconds = g_apply_prof.seconds;
+  *bytes = g_apply_prof.bytes;
+  *launches = g_apply_prof.launches;
+  return STTOPT_OK;
 }
 
 int sttopt_vcycle(const sttopt_mg* mg, const double* r, int64_t r_stride, double* z,

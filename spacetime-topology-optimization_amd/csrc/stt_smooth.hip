@@ -135,8 +135,8 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   __shared__ Vec<float, D> sI[POST ? 1 : kTileY][POST ? 1 : kTileX];
   __shared__ double red[kWaves];
   const int t = xcd_remap(blockIdx.x, a.total);
-  const int row = t / a.tiles;
-  const int tt = t - row * a.tiles;
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
   const int by = tt / a.ntx, bx = tt - by * a.ntx;
   // origin of the first stencil's 64 x 16 nodes; the staged halo starts one node
