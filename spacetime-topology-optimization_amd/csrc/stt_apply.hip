@@ -696,8 +696,10 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   const bool p22 = op.pkx == 2 && op.pky == 2;
   const bool param_tile = src == kSrcParam && p22 && sizeof(T) == sizeof(float);
   a.tiles = (src == kSrcDensity || param_tile) ? op.tiles() : op.tiles_small();
-  if (src != kSrcDensity && (dot != kNoDot || mode == kSmoothFirst)) {
-    set_error("stencil: reduction partials and the fused first sweeps are level-0 only");
+  if ((src != kSrcDensity && dot != kNoDot) ||
+      (mode == kSmoothFirst && src != kSrcDensity && !param_tile)) {
+    set_error("stencil: reduction partials are level-0 only, the fused first sweeps need "
+              "the LDS tile (level 0, or level 1 at 2 x 2 in fp32)");
     return STTOPT_EINVAL;
   }
   if (mode == kSmoothFirst && (b == nullptr || invd == nullptr)) {

@@ -394,7 +394,17 @@ struct VCycle {
     T* x = buf(L.vx);
     T* alt = buf(L.vx2);
     const int ns = mg->n_smooth;
-    if (ns >= 1) {
+    // the fp32 level 1 at 2 x 2 runs on the LDS tile: its first two sweeps in one pass
+    const bool tile_first = sizeof(T) == sizeof(float) && ns >= 2 && op[l].ptab != nullptr &&
+                            op[l].pkx == 2 && op[l].pky == 2 &&
+                            (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0;
+    if (tile_first) {
+      if (int e = stencil<T, T, T>(op[l], rows, kSmoothFirst, false, kNoDot,
+                                   static_cast<const T*>(nullptr), 0, b, nl, invd(l), sd(l), x,
+                                   nl, mg->omega, active, nullptr, s))
+        return e;
+      if (int e = sweeps(l, ns - 2, x, alt, b, nl)) return e;
+    } else if (ns >= 1) {
       if (int e = jacobi_first<T, T>(nl, rows, b, nl, invd(l), sd(l), mg->omega, L.mask, x,
                                      nl, nullptr, 0, active, s))
         return e;
