@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only a plain copy of known size and the fp64 / fp32 K(rho)u "
                          "apply (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE), then exit")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="before the warm-up, three plain device copies of known size (the "
+                         "scale of rocprofv3's FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--next-rows", action="store_true",
                     help="time the kernels around the solve (density filter, Heaviside "
                          "projection, heat-path gradient fields and load) on the same grid "
@@ -90,6 +93,8 @@ def main():
         (c + sum(stages)).backward()
         return x.grad, t.grad
 
+    if args.calibration_copy:
+        calibration_copy(torch, p, rows, dev)
     for i in range(args.warmup):
         step()
         print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -186,16 +191,25 @@ def _apply_args(torch, _lib, p, rows, dev, dtype, code, mode=None):
     return args, (u, dens, y, bits), stream
 
 
-def kernels_only(torch, _lib, p, rows, dev):
-    """A plain device copy of known size (the counter calibration: its FETCH_SIZE /
-    WRITE_SIZE against the bytes it moves), then the fp64 and fp32 K(rho)u applies of
-    the outer PCG on all rows: 5 launches each, then 20 timed with HIP events."""
-    lib = _lib.load()
+def calibration_copy(torch, p, rows, dev):
+    """Three plain device copies (torch `copy_`) of rows x (2 ndof + nel) fp64 values:
+    what rocprofv3's FETCH_SIZE / WRITE_SIZE read for them, against the bytes they
+    move, scales the counters of the other kernels to bytes."""
     n = rows * (2 * p.ndof + p.nel)
     src = torch.randn(n, device=dev, dtype=torch.float64)
     dst = torch.empty_like(src)
     for _ in range(3):
         dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"# copy: {n * 8} bytes read, {n * 8} bytes written per launch", file=sys.stderr,
+          flush=True)
+
+
+def kernels_only(torch, _lib, p, rows, dev):
+    """The counter calibration copy, then the fp64 and fp32 K(rho)u applies of the
+    outer PCG on all rows: 5 launches each, then 20 timed with HIP events."""
+    lib = _lib.load()
+    calibration_copy(torch, p, rows, dev)
     for dtype, code in ((torch.float64, _lib.F64), (torch.float32, _lib.F32)):
         args, keep, stream = _apply_args(torch, _lib, p, rows, dev, dtype, code)
         for _ in range(5):
@@ -212,7 +226,6 @@ def kernels_only(torch, _lib, p, rows, dev):
               f"{nbytes / us / 1e3:.0f} GB/s ({nbytes} algorithmic bytes)", flush=True)
         del keep
     torch.cuda.synchronize()
-    print(f"# copy: {n * 8} bytes read, {n * 8} bytes written per launch", flush=True)
 
 
 def next_rows(torch, _lib, nelx, nely, rows, dev, launches=50, rmin=3.0):
@@ -302,10 +315,10 @@ def next_rows(torch, _lib, nelx, nely, rows, dev, launches=50, rmin=3.0):
 
 
 def pmc_traffic():
-    """HBM bytes per launch of the fp64 apply from the committed counter summary:
-    FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each scaled by the ratio of the
-    bytes a plain copy moves to what the same counter reads for it.  None when the
-    summary is absent."""
+    """HBM bytes per launch of the outer PCG's fp64 apply from the committed counter
+    summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over that kernel's
+    dispatches in one bench step), each scaled by the ratio of the bytes a plain copy
+    moves to what the same counter reads for it.  None when the summary is absent."""
     try:
         with open(PMC_SUMMARY) as fh:
             kv = dict(line.split(":", 1) for line in fh if line.startswith("pmc."))
