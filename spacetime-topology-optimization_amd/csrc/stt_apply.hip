@@ -255,6 +255,11 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   constexpr bool kFirst = MODE == kSmoothFirst;
   __shared__ Vec<T, D> sBc[kFirst ? kTileY : 1][kFirst ? kTileX : 1];
   __shared__ Vec<T, D> sIc[kFirst ? kTileY : 1][kFirst ? kTileX : 1];
+  // free-dof words of the output rows: row r's dofs (gy * nnx + x0) D .. + 64 D - 1 span
+  // at most kMW consecutive 32-bit words, the first at (gy * nnx + x0) D / 32
+  constexpr int kMW = kTileX * D / 32 + 1;
+  constexpr bool kOutMask = MODE != kMatvec;
+  __shared__ uint32_t sMW[kOutMask ? kTileY : 1][kOutMask ? kMW : 1];
   const int t = xcd_remap(blockIdx.x, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
@@ -381,11 +386,19 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
     }
     if (tix < kTailU) tu = finish(tb, td, ti, tk, tgy, tgx);
   }
-  // Epilogue operands of this thread's output nodes (mask words, b, D^-1) are loaded
-  // now, before the barrier, so that the epilogue does not wait for them.
+  // Epilogue operands of this thread's output nodes (b, D^-1) are loaded now, before
+  // the barrier, so that the epilogue does not wait for them; the output rows' free-dof
+  // words are loaded by the first kTileY * kMW threads (one word each) and pass through
+  // LDS.
   constexpr bool kSweep = MODE == kSmooth;  // kSmoothFirst reads them from LDS
   constexpr bool kNeedB = kSweep || MODE == kResid;
-  uint32_t ow[NR];
+  const bool has_mask = kOutMask && a.mask != nullptr;
+  uint32_t mw = ~0u;
+  if (has_mask && tix < kTileY * kMW) {
+    const int r = tix / kMW, gy = y0 + r;
+    const long long w = (((long long)gy * a.nnx + x0) * D >> 5) + (tix - r * kMW);
+    if (gy < a.nny && w <= ((long long)a.nny * a.nnx * D - 1) >> 5) mw = a.mask[w];
+  }
   Vec<TB, D> ob[kNeedB ? NR : 1];
   Vec<T, D> oi[kSweep ? NR : 1];
   {
@@ -394,8 +407,6 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
     for (int j = 0; j < NR; ++j) {
       const bool in = gyo + j < a.nny && gxo < a.nnx;
       const long long node = (long long)(gyo + j) * a.nnx + gxo;
-      ow[j] = ~0u;
-      if (MODE != kMatvec && a.mask != nullptr && in) ow[j] = a.mask[(node * D) >> 5];
       if constexpr (kNeedB) {
 #pragma unroll
         for (int c = 0; c < D; ++c) ob[j].v[c] = TB(0);
@@ -446,6 +457,9 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   } else if (tix < kTail) {
     if constexpr (kParam) sRf[ti][tk] = tr2;
     else sR[ti][tk] = tr;
+  }
+  if constexpr (kOutMask) {
+    if (tix < kTileY * kMW) sMW[tix / kMW][tix % kMW] = mw;
   }
   __syncthreads();
 
@@ -603,7 +617,9 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
 #pragma unroll
         for (int ii = 0; ii < D; ++ii) acc[j][ii] += rd.v[ii] * uc.v[ii];
       }
-      const uint32_t word = ow[j];
+      uint32_t word = ~0u;
+      if (has_mask)
+        word = sMW[i0 + j][((node * D) >> 5) - ((((long long)gy * a.nnx + x0) * D) >> 5)];
       Vec<TB, D> bv;
       Vec<T, D> iv;
 #pragma unroll

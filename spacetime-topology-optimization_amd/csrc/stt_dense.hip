@@ -104,18 +104,43 @@ __global__ __launch_bounds__(256) void sweep_pivot_kernel(int n, int k0, int bk,
 
 constexpr int kKC = 16;  // k rows per LDS chunk
 
-// out[x][y] += sum_{k < bk} X(k, 4 ti + x) Y(k, 4 tj + y) for the 64 x 64 tile of a
-// 256-thread block (ti = t / 16, tj = t % 16).  X and Y are k-major (row k at
-// X + k ldx) with xcols / ycols valid columns; they pass through LDS 16 rows at a time
-// (zero past bk and past the valid columns), the next chunk's loads in flight while the
-// current one is multiplied.  Returns after a barrier-free last chunk: callers that reuse
-// sX / sY synchronize first.
+// The 64 x 64 tile of a 256-thread block is computed on the fp64 matrix cores: wave w
+// owns the 32 x 32 quadrant (w / 2, w % 2) as 2 x 2 products of 16 x 16 blocks
+// (v_mfma_f64_16x16x4f64: lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16) of a
+// 16 x 4 by 4 x 16 step and holds D(l / 16 + 4 r, l % 16), r < 4).  Entry q of a
+// thread's 16 results (q = 8 ib + 4 jb + r) is tile element (tile_row(q), tile_col(q)).
+typedef double d4_t __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ int tile_row(int q) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  return 32 * (w >> 1) + 16 * (q >> 3) + (l >> 4) + 4 * (q & 3);
+}
+__device__ __forceinline__ int tile_col(int q) {
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  return 32 * (w & 1) + 16 * ((q >> 2) & 1) + (l & 15);
+}
+
+// out[q] += sum_{k < bk} X(k, tile_row(q)) Y(k, tile_col(q)) for the 64 x 64 tile.  X
+// and Y are k-major (row k at X + k ldx) with xcols / ycols valid columns; they pass
+// through LDS 16 rows at a time (zero past bk and past the valid columns), the next
+// chunk's loads in flight while the current one is multiplied.  Returns after a
+// barrier-free last chunk: callers that reuse sX / sY synchronize first.
 __device__ __forceinline__ void gemm_tile_kmajor(const double* __restrict__ X, long long ldx,
                                                  int xcols, const double* __restrict__ Y,
                                                  long long ldy, int ycols, int bk,
-                                                 double (&out)[4][4], double* sX, double* sY) {
+                                                 double (&out)[16], double* sX, double* sY) {
   constexpr int kPer = kKC * kSB / 256;  // elements of each chunk per thread
-  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int ra = 32 * (w >> 1) + (l & 15), cb = 32 * (w & 1) + (l & 15), kl = l >> 4;
+  d4_t acc[2][2];
+#pragma unroll
+  for (int ib = 0; ib < 2; ++ib) {
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[ib][jb][r] = out[8 * ib + 4 * jb + r];
+    }
+  }
   double nx[kPer], ny[kPer];
   auto fetch = [&](int kc) {
 #pragma unroll
@@ -137,19 +162,24 @@ __device__ __forceinline__ void gemm_tile_kmajor(const double* __restrict__ X, l
     }
     __syncthreads();
     if (kc + kKC < bk) fetch(kc + kKC);
-#pragma unroll 4
-    for (int k = 0; k < kKC; ++k) {
-      const double2 a01 = *reinterpret_cast<const double2*>(sX + k * kSB + 4 * ti);
-      const double2 a23 = *reinterpret_cast<const double2*>(sX + k * kSB + 4 * ti + 2);
-      const double2 b01 = *reinterpret_cast<const double2*>(sY + k * kSB + 4 * tj);
-      const double2 b23 = *reinterpret_cast<const double2*>(sY + k * kSB + 4 * tj + 2);
-      const double ra[4] = {a01.x, a01.y, a23.x, a23.y};
-      const double pb[4] = {b01.x, b01.y, b23.x, b23.y};
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
+    for (int kk = 0; kk < kKC; kk += 4) {
+      const double* xr = sX + (kk + kl) * kSB;
+      const double* yr = sY + (kk + kl) * kSB;
+      const double a0 = xr[ra], a1 = xr[ra + 16];
+      const double b0 = yr[cb], b1 = yr[cb + 16];
+      acc[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc[0][0], 0, 0, 0);
+      acc[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc[0][1], 0, 0, 0);
+      acc[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc[1][0], 0, 0, 0);
+      acc[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc[1][1], 0, 0, 0);
+    }
+  }
 #pragma unroll
-        for (int y = 0; y < 4; ++y) out[x][y] += ra[x] * pb[y];
-      }
+  for (int ib = 0; ib < 2; ++ib) {
+#pragma unroll
+    for (int jb = 0; jb < 2; ++jb) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) out[8 * ib + 4 * jb + r] = acc[ib][jb][r];
     }
   }
 }
@@ -173,18 +203,13 @@ __global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
     const int k = e / kSB, c = e & (kSB - 1);
     if (k < bk && c < bj) Rr[(long long)k * n + c] = Ak[(long long)k * n + c];
   }
-  double out[4][4] = {};
+  double out[16] = {};
   // P(i, j) = sum_k D(i, k) R(k, j); D is symmetric, so its rows serve as X(k, i)
   gemm_tile_kmajor(D + (long long)row * kSB * kSB, kSB, bk, Ak, n, bj, bk, out, sX, sY);
-  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
 #pragma unroll
-  for (int x = 0; x < 4; ++x) {
-    const int i = 4 * ti + x;
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int j = 4 * tj + y;
-      if (i < bk && j < bj) Pr[(long long)i * n + j] = out[x][y];
-    }
+  for (int q = 0; q < 16; ++q) {
+    const int i = tile_row(q), j = tile_col(q);
+    if (i < bk && j < bj) Pr[(long long)i * n + j] = out[q];
   }
 }
 
@@ -224,68 +249,45 @@ __global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0
   const double* Rr = R + (long long)row * kSB * n;
   const double* Pr = P + (long long)row * kSB * n;
   const double* Dr = D + (long long)row * kSB * kSB;
-  const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
-  double out[4][4] = {};
+  double out[16] = {};
   if (I != kb && J != kb) {
     // A_IJ - R_KI^T P_KJ  (R_KI^T = A_IK before this step)
     gemm_tile_kmajor(Rr + i0, n, bi, Pr + j0, n, bj, bk, out, sX, sY);
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int i = 4 * ti + x;
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        const int j = 4 * tj + y;
-        out[x][y] = (i < bi && j < bj) ? A[(long long)(i0 + i) * n + (j0 + j)] - out[x][y] : 0.0;
-      }
+    for (int q = 0; q < 16; ++q) {
+      const int i = tile_row(q), j = tile_col(q);
+      out[q] = (i < bi && j < bj) ? A[(long long)(i0 + i) * n + (j0 + j)] - out[q] : 0.0;
     }
   } else if (I == kb && J == kb) {
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-#pragma unroll
-      for (int y = 0; y < 4; ++y) out[x][y] = -Dr[(4 * ti + x) * kSB + 4 * tj + y];
-    }
+    for (int q = 0; q < 16; ++q) out[q] = -Dr[tile_row(q) * kSB + tile_col(q)];
   } else if (I == kb) {
     // (K, J) = P_KJ
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int i = 4 * ti + x;
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        const int j = 4 * tj + y;
-        out[x][y] = (i < bi && j < bj) ? Pr[(long long)i * n + j0 + j] : 0.0;
-      }
+    for (int q = 0; q < 16; ++q) {
+      const int i = tile_row(q), j = tile_col(q);
+      out[q] = (i < bi && j < bj) ? Pr[(long long)i * n + j0 + j] : 0.0;
     }
   } else {
     // (I, K) = A_IK D = P_KI^T
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-      const int i = 4 * ti + x;
-#pragma unroll
-      for (int y = 0; y < 4; ++y) {
-        const int j = 4 * tj + y;
-        out[x][y] = (i < bi && j < bj) ? Pr[(long long)j * n + i0 + i] : 0.0;
-      }
+    for (int q = 0; q < 16; ++q) {
+      const int i = tile_row(q), j = tile_col(q);
+      out[q] = (i < bi && j < bj) ? Pr[(long long)j * n + i0 + i] : 0.0;
     }
   }
 #pragma unroll
-  for (int x = 0; x < 4; ++x) {
-    const int i = 4 * ti + x;
-#pragma unroll
-    for (int y = 0; y < 4; ++y) {
-      const int j = 4 * tj + y;
-      out[x][y] *= sign;
-      if (i < bi && j < bj) A[(long long)(i0 + i) * n + (j0 + j)] = out[x][y];
-    }
+  for (int q = 0; q < 16; ++q) {
+    const int i = tile_row(q), j = tile_col(q);
+    out[q] *= sign;
+    if (i < bi && j < bj) A[(long long)(i0 + i) * n + (j0 + j)] = out[q];
   }
   if (ahead && I == kn && J == kn) {
     // look-ahead: D_{K+1} = (this updated tile)^-1, re-laid out through LDS into the
     // pivot layout (column t % 64, rows t / 64 + 4 q)
     __syncthreads();  // the product's last chunk is consumed before sT overwrites it
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-#pragma unroll
-      for (int y = 0; y < 4; ++y) sT[4 * ti + x][4 * tj + y] = out[x][y];
-    }
+    for (int q = 0; q < 16; ++q) sT[tile_row(q)][tile_col(q)] = out[q];
     __syncthreads();
     const int jj = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
     double m[16];
@@ -306,11 +308,9 @@ __global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0
     // mirror tile (J, I): element (r, c) = out(c, r), written row by row
     __syncthreads();  // the product's last chunk is consumed before sT overwrites it
 #pragma unroll
-    for (int x = 0; x < 4; ++x) {
-#pragma unroll
-      for (int y = 0; y < 4; ++y) sT[4 * ti + x][4 * tj + y] = out[x][y];
-    }
+    for (int q = 0; q < 16; ++q) sT[tile_row(q)][tile_col(q)] = out[q];
     __syncthreads();
+    const int ti = threadIdx.x >> 4, tj = threadIdx.x & 15;
 #pragma unroll
     for (int x = 0; x < 4; ++x) {
       const int r = 4 * ti + x;
