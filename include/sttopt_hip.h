@@ -52,9 +52,8 @@ extern "C" {
 #define STTOPT_KEFF_UPPER 1
 
 /* sttopt_level.flags (level 0 only) */
-#define STTOPT_SEPARATE_SWEEPS 1 /* fp32 V-cycle: one pass per smoothing sweep at levels
-                                    0 and 1 instead of the fused passes (same results;
-                                    for testing) */
+#define STTOPT_SEPARATE_SWEEPS 1 /* one pass per smoothing sweep at every level instead
+                                    of the fused passes (same results; for testing) */
 
 /* operator modes for sttopt_apply */
 #define STTOPT_MODE_MATVEC 0   /* y = K u (no projection)                 */

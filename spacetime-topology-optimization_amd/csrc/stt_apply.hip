@@ -84,6 +84,9 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 
   if (c < a.nnx && r < a.nny) {
     const T* u = a.u + row * a.su;
+    // kSmoothFirst: the input is the first sweep from zero, x1 = w D^-1 b at free dofs
+    // (the expression of jacobi_first_kernel), formed here at each neighbour
+    const bool first = a.mode == kSmoothFirst;
     // 3 x 3 node neighbourhood, zero outside the grid (and at fixed dofs when the
     // input has to be projected).
     T U[3][3][D];
@@ -95,14 +98,20 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
         const bool ok = rr >= 0 && rr < a.nny && cc >= 0 && cc < a.nnx;
         const long long node = (long long)rr * a.nnx + cc;
         Vec<T, D> w;
-        if (ok) {
+        if (ok && first) {
+          const Vec<TB, D> bn = load_node<TB, D>(a.b + row * a.sb, node);
+          const Vec<T, D> dn = load_node<T, D>(a.invd + row * a.sd, node);
+#pragma unroll
+          for (int k = 0; k < D; ++k) w.v[k] = a.omega * dn.v[k] * static_cast<T>(bn.v[k]);
+        } else if (ok) {
           w = load_node<T, D>(u, node);
         } else {
 #pragma unroll
           for (int k = 0; k < D; ++k) w.v[k] = T(0);
         }
-        const uint32_t word =
-            (a.proj_in && ok && a.mask != nullptr) ? a.mask[(node * D) >> 5] : ~0u;
+        const uint32_t word = ((a.proj_in || first) && ok && a.mask != nullptr)
+                                  ? a.mask[(node * D) >> 5]
+                                  : ~0u;
 #pragma unroll
         for (int k = 0; k < D; ++k)
           U[dr][dc][k] = ((word >> ((node * D + k) & 31)) & 1u) ? w.v[k] : T(0);
@@ -201,8 +210,9 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
       bv.v[i] = TB(0);
       iv.v[i] = T(0);
     }
-    if (a.mode == kSmooth || a.mode == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
-    if (a.mode == kSmooth) iv = load_node<T, D>(a.invd + row * a.sd, node);
+    const bool sweep = a.mode == kSmooth || first;
+    if (sweep || a.mode == kResid) bv = load_node<TB, D>(a.b + row * a.sb, node);
+    if (sweep) iv = load_node<T, D>(a.invd + row * a.sd, node);
     Vec<TY, D> yo;
 #pragma unroll
     for (int i = 0; i < D; ++i) {
@@ -213,7 +223,7 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
           out = acc[i];
         } else if (a.mode == kResid) {
           out = static_cast<T>(bv.v[i]) - acc[i];
-        } else {  // kSmooth
+        } else {  // kSmooth, kSmoothFirst
           out = U[1][1][i] + a.omega * iv.v[i] * (static_cast<T>(bv.v[i]) - acc[i]);
         }
       }
@@ -712,10 +722,8 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   const bool p22 = op.pkx == 2 && op.pky == 2;
   const bool param_tile = src == kSrcParam && p22 && sizeof(T) == sizeof(float);
   a.tiles = (src == kSrcDensity || param_tile) ? op.tiles() : op.tiles_small();
-  if ((src != kSrcDensity && dot != kNoDot) ||
-      (mode == kSmoothFirst && src != kSrcDensity && !param_tile)) {
-    set_error("stencil: reduction partials are level-0 only, the fused first sweeps need "
-              "the LDS tile (level 0, or level 1 at 2 x 2 in fp32)");
+  if (src != kSrcDensity && dot != kNoDot) {
+    set_error("stencil: reduction partials are level-0 only");
     return STTOPT_EINVAL;
   }
   if (mode == kSmoothFirst && (b == nullptr || invd == nullptr)) {

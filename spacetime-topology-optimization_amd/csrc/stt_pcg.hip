@@ -394,11 +394,12 @@ struct VCycle {
     T* x = buf(L.vx);
     T* alt = buf(L.vx2);
     const int ns = mg->n_smooth;
-    // the fp32 level 1 at 2 x 2 runs on the LDS tile: its first two sweeps in one pass
-    const bool tile_first = sizeof(T) == sizeof(float) && ns >= 2 && op[l].ptab != nullptr &&
-                            op[l].pkx == 2 && op[l].pky == 2 &&
-                            (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0;
-    if (tile_first) {
+    // the first two sweeps in one pass (the first one, from zero, formed where the
+    // second reads it); the LDS tile kernel for the fp32 level 1 at 2 x 2, the
+    // one-node-per-thread kernel elsewhere
+    const bool fuse_first =
+        ns >= 2 && (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0;
+    if (fuse_first) {
       if (int e = stencil<T, T, T>(op[l], rows, kSmoothFirst, false, kNoDot,
                                    static_cast<const T*>(nullptr), 0, b, nl, invd(l), sd(l), x,
                                    nl, mg->omega, active, nullptr, s))

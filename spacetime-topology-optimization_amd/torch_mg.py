@@ -494,8 +494,8 @@ class VCycle:
     :param n_smooth: pre- and post-smoothing sweeps per level
     :param gamma: coarse visits per level (1: V-cycle, 2: W-cycle)
     :param dtype: V-cycle precision (default `VCYCLE_DTYPE`)
-    :param separate_sweeps: run every level-0 and level-1 smoothing sweep of the fp32
-        cycle as its own pass instead of the fused passes (same results; for testing)
+    :param separate_sweeps: run every smoothing sweep as its own pass instead of the
+        fused passes (same results; for testing)
     """
 
     def __init__(self, levels, *, omega: float = 0.6, n_smooth: int = 2, gamma: int = 1,

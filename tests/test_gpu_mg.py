@@ -368,23 +368,30 @@ def test_fp32_vcycle_packed_coarse_matrices_equal_full(dev):
     assert err < 1e-5
 
 
-@pytest.mark.parametrize("nelx, nely, reaction", [(64, 48, False), (64, 48, True), (124, 28, False),
-                                                  (122, 26, False), (16, 12, True), (4, 2, False)])
-def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction):
-    """The fp32 V-cycle's level-0 two-sweep passes (pre: two sweeps and the residual;
-    post: prolongation and two sweeps) equal one pass per sweep bit for bit, on grids
-    whose node counts fall on and next to the 62 x 14 output tiles (125 = 2 * 62 + 1,
-    29 = 2 * 14 + 1, 123, 27), with and without a lumped reaction diagonal."""
+@pytest.mark.parametrize("nelx, nely, reaction, coarse",
+                         [(64, 48, False, None), (64, 48, True, None), (124, 28, False, None),
+                          (122, 26, False, None), (16, 12, True, 2), (4, 2, False, 2),
+                          (64, 48, False, 20), (64, 48, True, 20)])
+def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coarse):
+    """The fp32 V-cycle's fused passes equal one pass per sweep bit for bit: level 0's
+    two-sweep passes (pre: two sweeps and the residual; post: prolongation and two
+    sweeps) on grids whose node counts fall on and next to the 62 x 14 output tiles
+    (125 = 2 * 62 + 1, 29 = 2 * 14 + 1, 123, 27), and the first two sweeps in one pass
+    on the coarse levels (level 1 on the LDS tile, deeper levels, reached with a small
+    coarsest level, on the one-node-per-thread kernel), with and without a lumped
+    reaction diagonal."""
     from sttopt_amd import torch_fem, torch_mg
 
     s = _setup(nelx, nely, dev)
     rng = np.random.default_rng(nelx * 1000 + nely)
     x = rng.uniform(0.05, 1, (3, nelx * nely))
-    kw = {"max_coarse_elements": 2} if nelx * nely < 64 else {}
+    kw = {} if coarse is None else {"max_coarse_elements": coarse}
     if reaction:
         kw["reaction"] = s["t"](rng.uniform(0.0, 1e-2, (3, nelx * nely)))
     levels = _hip_hierarchy(s, nelx, nely, x, **kw)
     assert (levels[0].kx, levels[0].ky) == (2, 2)
+    if coarse == 20:
+        assert len(levels) == 5  # levels 2 and 3 smooth on the one-node-per-thread kernel
     if reaction:
         assert levels[0].reaction_diag is not None
     r = torch_fem.project(s["t"](rng.standard_normal((3, s["ndof"]))), s["mask_t"])
