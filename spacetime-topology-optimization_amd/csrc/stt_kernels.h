@@ -98,11 +98,18 @@ int coarse_solve(long long n, int rows, const double* cinv, long long scinv,
 // b (fp64, rounded to fp32 as the cycle's copy of b would be) and r = b - A x.  post:
 // z = two sweeps from x + P ec (ec on the 2 x 2 coarser grid with cnnx nodes per row);
 // partial, when given, receives the per-tile r.z sums (free dofs).  Both need the
-// level-0 free-dof mask and rows below 4 GiB (32-bit offsets).
+// level-0 free-dof mask and rows below 4 GiB (32-bit offsets).  pre with ap != nullptr
+// first takes the PCG update: its b is b - alpha[row] ap, written to rnew (not b: the
+// tiles read b's halo) with the per-tile sums of its squares in rr and, with zold,
+// of its products with zold in rzo.
 int pair_tiles(const OpDesc& op);
 int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
                     const float* invd, long long sd, float* x, long long sx, float* r,
-                    long long sr, double omega, const int* active, hipStream_t s);
+                    long long sr, double omega, const int* active, hipStream_t s,
+                    const double* ap = nullptr, long long sap = 0,
+                    const double* alpha = nullptr, double* rnew = nullptr, long long srn = 0,
+                    double* rr = nullptr, const float* zold = nullptr, long long szo = 0,
+                    double* rzo = nullptr);
 template <typename ZT>
 int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
                      const float* ec, long long sec, int cnnx, const double* b, long long sb,

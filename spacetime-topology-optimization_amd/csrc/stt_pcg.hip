@@ -8,6 +8,7 @@
 // active count) every `check_every` iterations instead of syncing on every
 // iteration.  All inner products and norms are reduced in fp64, in a fixed order
 // (per-tile partials, then one workgroup per row), so results are deterministic.
+#include <utility>
 #include <vector>
 
 #include "stt_kernels.h"
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
         if (zr != nullptr) zv = load_node<ZT, D>(zr, node);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
-          rv.v[k] -= a * av.v[k];
+          rv.v[k] = fma(-a, av.v[k], rv.v[k]);
           rr += rv.v[k] * rv.v[k];
           rz += rv.v[k] * (double)zv.v[k];
         }
@@ -452,6 +453,41 @@ struct VCycle {
                           active, s);
   }
 
+  // The pair cycle split at the PCG's convergence check.  pair_pre_update: the residual
+  // update rnew = r - alpha Ap (per-tile sums of its squares in rr, and of rnew.zold in
+  // rzo when zold is given) in the pre-smoothing pass; pair_rest: the coarse visit and
+  // the post pass on that residual, z and the r.z partials.  Only for pair() (the fp32
+  // cycle).
+  int pair_pre_update(const double* r, const double* Ap, const double* alpha, double* rnew,
+                      double* rr, const T* zold, double* rzo) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+      const sttopt_level& L = mg->levels[0];
+      const long long nl = n[0];
+      return smooth_pair_pre(op[0], rows, r, nl, invd(0), sd(0), buf(L.vx), nl, buf(L.vr),
+                             nl, mg->omega, active, s, Ap, nl, alpha, rnew, nl, rr, zold,
+                             nl, rzo);
+    }
+    set_error("vcycle: the fused residual update needs the fp32 pair cycle");
+    return STTOPT_EINVAL;
+  }
+  template <typename ZT>
+  int pair_rest(const double* r, long long rs, ZT* z, long long zs, double* part) {
+    if constexpr (sizeof(T) == sizeof(float)) {
+      const sttopt_level& L = mg->levels[0];
+      const sttopt_level& C = mg->levels[1];
+      const long long nl = n[0], nc = n[1];
+      if (int e = restrict_to<T>(mg->d, C.nelx, C.nely, 2, 2, rows, buf(L.vr), nl, C.mask,
+                                 buf(C.vb), nc, active, s))
+        return e;
+      T* ec = nullptr;
+      if (int e = cycle(1, buf(C.vb), &ec)) return e;
+      return smooth_pair_post<ZT>(op[0], rows, buf(L.vx), nl, ec, nc, C.nelx + 1, r, rs,
+                                  invd(0), sd(0), z, zs, mg->omega, active, part, s);
+    }
+    set_error("vcycle: the pair cycle needs the fp32 V-cycle");
+    return STTOPT_EINVAL;
+  }
+
   // z = M(r) on level 0.  r fp64; z fp64 or in the V-cycle precision (ZT = T: the
   // cycle's last sweep computes in T, so storing z in T loses nothing).  When part !=
   // nullptr, writes the tile partials of r.z (free dofs) for the PCG.  tmp64: rows x
@@ -481,19 +517,10 @@ struct VCycle {
     }
     if constexpr (sizeof(T) == sizeof(float)) {
       if (pair()) {
-        const sttopt_level& C = mg->levels[1];
-        const long long nc = n[1];
-        T* x = buf(L.vx);
-        if (int e = smooth_pair_pre(op[0], rows, r, rs, invd(0), sd(0), x, nl, buf(L.vr), nl,
-                                    mg->omega, active, s))
+        if (int e = smooth_pair_pre(op[0], rows, r, rs, invd(0), sd(0), buf(L.vx), nl,
+                                    buf(L.vr), nl, mg->omega, active, s))
           return e;
-        if (int e = restrict_to<T>(mg->d, C.nelx, C.nely, 2, 2, rows, buf(L.vr), nl, C.mask,
-                                   buf(C.vb), nc, active, s))
-          return e;
-        T* ec = nullptr;
-        if (int e = cycle(1, buf(C.vb), &ec)) return e;
-        return smooth_pair_post<ZT>(op[0], rows, x, nl, ec, nc, C.nelx + 1, r, rs, invd(0),
-                                    sd(0), z, zs, mg->omega, active, part, s);
+        return pair_rest(r, rs, z, zs, part);
       }
     }
     // fine-level right-hand side in the V-cycle precision
@@ -680,6 +707,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   T* z = reinterpret_cast<T*>(work + rows * n);  // the preconditioned residual, in T
   double* p = work + 2 * rows * n;
   double* Ap = work + 3 * rows * n;
+  double* r2 = work + 4 * rows * n;  // the other residual buffer of the fused update
   TileGeom g{op.nnx(), op.nny(), op.ntx(), tiles, tiles * rows, mg->d};
   const dim3 tblock(kBlock), rgrid(rows);
   *n_iter = 0;
@@ -709,6 +737,12 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (int e = convert<T, double>(n, rows, z, n, p, n, sc.active, s)) return e;
   STT_CHECK_LAUNCH("pcg first preconditioner");
 
+  // The fp32 pair cycle takes the residual update r <- r - alpha Ap in its pre-smoothing
+  // pass (reading r and Ap at the staged nodes, writing the new residual to the other
+  // buffer), so the convergence check follows that pass; the rows that converge there
+  // skip the rest of the cycle.
+  const bool fused = vc.pair();
+  const int fused_tiles = pair_tiles(op);
   if (check_every < 1) check_every = 1;
   bool done = false;
   int k_last = 0;  // iteration whose x += alpha p is still pending
@@ -728,12 +762,19 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     }
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
                        sc.rz, sc.alpha);
-    STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, r, Ap,
-                    flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
-                    sc.part[3]);
+    const bool fuse_now = fused && k < max_iter;
+    if (fuse_now) {
+      if (int e = vc.pair_pre_update(r, Ap, sc.alpha, r2, sc.part[1],
+                                     flexible ? z : static_cast<T*>(nullptr), sc.part[3]))
+        return e;
+    } else {
+      STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, r, Ap,
+                      flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
+                      sc.part[3]);
+    }
     if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
-    hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, tiles, sc.part[1], rtol, k,
-                       sc.bsafe, resid, sc.active, iters, sc.counter);
+    hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, fuse_now ? fused_tiles : tiles,
+                       sc.part[1], rtol, k, sc.bsafe, resid, sc.active, iters, sc.counter);
     STT_CHECK_LAUNCH("pcg update");
     k_last = k;
     if (k % check_every == 0 || k == max_iter) {
@@ -744,9 +785,15 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       }
     }
     if (k == max_iter) break;
-    if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
-    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2], tiles,
-                       sc.part[3], flexible, sc.active, sc.rz, sc.beta);
+    if (fuse_now) {
+      if (int e = vc.pair_rest(r2, n, z, n, sc.part[2])) return e;
+      std::swap(r, r2);
+    } else if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) {
+      return e;
+    }
+    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2],
+                       fuse_now ? fused_tiles : tiles, sc.part[3], flexible, sc.active, sc.rz,
+                       sc.beta);
     STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
                     iters, k, 1);
     STT_CHECK_LAUNCH("pcg direction");
@@ -806,7 +853,8 @@ int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg) {
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {
   if (mg == nullptr || mg->nlevels < 1) return 0;
   const long long n = (long long)(mg->levels[0].nelx + 1) * (mg->levels[0].nely + 1) * mg->d;
-  return 4LL * mg->rows * n;
+  // r, z, p, Ap and the second residual buffer of the fused update
+  return 5LL * mg->rows * n;
 }
 
 int sttopt_profile_outer_apply(int32_t enable) {

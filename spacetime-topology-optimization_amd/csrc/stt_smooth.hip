@@ -49,6 +49,19 @@ struct PairArgs {
   float omega;
   const int* active;
   double* partial;        // post: r.z per tile (may be null)
+  // pre with the PCG residual update: the staged b is r - alpha Ap (r = b above, Ap
+  // fp64, alpha per row); the output nodes' fp64 values go to rnew, the tile sum of
+  // their squares to rr and, when zold is given (the flexible beta), the tile sum of
+  // their products with zold to rzo
+  const double* ap;
+  long long sap;
+  const double* alpha;
+  double* rnew;
+  long long srn;
+  double* rr;
+  const float* zold;
+  long long szo;
+  double* rzo;
   KE<float> ke;
 };
 
@@ -123,9 +136,11 @@ __device__ __forceinline__ Vec<T, D> zvec() {
   return z;
 }
 
-// RD: the operator carries a lumped reaction diagonal.
-template <typename ZT, int D, bool POST, bool RD>
+// RD: the operator carries a lumped reaction diagonal.  UPD (pre only): the pass also
+// takes the PCG's residual update r <- r - alpha Ap.
+template <typename ZT, int D, bool POST, bool RD, bool UPD>
 __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
+  static_assert(!(POST && UPD), "the residual update belongs to the pre pass");
   constexpr int NR = kRowsPerThread;
   __shared__ Vec<float, D> sU[kHY][kHX];          // input of the first stencil
   __shared__ float sR[kEY][kEX];
@@ -152,6 +167,10 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   const float* __restrict__ xin = POST ? a.x + row * a.sx : nullptr;
   const float* __restrict__ ec = POST ? a.ec + row * a.sec : nullptr;
   const float* __restrict__ rdg = RD ? a.rdiag + row * a.srd : nullptr;
+  const double* __restrict__ apr = UPD ? a.ap + row * a.sap : nullptr;
+  const double alpha = UPD ? a.alpha[row] : 0.0;
+  const bool flex = UPD && a.zold != nullptr;  // uniform
+  const float* __restrict__ zor = flex ? a.zold + row * a.szo : nullptr;
 
   // ---- staging: wavefront w takes halo rows w, w + 4, ...; lane l column x0 - 1 + l.
   // The two trailing node columns and the trailing element column are taken by the
@@ -176,8 +195,10 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   struct Raw {
     Vec<float, D> x, c00, c01, c10, c11;  // post: x2 and the coarse values around
     uint32_t word;
-    Vec<double, D> b;                      // pre: b and D^-1
+    Vec<double, D> b;                      // pre: b and D^-1 (and Ap with UPD)
     Vec<float, D> dv;
+    Vec<double, D> ap;
+    Vec<float, D> zo;
   };
   auto load_raw = [&](uint32_t gy, uint32_t gx) -> Raw {
     Raw r;
@@ -196,6 +217,10 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
     } else {
       r.b = ld<double, D>(b, node);
       r.dv = ld<float, D>(invd, node);
+      if constexpr (UPD) {
+        r.ap = ld<double, D>(apr, node);
+        r.zo = flex ? ld<float, D>(zor, node) : zvec<float, D>();
+      }
     }
     return r;
   };
@@ -204,6 +229,8 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   // even and 1/2, 1/2 at odd fine positions)
   struct Staged {
     Vec<float, D> u, bc, dv;
+    Vec<double, D> b64;  // pre with UPD: the updated fp64 b
+    Vec<float, D> zo;    // and zold
   };
   auto finish = [&](const Raw& r, uint32_t gy, uint32_t gx, bool ok) -> Staged {
     Staged s;
@@ -223,7 +250,11 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
     } else {
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        s.bc.v[k] = ok ? static_cast<float>(r.b.v[k]) : 0.0f;
+        // the update of pcg_update_kernel, same expression
+        const double bk = UPD ? fma(-alpha, r.ap.v[k], r.b.v[k]) : r.b.v[k];
+        s.b64.v[k] = bk;
+        s.zo.v[k] = r.zo.v[k];
+        s.bc.v[k] = ok ? static_cast<float>(bk) : 0.0f;
         s.dv.v[k] = ok ? r.dv.v[k] : 0.0f;
         s.u.v[k] = a.omega * s.dv.v[k] * s.bc.v[k];
       }
@@ -248,17 +279,33 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
              (uint32_t)min(tgx, a.nelx - 1)];
 
   // ---- staged values to LDS
+  // UPD: squares of the updated residual at this thread's output nodes, and its
+  // products with zold
+  double rr = 0.0, rzo = 0.0;
 #pragma unroll
   for (int n = 0; n < kUR; ++n) {
     const int i = wave + n * kWaves, gy = y0 - 1 + i;
     if (i < kHY) {
-      const Staged sg = finish(rw[n], clampy(gy), cxn, gy >= 0 && gy < nny && cnode);
+      const bool ok = gy >= 0 && gy < nny && cnode;
+      const Staged sg = finish(rw[n], clampy(gy), cxn, ok);
       sU[i][lane] = sg.u;
       if constexpr (!POST) {
         // halo node (i, lane) is node (i - 1, lane - 1) of the first stencil
         if (i >= 1 && i <= kTileY && lane >= 1) {
           sB[i - 1][lane - 1] = sg.bc;
           sI[i - 1][lane - 1] = sg.dv;
+        }
+      }
+      if constexpr (UPD) {
+        // output nodes (halo rows 2 .. kOutY + 1, columns 2 .. kOutX + 1): each is
+        // written by exactly one tile
+        if (ok && i >= 2 && i <= kOutY + 1 && lane >= 2 && lane <= kOutX + 1) {
+          st<double, D>(a.rnew + row * a.srn, (uint32_t)gy * (uint32_t)nnx + cxn, sg.b64);
+#pragma unroll
+          for (int k = 0; k < D; ++k) {
+            rr += sg.b64.v[k] * sg.b64.v[k];
+            rzo += sg.b64.v[k] * (double)sg.zo.v[k];
+          }
         }
       }
     }
@@ -432,6 +479,15 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
       if (threadIdx.x == 0) a.partial[(long long)row * a.tiles + tt] = sum;
     }
   }
+  if constexpr (UPD) {
+    const double sum = block_sum(rr, red);
+    if (threadIdx.x == 0) a.rr[(long long)row * a.tiles + tt] = sum;
+    if (flex) {
+      __syncthreads();
+      const double sz = block_sum(rzo, red);
+      if (threadIdx.x == 0) a.rzo[(long long)row * a.tiles + tt] = sz;
+    }
+  }
 }
 
 template <typename ZT>
@@ -453,16 +509,16 @@ PairArgs<ZT> pair_args(const OpDesc& op, int rows) {
   return a;
 }
 
-template <typename ZT, bool POST>
+template <typename ZT, bool POST, bool UPD>
 void launch_pair(const PairArgs<ZT>& a, int d, hipStream_t s) {
   const dim3 grid(a.total), block(kBlock);
   const bool rd = a.rdiag != nullptr;
   if (d == 2) {
-    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, false>), grid, block, 0, s, a);
+    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, true, UPD>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, false, UPD>), grid, block, 0, s, a);
   } else {
-    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, true>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, false>), grid, block, 0, s, a);
+    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, true, UPD>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, false, UPD>), grid, block, 0, s, a);
   }
 }
 
@@ -477,16 +533,29 @@ int pair_tiles(const OpDesc& op) {
 
 int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
                     const float* invd, long long sd, float* x, long long sx, float* r,
-                    long long sr, double omega, const int* active, hipStream_t s) {
+                    long long sr, double omega, const int* active, hipStream_t s,
+                    const double* ap, long long sap, const double* alpha, double* rnew,
+                    long long srn, double* rr, const float* zold, long long szo,
+                    double* rzo) {
   if (rows <= 0) return STTOPT_OK;
   if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) || !fits_u32(op)) {
     set_error("smooth_pair_pre: level-0 density operator with a free-dof mask, d = 1 or 2, "
               "rows below 4 GiB required");
     return STTOPT_EINVAL;
   }
+  const bool upd = ap != nullptr;
+  if (upd && (alpha == nullptr || rnew == nullptr || rr == nullptr || rnew == b ||
+              (zold != nullptr && rzo == nullptr))) {
+    set_error("smooth_pair_pre: the residual update needs alpha, a separate output and the "
+              "partial sums");
+    return STTOPT_EINVAL;
+  }
   if (op.d == 2 && !(pair_aligned(b, sb, sizeof(double)) && pair_aligned(invd, sd, sizeof(float)) &&
                      pair_aligned(x, sx, sizeof(float)) && pair_aligned(r, sr, sizeof(float)) &&
-                     pair_aligned(op.rdiag, op.srd, sizeof(float)))) {
+                     pair_aligned(op.rdiag, op.srd, sizeof(float)) &&
+                     pair_aligned(ap, sap, sizeof(double)) &&
+                     pair_aligned(rnew, srn, sizeof(double)) &&
+                     pair_aligned(zold, szo, sizeof(float)))) {
     set_error("smooth_pair_pre: d = 2 vectors need node-aligned pointers and even row strides");
     return STTOPT_EINVAL;
   }
@@ -501,7 +570,20 @@ int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
   a.sres = sr;
   a.omega = static_cast<float>(omega);
   a.active = active;
-  launch_pair<float, false>(a, op.d, s);
+  if (upd) {
+    a.ap = ap;
+    a.sap = sap;
+    a.alpha = alpha;
+    a.rnew = rnew;
+    a.srn = srn;
+    a.rr = rr;
+    a.zold = zold;
+    a.szo = szo;
+    a.rzo = rzo;
+    launch_pair<float, false, true>(a, op.d, s);
+  } else {
+    launch_pair<float, false, false>(a, op.d, s);
+  }
   STT_CHECK_LAUNCH("smooth_pair_kernel(pre)");
   return STTOPT_OK;
 }
@@ -539,7 +621,7 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
   a.omega = static_cast<float>(omega);
   a.active = active;
   a.partial = partial;
-  launch_pair<ZT, true>(a, op.d, s);
+  launch_pair<ZT, true, false>(a, op.d, s);
   STT_CHECK_LAUNCH("smooth_pair_kernel(post)");
   return STTOPT_OK;
 }
