@@ -47,9 +47,15 @@ extern "C" {
 
 /* layouts of coarse element matrices (entry-major, [rows][entries][nel]):
  * FULL: the (4d)^2 entries a * 4d + b; UPPER: the (4d)(4d+1)/2 entries a <= b of the
- * symmetric matrix, row-major over a (entry index a * 4d - a (a - 1) / 2 + b - a) */
+ * symmetric matrix, row-major over a (entry index a * 4d - a (a - 1) / 2 + b - a).
+ * NODE: the assembled operator as a node stencil, coefficient-major
+ * [rows][d(d+1)/2 + 4 d^2][nodes]: the upper triangle of the node's own d x d block,
+ * then its d x d blocks (row: the node's dof, column: the neighbour's) to the
+ * neighbours at (node row, column) + (0, 1), (1, -1), (1, 0), (1, 1); the other four
+ * neighbours' blocks are theirs, transposed (19 values per node at d = 2) */
 #define STTOPT_KEFF_FULL 0
 #define STTOPT_KEFF_UPPER 1
+#define STTOPT_KEFF_NODE 2
 
 /* sttopt_level.flags (level 0 only) */
 #define STTOPT_SEPARATE_SWEEPS 1 /* one pass per smoothing sweep at every level instead
@@ -131,7 +137,7 @@ typedef struct sttopt_level {
   int64_t rdiag_stride;
   const void* keff;
   int64_t keff_stride;
-  int32_t keff_layout;    /* STTOPT_KEFF_FULL or STTOPT_KEFF_UPPER                 */
+  int32_t keff_layout;    /* STTOPT_KEFF_FULL, STTOPT_KEFF_UPPER or STTOPT_KEFF_NODE */
   int32_t flags;          /* level 0: STTOPT_SEPARATE_SWEEPS or 0; others: 0      */
   /* coarser levels, alternative to keff: keff_E = sum over the pkx x pky level-0
    * elements g under E of density0_g * ptable[g], ptable: (pkx*pky) x (4d)^2 in the
@@ -175,6 +181,14 @@ typedef struct sttopt_mg {
  * reference (whose coarse matrices stay full, torch_mg.py:355-370). */
 int sttopt_pack_keff(int32_t dtype, int32_t d, int64_t nel, int32_t rows, const double* keff,
                      int64_t keff_stride, void* out, int64_t out_stride, void* stream);
+
+/* The STTOPT_KEFF_NODE form, in dtype, of fp64 FULL element matrices on an nelx x nely
+ * grid (each coupling summed in fp64 from (A + A^T) / 2, rounded once).  No
+ * counterpart in the reference (whose coarse levels apply element matrices,
+ * torch_mg.py:179-193). */
+int sttopt_node_stencil(int32_t dtype, int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                        const double* keff, int64_t keff_stride, void* out, int64_t out_stride,
+                        void* stream);
 
 /* Bytes of device scratch the PCG / V-cycle need for this hierarchy (per-row
  * scalars, reduction partials, flags).  levels[*].nelx/nely and rows must be set. */

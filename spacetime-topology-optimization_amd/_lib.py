@@ -17,7 +17,7 @@ from . import _build
 LIB_PATH = _build.LIB_PATH
 
 OK, EINVAL, EHIP, ENOCONV, ENOTSPD = 0, -1, -2, -3, -4
-KEFF_FULL, KEFF_UPPER = 0, 1
+KEFF_FULL, KEFF_UPPER, KEFF_NODE = 0, 1, 2
 SEPARATE_SWEEPS = 1  # sttopt_level.flags of level 0
 F64, F32 = 0, 1
 MODE_MATVEC, MODE_OPERATOR, MODE_OPERATOR_PROJECTED = 0, 1, 2
@@ -136,6 +136,8 @@ _SIGNATURES = {
     ),
     "sttopt_spd_inverse_work_count": (c_int64, [c_int32, c_int32]),
     "sttopt_pack_keff": (c_int32, [c_int32, c_int32, c_int64, c_int32, _P, c_int64, _P, c_int64, _P]),
+    "sttopt_node_stencil": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_int32, _P, c_int64,
+                                      _P, c_int64, _P]),
     "sttopt_spd_inverse": (c_int32, [c_int32, c_int32, _P, c_int64, _P, _P]),
     "sttopt_dense_assemble": (
         c_int32,

@@ -59,8 +59,9 @@ struct StencilArgs {
 
 // SRC: 0 = density * KE (level 0), 1 = explicit entry-major keff, 2 = keff
 // parameterized by the fine densities under each coarse element.
-// 3 = explicit keff, packed upper triangle.
-enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2, kSrcKeffUpper = 3 };
+// 3 = explicit keff, packed upper triangle.  4 = assembled node stencil
+// (STTOPT_KEFF_NODE).
+enum { kSrcDensity = 0, kSrcKeff = 1, kSrcParam = 2, kSrcKeffUpper = 3, kSrcNode = 4 };
 
 // Entry (r, c) of a K x K symmetric matrix in the packed upper-triangle order.
 __host__ __device__ constexpr int upper_index(int r, int c, int K) {
@@ -121,10 +122,57 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
     T acc[D];
 #pragma unroll
     for (int i = 0; i < D; ++i) acc[i] = T(0);
+    if constexpr (SRC == kSrcNode) {
+      // own block (upper triangle), the forward blocks E, NW, N, NE stored here, and the
+      // backward neighbours' forward blocks toward this node, transposed
+      constexpr int ND = D * (D + 1) / 2;
+      constexpr int fdr[4] = {0, 1, 1, 1}, fdc[4] = {1, -1, 0, 1};
+      const long long nn = (long long)a.nnx * a.nny;
+      const long long node = (long long)r * a.nnx + c;
+      const T* S = a.keff + row * a.sk;
+      int idx = 0;
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int k = i; k < D; ++k) {
+          const T v = S[(long long)idx * nn + node];
+          acc[i] += v * U[1][1][k];
+          if (k != i) acc[k] += v * U[1][1][i];
+          ++idx;
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const int cc = c + fdc[f];
+        if (r + fdr[f] < a.nny && cc >= 0 && cc < a.nnx) {
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+              acc[i] += S[(long long)(ND + (f * D + i) * D + k) * nn + node] *
+                        U[1 + fdr[f]][1 + fdc[f]][k];
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < 4; ++f) {
+        const int rr = r - fdr[f], cc = c - fdc[f];
+        if (rr >= 0 && cc >= 0 && cc < a.nnx) {
+          const long long m = (long long)rr * a.nnx + cc;
+#pragma unroll
+          for (int i = 0; i < D; ++i) {
+#pragma unroll
+            for (int k = 0; k < D; ++k)
+              acc[i] += S[(long long)(ND + (f * D + k) * D + i) * nn + m] *
+                        U[1 - fdr[f]][1 - fdc[f]][k];
+          }
+        }
+      }
+    }
     // q: element up-left, up-right, down-left, down-right of the node; the node is
     // that element's local BR, BL, TR, TL corner respectively.
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < 4 && SRC != kSrcNode; ++q) {
       const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
       if (er < 0 || er >= a.nely || ec < 0 || ec >= a.nelx) continue;
       const long long e = (long long)er * a.nelx + ec;
@@ -715,8 +763,9 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   a.nnx = op.nnx();
   a.nny = op.nny();
   const int src = op.ptab != nullptr ? kSrcParam
-                  : op.keff != nullptr ? (op.keff_upper ? kSrcKeffUpper : kSrcKeff)
-                                       : kSrcDensity;
+                  : op.keff != nullptr
+                      ? (op.keff_node ? kSrcNode : op.keff_upper ? kSrcKeffUpper : kSrcKeff)
+                      : kSrcDensity;
   a.ntx = op.ntx();
   // level 0, and in fp32 the parameterized level 1 at 2 x 2, run on 64 x 16 LDS tiles
   const bool p22 = op.pkx == 2 && op.pky == 2;
@@ -810,11 +859,13 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     if (src == kSrcParam) STT_PARAM(2);
     else if (src == kSrcKeff) STT_STENCIL(2, kSrcKeff);
     else if (src == kSrcKeffUpper) STT_STENCIL(2, kSrcKeffUpper);
+    else if (src == kSrcNode) STT_STENCIL(2, kSrcNode);
     else STT_TILE(2, kSrcDensity);
   } else if (op.d == 1) {
     if (src == kSrcParam) STT_PARAM(1);
     else if (src == kSrcKeff) STT_STENCIL(1, kSrcKeff);
     else if (src == kSrcKeffUpper) STT_STENCIL(1, kSrcKeffUpper);
+    else if (src == kSrcNode) STT_STENCIL(1, kSrcNode);
     else STT_TILE(1, kSrcDensity);
 #undef STT_STENCIL
 #undef STT_STENCIL22

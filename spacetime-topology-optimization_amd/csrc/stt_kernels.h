@@ -20,6 +20,7 @@ struct OpDesc {
   const void* keff = nullptr;
   long long sk = 0;
   int keff_upper = 0;  // keff holds the packed upper triangle (STTOPT_KEFF_UPPER)
+  int keff_node = 0;   // keff holds the assembled node stencil (STTOPT_KEFF_NODE)
   const void* rdiag = nullptr;
   long long srd = 0;
   const uint32_t* mask = nullptr;

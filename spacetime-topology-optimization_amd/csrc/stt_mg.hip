@@ -115,6 +115,62 @@ __global__ __launch_bounds__(256) void pack_keff_kernel(long long nel,
   }
 }
 
+// Assembled node stencil of symmetric entry-major element matrices (STTOPT_KEFF_NODE):
+// per node, the upper triangle of its own d x d block, then the d x d couplings to its
+// E, NW, N, NE neighbours (node row + 0/1, column + 1/-1/0/1), each the sum over the
+// elements holding both nodes of (A + A^T) / 2.  The other four neighbours' couplings
+// are those neighbours' forward blocks, transposed.  Sums in fp64, one rounding to TO.
+template <typename TO, int D>
+__global__ __launch_bounds__(256) void node_stencil_kernel(int nelx, int nely,
+                                                           const double* __restrict__ keff,
+                                                           long long sk, TO* __restrict__ out,
+                                                           long long so) {
+  constexpr int K = 4 * D, ND = D * (D + 1) / 2, NS = ND + 4 * D * D;
+  const int nnx = nelx + 1;
+  const long long nn = (long long)nnx * (nely + 1), nel = (long long)nelx * nely;
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (node >= nn) return;
+  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
+  const double* ke = keff + row * sk;
+  constexpr int fdr[4] = {0, 1, 1, 1}, fdc[4] = {1, -1, 0, 1};  // E, NW, N, NE
+  double acc[NS];
+#pragma unroll
+  for (int k = 0; k < NS; ++k) acc[k] = 0.0;
+  // local index of node (dr, dc) of an element (offsets from its first node row / column)
+  auto local = [](int dr, int dc) { return dr == 1 ? (dc == 0 ? 0 : 1) : (dc == 1 ? 2 : 3); };
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+    if (er < 0 || er >= nely || ec < 0 || ec >= nelx) continue;
+    const long long e = (long long)er * nelx + ec;
+    const int ln = local(r - er, c - ec);
+    auto sym = [&](int a, int b) {
+      return 0.5 * (ke[(long long)(a * K + b) * nel + e] + ke[(long long)(b * K + a) * nel + e]);
+    };
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < D; ++i) {
+#pragma unroll
+      for (int k = i; k < D; ++k) acc[idx++] += sym(ln * D + i, ln * D + k);
+    }
+#pragma unroll
+    for (int f = 0; f < 4; ++f) {
+      const int mr = r + fdr[f] - er, mc = c + fdc[f] - ec;
+      if (mr < 0 || mr > 1 || mc < 0 || mc > 1) continue;
+      const int lm = local(mr, mc);
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) acc[ND + (f * D + i) * D + k] += sym(ln * D + i, lm * D + k);
+      }
+    }
+  }
+  TO* o = out + row * so + node;
+#pragma unroll
+  for (int k = 0; k < NS; ++k) o[(long long)k * nn] = static_cast<TO>(acc[k]);
+}
+
 // ------------------------------------------------------------ transfers -----
 
 // Both transfers work on whole nodes (the D dofs as one vector).  KX = KY = 0 takes
@@ -469,6 +525,33 @@ int sttopt_pack_keff(int32_t dtype, int32_t d, int64_t nel, int32_t rows, const 
       hipLaunchKernelGGL((pack_keff_kernel<double, 4>), grid, block, 0, st, nel, keff, keff_stride, o, out_stride);
   }
   STT_CHECK_LAUNCH("pack_keff_kernel");
+  return STTOPT_OK;
+}
+
+int sttopt_node_stencil(int32_t dtype, int32_t d, int32_t nelx, int32_t nely, int32_t rows,
+                        const double* keff, int64_t keff_stride, void* out, int64_t out_stride,
+                        void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || rows < 0 || keff == nullptr ||
+      out == nullptr || (dtype != STTOPT_F64 && dtype != STTOPT_F32)) {
+    set_error("sttopt_node_stencil: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  const long long nn = (long long)(nelx + 1) * (nely + 1);
+  const dim3 grid(ceil_div(nn, 256), rows), block(256);
+  hipStream_t st = (hipStream_t)stream;
+#define STT_NODE(TO, DD)                                                                \
+  hipLaunchKernelGGL((node_stencil_kernel<TO, DD>), grid, block, 0, st, nelx, nely, keff, \
+                     keff_stride, static_cast<TO*>(out), out_stride)
+  if (dtype == STTOPT_F32) {
+    if (d == 2) STT_NODE(float, 2);
+    else STT_NODE(float, 1);
+  } else {
+    if (d == 2) STT_NODE(double, 2);
+    else STT_NODE(double, 1);
+  }
+#undef STT_NODE
+  STT_CHECK_LAUNCH("node_stencil_kernel");
   return STTOPT_OK;
 }
 

@@ -343,6 +343,7 @@ struct VCycle {
         o.keff = L.keff;
         o.sk = L.keff_stride;
         o.keff_upper = L.keff_layout == STTOPT_KEFF_UPPER;
+        o.keff_node = L.keff_layout == STTOPT_KEFF_NODE;
       }
       for (int i = 0; i < 64; ++i) o.ke[i] = m->ke[i];
       n[l] = o.ndof();

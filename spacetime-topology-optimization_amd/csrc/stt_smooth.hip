@@ -139,12 +139,19 @@ __device__ __forceinline__ Vec<T, D> zvec() {
 // RD: the operator carries a lumped reaction diagonal.  UPD (pre only): the pass also
 // takes the PCG's residual update r <- r - alpha Ap.
 template <typename ZT, int D, bool POST, bool RD, bool UPD>
-__global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
+__device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
   static_assert(!(POST && UPD), "the residual update belongs to the pre pass");
   constexpr int NR = kRowsPerThread;
-  __shared__ Vec<float, D> sU[kHY][kHX];          // input of the first stencil
+  // input of the first stencil, and its output with a one-node border (sX).  In the
+  // pre pass sX reuses sU's LDS once every thread holds its window in registers (the
+  // LDS of five workgroups per CU); the post pass, limited by its registers, keeps
+  // its own.
+  __shared__ Vec<float, D> sU[kHY][kHX];
+  __shared__ Vec<float, D> sXp[POST ? kTileY + 2 : 1][POST ? kTileX + 2 : 1];
+  static_assert(kHY == kTileY + 2 && kHX == kTileX + 2, "sX may alias sU");
+  Vec<float, D>(*const sX)[kTileX + 2] =
+      POST ? reinterpret_cast<Vec<float, D>(*)[kTileX + 2]>(&sXp[0][0]) : sU;
   __shared__ float sR[kEY][kEX];
-  __shared__ Vec<float, D> sX[kTileY + 2][kTileX + 2];  // its output, one-node border
   // pre: b and D^-1 of the first stencil's nodes, from the staging loads
   __shared__ Vec<float, D> sB[POST ? 1 : kTileY][POST ? 1 : kTileX];
   __shared__ Vec<float, D> sI[POST ? 1 : kTileY][POST ? 1 : kTileX];
@@ -352,14 +359,6 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
     }
   }
 
-  // border of sX: read only by nodes without output; kept at zero
-  if (tix < kTileX + 2) {
-    sX[0][tix] = zvec<float, D>();
-    sX[kTileY + 1][tix] = zvec<float, D>();
-  } else if (tix < kTileX + 2 + kTileY) {
-    sX[tix - (kTileX + 2) + 1][0] = zvec<float, D>();
-    sX[tix - (kTileX + 2) + 1][kTileX + 1] = zvec<float, D>();
-  }
   __syncthreads();
 
   const int i0 = wave * NR;
@@ -404,6 +403,15 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
     for (int r = 0; r < NR + 2; ++r) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) W[r][c] = sU[i0 + r][lane + c];
+    }
+    if constexpr (!POST) __syncthreads();  // every window is read: sU becomes sX
+    // border of sX: read only by nodes without output; kept at zero
+    if (tix < kTileX + 2) {
+      sX[0][tix] = zvec<float, D>();
+      sX[kTileY + 1][tix] = zvec<float, D>();
+    } else if (tix < kTileX + 2 + kTileY) {
+      sX[tix - (kTileX + 2) + 1][0] = zvec<float, D>();
+      sX[tix - (kTileX + 2) + 1][kTileX + 1] = zvec<float, D>();
     }
     float acc[NR][D];
     apply_window<D, NR>(W, R, ke_args, acc);
@@ -490,6 +498,18 @@ __global__ __launch_bounds__(kBlock) void smooth_pair_kernel(PairArgs<ZT> a) {
   }
 }
 
+// The pre pass runs at five waves per SIMD (at most 96 VGPRs; its LDS allows five
+// workgroups per CU); the post pass keeps four, where five would spill.
+template <int D, bool RD, bool UPD>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void smooth_pair_pre_kernel(PairArgs<float> a) {
+  smooth_pair_body<float, D, false, RD, UPD>(a);
+}
+template <typename ZT, int D, bool RD>
+__global__ __launch_bounds__(kBlock) void smooth_pair_post_kernel(PairArgs<ZT> a) {
+  smooth_pair_body<ZT, D, true, RD, false>(a);
+}
+
 template <typename ZT>
 PairArgs<ZT> pair_args(const OpDesc& op, int rows) {
   PairArgs<ZT> a{};
@@ -509,16 +529,28 @@ PairArgs<ZT> pair_args(const OpDesc& op, int rows) {
   return a;
 }
 
-template <typename ZT, bool POST, bool UPD>
-void launch_pair(const PairArgs<ZT>& a, int d, hipStream_t s) {
+template <bool UPD>
+void launch_pre(const PairArgs<float>& a, int d, hipStream_t s) {
   const dim3 grid(a.total), block(kBlock);
   const bool rd = a.rdiag != nullptr;
   if (d == 2) {
-    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, true, UPD>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 2, POST, false, UPD>), grid, block, 0, s, a);
+    if (rd) hipLaunchKernelGGL((smooth_pair_pre_kernel<2, true, UPD>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_pre_kernel<2, false, UPD>), grid, block, 0, s, a);
   } else {
-    if (rd) hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, true, UPD>), grid, block, 0, s, a);
-    else hipLaunchKernelGGL((smooth_pair_kernel<ZT, 1, POST, false, UPD>), grid, block, 0, s, a);
+    if (rd) hipLaunchKernelGGL((smooth_pair_pre_kernel<1, true, UPD>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_pre_kernel<1, false, UPD>), grid, block, 0, s, a);
+  }
+}
+template <typename ZT>
+void launch_post(const PairArgs<ZT>& a, int d, hipStream_t s) {
+  const dim3 grid(a.total), block(kBlock);
+  const bool rd = a.rdiag != nullptr;
+  if (d == 2) {
+    if (rd) hipLaunchKernelGGL((smooth_pair_post_kernel<ZT, 2, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_post_kernel<ZT, 2, false>), grid, block, 0, s, a);
+  } else {
+    if (rd) hipLaunchKernelGGL((smooth_pair_post_kernel<ZT, 1, true>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((smooth_pair_post_kernel<ZT, 1, false>), grid, block, 0, s, a);
   }
 }
 
@@ -580,11 +612,11 @@ int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
     a.zold = zold;
     a.szo = szo;
     a.rzo = rzo;
-    launch_pair<float, false, true>(a, op.d, s);
+    launch_pre<true>(a, op.d, s);
   } else {
-    launch_pair<float, false, false>(a, op.d, s);
+    launch_pre<false>(a, op.d, s);
   }
-  STT_CHECK_LAUNCH("smooth_pair_kernel(pre)");
+  STT_CHECK_LAUNCH("smooth_pair_pre_kernel");
   return STTOPT_OK;
 }
 
@@ -621,8 +653,8 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
   a.omega = static_cast<float>(omega);
   a.active = active;
   a.partial = partial;
-  launch_pair<ZT, true, false>(a, op.d, s);
-  STT_CHECK_LAUNCH("smooth_pair_kernel(post)");
+  launch_post<ZT>(a, op.d, s);
+  STT_CHECK_LAUNCH("smooth_pair_post_kernel");
   return STTOPT_OK;
 }
 

@@ -306,21 +306,24 @@ class _Level:
             if dtype == torch.float64:
                 out["keff"] = self._soa
             else:
-                # the fp32 V-cycle reads the symmetric matrices' upper triangle only
-                k = 4 * self.d
-                nel = self.nelx * self.nely
+                # the fp32 V-cycle applies the assembled operator as a node stencil: each
+                # node-pair coupling stored once (19 values per node at d = 2 against 36
+                # per element for the packed element matrices)
+                d = self.d
+                nn = (self.nelx + 1) * (self.nely + 1)
                 nrow = self._soa.shape[0]
-                packed = torch.empty(nrow, k * (k + 1) // 2, nel, dtype=dtype, device=dev)
+                node = torch.empty(nrow, d * (d + 1) // 2 + 4 * d * d, nn, dtype=dtype,
+                                   device=dev)
                 _lib.check(
-                    _lib.load().sttopt_pack_keff(
-                        _lib.dtype_code(dtype), self.d, nel, nrow,
-                        self._soa.data_ptr(), self._soa.stride(0), packed.data_ptr(),
-                        packed.stride(0), _lib.stream_handle(dev),
+                    _lib.load().sttopt_node_stencil(
+                        _lib.dtype_code(dtype), d, self.nelx, self.nely, nrow,
+                        self._soa.data_ptr(), self._soa.stride(0), node.data_ptr(),
+                        node.stride(0), _lib.stream_handle(dev),
                     ),
-                    "sttopt_pack_keff",
+                    "sttopt_node_stencil",
                 )
-                out["keff"] = packed
-                out["keff_layout"] = _lib.KEFF_UPPER
+                out["keff"] = node
+                out["keff_layout"] = _lib.KEFF_NODE
         self._copies[dtype] = out
         return out
 

@@ -343,9 +343,58 @@ def test_packed_upper_triangle_keff(dev):
             assert float((out.double() - want).abs().max()) <= tol * float(want.abs().max())
 
 
-def test_fp32_vcycle_packed_coarse_matrices_equal_full(dev):
-    """The fp32 V-cycle reading the packed upper triangles of the explicit coarse levels
-    equals the one reading the full matrices (the Galerkin matrices are symmetric)."""
+@pytest.mark.parametrize("d, nelx, nely", [(2, 7, 5), (1, 6, 9), (2, 1, 1)])
+def test_node_stencil_equals_assembled_operator(dev, d, nelx, nely):
+    """sttopt_node_stencil: every stored coefficient equals the entry of the operator
+    assembled densely from the same element matrices ((A + A^T) / 2 summed over the
+    elements through the element dof map): the node's own block (upper triangle) and
+    its blocks to the E, NW, N, NE neighbours, in the documented coefficient order; in
+    fp64 (to rounding) and fp32 (one rounding)."""
+    from sttopt_amd import _lib, torch_mg
+
+    lib = _lib.load()
+    rng = np.random.default_rng(d * 100 + nelx)
+    k, nel, rows = 4 * d, nelx * nely, 2
+    nnx, nny = nelx + 1, nely + 1
+    nn, ndof = nnx * nny, d * nnx * nny
+    full = torch.tensor(rng.standard_normal((rows, k * k, nel)), device=dev)
+    A = full.reshape(rows, k, k, nel).permute(0, 3, 1, 2)  # (rows, nel, k, k)
+    A = 0.5 * (A + A.transpose(2, 3))
+    edof = torch_mg.element_dof_map_tensor(nelx, nely, dev, d)
+    dense = torch.zeros(rows, ndof, ndof, dtype=torch.float64, device=dev)
+    for e in range(nel):
+        idx = edof[e]
+        dense[:, idx[:, None], idx[None, :]] += A[:, e]
+    ns = d * (d + 1) // 2 + 4 * d * d
+    want = torch.zeros(rows, ns, nn, dtype=torch.float64, device=dev)
+    for node in range(nn):
+        r, c = divmod(node, nnx)
+        col = 0
+        for i in range(d):
+            for kk in range(i, d):
+                want[:, col, node] = dense[:, d * node + i, d * node + kk]
+                col += 1
+        for dr, dc in ((0, 1), (1, -1), (1, 0), (1, 1)):
+            rr, cc = r + dr, c + dc
+            for i in range(d):
+                for kk in range(d):
+                    if 0 <= rr < nny and 0 <= cc < nnx:
+                        want[:, col, node] = dense[:, d * node + i, d * (rr * nnx + cc) + kk]
+                    col += 1
+    for dtype in (torch.float64, torch.float32):
+        out = torch.empty(rows, ns, nn, dtype=dtype, device=dev)
+        _lib.check(lib.sttopt_node_stencil(_lib.dtype_code(dtype), d, nelx, nely, rows,
+                                           full.data_ptr(), full.stride(0), out.data_ptr(),
+                                           out.stride(0), _lib.stream_handle(dev)), "node")
+        tol = 1e-14 if dtype == torch.float64 else 2e-7
+        err = float((out.double() - want).abs().max())
+        assert err <= tol * float(want.abs().max()), (dtype, err)
+
+
+def test_fp32_vcycle_node_stencil_coarse_levels_equal_full(dev):
+    """The fp32 V-cycle applying the explicit coarse levels as node stencils equals the
+    one reading the full element matrices, to fp32 rounding (the Galerkin matrices are
+    symmetric; the sums run in another order)."""
     from sttopt_amd import _lib, torch_fem, torch_mg
 
     nelx, nely = 64, 48
@@ -354,17 +403,17 @@ def test_fp32_vcycle_packed_coarse_matrices_equal_full(dev):
     r = torch_fem.project(s["t"](rng.standard_normal((3, s["ndof"]))), s["mask_t"])
     x = rng.uniform(0.05, 1, (3, nelx * nely))
     levels = _hip_hierarchy(s, nelx, nely, x, max_coarse_elements=12)
-    z_packed = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    z_node = torch_mg.VCycle(levels, dtype=torch.float32)(r)
     explicit = [lv for lv in levels[1:] if lv.keff is not None and "keff" in lv.vdata(torch.float32)]
-    assert explicit and all(lv.vdata(torch.float32).get("keff_layout") == _lib.KEFF_UPPER
+    assert explicit and all(lv.vdata(torch.float32).get("keff_layout") == _lib.KEFF_NODE
                             for lv in explicit)
     for lv in explicit:
         data = lv.vdata(torch.float32)
         data["keff"] = lv._soa.to(torch.float32)
         data.pop("keff_layout")
     z_full = torch_mg.VCycle(levels, dtype=torch.float32)(r)
-    err = float((z_packed - z_full).norm() / z_full.norm())
-    print(f"packed vs full fp32 V-cycle: {err:.2e}")
+    err = float((z_node - z_full).norm() / z_full.norm())
+    print(f"node stencil vs full fp32 V-cycle: {err:.2e}")
     assert err < 1e-5
 
 
