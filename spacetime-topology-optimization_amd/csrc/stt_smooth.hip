@@ -198,9 +198,28 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
   const int tgy = y0 - 1 + ti, tgx = x0 - 1 + tk;
   auto clampy = [&](int gy) -> uint32_t { return (uint32_t)min(max(gy, 0), nny - 1); };
 
+  // post: the coarse correction at the tile's nodes of the 2 x 2 coarser grid (at most
+  // kCY x kCX of them from (cy0, cx0), the coarse node at or before the halo's first
+  // fine node), staged in LDS
+  constexpr int kCY = kHY / 2 + 1, kCX = kHX / 2 + 1;
+  __shared__ Vec<float, D> sE[POST ? kCY : 1][POST ? kCX : 1];
+  const int cy0 = (y0 + 1) / 2 - 1, cx0 = (x0 + 1) / 2 - 1;  // floor((y0 - 1) / 2), y0 >= -1
+  constexpr int kCE = (kCY * kCX + kBlock - 1) / kBlock;
+  Vec<float, D> cev[POST ? kCE : 1];
+  if constexpr (POST) {
+    const int cnny = a.nely / 2 + 1;
+#pragma unroll
+    for (int m = 0; m < kCE; ++m) {
+      const int e = tix + m * kBlock, cy = cy0 + e / kCX, cx = cx0 + e % kCX;
+      cev[m] = zvec<float, D>();
+      if (e < kCY * kCX && cy >= 0 && cy < cnny && cx >= 0 && cx < a.cnnx)
+        cev[m] = ld<float, D>(ec, (uint32_t)cy * (uint32_t)a.cnnx + (uint32_t)cx);
+    }
+  }
+
   // loads of one staged node (gy, gx), both inside the grid
   struct Raw {
-    Vec<float, D> x, c00, c01, c10, c11;  // post: x2 and the coarse values around
+    Vec<float, D> x;  // post: x2
     uint32_t word;
     Vec<double, D> b;                      // pre: b and D^-1 (and Ap with UPD)
     Vec<float, D> dv;
@@ -213,14 +232,6 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
     if constexpr (POST) {
       r.x = ld<float, D>(xin, node);
       r.word = mask[(node * D) >> 5];
-      // the 2 x 2 coarse nodes around; at an even fine position the second one of a
-      // pair is the first one again (its weight is 0)
-      const uint32_t c00 = (gy >> 1) * (uint32_t)a.cnnx + (gx >> 1);
-      const uint32_t c10 = c00 + (gy & 1u) * (uint32_t)a.cnnx;
-      r.c00 = ld<float, D>(ec, c00);
-      r.c01 = ld<float, D>(ec, c00 + (gx & 1u));
-      r.c10 = ld<float, D>(ec, c10);
-      r.c11 = ld<float, D>(ec, c10 + (gx & 1u));
     } else {
       r.b = ld<double, D>(b, node);
       r.dv = ld<float, D>(invd, node);
@@ -246,10 +257,16 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
       const uint32_t bits = (r.word >> ((node * D) & 31)) & ((1u << D) - 1u);
       const float wy1 = (gy & 1u) ? 0.5f : 0.0f, wy0 = 1.0f - wy1;
       const float wx1 = (gx & 1u) ? 0.5f : 0.0f, wx0 = 1.0f - wx1;
+      // the 2 x 2 coarse nodes around; at an even fine position the second one of a
+      // pair is the first one again (its weight is 0)
+      const int ly = (int)(gy >> 1) - cy0, lx = (int)(gx >> 1) - cx0;
+      const int ly1 = ly + (int)(gy & 1u), lx1 = lx + (int)(gx & 1u);
+      const Vec<float, D> c00 = sE[ly][lx], c01 = sE[ly][lx1];
+      const Vec<float, D> c10 = sE[ly1][lx], c11 = sE[ly1][lx1];
 #pragma unroll
       for (int k = 0; k < D; ++k) {
-        const float top = wx0 * r.c00.v[k] + wx1 * r.c01.v[k];
-        const float bot = wx0 * r.c10.v[k] + wx1 * r.c11.v[k];
+        const float top = wx0 * c00.v[k] + wx1 * c01.v[k];
+        const float bot = wx0 * c10.v[k] + wx1 * c11.v[k];
         float u = r.x.v[k];
         if ((bits >> k) & 1u) u += wy0 * top + wy1 * bot;
         s.u.v[k] = ok ? u : 0.0f;
@@ -284,6 +301,15 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
   if (tix >= kTailU && tix < kTail)
     tr = rho[(uint32_t)min(max(tgy, 0), a.nely - 1) * (uint32_t)a.nelx +
              (uint32_t)min(tgx, a.nelx - 1)];
+
+  if constexpr (POST) {
+#pragma unroll
+    for (int m = 0; m < kCE; ++m) {
+      const int e = tix + m * kBlock;
+      if (e < kCY * kCX) sE[e / kCX][e % kCX] = cev[m];
+    }
+    __syncthreads();  // the prolongation below reads them
+  }
 
   // ---- staged values to LDS
   // UPD: squares of the updated residual at this thread's output nodes, and its
