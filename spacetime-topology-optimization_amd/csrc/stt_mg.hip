@@ -340,7 +340,7 @@ template int prolong_add<float>(int, int, int, int, int, int, const float*, long
 // One wavefront per output dof: the 64 lanes stream one row of the dense
 // inverse (coalesced) and reduce in fp64.
 
-template <typename T>
+template <typename T, bool PAIRS>
 __global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const double* cinv,
                                                            long long scinv, const T* b,
                                                            long long sb, const uint32_t* mask,
@@ -354,7 +354,19 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const do
   const double* Ai = cinv + row * scinv + i * n;
   const T* bb = b + row * sb;
   double acc = 0.0;
-  for (long long j = lane; j < n; j += kWave) acc += Ai[j] * (double)bb[j];
+  if (PAIRS) {
+    // 16-byte loads (n even and 16-byte aligned rows), several in flight per lane
+    const long long n2 = n >> 1;
+    const double2* A2 = reinterpret_cast<const double2*>(Ai);
+#pragma unroll 8
+    for (long long j = lane; j < n2; j += kWave) {
+      const double2 av = A2[j];
+      acc += av.x * (double)bb[2 * j] + av.y * (double)bb[2 * j + 1];
+    }
+  } else {
+#pragma unroll 8
+    for (long long j = lane; j < n; j += kWave) acc += Ai[j] * (double)bb[j];
+  }
   acc = wave_sum(acc);
   if (lane == 0) x[row * sx + i] = mask_free(mask, i) ? static_cast<T>(acc) : T(0);
 }
@@ -368,8 +380,14 @@ int coarse_solve(long long n, int rows, const double* cinv, long long scinv, con
     set_error("coarse_solve: coarsest level has no dense inverse");
     return STTOPT_EINVAL;
   }
-  hipLaunchKernelGGL((coarse_solve_kernel<T>), dim3(ceil_div(n, 4), rows), dim3(256), 0, s,
-                     n, cinv, scinv, b, sb, mask, x, sx, active);
+  const bool pairs = n % 2 == 0 && scinv % 2 == 0 &&
+                     (reinterpret_cast<uintptr_t>(cinv) & 15) == 0;
+  if (pairs)
+    hipLaunchKernelGGL((coarse_solve_kernel<T, true>), dim3(ceil_div(n, 4), rows), dim3(256),
+                       0, s, n, cinv, scinv, b, sb, mask, x, sx, active);
+  else
+    hipLaunchKernelGGL((coarse_solve_kernel<T, false>), dim3(ceil_div(n, 4), rows), dim3(256),
+                       0, s, n, cinv, scinv, b, sb, mask, x, sx, active);
   STT_CHECK_LAUNCH("coarse_solve_kernel");
   return STTOPT_OK;
 }
