@@ -22,6 +22,7 @@ torch.float64` (or pass `dtype=`) for the all-fp64 cycle of the reference.
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import dataclasses
 import functools
@@ -54,6 +55,28 @@ PARAM_MAX_CHILDREN = 4
 #: coarse element runs over its block, so very deep levels lack parallelism); deeper
 #: ones from the level above.
 GALERKIN_MAX_BLOCK = 64
+
+
+#: Per-position Galerkin tables of the coarsening sequences built so far.  They depend
+#: on the reference element matrix and the coarsening factors only, not on the
+#: densities, so one hierarchy build per step reuses them.  Keyed by the KE tensor
+#: object and its version counter (an in-place change invalidates); each entry keeps
+#: its KE alive, so the id cannot be reused while cached.
+_TABLES: collections.OrderedDict = collections.OrderedDict()
+_TABLES_MAX = 32
+
+
+def _cached_table(KE: Tensor, key: tuple, make):
+    full = (id(KE), KE._version, key)
+    hit = _TABLES.get(full)
+    if hit is not None and hit[0] is KE:
+        _TABLES.move_to_end(full)
+        return hit[1]
+    value = make()
+    _TABLES[full] = (KE, value)
+    while len(_TABLES) > _TABLES_MAX:
+        _TABLES.popitem(last=False)
+    return value
 
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
@@ -414,6 +437,7 @@ def build_hierarchy(
     k = 4 * d
     soa = None
     ptab = None  # (Ky, Kx, k, k) per-position matrices of the current coarsest level
+    seq: tuple = ()  # the coarsening factors so far (the tables' cache key)
     while True:
         top = levels[-1]
         if top.nelx * top.nely <= max_coarse_elements:
@@ -426,23 +450,32 @@ def build_hierarchy(
         csoa = torch.empty(rows, k * k, cnelx * cnely, dtype=torch.float64, device=dev)
         # per-position matrices of the new level: level-0 element (gy, gx) of the block
         # under one coarse element contributes density * ptab[gy, gx]
+        seq = seq + ((kx, ky),)
         if soa is None:
-            ptab = torch.einsum("mpi,pq,mqj->mij", S, KE64, S).reshape(ky, kx, k, k)
+            ptab = _cached_table(
+                KE, ("ptab", seq, str(dev), d),
+                lambda: torch.einsum("mpi,pq,mqj->mij", S, KE64, S).reshape(ky, kx, k, k))
         elif ptab is not None:
-            Ky, Kx = ptab.shape[:2]
-            nxt = torch.empty(ky, Ky, kx, Kx, k, k, dtype=torch.float64, device=dev)
-            for my in range(ky):
-                for mx in range(kx):
-                    Sm = S[my * kx + mx]
-                    nxt[my, :, mx] = torch.einsum("pi,yxpq,qj->yxij", Sm, ptab, Sm)
-            ptab = nxt.reshape(ky * Ky, kx * Kx, k, k)
+            def deeper(prev=ptab, S=S, kx=kx, ky=ky):
+                Ky, Kx = prev.shape[:2]
+                nxt = torch.empty(ky, Ky, kx, Kx, k, k, dtype=torch.float64, device=dev)
+                for my in range(ky):
+                    for mx in range(kx):
+                        Sm = S[my * kx + mx]
+                        nxt[my, :, mx] = torch.einsum("pi,yxpq,qj->yxij", Sm, prev, Sm)
+                return nxt.reshape(ky * Ky, kx * Kx, k, k)
+
+            ptab = _cached_table(KE, ("ptab", seq, str(dev), d), deeper)
         if soa is None or (react64 is None and ptab is not None
                            and ptab.shape[0] * ptab.shape[1] <= GALERKIN_MAX_BLOCK):
             # density form: straight from the level-0 densities (any depth)
             GM = None
             if soa is None and react64 is not None:
-                ME = torch.eye(k, dtype=torch.float64, device=dev) / 4
-                GM = torch.einsum("mpi,pq,mqj->mij", S, ME, S).contiguous()
+                GM = _cached_table(
+                    KE, ("reaction", (kx, ky), str(dev), d),
+                    lambda: torch.einsum("mpi,pq,mqj->mij", S,
+                                         torch.eye(k, dtype=torch.float64, device=dev) / 4,
+                                         S).contiguous())
             table = ptab.reshape(-1, k * k).contiguous()
             _lib.check(
                 lib.sttopt_galerkin(
