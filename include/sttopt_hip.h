@@ -276,8 +276,11 @@ int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v,
  * Rows retire as they converge (per-row device flags): iters[row] receives each
  * row's iteration count and resid[row] its final relative residual.  flexible != 0
  * uses the Polak-Ribiere beta (for a V-cycle in fp32).  check_every: iterations
- * between host checks of the active count.  Returns STTOPT_ENOCONV when a row is
- * still above rtol after max_iter iterations (x, iters, resid are still written). */
+ * between copies of the active-row count to the host; each is read one check late
+ * (the device keeps the next iteration queued), so the solve may launch one more
+ * iteration that finds no active row.  The stream is synchronized at the start, at
+ * max_iter and at the end.  Returns STTOPT_ENOCONV when a row is still above rtol
+ * after max_iter iterations (x, iters, resid are still written). */
 int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double* x,
                  int64_t x_stride, double rtol, int32_t max_iter, int32_t flexible,
                  int32_t check_every, double* work, int32_t* iters, double* resid,

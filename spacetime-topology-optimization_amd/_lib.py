@@ -7,8 +7,10 @@ checks that its tensors live on a ROCm device -- there is no CPU path.
 
 from __future__ import annotations
 
+import collections
 import ctypes
 import pathlib
+import threading
 
 import torch
 
@@ -308,12 +310,28 @@ def dtype_code(dtype: torch.dtype) -> int:
     raise TypeError(f"unsupported dtype {dtype}; the HIP kernels take float64 or float32")
 
 
+_KE_HOST: "collections.OrderedDict" = collections.OrderedDict()
+_KE_HOST_LOCK = threading.Lock()
+
+
 def ke_array(KE: torch.Tensor) -> ctypes.Array:
-    """The element matrix as a host double[64] (row-major, zero padded)."""
+    """The element matrix as a host double[64] (row-major, zero padded).  Cached per KE
+    tensor and version (reading a device KE back synchronizes the stream); an entry
+    keeps its KE alive, so the id cannot be reused while cached.  Callers only read
+    the array."""
+    key = (id(KE), KE._version)
+    with _KE_HOST_LOCK:
+        hit = _KE_HOST.get(key)
+        if hit is not None and hit[0] is KE:
+            return hit[1]
     flat = KE.detach().to("cpu", torch.float64).reshape(-1).tolist()
     if len(flat) not in (16, 64):
         raise ValueError(f"KE must be 4x4 or 8x8, got {tuple(KE.shape)}")
     arr = (c_double * 64)()
     for i, v in enumerate(flat):
         arr[i] = v
+    with _KE_HOST_LOCK:
+        _KE_HOST[key] = (KE, arr)
+        while len(_KE_HOST) > 16:
+            _KE_HOST.popitem(last=False)
     return arr

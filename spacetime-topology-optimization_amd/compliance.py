@@ -133,6 +133,36 @@ def gravity_compliance(xPhys, tPhys, KE, edofMat, Emin, Emax, penal, ti, C, beta
 # -------------------------------------------------------------- fused path --
 
 _fe_cache = WeakTensorKeyDictionary()
+# free-dof masks per freedofs tensor (value: version, ndof, device, mask) and stage-time
+# tensors per (times, device): rebuilding them each step copies host data to the
+# device, which waits for the stream.  A new mask is complete (its stream synchronized)
+# before it is shared, so any stream may read it; the times tensor is copied from host
+# memory synchronously.
+_mask_cache = WeakTensorKeyDictionary()
+_times_cache: dict = {}
+
+
+def _free_mask_cached(ndof: int, freedofs, device) -> Tensor:
+    if not isinstance(freedofs, Tensor):
+        return torch_fem.free_mask(ndof, freedofs, device=device)
+    hit = _mask_cache.get(freedofs)
+    if hit is not None and hit[0] == freedofs._version and hit[1] == ndof and hit[2] == device:
+        return hit[3]
+    mask = torch_fem.free_mask(ndof, freedofs, device=device)
+    if mask.is_cuda:
+        torch.cuda.current_stream(mask.device).synchronize()
+    _mask_cache[freedofs] = (freedofs._version, ndof, device, mask)
+    return mask
+
+
+def _stage_times_tensor(times: list, dev) -> Tensor:
+    key = (tuple(times), str(dev))
+    hit = _times_cache.get(key)
+    if hit is None:
+        if len(_times_cache) > 32:
+            _times_cache.clear()
+        hit = _times_cache[key] = torch.tensor(times, dtype=torch.float64, device=dev)
+    return hit
 
 
 def gravity_weight(C, nelx: int, nely: int) -> float:
@@ -183,7 +213,7 @@ class _SpaceTimeCompliance(torch.autograd.Function):
         ns = len(times)
         x = xPhys.detach().to(torch.float64).contiguous().reshape(-1)
         t = tPhys.detach().to(torch.float64).contiguous().reshape(-1)
-        tk = torch.tensor(times, dtype=torch.float64, device=dev) if ns else None
+        tk = _stage_times_tensor(times, dev) if ns else None
         fext = F.detach().to(torch.float64).contiguous()
         density = torch.empty(1 + ns, nel, dtype=torch.float64, device=dev)
         load = torch.empty(1 + ns, ndof, dtype=torch.float64, device=dev)
@@ -265,7 +295,7 @@ def batched_whole_and_gravity_compliance(
     batched solve, `(c, [cg, ...], U)` (compliance.py:198-253)."""
     _lib.require_device(xPhys, tPhys, F)
     nely, nelx = xPhys.shape
-    mask = torch_fem.free_mask(ndof, freedofs, device=xPhys.device)
+    mask = _free_mask_cached(ndof, freedofs, xPhys.device)
     if _solve_fe_batched is not _FUSED_SOLVER or KE.shape[-1] != 8:
         return _batched_composed(xPhys, tPhys, KE, edofMat, Emin, Emax, penal, mask, F, ndof,
                                  C, beta_t, stage_times, x0)

@@ -650,6 +650,17 @@ static int* pinned_counter() {
   return p;
 }
 
+// Two events of the lagged convergence check (one per pinned slot), created once per
+// thread.
+static hipEvent_t* lag_events() {
+  static thread_local hipEvent_t ev[2] = {nullptr, nullptr};
+  for (int i = 0; i < 2; ++i) {
+    if (ev[i] == nullptr && hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
+      ev[i] = nullptr;
+  }
+  return (ev[0] != nullptr && ev[1] != nullptr) ? ev : nullptr;
+}
+
 int read_device_int(const int* dev, hipStream_t s, int* out) {
   int* h = pinned_counter();
   if (h == nullptr) {
@@ -745,6 +756,13 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   const bool fused = vc.pair();
   const int fused_tiles = pair_tiles(op);
   if (check_every < 1) check_every = 1;
+  int* lag_host = pinned_counter();
+  hipEvent_t* lag_ev = lag_events();
+  if (lag_host == nullptr || lag_ev == nullptr) {
+    set_error("sttopt_mgpcg: pinned readback word or events unavailable");
+    return STTOPT_EHIP;
+  }
+  int checks = 0;  // active-row counts copied back so far
   bool done = false;
   int k_last = 0;  // iteration whose x += alpha p is still pending
   for (int k = 1; k <= max_iter; ++k) {
@@ -778,14 +796,35 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
                        sc.part[1], rtol, k, sc.bsafe, resid, sc.active, iters, sc.counter);
     STT_CHECK_LAUNCH("pcg update");
     k_last = k;
-    if (k % check_every == 0 || k == max_iter) {
+    if (k == max_iter) {
       if (int e = read_device_int(sc.counter, s, &remaining)) return e;
-      if (remaining == 0) {
-        done = true;
-        break;
-      }
+      done = remaining == 0;
+      break;
     }
-    if (k == max_iter) break;
+    if (k % check_every == 0) {
+      // This iteration's count of active rows goes to pinned host memory without a
+      // wait; the host reads the previous check's count, by which time this iteration
+      // is queued behind it, so the device never drains.  When that count is 0 every
+      // row converged there: this iteration's launches found no active row, and the
+      // converged rows' last x += alpha p was in that iteration's direction update.
+      // checks alternate between two pinned slots and events
+      const int cur = checks & 1, prev = cur ^ 1;
+      hipError_t e = hipMemcpyAsync(lag_host + 1 + cur, sc.counter, sizeof(int),
+                                    hipMemcpyDeviceToHost, s);
+      if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(active rows)");
+      e = hipEventRecord(lag_ev[cur], s);
+      if (e != hipSuccess) return hip_status(e, "hipEventRecord");
+      if (checks > 0) {
+        e = hipEventSynchronize(lag_ev[prev]);
+        if (e != hipSuccess) return hip_status(e, "hipEventSynchronize");
+        if (reinterpret_cast<volatile int*>(lag_host)[1 + prev] == 0) {
+          done = true;
+          k_last = 0;
+          break;
+        }
+      }
+      ++checks;
+    }
     if (fuse_now) {
       if (int e = vc.pair_rest(r2, n, z, n, sc.part[2])) return e;
       std::swap(r, r2);
@@ -822,6 +861,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       if (e != hipSuccess) return hip_status(e, "hipEventElapsedTime");
       int act = 0;
       for (int r = 0; r < rows; ++r) act += h[r] >= g_apply_prof.iter_of[i] ? 1 : 0;
+      if (act == 0) continue;  // the lagged check's launch after convergence: no rows
       g_apply_prof.seconds += ms * 1e-3;
       g_apply_prof.bytes += per_row * act;
       g_apply_prof.launches += 1;

@@ -26,6 +26,7 @@ import collections
 import ctypes
 import dataclasses
 import functools
+import threading
 from dataclasses import dataclass, field
 
 import torch
@@ -61,22 +62,32 @@ GALERKIN_MAX_BLOCK = 64
 #: on the reference element matrix and the coarsening factors only, not on the
 #: densities, so one hierarchy build per step reuses them.  Keyed by the KE tensor
 #: object and its version counter (an in-place change invalidates); each entry keeps
-#: its KE alive, so the id cannot be reused while cached.
+#: its KE alive, so the id cannot be reused while cached.  A new table is complete on
+#: the device before it is shared (its stream is synchronized once), and the device is
+#: synchronized before an entry is dropped, so tables are safe to read from any stream
+#: (the uneven-batch solver runs buckets on their own streams and threads).
 _TABLES: collections.OrderedDict = collections.OrderedDict()
 _TABLES_MAX = 32
+_TABLES_LOCK = threading.Lock()
 
 
 def _cached_table(KE: Tensor, key: tuple, make):
     full = (id(KE), KE._version, key)
-    hit = _TABLES.get(full)
-    if hit is not None and hit[0] is KE:
-        _TABLES.move_to_end(full)
-        return hit[1]
-    value = make()
-    _TABLES[full] = (KE, value)
-    while len(_TABLES) > _TABLES_MAX:
-        _TABLES.popitem(last=False)
-    return value
+    with _TABLES_LOCK:
+        hit = _TABLES.get(full)
+        if hit is not None and hit[0] is KE:
+            _TABLES.move_to_end(full)
+            return hit[1]
+        value = make()
+        if value.is_cuda:
+            torch.cuda.current_stream(value.device).synchronize()
+        _TABLES[full] = (KE, value)
+        if len(_TABLES) > _TABLES_MAX:
+            if value.is_cuda:
+                torch.cuda.synchronize(value.device)
+            while len(_TABLES) > _TABLES_MAX:
+                _TABLES.popitem(last=False)
+        return value
 
 
 def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
