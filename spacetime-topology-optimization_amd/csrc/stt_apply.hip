@@ -85,8 +85,10 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 
   if (c < a.nnx && r < a.nny) {
     const T* u = a.u + row * a.su;
-    // kSmoothFirst: the input is the first sweep from zero, x1 = w D^-1 b at free dofs
-    // (the expression of jacobi_first_kernel), formed here at each neighbour
+    // kSmoothFirst: the input is the first sweep from zero, x1 = w D^-1 b (the
+    // expression of jacobi_first_kernel), formed here at each neighbour; D^-1 is 0 at
+    // fixed dofs, so x1 is zero there without the mask (a -0 at most, which sums and
+    // compares as +0)
     const bool first = a.mode == kSmoothFirst;
     // 3 x 3 node neighbourhood, zero outside the grid (and at fixed dofs when the
     // input has to be projected).
@@ -110,9 +112,8 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
 #pragma unroll
           for (int k = 0; k < D; ++k) w.v[k] = T(0);
         }
-        const uint32_t word = ((a.proj_in || first) && ok && a.mask != nullptr)
-                                  ? a.mask[(node * D) >> 5]
-                                  : ~0u;
+        const uint32_t word =
+            (a.proj_in && ok && a.mask != nullptr) ? a.mask[(node * D) >> 5] : ~0u;
 #pragma unroll
         for (int k = 0; k < D; ++k)
           U[dr][dc][k] = ((word >> ((node * D + k) & 31)) & 1u) ? w.v[k] : T(0);

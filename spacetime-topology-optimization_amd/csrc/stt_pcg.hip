@@ -397,10 +397,16 @@ struct VCycle {
     T* alt = buf(L.vx2);
     const int ns = mg->n_smooth;
     // the first two sweeps in one pass (the first one, from zero, formed where the
-    // second reads it); the LDS tile kernel for the fp32 level 1 at 2 x 2, the
-    // one-node-per-thread kernel elsewhere
-    const bool fuse_first =
-        ns >= 2 && (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0;
+    // second reads it): on the LDS tile of the fp32 level 1 at 2 x 2, and on the
+    // one-node-per-thread kernel for small levels, where the launch it saves costs
+    // more than gathering b and D^-1 instead of x at the 9 neighbours (on a 256 x 128
+    // level the separate first sweep and pass take 5 + 17 us, the fused pass 28 us)
+    constexpr long long kFuseFirstMaxNodes = 16384;
+    const bool tile_level = sizeof(T) == sizeof(float) && op[l].ptab != nullptr &&
+                            op[l].pkx == 2 && op[l].pky == 2;
+    const bool fuse_first = ns >= 2 &&
+                            (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0 &&
+                            (tile_level || op[l].nnodes() <= kFuseFirstMaxNodes);
     if (fuse_first) {
       if (int e = stencil<T, T, T>(op[l], rows, kSmoothFirst, false, kNoDot,
                                    static_cast<const T*>(nullptr), 0, b, nl, invd(l), sd(l), x,
