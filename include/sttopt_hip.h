@@ -48,11 +48,12 @@ extern "C" {
 /* layouts of coarse element matrices (entry-major, [rows][entries][nel]):
  * FULL: the (4d)^2 entries a * 4d + b; UPPER: the (4d)(4d+1)/2 entries a <= b of the
  * symmetric matrix, row-major over a (entry index a * 4d - a (a - 1) / 2 + b - a).
- * NODE: the assembled operator as a node stencil, coefficient-major
- * [rows][d(d+1)/2 + 4 d^2][nodes]: the upper triangle of the node's own d x d block,
- * then its d x d blocks (row: the node's dof, column: the neighbour's) to the
- * neighbours at (node row, column) + (0, 1), (1, -1), (1, 0), (1, 1); the other four
- * neighbours' blocks are theirs, transposed (19 values per node at d = 2) */
+ * NODE: the assembled operator as a node stencil, one record per node
+ * [rows][nodes][4 + 4 d^2]: the upper triangle of the node's own d x d block (row-major,
+ * zero padded to 4 values), then its d x d blocks (row: the node's dof, column: the
+ * neighbour's) to the neighbours at (node row, column) + (0, 1), (1, -1), (1, 0),
+ * (1, 1); the other four neighbours' blocks are theirs, transposed (20 values per node
+ * at d = 2; records and blocks 16-byte aligned for vector loads) */
 #define STTOPT_KEFF_FULL 0
 #define STTOPT_KEFF_UPPER 1
 #define STTOPT_KEFF_NODE 2

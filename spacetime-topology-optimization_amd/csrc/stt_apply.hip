@@ -68,6 +68,33 @@ __host__ __device__ constexpr int upper_index(int r, int c, int K) {
   return r <= c ? r * K - r * (r - 1) / 2 + (c - r) : c * K - c * (c - 1) / 2 + (r - c);
 }
 
+// N consecutive values starting 16-byte aligned (fp32: N a multiple of 4 or N = 1;
+// fp64: N even or N = 1), with vector loads.
+template <typename T, int N>
+__device__ __forceinline__ void load_record(const T* p, T (&out)[N]) {
+  constexpr int W = 16 / (int)sizeof(T);  // values per 16-byte load
+  if constexpr (N % W == 0) {
+    using V = HIP_vector_type<T, W>;
+    const V* pv = reinterpret_cast<const V*>(p);
+#pragma unroll
+    for (int q = 0; q < N / W; ++q) {
+      const V v = pv[q];
+      if constexpr (W == 4) {
+        out[4 * q] = v.x;
+        out[4 * q + 1] = v.y;
+        out[4 * q + 2] = v.z;
+        out[4 * q + 3] = v.w;
+      } else {
+        out[2 * q] = v.x;
+        out[2 * q + 1] = v.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < N; ++q) out[q] = p[q];
+  }
+}
+
 // Coarse-level stencil (SRC 1 or 2): one node per thread on 64 x 4 node tiles.
 // PKX, PKY > 0 fix the parameterized block at compile time (fully unrolled).
 template <typename T, typename TB, typename TY, int D, int SRC, int PKX = 0, int PKY = 0>
@@ -125,21 +152,22 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
     for (int i = 0; i < D; ++i) acc[i] = T(0);
     if constexpr (SRC == kSrcNode) {
       // own block (upper triangle), the forward blocks E, NW, N, NE stored here, and the
-      // backward neighbours' forward blocks toward this node, transposed
-      constexpr int ND = D * (D + 1) / 2;
+      // backward neighbours' forward blocks toward this node, transposed; each record
+      // and block is read with vector loads
+      constexpr int NS = node_stencil_stride(D), ND = 4, B = D * D;
       constexpr int fdr[4] = {0, 1, 1, 1}, fdc[4] = {1, -1, 0, 1};
-      const long long nn = (long long)a.nnx * a.nny;
-      const long long node = (long long)r * a.nnx + c;
       const T* S = a.keff + row * a.sk;
+      const long long node = (long long)r * a.nnx + c;
+      T own[NS];
+      load_record<T, NS>(S + node * NS, own);
       int idx = 0;
 #pragma unroll
       for (int i = 0; i < D; ++i) {
 #pragma unroll
         for (int k = i; k < D; ++k) {
-          const T v = S[(long long)idx * nn + node];
+          const T v = own[idx++];
           acc[i] += v * U[1][1][k];
           if (k != i) acc[k] += v * U[1][1][i];
-          ++idx;
         }
       }
 #pragma unroll
@@ -150,8 +178,7 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
           for (int i = 0; i < D; ++i) {
 #pragma unroll
             for (int k = 0; k < D; ++k)
-              acc[i] += S[(long long)(ND + (f * D + i) * D + k) * nn + node] *
-                        U[1 + fdr[f]][1 + fdc[f]][k];
+              acc[i] += own[ND + (f * D + i) * D + k] * U[1 + fdr[f]][1 + fdc[f]][k];
           }
         }
       }
@@ -159,13 +186,12 @@ __global__ __launch_bounds__(kBlock) void stencil_kernel(StencilArgs<T, TB, TY> 
       for (int f = 0; f < 4; ++f) {
         const int rr = r - fdr[f], cc = c - fdc[f];
         if (rr >= 0 && cc >= 0 && cc < a.nnx) {
-          const long long m = (long long)rr * a.nnx + cc;
+          T blk[B];
+          load_record<T, B>(S + ((long long)rr * a.nnx + cc) * NS + ND + f * B, blk);
 #pragma unroll
           for (int i = 0; i < D; ++i) {
 #pragma unroll
-            for (int k = 0; k < D; ++k)
-              acc[i] += S[(long long)(ND + (f * D + k) * D + i) * nn + m] *
-                        U[1 - fdr[f]][1 - fdc[f]][k];
+            for (int k = 0; k < D; ++k) acc[i] += blk[k * D + i] * U[1 - fdr[f]][1 - fdc[f]][k];
           }
         }
       }

@@ -5,6 +5,11 @@
 
 namespace stt {
 
+// Values per node of the STTOPT_KEFF_NODE layout: the own block's upper triangle padded
+// to 4, then the four forward d x d blocks (20 at d = 2, 8 at d = 1; records and blocks
+// start 16-byte aligned in fp32).
+__host__ __device__ constexpr int node_stencil_stride(int d) { return 4 + 4 * d * d; }
+
 // kSmoothFirst (level 0 only): two damped-Jacobi sweeps from x = 0 in one pass, the
 // first one (x0 = w D^-1 b) formed while staging the tile; also copies b into bcopy in
 // the stencil precision when given.

@@ -116,16 +116,17 @@ __global__ __launch_bounds__(256) void pack_keff_kernel(long long nel,
 }
 
 // Assembled node stencil of symmetric entry-major element matrices (STTOPT_KEFF_NODE):
-// per node, the upper triangle of its own d x d block, then the d x d couplings to its
-// E, NW, N, NE neighbours (node row + 0/1, column + 1/-1/0/1), each the sum over the
-// elements holding both nodes of (A + A^T) / 2.  The other four neighbours' couplings
-// are those neighbours' forward blocks, transposed.  Sums in fp64, one rounding to TO.
+// one record of node_stencil_stride(d) values per node: the upper triangle of its own
+// d x d block (zero padded to 4 values), then the d x d couplings to its E, NW, N, NE
+// neighbours (node row + 0/1, column + 1/-1/0/1), each the sum over the elements
+// holding both nodes of (A + A^T) / 2.  The other four neighbours' couplings are those
+// neighbours' forward blocks, transposed.  Sums in fp64, one rounding to TO.
 template <typename TO, int D>
 __global__ __launch_bounds__(256) void node_stencil_kernel(int nelx, int nely,
                                                            const double* __restrict__ keff,
                                                            long long sk, TO* __restrict__ out,
                                                            long long so) {
-  constexpr int K = 4 * D, ND = D * (D + 1) / 2, NS = ND + 4 * D * D;
+  constexpr int K = 4 * D, ND = 4, NS = node_stencil_stride(D);
   const int nnx = nelx + 1;
   const long long nn = (long long)nnx * (nely + 1), nel = (long long)nelx * nely;
   const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -166,9 +167,9 @@ __global__ __launch_bounds__(256) void node_stencil_kernel(int nelx, int nely,
       }
     }
   }
-  TO* o = out + row * so + node;
+  TO* o = out + row * so + node * NS;
 #pragma unroll
-  for (int k = 0; k < NS; ++k) o[(long long)k * nn] = static_cast<TO>(acc[k]);
+  for (int k = 0; k < NS; ++k) o[k] = static_cast<TO>(acc[k]);
 }
 
 // ------------------------------------------------------------ transfers -----

@@ -341,13 +341,12 @@ class _Level:
                 out["keff"] = self._soa
             else:
                 # the fp32 V-cycle applies the assembled operator as a node stencil: each
-                # node-pair coupling stored once (19 values per node at d = 2 against 36
-                # per element for the packed element matrices)
+                # node-pair coupling stored once (one 20-value record per node at d = 2
+                # against 36 values per element for the packed element matrices)
                 d = self.d
                 nn = (self.nelx + 1) * (self.nely + 1)
                 nrow = self._soa.shape[0]
-                node = torch.empty(nrow, d * (d + 1) // 2 + 4 * d * d, nn, dtype=dtype,
-                                   device=dev)
+                node = torch.empty(nrow, nn, 4 + 4 * d * d, dtype=dtype, device=dev)
                 _lib.check(
                     _lib.load().sttopt_node_stencil(
                         _lib.dtype_code(dtype), d, self.nelx, self.nely, nrow,

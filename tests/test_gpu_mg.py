@@ -347,9 +347,9 @@ def test_packed_upper_triangle_keff(dev):
 def test_node_stencil_equals_assembled_operator(dev, d, nelx, nely):
     """sttopt_node_stencil: every stored coefficient equals the entry of the operator
     assembled densely from the same element matrices ((A + A^T) / 2 summed over the
-    elements through the element dof map): the node's own block (upper triangle) and
-    its blocks to the E, NW, N, NE neighbours, in the documented coefficient order; in
-    fp64 (to rounding) and fp32 (one rounding)."""
+    elements through the element dof map): per node, its own block (upper triangle,
+    zero padded to 4 values) and its blocks to the E, NW, N, NE neighbours, in the
+    documented record order; in fp64 (to rounding) and fp32 (one rounding)."""
     from sttopt_amd import _lib, torch_mg
 
     lib = _lib.load()
@@ -365,24 +365,25 @@ def test_node_stencil_equals_assembled_operator(dev, d, nelx, nely):
     for e in range(nel):
         idx = edof[e]
         dense[:, idx[:, None], idx[None, :]] += A[:, e]
-    ns = d * (d + 1) // 2 + 4 * d * d
-    want = torch.zeros(rows, ns, nn, dtype=torch.float64, device=dev)
+    ns = 4 + 4 * d * d
+    want = torch.zeros(rows, nn, ns, dtype=torch.float64, device=dev)
     for node in range(nn):
         r, c = divmod(node, nnx)
         col = 0
         for i in range(d):
             for kk in range(i, d):
-                want[:, col, node] = dense[:, d * node + i, d * node + kk]
+                want[:, node, col] = dense[:, d * node + i, d * node + kk]
                 col += 1
+        col = 4
         for dr, dc in ((0, 1), (1, -1), (1, 0), (1, 1)):
             rr, cc = r + dr, c + dc
             for i in range(d):
                 for kk in range(d):
                     if 0 <= rr < nny and 0 <= cc < nnx:
-                        want[:, col, node] = dense[:, d * node + i, d * (rr * nnx + cc) + kk]
+                        want[:, node, col] = dense[:, d * node + i, d * (rr * nnx + cc) + kk]
                     col += 1
     for dtype in (torch.float64, torch.float32):
-        out = torch.empty(rows, ns, nn, dtype=dtype, device=dev)
+        out = torch.empty(rows, nn, ns, dtype=dtype, device=dev)
         _lib.check(lib.sttopt_node_stencil(_lib.dtype_code(dtype), d, nelx, nely, rows,
                                            full.data_ptr(), full.stride(0), out.data_ptr(),
                                            out.stride(0), _lib.stream_handle(dev)), "node")
