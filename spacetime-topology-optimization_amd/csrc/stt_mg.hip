@@ -222,6 +222,88 @@ __global__ __launch_bounds__(256) void restrict_kernel(int cnelx, int cnely, int
   store_node<T, D>(coarse + row * sc, cnode, out);
 }
 
+// 2 x 2 restriction with one load per fine node and row: each thread loads the fine
+// nodes 2J and 2J + 1 of the three fine rows around its coarse node J; node 2J - 1 is
+// the left lane's 2J' + 1 (a wave-wide DPP shift), loaded directly only by lane 0.  The
+// sums run in the order of restrict_kernel (dx = -1, 0, 1), so results are identical.
+template <typename T>
+__device__ __forceinline__ T lane_from_left(T v) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(
+        __builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x138, 0xF, 0xF, false));
+  } else {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), 0x138, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), 0x138, 0xF, 0xF, false);
+    return __hiloint2double(hi, lo);
+  }
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void restrict22_kernel(int cnelx, int cnely,
+                                                         const T* __restrict__ fine,
+                                                         long long sf,
+                                                         const uint32_t* __restrict__ cmask,
+                                                         T* __restrict__ coarse, long long sc,
+                                                         const int* __restrict__ active) {
+  const int row = blockIdx.y;
+  if (active != nullptr && active[row] == 0) return;  // whole block
+  const int cnnx = cnelx + 1, cnny = cnely + 1;
+  const int fnnx = 2 * cnelx + 1, fnny = 2 * cnely + 1;
+  const long long cn = (long long)cnnx * cnny;
+  const long long cnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = cnode < cn;  // every lane takes part in the shifts
+  const long long cnc = valid ? cnode : cn - 1;
+  const int I = (int)(cnc / cnnx), J = (int)(cnc - (long long)I * cnnx);
+  const int lane = threadIdx.x & (kWave - 1);
+  const bool own_left = lane == 0;  // no left lane in this wavefront
+  const T* f = fine + row * sf;
+  T acc[D];
+#pragma unroll
+  for (int k = 0; k < D; ++k) acc[k] = T(0);
+#pragma unroll
+  for (int dy = -1; dy <= 1; ++dy) {
+    const int R = 2 * I + dy;
+    const bool rin = R >= 0 && R < fnny;
+    const long long base = (long long)(rin ? R : 0) * fnnx;
+    Vec<T, D> a, b, l;
+#pragma unroll
+    for (int k = 0; k < D; ++k) a.v[k] = b.v[k] = l.v[k] = T(0);
+    if (rin) a = load_node<T, D>(f, base + 2 * J);
+    if (rin && 2 * J + 1 < fnnx) b = load_node<T, D>(f, base + 2 * J + 1);
+#pragma unroll
+    for (int k = 0; k < D; ++k) l.v[k] = lane_from_left(b.v[k]);
+    if (own_left && rin && J > 0) l = load_node<T, D>(f, base + 2 * J - 1);
+    if (J == 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) l.v[k] = T(0);
+    }
+    if (!rin) continue;
+    const T wy = T(2 - (dy < 0 ? -dy : dy)) / T(2);
+    T racc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) racc[k] = T(0);
+    const T wh = T(1) / T(2), w1 = T(2) / T(2);
+    if (J > 0) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) racc[k] += wh * l.v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) racc[k] += w1 * a.v[k];
+    if (2 * J + 1 < fnnx) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) racc[k] += wh * b.v[k];
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[k] += wy * racc[k];
+  }
+  if (!valid) return;
+  const uint32_t word = cmask != nullptr ? cmask[(cnode * D) >> 5] : ~0u;
+  Vec<T, D> out;
+#pragma unroll
+  for (int k = 0; k < D; ++k) out.v[k] = ((word >> ((cnode * D + k) & 31)) & 1u) ? acc[k] : T(0);
+  store_node<T, D>(coarse + row * sc, cnode, out);
+}
+
 template <typename T, int D, int KX, int KY>
 __global__ __launch_bounds__(256) void prolong_add_kernel(int cnelx, int cnely, int kx_,
                                                           int ky_, const T* __restrict__ coarse,
@@ -278,15 +360,15 @@ int restrict_to(int d, int cnelx, int cnely, int kx, int ky, int rows, const T* 
   const bool k22 = kx == 2 && ky == 2;
   if (d == 2) {
     if (k22)
-      hipLaunchKernelGGL((restrict_kernel<T, 2, 2, 2>), grid, block, 0, s, cnelx, cnely, kx,
-                         ky, fine, sf, cmask, coarse, sc, active);
+      hipLaunchKernelGGL((restrict22_kernel<T, 2>), grid, block, 0, s, cnelx, cnely, fine, sf,
+                         cmask, coarse, sc, active);
     else
       hipLaunchKernelGGL((restrict_kernel<T, 2, 0, 0>), grid, block, 0, s, cnelx, cnely, kx,
                          ky, fine, sf, cmask, coarse, sc, active);
   } else {
     if (k22)
-      hipLaunchKernelGGL((restrict_kernel<T, 1, 2, 2>), grid, block, 0, s, cnelx, cnely, kx,
-                         ky, fine, sf, cmask, coarse, sc, active);
+      hipLaunchKernelGGL((restrict22_kernel<T, 1>), grid, block, 0, s, cnelx, cnely, fine, sf,
+                         cmask, coarse, sc, active);
     else
       hipLaunchKernelGGL((restrict_kernel<T, 1, 0, 0>), grid, block, 0, s, cnelx, cnely, kx,
                          ky, fine, sf, cmask, coarse, sc, active);
