@@ -318,7 +318,9 @@ int sttopt_stage_prologue(int32_t nelx, int32_t nely, int32_t nstage,
 
 /* Space-time compliance epilogue and its hand-derived adjoint
  * (compliance.py:244-253; tests/reference/compliance.py:24-110).
- * comp[row] = sum_e SIMP(xt_row)_e * c_e(U_row).  When dx != NULL, also
+ * When comp != NULL: comp[row] = sum_e SIMP(xt_row)_e * c_e(U_row), with the
+ * SIMP densities read from density ([1+nstage][nel], as sttopt_stage_prologue wrote
+ * them; required with comp).  When dx != NULL, also
  * dx[e] = sum_rows g[row] dc_row/dx_e and dt[e] = sum_rows g[row] dc_row/dt_e with
  * the closed forms of the self-adjoint compliance (lambda = 2U), including the
  * gravity load term -2 (C^T U_y)_e.  scratch: sttopt_stage_scratch_count doubles. */
@@ -326,8 +328,8 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage,
                           const double* x, const double* t, const double* stage_times,
                           double beta, double emin, double emax, double penal,
                           double fe, const double* ke, const double* u,
-                          const double* g, double* comp, double* dx, double* dt,
-                          double* scratch, void* stream);
+                          const double* density, const double* g, double* comp,
+                          double* dx, double* dt, double* scratch, void* stream);
 int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage);
 
 /* HIP-event timing of the outer PCG's K(rho)p launches inside sttopt_mgpcg (off by

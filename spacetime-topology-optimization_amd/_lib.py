@@ -221,7 +221,7 @@ _SIGNATURES = {
     "sttopt_stage_epilogue": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,
-         c_double, _P, _P, _P, _P, _P, _P, _P, _P],
+         c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     ),
     "sttopt_stage_scratch_count": (c_int64, [c_int32, c_int32, c_int32]),
 }

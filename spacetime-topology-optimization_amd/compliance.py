@@ -245,8 +245,8 @@ class _SpaceTimeCompliance(torch.autograd.Function):
         _lib.check(
             lib.sttopt_stage_epilogue(
                 nelx, nely, ns, x.data_ptr(), t.data_ptr(), _lib.ptr(tk), beta_t, Emin, Emax,
-                penal, fe, _lib.ke_array(ke), U.data_ptr(), None, comp.data_ptr(), None, None,
-                scratch.data_ptr(), stream,
+                penal, fe, _lib.ke_array(ke), U.data_ptr(), density.data_ptr(), None,
+                comp.data_ptr(), None, None, scratch.data_ptr(), stream,
             ),
             "sttopt_stage_epilogue",
         )
@@ -266,8 +266,8 @@ class _SpaceTimeCompliance(torch.autograd.Function):
         _lib.check(
             _lib.load().sttopt_stage_epilogue(
                 nelx, nely, ns, x.data_ptr(), t.data_ptr(), _lib.ptr(tk), beta_t, Emin, Emax,
-                penal, fe, _lib.ke_array(ke), U.data_ptr(), g.data_ptr(), None, dx.data_ptr(),
-                dt.data_ptr(), None, _lib.stream_handle(dev),
+                penal, fe, _lib.ke_array(ke), U.data_ptr(), None, g.data_ptr(), None,
+                dx.data_ptr(), dt.data_ptr(), None, _lib.stream_handle(dev),
             ),
             "sttopt_stage_epilogue",
         )

@@ -19,9 +19,20 @@ __device__ __forceinline__ void gather_elem(const double* u, int nnx, int er, in
   }
 }
 
-// sum_i (sum_j a_j KE[j][i]) b_i  -- the reference's (a_e @ KE) . b_e
+// Copies the (4d)^2 KE entries into LDS; the caller synchronizes before reading them.
+// Every lane of a wavefront then reads the same entry at once (an LDS broadcast):
+// held in scalar registers, the 64 fp64 entries do not fit alongside the rest of the
+// kernel, and the compiler spills them to vector lanes and reads them back one
+// v_readlane at a time.
 template <int D>
-__device__ __forceinline__ double ke_form(const KE<double>& ke, const double* a,
+__device__ __forceinline__ void stage_ke(const KE<double>& ke, double* ske) {
+  constexpr int K = 4 * D;
+  for (int i = threadIdx.x; i < K * K; i += blockDim.x) ske[i] = ke.v[i];
+}
+
+// sum_i (sum_j a_j KE[j][i]) b_i  -- the reference's (a_e @ KE) . b_e; ke in LDS
+template <int D>
+__device__ __forceinline__ double ke_form(const double* ke, const double* a,
                                           const double* b) {
   constexpr int K = 4 * D;
   double s = 0.0;
@@ -29,7 +40,7 @@ __device__ __forceinline__ double ke_form(const KE<double>& ke, const double* a,
   for (int i = 0; i < K; ++i) {
     double t = 0.0;
 #pragma unroll
-    for (int j = 0; j < K; ++j) t += a[j] * ke.v[j * K + i];
+    for (int j = 0; j < K; ++j) t += a[j] * ke[j * K + i];
     s += t * b[i];
   }
   return s;
@@ -43,6 +54,9 @@ __global__ __launch_bounds__(256) void adjoint_grad_kernel(int nelx, int nely,
                                                            long long sgd, double* gr,
                                                            long long sgr) {
   constexpr int K = 4 * D;
+  __shared__ double ske[K * K];
+  stage_ke<D>(ke, ske);
+  __syncthreads();
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = blockIdx.y;
   if (e >= (long long)nelx * nely) return;
@@ -50,7 +64,7 @@ __global__ __launch_bounds__(256) void adjoint_grad_kernel(int nelx, int nely,
   double ue[K], le[K];
   gather_elem<D>(u + row * su, nelx + 1, er, ec, ue);
   gather_elem<D>(lam + row * sl, nelx + 1, er, ec, le);
-  if (gd != nullptr) gd[row * sgd + e] = -ke_form<D>(ke, le, ue);
+  if (gd != nullptr) gd[row * sgd + e] = -ke_form<D>(ske, le, ue);
   if (gr != nullptr) {
     double s = 0.0;
 #pragma unroll
@@ -65,13 +79,16 @@ __global__ __launch_bounds__(256) void strain_energy_kernel(int nelx, int nely,
                                                             KE<double> ke, double* ce,
                                                             long long sce) {
   constexpr int K = 4 * D;
+  __shared__ double ske[K * K];
+  stage_ke<D>(ke, ske);
+  __syncthreads();
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = blockIdx.y;
   if (e >= (long long)nelx * nely) return;
   const int er = (int)(e / nelx), ec = (int)(e - (long long)er * nelx);
   double ue[K];
   gather_elem<D>(u + row * su, nelx + 1, er, ec, ue);
-  ce[row * sce + e] = ke_form<D>(ke, ue, ue);
+  ce[row * sce + e] = ke_form<D>(ske, ue, ue);
 }
 
 // ------------------------------------------------------- space-time stages --
@@ -126,90 +143,82 @@ __device__ __forceinline__ double simp(double x, const StageParams& p) {
   return p.emin + pow_p(x, p.penal) * (p.emax - p.emin);
 }
 
-__global__ __launch_bounds__(256) void stage_density_kernel(StageParams p, const double* x,
-                                                            const double* t,
-                                                            const double* times,
-                                                            double* density) {
-  const long long nel = (long long)p.nelx * p.nely;
-  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int row = blockIdx.y;
-  if (e >= nel) return;
-  double xe = x[e];
-  if (row > 0) {
-    const MaskRow k = mask_row(times[row - 1], p.beta);
-    double m, dm;
-    stage_mask(t[e], k, p.beta, m, dm);
-    xe *= m;
-  }
-  density[row * nel + e] = simp(xe, p);
-}
-
-// Self-weight load of stage rows (row 0 copies the external load):
-// F_y(node) = -fe/4 * sum of x*m over the node's elements, F_x = 0.  A 64 x 4 node tile
-// per block: the 65 x 5 elements around it get x*m once, in LDS.
-__global__ __launch_bounds__(256) void stage_load_kernel(StageParams p, const double* x,
-                                                         const double* t, const double* times,
-                                                         const double* fext, double* load) {
+// Stage densities and self-weight loads, one 64 x 4 node tile of row blockIdx.y per
+// block.  x*m of the 65 x 5 elements around the tile is formed once, in LDS; the block
+// writes the SIMP densities of the elements whose top-left node lies in the tile and
+// the loads of the tile's nodes, F_y(node) = -fe/4 * sum of x*m over the node's
+// elements, F_x = 0.  Row 0 (the whole structure) takes x unmasked and copies the
+// external load.  The stage constants of the row are formed once per block.
+__global__ __launch_bounds__(256) void stage_prologue_kernel(StageParams p, const double* x,
+                                                             const double* t,
+                                                             const double* times,
+                                                             const double* fext,
+                                                             double* density, double* load) {
   __shared__ double sxm[5][65];
+  __shared__ MaskRow sk;
   const int nnx = p.nelx + 1, nny = p.nely + 1;
   const long long ndof = 2LL * nnx * nny;
+  const long long nel = (long long)p.nelx * p.nely;
   const int tnx = (nnx + 63) / 64;
   const int bx = blockIdx.x % tnx, by = blockIdx.x / tnx;
   const int c0 = bx * 64, r0 = by * 4;
-  const int c = c0 + (threadIdx.x & 63), r = r0 + (threadIdx.x >> 6);
+  const int lr = threadIdx.x >> 6, lc = threadIdx.x & 63;
+  const int c = c0 + lc, r = r0 + lr;
   const int row = blockIdx.y;
   const bool own = r < nny && c < nnx;
   const long long node = (long long)r * nnx + c;
+  const bool own_e = r < p.nely && c < p.nelx;  // the element whose top-left node is (r, c)
+  const long long e = (long long)r * p.nelx + c;
   if (row == 0) {
     if (own) {
       load[2 * node] = fext[2 * node];
       load[2 * node + 1] = fext[2 * node + 1];
     }
+    if (own_e) density[e] = simp(x[e], p);
     return;  // uniform: the whole block is row 0
   }
-  const MaskRow k = mask_row(times[row - 1], p.beta);
+  if (threadIdx.x == 0) sk = mask_row(times[row - 1], p.beta);
+  __syncthreads();
+  const MaskRow k = sk;
   for (int i = threadIdx.x; i < 5 * 65; i += 256) {
     const int er = r0 - 1 + i / 65, ec = c0 - 1 + i % 65;
     double v = 0.0;
     if (er >= 0 && er < p.nely && ec >= 0 && ec < p.nelx) {
-      const long long e = (long long)er * p.nelx + ec;
+      const long long ei = (long long)er * p.nelx + ec;
       double m, dm;
-      stage_mask(t[e], k, p.beta, m, dm);
-      v = x[e] * m;
+      stage_mask(t[ei], k, p.beta, m, dm);
+      v = x[ei] * m;
     }
     sxm[i / 65][i % 65] = v;
   }
   __syncthreads();
+  if (own_e) density[row * nel + e] = simp(sxm[lr + 1][lc + 1], p);
   if (!own) return;
-  const int lr = threadIdx.x >> 6, lc = threadIdx.x & 63;
   const double acc = sxm[lr][lc] + sxm[lr][lc + 1] + sxm[lr + 1][lc] + sxm[lr + 1][lc + 1];
   load[row * ndof + 2 * node] = 0.0;
   load[row * ndof + 2 * node + 1] = -(p.fe / 4.0) * acc;
 }
 
-// Per-row compliance partials: sum_e SIMP(xt_row)_e * (U_e^T KE U_e).
-__global__ __launch_bounds__(256) void stage_comp_kernel(StageParams p, const double* x,
-                                                         const double* t, const double* times,
+// Per-row compliance partials: sum_e density_e * (U_e^T KE U_e), with the SIMP
+// densities of the prologue (SIMP(x m) of the row's stage mask m).
+__global__ __launch_bounds__(256) void stage_comp_kernel(StageParams p,
+                                                         const double* __restrict__ density,
                                                          KE<double> ke, const double* u,
                                                          double* part, int nblk) {
   __shared__ double red[4];
+  __shared__ double ske[64];
   const long long nel = (long long)p.nelx * p.nely;
   const long long ndof = 2LL * (p.nelx + 1) * (p.nely + 1);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const int row = blockIdx.y;
+  stage_ke<2>(ke, ske);
+  __syncthreads();
   double v = 0.0;
   if (e < nel) {
     const int er = (int)(e / p.nelx), ec = (int)(e - (long long)er * p.nelx);
     double ue[8];
     gather_elem<2>(u + row * ndof, p.nelx + 1, er, ec, ue);
-    double xe = x[e];
-    if (row > 0) {
-      const MaskRow k = mask_row(times[row - 1], p.beta);
-      double m, dm;
-      stage_mask(t[e], k, p.beta, m, dm);
-      xe *= m;
-    }
-    v = simp(xe, p) * ke_form<2>(ke, ue, ue);
+    v = density[row * nel + e] * ke_form<2>(ske, ue, ue);
   }
   const double s = block_sum(v, red);
   if (threadIdx.x == 0) part[(long long)row * nblk + blockIdx.x] = s;
@@ -237,6 +246,9 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(StageParams p, const
                                                             double* dt) {
   constexpr int kChunk = 64;  // stage constants staged in LDS per chunk of stages
   __shared__ MaskRow rows[kChunk];
+  __shared__ double ske[64];
+  stage_ke<2>(ke, ske);
+  __syncthreads();
   const long long nel = (long long)p.nelx * p.nely;
   const long long ndof = 2LL * (p.nelx + 1) * (p.nely + 1);
   const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -247,7 +259,7 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(StageParams p, const
   const double xe = x[ee], te = p.nstage > 0 ? t[ee] : 0.0;
   double ue[8];
   gather_elem<2>(u, p.nelx + 1, er, ec, ue);
-  double gx = g[0] * (-p.penal * de * pow_p(xe, p.penal - 1.0) * ke_form<2>(ke, ue, ue));
+  double gx = g[0] * (-p.penal * de * pow_p(xe, p.penal - 1.0) * ke_form<2>(ske, ue, ue));
   double gt = 0.0;
   for (int k0 = 1; k0 <= p.nstage; k0 += kChunk) {
     const int nk = min(kChunk, p.nstage - k0 + 1);
@@ -261,7 +273,7 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(StageParams p, const
       double m, dm;
       stage_mask(te, rows[j], p.beta, m, dm);
       const double xt = xe * m;
-      const double A = -p.penal * de * pow_p(xt, p.penal - 1.0) * ke_form<2>(ke, ue, ue);
+      const double A = -p.penal * de * pow_p(xt, p.penal - 1.0) * ke_form<2>(ske, ue, ue);
       // (C^T U_y)_e = fe/4 * sum of the element's four nodal y-displacements
       const double cu = (p.fe / 4.0) * (ue[1] + ue[3] + ue[5] + ue[7]);
       const double s = A - 2.0 * cu;
@@ -338,13 +350,10 @@ int sttopt_stage_prologue(int32_t nelx, int32_t nely, int32_t nstage, const doub
     return STTOPT_EINVAL;
   }
   StageParams p{nelx, nely, nstage, beta, emin, emax, penal, fe};
-  const long long nel = (long long)nelx * nely;
   hipStream_t s = (hipStream_t)stream;
-  hipLaunchKernelGGL(stage_density_kernel, dim3(ceil_div(nel, 256), 1 + nstage), dim3(256), 0,
-                     s, p, x, t, stage_times, density);
-  const int load_tiles = ceil_div(nelx + 1, 64) * ceil_div(nely + 1, 4);
-  hipLaunchKernelGGL(stage_load_kernel, dim3(load_tiles, 1 + nstage), dim3(256), 0, s, p, x,
-                     t, stage_times, f_ext, load);
+  const int tiles = ceil_div(nelx + 1, 64) * ceil_div(nely + 1, 4);
+  hipLaunchKernelGGL(stage_prologue_kernel, dim3(tiles, 1 + nstage), dim3(256), 0, s, p, x,
+                     t, stage_times, f_ext, density, load);
   STT_CHECK_LAUNCH("stage prologue");
   return STTOPT_OK;
 }
@@ -357,8 +366,9 @@ int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage) {
 int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage, const double* x,
                           const double* t, const double* stage_times, double beta,
                           double emin, double emax, double penal, double fe, const double* ke,
-                          const double* u, const double* g, double* comp, double* dx,
-                          double* dt, double* scratch, void* stream) {
+                          const double* u, const double* density, const double* g,
+                          double* comp, double* dx, double* dt, double* scratch,
+                          void* stream) {
   if (nelx <= 0 || nely <= 0 || nstage < 0 || !x || !ke || !u ||
       (nstage > 0 && (!t || !stage_times))) {
     set_error("sttopt_stage_epilogue: invalid arguments");
@@ -370,12 +380,12 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage, const doub
   const int nblk = ceil_div(nel, 256);
   hipStream_t s = (hipStream_t)stream;
   if (comp != nullptr) {
-    if (scratch == nullptr) {
-      set_error("sttopt_stage_epilogue: compliance needs scratch");
+    if (scratch == nullptr || density == nullptr) {
+      set_error("sttopt_stage_epilogue: compliance needs scratch and the stage densities");
       return STTOPT_EINVAL;
     }
-    hipLaunchKernelGGL(stage_comp_kernel, dim3(nblk, 1 + nstage), dim3(256), 0, s, p, x, t,
-                       stage_times, k, u, scratch, nblk);
+    hipLaunchKernelGGL(stage_comp_kernel, dim3(nblk, 1 + nstage), dim3(256), 0, s, p, density,
+                       k, u, scratch, nblk);
     hipLaunchKernelGGL(row_finalize_kernel, dim3(1 + nstage), dim3(256), 0, s, scratch, nblk,
                        comp);
     STT_CHECK_LAUNCH("stage compliance");
