@@ -256,117 +256,108 @@ __global__ __launch_bounds__(256) void mma_rows_reduce_kernel(int m, int nchunk,
 
 // ------------------------------------------------------------------- Gram ----
 
-constexpr int kGJ = 64;  // columns per LDS stage
+// part[chunk][i][k] (i >= k) = sum over the chunk's columns j of GG_ij * (GG_kj / diagx_j),
+// GG = P diag(uxinv2) - Q diag(xlinv2): an m x m x (chunk) product on the fp64 matrix
+// cores.  The rows are padded to nt 16-row tiles; wave w owns the lower tile pairs
+// (I, K) = w, w + 4, ... in the order (0,0), (1,0), (1,1), (2,0), ... and accumulates
+// each over all of the chunk's columns (v_mfma_f64_16x16x4f64: lane l supplies
+// A(l % 16, l / 16) and B(l / 16, l % 16) of a 16 x 4 by 4 x 16 step and holds
+// D(l / 16 + 4 r, l % 16), r < 4).  GG and GG / diagx pass through LDS SW columns at a
+// time, k-major ([column][row], rows padded to 16 nt + 1); thread t forms column t % SW
+// for rows t / SW + (256 / SW) q, and the next stage's P, Q and column factors are in
+// flight while the current one is multiplied.  PW: tile pairs per wave at most.
+typedef double gram_d4 __attribute__((ext_vector_type(4)));
+constexpr int kGramRows = 8;  // rows per thread and stage: m <= 16 nt <= 256 / SW * 8
 
-// Dynamic LDS of the Gram kernel (doubles): the two stage arrays [m][kGJ + 1], reused
-// afterwards for the column groups' partial blocks [256][17].
+__host__ __device__ inline int gram_tiles(int m) { return (m + 15) / 16; }
+__host__ __device__ inline int gram_stage_cols(int m) {
+  const int nt = gram_tiles(m);
+  return nt <= 2 ? 64 : (nt <= 4 ? 32 : 16);
+}
 __host__ __device__ inline int gram_lds_doubles(int m) {
-  const int stage = 2 * m * (kGJ + 1), red = 256 * 17;
-  return stage > red ? stage : red;
+  return 2 * gram_stage_cols(m) * (16 * gram_tiles(m) + 1);
 }
 
-// part[chunk][i][k] (i >= k, 4 x 4 blocks) = sum over the chunk's columns of
-// GG_ij * (GG_kj / diagx_j), GG = P diag(uxinv2) - Q diag(xlinv2).  With nblk <= 256
-// lower-triangle blocks, the block's threads form 256 / nblk column groups (thread t:
-// block t % nblk, group t / nblk) whose partial blocks are summed through LDS at the
-// end; with more blocks each thread takes up to 3 of them over all columns.
+template <int SW, int PW>
 __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
   extern __shared__ double gsm[];
   const int m = D.m, n = D.n;
-  double* sA = gsm;                   // [m][kGJ + 1]
-  double* sB = gsm + m * (kGJ + 1);   // [m][kGJ + 1]
-  double* sR = gsm;                   // [256][17], after the last stage
+  const int nt = gram_tiles(m), ld = 16 * nt + 1;
+  double* sA = gsm;            // [SW][ld]: GG
+  double* sB = gsm + SW * ld;  // [SW][ld]: GG / diagx
   const int ch = blockIdx.x;
   const long long per = (n + D.gchunk - 1) / D.gchunk;
   const long long j0 = ch * per, j1 = j0 + per < n ? j0 + per : n;
-  const int nb = (m + 3) / 4, nblk = nb * (nb + 1) / 2;
-  const int groups = nblk <= 256 ? 256 / nblk : 1;
-  const int grp = nblk <= 256 ? (int)threadIdx.x / nblk : 0;
-  constexpr int kMaxBlk = 3;  // blocks per thread: 528 for m = 126 over 256 threads
-  double acc[kMaxBlk][4][4];
-  int bi[kMaxBlk], bk[kMaxBlk];
-#pragma unroll
-  for (int q = 0; q < kMaxBlk; ++q) {
-    const int blk = nblk <= 256 ? (q == 0 && grp < groups ? (int)threadIdx.x % nblk : -1)
-                                : (int)threadIdx.x + q * 256;
-    int I = 0;
-    bi[q] = bk[q] = -1;
-    if (blk >= 0 && blk < nblk) {
-      while ((I + 1) * (I + 2) / 2 <= blk) ++I;
-      bi[q] = I;
-      bk[q] = blk - I * (I + 1) / 2;
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int c = 0; c < 4; ++c) acc[q][r][c] = 0.0;
+  constexpr int kStep = 256 / SW;
+  const int jj = threadIdx.x % SW, r0 = threadIdx.x / SW;
+  // rows m .. 16 nt - 1 stay zero in every stage
+  for (int e = threadIdx.x; e < SW * ld; e += 256) {
+    const int i = e % ld;
+    if (i >= m) sA[e] = sB[e] = 0.0;
   }
-  constexpr int kLdS = kGJ + 1;
-  for (long long jb = j0; jb < j1; jb += kGJ) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < m * kGJ; e += blockDim.x) {
-      const int i = e / kGJ, jj = e - i * kGJ;
-      const long long j = jb + jj;
-      double g = 0.0, gs = 0.0;
-      if (j < j1) {
-        g = D.P[(long long)i * n + j] * D.uxinv2[j] - D.Q[(long long)i * n + j] * D.xlinv2[j];
-        gs = g * (1.0 / D.diagx[j]);
+  double nP[kGramRows], nQ[kGramRows], nux = 0.0, nxl = 0.0, ndg = 1.0;
+  auto fetch = [&](long long jb) {
+    const long long j = jb + jj;
+    const bool in = j < j1;
+    nux = in ? D.uxinv2[j] : 0.0;
+    nxl = in ? D.xlinv2[j] : 0.0;
+    ndg = in ? D.diagx[j] : 1.0;
+#pragma unroll
+    for (int q = 0; q < kGramRows; ++q) {
+      const int i = r0 + kStep * q;
+      nP[q] = (in && i < m) ? D.P[(long long)i * n + j] : 0.0;
+      nQ[q] = (in && i < m) ? D.Q[(long long)i * n + j] : 0.0;
+    }
+  };
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), l = threadIdx.x & 63;
+  const int kl = l >> 4, lr = l & 15;
+  const int npair = nt * (nt + 1) / 2;
+  int pI[PW], pK[PW];
+#pragma unroll
+  for (int q = 0; q < PW; ++q) {
+    const int p = w + 4 * q;
+    int I = 0;
+    while ((I + 1) * (I + 2) / 2 <= p) ++I;
+    pI[q] = I;
+    pK[q] = p - I * (I + 1) / 2;
+  }
+  gram_d4 acc[PW];
+#pragma unroll
+  for (int q = 0; q < PW; ++q) acc[q] = gram_d4{0.0, 0.0, 0.0, 0.0};
+  fetch(j0);
+  for (long long jb = j0; jb < j1; jb += SW) {
+    __syncthreads();  // the previous stage is consumed (and the zero rows written)
+    const double dinv = 1.0 / ndg;
+#pragma unroll
+    for (int q = 0; q < kGramRows; ++q) {
+      const int i = r0 + kStep * q;
+      if (i < m) {
+        const double g = nP[q] * nux - nQ[q] * nxl;
+        sA[jj * ld + i] = g;
+        sB[jj * ld + i] = g * dinv;
       }
-      sA[i * kLdS + jj] = g;
-      sB[i * kLdS + jj] = gs;
     }
     __syncthreads();
+    if (jb + SW < j1) fetch(jb + SW);
 #pragma unroll
-    for (int q = 0; q < kMaxBlk; ++q) {
-      if (bi[q] < 0) continue;
-      const int i0 = 4 * bi[q], k0 = 4 * bk[q];
-      for (int jj = grp; jj < kGJ; jj += groups) {
-        double ga[4], gb[4];
+    for (int q = 0; q < PW; ++q) {
+      if (w + 4 * q >= npair) continue;  // uniform across the wavefront
+      const double* a = sA + kl * ld + 16 * pI[q] + lr;
+      const double* b = sB + kl * ld + 16 * pK[q] + lr;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          ga[r] = i0 + r < m ? sA[(i0 + r) * kLdS + jj] : 0.0;
-          gb[r] = k0 + r < m ? sB[(k0 + r) * kLdS + jj] : 0.0;
-        }
-#pragma unroll
-        for (int r = 0; r < 4; ++r)
-#pragma unroll
-          for (int c = 0; c < 4; ++c) acc[q][r][c] += ga[r] * gb[c];
-      }
+      for (int kk = 0; kk < SW; kk += 4)
+        acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk * ld], b[kk * ld], acc[q], 0, 0, 0);
     }
   }
   double* out = D.grampart + (long long)ch * m * m;
-  if (groups > 1) {
-    // sum the column groups' partial blocks: group g's thread for block b is g nblk + b
-    __syncthreads();
-    if (bi[0] >= 0) {
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
+  for (int q = 0; q < PW; ++q) {
+    if (w + 4 * q >= npair) continue;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) sR[threadIdx.x * 17 + r * 4 + c] = acc[0][r][c];
+    for (int r = 0; r < 4; ++r) {
+      const int i = 16 * pI[q] + kl + 4 * r, k = 16 * pK[q] + lr;
+      if (i < m && k <= i) out[i * m + k] = acc[q][r];
     }
-    __syncthreads();
-    if ((int)threadIdx.x < nblk) {
-      const int i0 = 4 * bi[0], k0 = 4 * bk[0];
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-          double v = 0.0;
-          for (int g = 0; g < groups; ++g) v += sR[(g * nblk + threadIdx.x) * 17 + r * 4 + c];
-          if (i0 + r < m && k0 + c < m) out[(i0 + r) * m + (k0 + c)] = v;
-        }
-    }
-    return;
-  }
-#pragma unroll
-  for (int q = 0; q < kMaxBlk; ++q) {
-    if (bi[q] < 0) continue;
-    const int i0 = 4 * bi[q], k0 = 4 * bk[q];
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int c = 0; c < 4; ++c)
-        if (i0 + r < m && k0 + c < m) out[(i0 + r) * m + (k0 + c)] = acc[q][r][c];
   }
 }
 
@@ -888,13 +879,15 @@ int sttopt_mma_subsolv(int32_t m, int32_t n, const double* low, const double* up
       hipLaunchKernelGGL(mma_rows_reduce_kernel, dim3(m, 2), dim3(256), 0, s, m, D.nchunk,
                          D.rowpart, D.gvec, D.ggd);
       if (mlt) {
+        // at most 2 x 64 x 33 doubles (34 KB) for every m <= 126
         const size_t lds = sizeof(double) * gram_lds_doubles(m);
-        if (lds > 65536) {
-          hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(mma_gram_kernel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-          if (e != hipSuccess) return hip_status(e, "hipFuncSetAttribute(mma_gram_kernel)");
-        }
-        hipLaunchKernelGGL(mma_gram_kernel, dim3(D.gchunk), dim3(256), lds, s, D);
+        const int nt = gram_tiles(m);
+        if (nt <= 2)
+          hipLaunchKernelGGL((mma_gram_kernel<64, 1>), dim3(D.gchunk), dim3(256), lds, s, D);
+        else if (nt <= 4)
+          hipLaunchKernelGGL((mma_gram_kernel<32, 3>), dim3(D.gchunk), dim3(256), lds, s, D);
+        else
+          hipLaunchKernelGGL((mma_gram_kernel<16, 9>), dim3(D.gchunk), dim3(256), lds, s, D);
         hipLaunchKernelGGL(mma_gram_reduce_kernel, dim3(m * m), dim3(256), 0, s, D);
       }
       hipLaunchKernelGGL(mma_small_kernel<kSmSolve>, dim3(1), dim3(256), 0, s, D, epsi, 0, mlt);
