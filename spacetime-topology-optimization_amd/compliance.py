@@ -253,6 +253,8 @@ class _SpaceTimeCompliance(torch.autograd.Function):
         ctx.save_for_backward(x, t, U)
         ctx.consts = (tk, beta_t, Emin, Emax, penal, fe, ke, nelx, nely, ns)
         ctx.mark_non_differentiable(U)
+        # U's gradient is never used: no rows x ndof zero tensor for it in the backward
+        ctx.set_materialize_grads(False)
         return comp.to(xPhys.dtype), U
 
     @staticmethod
@@ -260,6 +262,8 @@ class _SpaceTimeCompliance(torch.autograd.Function):
         x, t, U = ctx.saved_tensors
         tk, beta_t, Emin, Emax, penal, fe, ke, nelx, nely, ns = ctx.consts
         dev = x.device
+        if g_comp is None:
+            return (None,) * 12
         g = g_comp.detach().to(torch.float64).contiguous()
         dx = torch.empty(nelx * nely, dtype=torch.float64, device=dev)
         dt = torch.empty(nelx * nely, dtype=torch.float64, device=dev)
@@ -305,7 +309,10 @@ def batched_whole_and_gravity_compliance(
         xPhys, tPhys, KE, Emin, Emax, penal, mask, F, fe, beta_t,
         [float(v) for v in stage_times], None if x0 is None else x0.detach(),
     )
-    return comp[0], [comp[1 + i] for i in range(len(stage_times))], U
+    # one unbind: its backward stacks the scalar gradients in one kernel, where one
+    # indexing view per stage would zero-fill and scatter one small tensor each
+    parts = comp.unbind(0)
+    return parts[0], list(parts[1:]), U
 
 
 def _batched_composed(xPhys, tPhys, KE, edofMat, Emin, Emax, penal, mask, F, ndof, C,
