@@ -231,8 +231,8 @@ class _Level:
     reaction_diag: Tensor | None = None
     kx: int = 1
     ky: int = 1
-    # coarsest level: the assembled dense operator (*batch, n, n); its Cholesky factor
-    # `chol` is formed from it on first access (the V-cycle uses `_cinv`)
+    # coarsest level: the assembled dense operator (*batch, n, n) and its Cholesky factor
+    # `chol`, both formed on first access of `chol` (the V-cycle uses `_cinv`)
     _dense: Tensor | None = field(default=None, repr=False)
     _chol: Tensor | None = field(default=None, repr=False)
     # device storage used by the kernels
@@ -257,6 +257,9 @@ class _Level:
     def chol(self) -> Tensor | None:
         """Lower Cholesky factor of the coarsest level's dense operator (the reference's
         `_Level.chol`, torch_mg.py:397), computed on first access."""
+        if self._chol is None and self._dense is None and self._cinv is not None:
+            dense = _assemble_dense(self, max(self.rows, 1))
+            self._dense = dense.reshape(*self._dbatch, self.ndof, self.ndof)
         if self._chol is None and self._dense is not None:
             self._chol = torch.linalg.cholesky(self._dense)
         return self._chol
@@ -361,17 +364,13 @@ class _Level:
         return out
 
 
-def _dense_factor(level: _Level, rows: int) -> None:
-    """Dense projected operator of `level` and its inverse (torch_mg._dense_cholesky,
-    torch_mg.py:269-290; the inverse replaces the factor's two triangular solves per
-    coarse solve, torch_mg.py:448-451).
-
-    :raises torch.linalg.LinAlgError: if the operator is not positive definite (the
-        reference's `torch.linalg.cholesky` raises the same)
-    """
+def _assemble_dense(level: _Level, rows: int) -> Tensor:
+    """The coarsest level's dense projected operator, `(rows, n, n)` fp64
+    (torch_mg._dense_cholesky's assembly, torch_mg.py:269-290)."""
     dev = level.mask.device
     n = level.ndof
     dense = torch.empty(rows, n, n, dtype=torch.float64, device=dev)
+    keep = None
     if level.keff is not None:
         args = (level._soa.data_ptr(), level._soa.stride(0), None, 0, None, 0, None)
     else:
@@ -379,23 +378,39 @@ def _dense_factor(level: _Level, rows: int) -> None:
         rd = level.reaction_diag
         rd = None if rd is None else rd.detach().to(torch.float64).reshape(rows, -1).contiguous()
         ke_dev = level.KE.detach().to(device=dev, dtype=torch.float64).contiguous()
-        level._copies["_dense_keep"] = (den, rd, ke_dev)
+        keep = (den, rd, ke_dev)
         args = (None, 0, den.data_ptr(), den.stride(0), _lib.ptr(rd),
                 0 if rd is None else rd.stride(0), ke_dev.data_ptr())
-    lib = _lib.load()
-    stream = _lib.stream_handle(dev)
     _lib.check(
-        lib.sttopt_dense_assemble(
+        _lib.load().sttopt_dense_assemble(
             level.d, level.nelx, level.nely, rows, *args, level.bits().data_ptr(),
-            dense.data_ptr(), n * n, stream,
+            dense.data_ptr(), n * n, _lib.stream_handle(dev),
         ),
         "sttopt_dense_assemble",
     )
-    level._copies.pop("_dense_keep", None)
-    inv = dense.clone()
+    # the temporaries in `keep` may be freed now: the caching allocator hands their
+    # memory only to later work on this stream
+    del keep
+    return dense
+
+
+def _dense_factor(level: _Level, rows: int) -> None:
+    """Inverse of the coarsest level's dense projected operator (torch_mg._dense_cholesky,
+    torch_mg.py:269-290; the inverse replaces the factor's two triangular solves per
+    coarse solve, torch_mg.py:448-451).  The operator is assembled into the buffer that
+    is then inverted in place; `chol` assembles it again if a caller asks for it.
+
+    :raises torch.linalg.LinAlgError: if the operator is not positive definite (the
+        reference's `torch.linalg.cholesky` raises the same)
+    """
+    dev = level.mask.device
+    n = level.ndof
+    inv = _assemble_dense(level, rows)
+    lib = _lib.load()
     work = torch.empty(lib.sttopt_spd_inverse_work_count(n, rows), dtype=torch.float64,
                        device=dev)
-    status = lib.sttopt_spd_inverse(n, rows, inv.data_ptr(), n * n, work.data_ptr(), stream)
+    status = lib.sttopt_spd_inverse(n, rows, inv.data_ptr(), n * n, work.data_ptr(),
+                                    _lib.stream_handle(dev))
     if status == _lib.ENOTSPD:
         raise torch.linalg.LinAlgError(
             "linalg.cholesky: The factorization could not be completed because the input is "
@@ -403,7 +418,7 @@ def _dense_factor(level: _Level, rows: int) -> None:
         )
     _lib.check(status, "sttopt_spd_inverse")
     level._cinv = inv
-    level._dense = dense.reshape(*level._dbatch, n, n)
+    level._dense = None
     level._chol = None
 
 
