@@ -247,10 +247,14 @@ __global__ __launch_bounds__(kBlock) void fin_init_kernel(int tiles, const doubl
   }
 }
 
+// fin_rz and fin_beta also zero the active-row counter for the next fin_resid (the
+// host's copy of the count, if any, is queued before them).
 __global__ __launch_bounds__(kBlock) void fin_rz_kernel(int tiles, const double* prz,
-                                                        const int* active, double* rz) {
+                                                        const int* active, double* rz,
+                                                        int* counter) {
   __shared__ double red[kBlock / kWave];
   const int row = blockIdx.x;
+  if (row == 0 && threadIdx.x == 0) *counter = 0;
   if (active[row] == 0) return;
   const double s = row_sum(prz, tiles, row, red);
   if (threadIdx.x == 0) rz[row] = s;
@@ -291,9 +295,10 @@ __global__ __launch_bounds__(kBlock) void fin_beta_kernel(int tiles, const doubl
                                                           int tiles_o, const double* przo,
                                                           int flexible,
                                                           const int* active, double* rz,
-                                                          double* beta) {
+                                                          double* beta, int* counter) {
   __shared__ double red[kBlock / kWave];
   const int row = blockIdx.x;
+  if (row == 0 && threadIdx.x == 0) *counter = 0;
   if (active[row] == 0) return;
   const double rzn = row_sum(prz, tiles, row, red);
   double rzo = 0.0;
@@ -695,9 +700,11 @@ static ApplyProfile g_apply_prof;
 
 static int prof_events(size_t pairs) {
   while (g_apply_prof.pool.size() < 2 * pairs) {
+    // timing only: no system-scope fence (cache writeback and invalidation) at each
+    // record, which would stall the launches that follow
     hipEvent_t e;
-    hipError_t st = hipEventCreate(&e);
-    if (st != hipSuccess) return hip_status(st, "hipEventCreate");
+    hipError_t st = hipEventCreateWithFlags(&e, hipEventDisableSystemFence);
+    if (st != hipSuccess) return hip_status(st, "hipEventCreateWithFlags");
     g_apply_prof.pool.push_back(e);
   }
   return STTOPT_OK;
@@ -751,7 +758,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   const int rz_tiles = vc.rz_tiles();
   if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
   hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2], sc.active,
-                     sc.rz);
+                     sc.rz, sc.counter);
   if (int e = convert<T, double>(n, rows, z, n, p, n, sc.active, s)) return e;
   STT_CHECK_LAUNCH("pcg first preconditioner");
 
@@ -797,7 +804,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
                       flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
                       sc.part[3]);
     }
-    if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
+    // the counter was zeroed by fin_rz (first iteration) or by the last fin_beta
     hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, fuse_now ? fused_tiles : tiles,
                        sc.part[1], rtol, k, sc.bsafe, resid, sc.active, iters, sc.counter);
     STT_CHECK_LAUNCH("pcg update");
@@ -839,7 +846,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     }
     hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2],
                        fuse_now ? fused_tiles : tiles, sc.part[3], flexible, sc.active, sc.rz,
-                       sc.beta);
+                       sc.beta, sc.counter);
     STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
                     iters, k, 1);
     STT_CHECK_LAUNCH("pcg direction");
