@@ -270,11 +270,15 @@ __global__ __launch_bounds__(kBlock) void fin_alpha_kernel(int tiles, const doub
   if (threadIdx.x == 0) alpha[row] = safe_div(rz[row], pap);
 }
 
+// host_flag (pinned host memory, or null): every row still active after this check
+// stores check_id there (a vector store at system scope), so the host finds check_id
+// in it exactly when a row is left; the check ids are never reused.
 __global__ __launch_bounds__(kBlock) void fin_resid_kernel(int tiles, const double* prr,
                                                            double rtol, int k,
                                                            const double* bsafe,
                                                            double* resid, int* active,
-                                                           int* iters, int* counter) {
+                                                           int* iters, int* counter,
+                                                           int* host_flag, int check_id) {
   __shared__ double red[kBlock / kWave];
   const int row = blockIdx.x;
   if (active[row] == 0) return;
@@ -287,6 +291,8 @@ __global__ __launch_bounds__(kBlock) void fin_resid_kernel(int tiles, const doub
       active[row] = 0;
     } else {
       atomicAdd(counter, 1);
+      if (host_flag != nullptr)
+        __hip_atomic_store(host_flag, check_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     }
   }
 }
@@ -652,11 +658,17 @@ static int validate(const sttopt_mg* mg) {
   return STTOPT_OK;
 }
 
+// 16 words of pinned, device-mapped, coherent host memory per host thread: word 0 for
+// synchronous readbacks, words 1-2 for the marks of the lagged convergence check.
 static int* pinned_counter() {
   static thread_local int* p = nullptr;
   if (p == nullptr) {
-    if (hipHostMalloc(reinterpret_cast<void**>(&p), 64, hipHostMallocDefault) != hipSuccess)
+    if (hipHostMalloc(reinterpret_cast<void**>(&p), 64,
+                      hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
       p = nullptr;
+    } else {
+      for (int i = 0; i < 16; ++i) p[i] = 0;  // no check id is 0
+    }
   }
   return p;
 }
@@ -670,6 +682,14 @@ static hipEvent_t* lag_events() {
       ev[i] = nullptr;
   }
   return (ev[0] != nullptr && ev[1] != nullptr) ? ev : nullptr;
+}
+
+// Ids of the lagged convergence checks: unique per host thread (and so per pinned
+// slot pair), so a slot never holds a matching id from an earlier check or solve.
+static int next_check_id() {
+  static thread_local int id = 0;
+  id = id == 0x7fffffff ? 1 : id + 1;
+  return id;
 }
 
 int read_device_int(const int* dev, hipStream_t s, int* out) {
@@ -771,11 +791,14 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (check_every < 1) check_every = 1;
   int* lag_host = pinned_counter();
   hipEvent_t* lag_ev = lag_events();
-  if (lag_host == nullptr || lag_ev == nullptr) {
-    set_error("sttopt_mgpcg: pinned readback word or events unavailable");
+  int* lag_dev = nullptr;  // the device's address of lag_host
+  if (lag_host == nullptr || lag_ev == nullptr ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&lag_dev), lag_host, 0) != hipSuccess) {
+    set_error("sttopt_mgpcg: pinned readback words or events unavailable");
     return STTOPT_EHIP;
   }
-  int checks = 0;  // active-row counts copied back so far
+  int checks = 0;  // lagged checks issued so far
+  int check_ids[2] = {0, 0};  // the id each pinned slot is expected to hold
   bool done = false;
   int k_last = 0;  // iteration whose x += alpha p is still pending
   for (int k = 1; k <= max_iter; ++k) {
@@ -805,8 +828,12 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
                       sc.part[3]);
     }
     // the counter was zeroed by fin_rz (first iteration) or by the last fin_beta
+    const bool check = k < max_iter && k % check_every == 0;
+    const int cur = checks & 1, prev = cur ^ 1;
+    if (check) check_ids[cur] = next_check_id();
     hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, fuse_now ? fused_tiles : tiles,
-                       sc.part[1], rtol, k, sc.bsafe, resid, sc.active, iters, sc.counter);
+                       sc.part[1], rtol, k, sc.bsafe, resid, sc.active, iters, sc.counter,
+                       check ? lag_dev + 1 + cur : nullptr, check ? check_ids[cur] : 0);
     STT_CHECK_LAUNCH("pcg update");
     k_last = k;
     if (k == max_iter) {
@@ -814,23 +841,19 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       done = remaining == 0;
       break;
     }
-    if (k % check_every == 0) {
-      // This iteration's count of active rows goes to pinned host memory without a
-      // wait; the host reads the previous check's count, by which time this iteration
-      // is queued behind it, so the device never drains.  When that count is 0 every
-      // row converged there: this iteration's launches found no active row, and the
-      // converged rows' last x += alpha p was in that iteration's direction update.
-      // checks alternate between two pinned slots and events
-      const int cur = checks & 1, prev = cur ^ 1;
-      hipError_t e = hipMemcpyAsync(lag_host + 1 + cur, sc.counter, sizeof(int),
-                                    hipMemcpyDeviceToHost, s);
-      if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(active rows)");
-      e = hipEventRecord(lag_ev[cur], s);
+    if (check) {
+      // fin_resid marks this check in pinned host memory if a row is left, without a
+      // wait; the host reads the previous check's mark, by which time this iteration
+      // is queued behind it, so the device never drains.  An unmarked check means
+      // every row converged there: this iteration's launches found no active row, and
+      // the converged rows' last x += alpha p was in that iteration's direction update.
+      // Checks alternate between two pinned slots and events.
+      hipError_t e = hipEventRecord(lag_ev[cur], s);
       if (e != hipSuccess) return hip_status(e, "hipEventRecord");
       if (checks > 0) {
         e = hipEventSynchronize(lag_ev[prev]);
         if (e != hipSuccess) return hip_status(e, "hipEventSynchronize");
-        if (reinterpret_cast<volatile int*>(lag_host)[1 + prev] == 0) {
+        if (reinterpret_cast<volatile int*>(lag_host)[1 + prev] != check_ids[prev]) {
           done = true;
           k_last = 0;
           break;
