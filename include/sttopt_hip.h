@@ -277,7 +277,8 @@ int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v,
  * Rows retire as they converge (per-row device flags): iters[row] receives each
  * row's iteration count and resid[row] its final relative residual.  flexible != 0
  * uses the Polak-Ribiere beta (for a V-cycle in fp32).  check_every: iterations
- * between copies of the active-row count to the host; each is read one check late
+ * between convergence checks (a mark a kernel stores in mapped pinned host memory
+ * when rows are still active); each is read one check late
  * (the device keeps the next iteration queued), so the solve may launch one more
  * iteration that finds no active row.  The stream is synchronized at the start, at
  * max_iter and at the end.  Returns STTOPT_ENOCONV when a row is still above rtol
