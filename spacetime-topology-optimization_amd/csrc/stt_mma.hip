@@ -256,20 +256,25 @@ __global__ __launch_bounds__(256) void mma_rows_reduce_kernel(int m, int nchunk,
 
 // ------------------------------------------------------------------- Gram ----
 
-// part[chunk][i][k] (i >= k) = sum over the chunk's columns j of GG_ij * (GG_kj / diagx_j),
-// GG = P diag(uxinv2) - Q diag(xlinv2): an m x m x (chunk) product on the fp64 matrix
-// cores.  The rows are padded to nt 16-row tiles; wave w owns the lower tile pairs
-// (I, K) = w, w + 4, ... in the order (0,0), (1,0), (1,1), (2,0), ... and accumulates
-// each over all of the chunk's columns (v_mfma_f64_16x16x4f64: lane l supplies
-// A(l % 16, l / 16) and B(l / 16, l % 16) of a 16 x 4 by 4 x 16 step and holds
-// D(l / 16 + 4 r, l % 16), r < 4).  GG and GG / diagx pass through LDS SW columns at a
-// time, k-major ([column][row], rows padded to 16 nt + 1); thread t forms column t % SW
-// for rows t / SW + (256 / SW) q, and the next stage's P, Q and column factors are in
-// flight while the current one is multiplied.  PW: tile pairs per wave at most.
+// One pass over P and Q for the Newton system: per column chunk ch,
+//   part[ch][i][k] (i >= k, i, k < m) = sum_j GG_ij GG_kj / diagx_j,
+//   part[ch][m][k] = sum_j delx_j GG_kj / diagx_j          (GG delx/diagx, k < m),
+//   part[ch][m + 1][k] = sum_j P_kj uxinv1_j + Q_kj xlinv1_j  (the row sums gvec),
+// GG = P diag(uxinv2) - Q diag(xlinv2), with row stride m + 1.  The first two are one
+// product on the fp64 matrix cores over m + 1 rows (the extra row is delx); the rows are
+// padded to nt 16-row tiles and wave w owns the lower tile pairs (I, K) = w, w + 4, ...
+// in the order (0,0), (1,0), (1,1), (2,0), ..., accumulating each over the chunk's
+// columns (v_mfma_f64_16x16x4f64: lane l supplies A(l % 16, l / 16) and B(l / 16, l % 16)
+// of a 16 x 4 by 4 x 16 step and holds D(l / 16 + 4 r, l % 16), r < 4).  The rows and
+// the rows / diagx pass through LDS SW columns at a time, k-major ([column][row], rows
+// padded to 16 nt + 1); thread t forms column t % SW for rows t / SW + (256 / SW) q and
+// also sums its P and Q values into gvec, and the next stage's loads are in flight while
+// the current one is multiplied.  PW: tile pairs per wave at most.
+// R: rows per thread and stage, m + 1 <= 256 / SW * R (R = 8 covers every m of the
+// tile count; R = 5 keeps m + 1 <= 20 at four waves per SIMD).
 typedef double gram_d4 __attribute__((ext_vector_type(4)));
-constexpr int kGramRows = 8;  // rows per thread and stage: m <= 16 nt <= 256 / SW * 8
 
-__host__ __device__ inline int gram_tiles(int m) { return (m + 15) / 16; }
+__host__ __device__ inline int gram_tiles(int m) { return (m + 16) / 16; }  // m + 1 rows
 __host__ __device__ inline int gram_stage_cols(int m) {
   const int nt = gram_tiles(m);
   return nt <= 2 ? 64 : (nt <= 4 ? 32 : 16);
@@ -277,33 +282,41 @@ __host__ __device__ inline int gram_stage_cols(int m) {
 __host__ __device__ inline int gram_lds_doubles(int m) {
   return 2 * gram_stage_cols(m) * (16 * gram_tiles(m) + 1);
 }
+__host__ __device__ inline long long gram_part_doubles(int m) {
+  return (long long)(m + 1) * (m + 1) + m;
+}
 
-template <int SW, int PW>
-__global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
+template <int SW, int PW, int R>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(R == 5 ? 4 : 1)))
+void mma_gram_kernel(MmaDev D) {
   extern __shared__ double gsm[];
-  const int m = D.m, n = D.n;
+  const int m = D.m, n = D.n, me = m + 1;
   const int nt = gram_tiles(m), ld = 16 * nt + 1;
-  double* sA = gsm;            // [SW][ld]: GG
-  double* sB = gsm + SW * ld;  // [SW][ld]: GG / diagx
+  double* sA = gsm;            // [SW][ld]: GG and delx
+  double* sB = gsm + SW * ld;  // [SW][ld]: the same / diagx
   const int ch = blockIdx.x;
   const long long per = (n + D.gchunk - 1) / D.gchunk;
   const long long j0 = ch * per, j1 = j0 + per < n ? j0 + per : n;
   constexpr int kStep = 256 / SW;
   const int jj = threadIdx.x % SW, r0 = threadIdx.x / SW;
-  // rows m .. 16 nt - 1 stay zero in every stage
+  // rows m + 1 .. 16 nt - 1 stay zero in every stage
   for (int e = threadIdx.x; e < SW * ld; e += 256) {
     const int i = e % ld;
-    if (i >= m) sA[e] = sB[e] = 0.0;
+    if (i >= me) sA[e] = sB[e] = 0.0;
   }
-  double nP[kGramRows], nQ[kGramRows], nux = 0.0, nxl = 0.0, ndg = 1.0;
+  double nP[R], nQ[R], nux = 0.0, nxl = 0.0, ndg = 1.0, ndl = 0.0;
+  double nu1 = 0.0, nx1 = 0.0;
   auto fetch = [&](long long jb) {
     const long long j = jb + jj;
     const bool in = j < j1;
     nux = in ? D.uxinv2[j] : 0.0;
     nxl = in ? D.xlinv2[j] : 0.0;
     ndg = in ? D.diagx[j] : 1.0;
+    ndl = in ? D.delx[j] : 0.0;
+    nu1 = in ? D.uxinv1[j] : 0.0;
+    nx1 = in ? D.xlinv1[j] : 0.0;
 #pragma unroll
-    for (int q = 0; q < kGramRows; ++q) {
+    for (int q = 0; q < R; ++q) {
       const int i = r0 + kStep * q;
       nP[q] = (in && i < m) ? D.P[(long long)i * n + j] : 0.0;
       nQ[q] = (in && i < m) ? D.Q[(long long)i * n + j] : 0.0;
@@ -324,17 +337,24 @@ __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
   gram_d4 acc[PW];
 #pragma unroll
   for (int q = 0; q < PW; ++q) acc[q] = gram_d4{0.0, 0.0, 0.0, 0.0};
+  double gv[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) gv[q] = 0.0;
   fetch(j0);
   for (long long jb = j0; jb < j1; jb += SW) {
     __syncthreads();  // the previous stage is consumed (and the zero rows written)
     const double dinv = 1.0 / ndg;
 #pragma unroll
-    for (int q = 0; q < kGramRows; ++q) {
+    for (int q = 0; q < R; ++q) {
       const int i = r0 + kStep * q;
       if (i < m) {
         const double g = nP[q] * nux - nQ[q] * nxl;
         sA[jj * ld + i] = g;
         sB[jj * ld + i] = g * dinv;
+        gv[q] += nP[q] * nu1 + nQ[q] * nx1;
+      } else if (i == m) {
+        sA[jj * ld + i] = ndl;
+        sB[jj * ld + i] = ndl * dinv;
       }
     }
     __syncthreads();
@@ -344,36 +364,65 @@ __global__ __launch_bounds__(256) void mma_gram_kernel(MmaDev D) {
       if (w + 4 * q >= npair) continue;  // uniform across the wavefront
       const double* a = sA + kl * ld + 16 * pI[q] + lr;
       const double* b = sB + kl * ld + 16 * pK[q] + lr;
-#pragma unroll
+      // four steps at a time: their LDS operands only are live (occupancy)
+#pragma unroll 4
       for (int kk = 0; kk < SW; kk += 4)
         acc[q] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk * ld], b[kk * ld], acc[q], 0, 0, 0);
     }
   }
-  double* out = D.grampart + (long long)ch * m * m;
+  double* out = D.grampart + (long long)ch * gram_part_doubles(m);
 #pragma unroll
   for (int q = 0; q < PW; ++q) {
     if (w + 4 * q >= npair) continue;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int i = 16 * pI[q] + kl + 4 * r, k = 16 * pK[q] + lr;
-      if (i < m && k <= i) out[i * m + k] = acc[q][r];
+      if (i < me && k <= i) out[i * me + k] = acc[q][r];
     }
+  }
+  // gvec: the SW column threads of each row group, summed in column order through LDS
+  __syncthreads();  // the last stage's operands are consumed
+  double* red = gsm;  // [16 nt][SW]
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const int i = r0 + kStep * q;
+    if (i < m) red[i * SW + jj] = gv[q];
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < m; i += 256) {
+    double t = 0.0;
+    for (int c = 0; c < SW; ++c) t += red[i * SW + c];
+    out[me * me + i] = t;
   }
 }
 
-// alam[i][k] = sum over chunks, one workgroup per entry with i >= k (every such entry is
-// written by the Gram kernel's blocks)
+// Sums over the chunks of the Gram pass, one workgroup per output: alam[i][k] (i >= k;
+// every such entry is written by the Gram kernel's blocks), then ggd[k], then gvec[k].
 __global__ __launch_bounds__(256) void mma_gram_reduce_kernel(MmaDev D) {
   __shared__ double red[256 / kWave];
-  const int m = D.m;
+  const int m = D.m, me = m + 1;
   const int e = blockIdx.x;
-  const int i = e / m, k = e - i * m;
-  if (i < k) return;  // uniform per block
+  long long src;
+  double* dst;
+  if (e < m * m) {
+    const int i = e / m, k = e - i * m;
+    if (i < k) return;  // uniform per block
+    src = (long long)i * me + k;
+    dst = D.alam + e;
+  } else if (e < m * m + m) {
+    const int k = e - m * m;
+    src = (long long)m * me + k;
+    dst = D.ggd + k;
+  } else {
+    const int k = e - m * m - m;
+    src = (long long)me * me + k;
+    dst = D.gvec + k;
+  }
+  const long long stride = gram_part_doubles(m);
   double s = 0.0;
-  for (int ch = threadIdx.x; ch < D.gchunk; ch += blockDim.x)
-    s += D.grampart[(long long)ch * m * m + e];
+  for (int ch = threadIdx.x; ch < D.gchunk; ch += blockDim.x) s += D.grampart[ch * stride + src];
   s = block_sum(s, red);
-  if (threadIdx.x == 0) D.alam[e] = s;
+  if (threadIdx.x == 0) *dst = s;
 }
 
 // ----------------------------------------------------------- small (1 WG) ----
@@ -792,7 +841,7 @@ static void carve_mma(MmaDev& D, double* work) {
   D.xlc = take(n);
   D.colpart = take((long long)kColBlocks * kColSlots);
   D.rowpart = take(2LL * m * D.nchunk);
-  D.grampart = take((long long)D.gchunk * m * m);
+  D.grampart = take((long long)D.gchunk * gram_part_doubles(m));
   D.alam = take((long long)m * m);
   D.gvec = take(m);
   D.ggd = take(m);
@@ -874,21 +923,25 @@ int sttopt_mma_subsolv(int32_t m, int32_t n, const double* low, const double* up
       break;
     case 2:  // Newton direction and step bound
       hipLaunchKernelGGL(mma_col_kernel<kColPass1>, dim3(cb), dim3(kCol), 0, s, D, epsi, 0);
-      hipLaunchKernelGGL(mma_gemv_kernel, gv, dim3(256), 0, s, m, n, P, Q, D.uxinv1, D.xlinv1,
-                         D.wA, D.wB, 2, D.rowpart, D.nchunk);
-      hipLaunchKernelGGL(mma_rows_reduce_kernel, dim3(m, 2), dim3(256), 0, s, m, D.nchunk,
-                         D.rowpart, D.gvec, D.ggd);
       if (mlt) {
-        // at most 2 x 64 x 33 doubles (34 KB) for every m <= 126
+        // the Gram pass also forms gvec and GG delx/diagx; at most 2 x 64 x 33 doubles
+        // (34 KB) of LDS for every m <= 126
         const size_t lds = sizeof(double) * gram_lds_doubles(m);
         const int nt = gram_tiles(m);
-        if (nt <= 2)
-          hipLaunchKernelGGL((mma_gram_kernel<64, 1>), dim3(D.gchunk), dim3(256), lds, s, D);
+        if (m + 1 <= 20)
+          hipLaunchKernelGGL((mma_gram_kernel<64, 1, 5>), dim3(D.gchunk), dim3(256), lds, s, D);
+        else if (nt <= 2)
+          hipLaunchKernelGGL((mma_gram_kernel<64, 1, 8>), dim3(D.gchunk), dim3(256), lds, s, D);
         else if (nt <= 4)
-          hipLaunchKernelGGL((mma_gram_kernel<32, 3>), dim3(D.gchunk), dim3(256), lds, s, D);
+          hipLaunchKernelGGL((mma_gram_kernel<32, 3, 8>), dim3(D.gchunk), dim3(256), lds, s, D);
         else
-          hipLaunchKernelGGL((mma_gram_kernel<16, 9>), dim3(D.gchunk), dim3(256), lds, s, D);
-        hipLaunchKernelGGL(mma_gram_reduce_kernel, dim3(m * m), dim3(256), 0, s, D);
+          hipLaunchKernelGGL((mma_gram_kernel<16, 9, 8>), dim3(D.gchunk), dim3(256), lds, s, D);
+        hipLaunchKernelGGL(mma_gram_reduce_kernel, dim3(m * m + 2 * m), dim3(256), 0, s, D);
+      } else {
+        hipLaunchKernelGGL(mma_gemv_kernel, gv, dim3(256), 0, s, m, n, P, Q, D.uxinv1,
+                           D.xlinv1, D.wA, D.wB, 2, D.rowpart, D.nchunk);
+        hipLaunchKernelGGL(mma_rows_reduce_kernel, dim3(m, 2), dim3(256), 0, s, m, D.nchunk,
+                           D.rowpart, D.gvec, D.ggd);
       }
       hipLaunchKernelGGL(mma_small_kernel<kSmSolve>, dim3(1), dim3(256), 0, s, D, epsi, 0, mlt);
       if (mlt)
