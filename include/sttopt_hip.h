@@ -272,8 +272,9 @@ int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v,
                        int64_t v_stride, void* y, int64_t y_stride, void* stream);
 
 /* Batched multigrid-preconditioned CG on rows of b (torch_fem.pcg with the
- * torch_mg levels, torch_fem.py:201-343).  x holds the warm start on entry (zeros
- * for a cold start; its fixed dofs are ignored) and the solution on return.
+ * torch_mg levels, torch_fem.py:201-343).  x holds the warm start on entry (its fixed
+ * dofs are ignored) and the solution on return; with cold_start != 0 the start is
+ * x = 0 and x is not read (r = P b and x = 0 are written in one pass).
  * Rows retire as they converge (per-row device flags): iters[row] receives each
  * row's iteration count and resid[row] its final relative residual.  flexible != 0
  * uses the Polak-Ribiere beta (for a V-cycle in fp32).  check_every: iterations
@@ -285,8 +286,8 @@ int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v,
  * after max_iter iterations (x, iters, resid are still written). */
 int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double* x,
                  int64_t x_stride, double rtol, int32_t max_iter, int32_t flexible,
-                 int32_t check_every, double* work, int32_t* iters, double* resid,
-                 int32_t* n_iter, void* stream);
+                 int32_t check_every, int32_t cold_start, double* work, int32_t* iters,
+                 double* resid, int32_t* n_iter, void* stream);
 
 /* Work vectors sttopt_mgpcg needs: returns the count of doubles ([rows][ndof0] x 5). */
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg);

@@ -160,7 +160,8 @@ _SIGNATURES = {
     "sttopt_level_apply": (c_int32, [_P, c_int32, _P, c_int64, _P, c_int64, _P]),
     "sttopt_mgpcg": (
         c_int32,
-        [_P, _P, c_int64, _P, c_int64, c_double, c_int32, c_int32, c_int32, _P, _P, _P, _P, _P],
+        [_P, _P, c_int64, _P, c_int64, c_double, c_int32, c_int32, c_int32, c_int32, _P, _P,
+         _P, _P, _P],
     ),
     "sttopt_mgpcg_work_count": (c_int64, [_P]),
     "sttopt_adjoint_grad": (

@@ -167,6 +167,42 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
   }
 }
 
+// Cold start (x0 = 0): x = 0, r = P b and the partials of r.r over free dofs (= b.b
+// there), summed as dot_kernel sums them.
+template <int D, typename Unused>
+__global__ __launch_bounds__(kBlock) void cold_init_kernel(TileGeom g, long long n,
+                                                           const double* b, long long sb,
+                                                           const uint32_t* mask, double* x,
+                                                           long long sx, double* r,
+                                                           double* part) {
+  __shared__ double red[kBlock / kWave];
+  int row, tt, c, r0;
+  tile_origin(g, row, tt, c, r0);
+  double acc = 0.0;
+  if (c < g.nnx) {
+#pragma unroll
+    for (int j = 0; j < kRowsPerThread; ++j) {
+      if (r0 + j < g.nny) {
+        const long long node = (long long)(r0 + j) * g.nnx + c;
+        const uint32_t word = mask != nullptr ? mask[(node * D) >> 5] : ~0u;
+        const Vec<double, D> bv = load_node<double, D>(b + row * sb, node);
+        Vec<double, D> rv, zero;
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const bool fr = (word >> ((node * D + k) & 31)) & 1u;
+          rv.v[k] = fr ? bv.v[k] : 0.0;
+          zero.v[k] = 0.0;
+          if (fr) acc += bv.v[k] * bv.v[k];
+        }
+        store_node<double, D>(r + row * n, node, rv);
+        store_node<double, D>(x + row * sx, node, zero);
+      }
+    }
+  }
+  const double s = block_sum(acc, red);
+  if (threadIdx.x == 0) part[(long long)row * g.tiles + tt] = s;
+}
+
 // partial sums of a.b over free dofs (b == nullptr: a.a), optionally only of
 // active rows.
 template <int D, typename BT>
@@ -733,7 +769,8 @@ static int prof_events(size_t pairs) {
 template <typename T>
 static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double* x,
                       long long xs, double rtol, int max_iter, int flexible, int check_every,
-                      double* work, int* iters, double* resid, int* n_iter, hipStream_t s) {
+                      int cold, double* work, int* iters, double* resid, int* n_iter,
+                      hipStream_t s) {
   const int rows = mg->rows;
   OpDesc op;
   op.d = mg->d;
@@ -757,18 +794,24 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   const dim3 tblock(kBlock), rgrid(rows);
   *n_iter = 0;
 
-  // x <- P x0;  r = P b - A x;  relative residual; active flags
-  if (int e = project_f64(n, rows, op.mask, x, xs, s)) return e;
-  if (int e = stencil<double, double, double>(op, rows, kResid, false, kNoDot, x, xs, b, bs,
-                                              nullptr, 0, r, n, 0.0, nullptr, nullptr, s))
-    return e;
-  STT_TILE_LAUNCH(dot_kernel, double, g, s, n, b, bs, static_cast<const double*>(nullptr), 0, op.mask,
-                     nullptr, sc.part[4]);
-  STT_TILE_LAUNCH(dot_kernel, double, g, s, n, r, n, static_cast<const double*>(nullptr), 0, op.mask,
-                     nullptr, sc.part[1]);
+  // x <- P x0;  r = P b - A x;  relative residual; active flags.  A cold start
+  // (x0 = 0) writes x = 0 and r = P b in one pass, whose r.r partials are b.b's too.
+  if (cold) {
+    STT_TILE_LAUNCH(cold_init_kernel, double, g, s, n, b, bs, op.mask, x, xs, r, sc.part[4]);
+  } else {
+    if (int e = project_f64(n, rows, op.mask, x, xs, s)) return e;
+    if (int e = stencil<double, double, double>(op, rows, kResid, false, kNoDot, x, xs, b, bs,
+                                                nullptr, 0, r, n, 0.0, nullptr, nullptr, s))
+      return e;
+    STT_TILE_LAUNCH(dot_kernel, double, g, s, n, b, bs, static_cast<const double*>(nullptr), 0,
+                    op.mask, nullptr, sc.part[4]);
+    STT_TILE_LAUNCH(dot_kernel, double, g, s, n, r, n, static_cast<const double*>(nullptr), 0,
+                    op.mask, nullptr, sc.part[1]);
+  }
   if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
-  hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4], sc.part[1],
-                     rtol, sc.bsafe, resid, sc.active, iters, sc.counter);
+  hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4],
+                     cold ? sc.part[4] : sc.part[1], rtol, sc.bsafe, resid, sc.active, iters,
+                     sc.counter);
   STT_CHECK_LAUNCH("pcg init");
   int remaining = 0;
   if (int e = read_device_int(sc.counter, s, &remaining)) return e;
@@ -993,8 +1036,8 @@ int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v, int64_
 
 int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double* x,
                  int64_t x_stride, double rtol, int32_t max_iter, int32_t flexible,
-                 int32_t check_every, double* work, int32_t* iters, double* resid,
-                 int32_t* n_iter, void* stream) {
+                 int32_t check_every, int32_t cold_start, double* work, int32_t* iters,
+                 double* resid, int32_t* n_iter, void* stream) {
   if (int e = validate(mg)) return e;
   if (b == nullptr || x == nullptr || work == nullptr || iters == nullptr ||
       resid == nullptr || n_iter == nullptr || mg->scratch == nullptr ||
@@ -1015,9 +1058,9 @@ int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double*
   hipStream_t s = (hipStream_t)stream;
   if (mg->vdtype == STTOPT_F64)
     return mgpcg_impl<double>(mg, b, b_stride, x, x_stride, rtol, max_iter, flexible,
-                              check_every, work, iters, resid, n_iter, s);
+                              check_every, cold_start, work, iters, resid, n_iter, s);
   return mgpcg_impl<float>(mg, b, b_stride, x, x_stride, rtol, max_iter, flexible,
-                           check_every, work, iters, resid, n_iter, s);
+                           check_every, cold_start, work, iters, resid, n_iter, s);
 }
 
 }  // extern "C"

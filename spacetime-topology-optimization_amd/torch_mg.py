@@ -690,7 +690,7 @@ def native_mgpcg(vcycle: VCycle, b: Tensor, *, rtol: float, max_iter: int,
     mg, _, work = vcycle._native(rows)
     b2, bs = _grid.rows_of(b, lv0.ndof, batch, torch.float64)
     if x0 is None:
-        x = torch.zeros(rows, lv0.ndof, dtype=torch.float64, device=dev)
+        x = torch.empty(rows, lv0.ndof, dtype=torch.float64, device=dev)  # written as 0
     else:
         x = x0.detach().to(torch.float64).expand(*batch, lv0.ndof).reshape(rows, lv0.ndof).clone()
     iters = torch.zeros(rows, dtype=torch.int32, device=dev)
@@ -699,8 +699,8 @@ def native_mgpcg(vcycle: VCycle, b: Tensor, *, rtol: float, max_iter: int,
     flexible = 1 if vcycle.dtype != torch.float64 else 0
     status = _lib.load().sttopt_mgpcg(
         ctypes.byref(mg), b2.data_ptr(), bs, x.data_ptr(), lv0.ndof, float(rtol),
-        int(max_iter), flexible, CHECK_EVERY, work.data_ptr(), iters.data_ptr(),
-        resid.data_ptr(), ctypes.byref(n_iter), _lib.stream_handle(dev),
+        int(max_iter), flexible, CHECK_EVERY, 1 if x0 is None else 0, work.data_ptr(),
+        iters.data_ptr(), resid.data_ptr(), ctypes.byref(n_iter), _lib.stream_handle(dev),
     )
     if info is not None:
         info["row_iters"] = int(iters.sum())

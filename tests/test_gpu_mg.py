@@ -237,6 +237,39 @@ def test_warm_start_and_exact_warm_start(dev):
     assert n0 == 0 and torch.equal(U3, U)
 
 
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_cold_start_equals_zero_warm_start(dev, dtype):
+    """x0 = None takes the cold-start initialization (x = 0 and r = P b in one pass, x
+    not read); a warm start from zeros takes the projection and residual apply.  Both
+    give the same iterations and bit-identical solutions, on a batch whose fixed dofs
+    carry nonzero loads (dropped by the projection) and with a zero-load row that
+    converges at initialization."""
+    from sttopt_amd import torch_mg
+
+    nelx, nely = 18, 12
+    s = _setup(nelx, nely, dev)
+    x = s["t"](np.stack([binary_design(nelx, nely), np.full(nelx * nely, 0.5),
+                         np.full(nelx * nely, 0.7)]))
+    rng = np.random.default_rng(11)
+    F = s["t"](rng.standard_normal((3, s["ndof"])))
+    F[2] = 0.0
+    args = (s["edof_t"], s["KE_t"], EMIN, EMAX, PENAL, s["mask_t"], nelx, nely)
+    old = torch_mg.VCYCLE_DTYPE
+    torch_mg.VCYCLE_DTYPE = getattr(torch, dtype)
+    try:
+        info_c, info_w = {}, {}
+        Uc, nc = torch_mg.solve(F, x, *args, rtol=1e-10, info=info_c, **MG_KW)
+        Uw, nw = torch_mg.solve(F, x, *args, rtol=1e-10, x0=torch.zeros_like(F), info=info_w,
+                                **MG_KW)
+    finally:
+        torch_mg.VCYCLE_DTYPE = old
+    assert nc == nw > 0
+    assert info_c["row_iter_list"] == info_w["row_iter_list"]
+    assert info_c["row_iter_list"][2] == 0
+    assert torch.equal(Uc, Uw)
+    assert torch.all(Uc[2] == 0)
+
+
 def test_nonconvergence_raises_with_original_row_indices(dev):
     from sttopt_amd import torch_fem, torch_mg
 
