@@ -212,7 +212,10 @@ __global__ __launch_bounds__(kCol) void mma_col_kernel(MmaDev D, double epsi, in
 
 // ------------------------------------------------------------------- rows ----
 
-// part[v][i][chunk] = sum over the chunk's columns j of P_ij A_v[j] + Q_ij B_v[j]
+// part[v][i][chunk] = sum over the chunk's columns j of P_ij A_v[j] + Q_ij B_v[j], for
+// the kGemvRows rows i = blockIdx.y kGemvRows + q: the column values are read once for
+// all of them; each row's sum runs in the same order as with one row per block.
+constexpr int kGemvRows = 4;
 __global__ __launch_bounds__(256) void mma_gemv_kernel(int m, int n, const double* __restrict__ P,
                                                        const double* __restrict__ Q,
                                                        const double* A0, const double* B0,
@@ -220,23 +223,37 @@ __global__ __launch_bounds__(256) void mma_gemv_kernel(int m, int n, const doubl
                                                        int nv, double* __restrict__ part,
                                                        int nchunk) {
   __shared__ double red[256 / kWave];
-  const int i = blockIdx.y, ch = blockIdx.x;
+  const int i0 = blockIdx.y * kGemvRows, ch = blockIdx.x;
   const long long per = (n + nchunk - 1) / nchunk;
   const long long j0 = ch * per, j1 = j0 + per < n ? j0 + per : n;
-  const double* Pr = P + (long long)i * n;
-  const double* Qr = Q + (long long)i * n;
-  double s0 = 0.0, s1 = 0.0;
+  double s0[kGemvRows], s1[kGemvRows];
+#pragma unroll
+  for (int q = 0; q < kGemvRows; ++q) s0[q] = s1[q] = 0.0;
   for (long long j = j0 + threadIdx.x; j < j1; j += blockDim.x) {
-    const double p = Pr[j], q = Qr[j];
-    s0 += p * A0[j] + q * B0[j];
-    if (nv > 1) s1 += p * A1[j] + q * B1[j];
+    const double a0 = A0[j], b0 = B0[j];
+    const double a1 = nv > 1 ? A1[j] : 0.0, b1 = nv > 1 ? B1[j] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kGemvRows; ++q) {
+      const int i = i0 + q;
+      if (i < m) {
+        const double p = P[(long long)i * n + j], qv = Q[(long long)i * n + j];
+        s0[q] += p * a0 + qv * b0;
+        if (nv > 1) s1[q] += p * a1 + qv * b1;
+      }
+    }
   }
-  s0 = block_sum(s0, red);
-  if (threadIdx.x == 0) part[(long long)i * nchunk + ch] = s0;
-  if (nv > 1) {
-    __syncthreads();
-    s1 = block_sum(s1, red);
-    if (threadIdx.x == 0) part[((long long)m + i) * nchunk + ch] = s1;
+#pragma unroll
+  for (int q = 0; q < kGemvRows; ++q) {
+    const int i = i0 + q;
+    if (i >= m) break;  // uniform
+    if (q > 0) __syncthreads();
+    const double t0 = block_sum(s0[q], red);
+    if (threadIdx.x == 0) part[(long long)i * nchunk + ch] = t0;
+    if (nv > 1) {
+      __syncthreads();
+      const double t1 = block_sum(s1[q], red);
+      if (threadIdx.x == 0) part[((long long)m + i) * nchunk + ch] = t1;
+    }
   }
 }
 
@@ -905,7 +922,7 @@ int sttopt_mma_subsolv(int32_t m, int32_t n, const double* low, const double* up
   hipStream_t s = (hipStream_t)stream;
   const int cb = col_blocks(n);
   const int mlt = m < n;
-  const dim3 gv(D.nchunk, m);
+  const dim3 gv(D.nchunk, (m + kGemvRows - 1) / kGemvRows);
   switch (op) {
     case 0:  // initial point
       hipLaunchKernelGGL(mma_col_kernel<kColInit>, dim3(cb), dim3(kCol), 0, s, D, epsi, 0);
@@ -983,7 +1000,8 @@ int sttopt_mma_prepare(int32_t m, int32_t n, int32_t iteration, const double* xv
   hipLaunchKernelGGL(mma_prep_kernel, dim3(col_blocks(n)), dim3(256), 0, s, A);
   const int nchunk = n < 4096 ? 1 : (n / 4096 < 256 ? n / 4096 : 256);
   double* part = work + 2LL * n;
-  hipLaunchKernelGGL(mma_gemv_kernel, dim3(nchunk, m), dim3(256), 0, s, m, n, P, Q, work,
+  hipLaunchKernelGGL(mma_gemv_kernel, dim3(nchunk, (m + kGemvRows - 1) / kGemvRows), dim3(256),
+                     0, s, m, n, P, Q, work,
                      work + n, nullptr, nullptr, 1, part, nchunk);
   hipLaunchKernelGGL(mma_b_kernel, dim3(1), dim3(256), 0, s, m, nchunk, part, fval, b);
   STT_CHECK_LAUNCH("mma_prepare kernels");
