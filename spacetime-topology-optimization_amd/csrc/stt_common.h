@@ -113,6 +113,15 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// tanh(y) from the identity tanh a = (1 - e^-2a) / (1 + e^-2a) = -m / (2 + m), with
+// m = expm1(-2a), a = |y| and the sign of y: one expm1 and one division (74 VALU
+// instructions against 171 for the library tanh).  expm1 keeps the relative accuracy
+// near 0; beyond |y| ~ 19 the result is exactly +-1.
+__device__ __forceinline__ double tanh_expm1(double y) {
+  const double m = expm1(-2.0 * fabs(y));
+  return copysign(-m / (2.0 + m), y);
+}
+
 // Sum over a block of kBlock threads; result valid in thread 0.
 __device__ __forceinline__ double block_sum(double v, double* lds /* >= kBlock/kWave */) {
   v = wave_sum(v);

@@ -215,7 +215,7 @@ __global__ __launch_bounds__(256) void heaviside_kernel(long long n, double beta
   if (isinf(beta)) {
     out = v > eta ? 1.0 : (v < eta ? 0.0 : 0.5);
   } else {
-    const double th = tanh(beta * (v - eta));
+    const double th = tanh_expm1(beta * (v - eta));
     out = derivative ? beta * (1.0 - th * th) / denom : (te + th) / denom;
   }
   if (g != nullptr) out *= g[i];

@@ -106,8 +106,10 @@ struct MaskRow {
 __device__ __forceinline__ MaskRow mask_row(double tk, double beta) {
   MaskRow r;
   r.tk = tk;
-  r.a = tanh(beta * tk);
-  r.den = r.a + tanh(beta * (1.0 - tk));
+  // the same tanh as stage_mask's: at t = 1 its numerator then equals den exactly, so the
+  // mask is exactly 0 there and the clamp does not act
+  r.a = tanh_expm1(beta * tk);
+  r.den = r.a + tanh_expm1(beta * (1.0 - tk));
   return r;
 }
 
@@ -115,7 +117,7 @@ __device__ __forceinline__ MaskRow mask_row(double tk, double beta) {
 // (tests/reference/compliance.time_mask_derivative), zero where the clamp acts.
 __device__ __forceinline__ void stage_mask(double t, const MaskRow& k, double beta, double& m,
                                            double& dm) {
-  const double th = tanh(beta * (t - k.tk));
+  const double th = tanh_expm1(beta * (t - k.tk));
   const double v = 1.0 - (k.a + th) / k.den;
   if (v >= 0.0) {
     m = v;
