@@ -6,7 +6,8 @@
 
 namespace stt {
 
-// Gathers the 4d element values of element (er, ec) in local order BL, BR, TR, TL.
+// Gathers the 4d element values of element (er, ec) in local order BL, BR, TR, TL, one
+// vector load per node (u node-aligned: the d = 2 entry points check it).
 template <int D>
 __device__ __forceinline__ void gather_elem(const double* u, int nnx, int er, int ec,
                                             double* ue) {
@@ -14,8 +15,9 @@ __device__ __forceinline__ void gather_elem(const double* u, int nnx, int er, in
   const long long nodes[4] = {tl + nnx, tl + nnx + 1, tl + 1, tl};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
+    const Vec<double, D> v = load_node<double, D>(u, nodes[j]);
 #pragma unroll
-    for (int k = 0; k < D; ++k) ue[j * D + k] = u[nodes[j] * D + k];
+    for (int k = 0; k < D; ++k) ue[j * D + k] = v.v[k];
   }
 }
 
@@ -304,6 +306,11 @@ int sttopt_adjoint_grad(int32_t d, int32_t nelx, int32_t nely, int32_t rows, con
     set_error("sttopt_adjoint_grad: invalid arguments");
     return STTOPT_EINVAL;
   }
+  if (d == 2 && !(pair_aligned(u, u_stride, sizeof(double)) &&
+                  pair_aligned(lam, lam_stride, sizeof(double)))) {
+    set_error("sttopt_adjoint_grad: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
   if (rows == 0 || (grad_density == nullptr && grad_reaction == nullptr)) return STTOPT_OK;
   const long long nel = (long long)nelx * nely;
   dim3 grid(ceil_div(nel, 256), rows), block(256);
@@ -325,6 +332,10 @@ int sttopt_strain_energy(int32_t d, int32_t nelx, int32_t nely, int32_t rows, co
   if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || u == nullptr || ke == nullptr ||
       ce == nullptr || rows < 0) {
     set_error("sttopt_strain_energy: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (d == 2 && !pair_aligned(u, u_stride, sizeof(double))) {
+    set_error("sttopt_strain_energy: d = 2 vectors need node-aligned pointers and even row strides");
     return STTOPT_EINVAL;
   }
   if (rows == 0) return STTOPT_OK;
@@ -374,6 +385,10 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage, const doub
   if (nelx <= 0 || nely <= 0 || nstage < 0 || !x || !ke || !u ||
       (nstage > 0 && (!t || !stage_times))) {
     set_error("sttopt_stage_epilogue: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (!pair_aligned(u, 2, sizeof(double))) {
+    set_error("sttopt_stage_epilogue: u needs a node-aligned (16-byte) pointer");
     return STTOPT_EINVAL;
   }
   StageParams p{nelx, nely, nstage, beta, emin, emax, penal, fe};
