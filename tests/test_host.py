@@ -54,6 +54,15 @@ def test_library_reports_errors_without_touching_the_gpu():
     st = lib.sttopt_apply(_lib.F64, 0, 2, 4, 4, 2, 4096, ndof + 1, 8192, 16, ke, None, None, 0,
                           12288, ndof, None)
     assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
+    # the element gathers of the gradient kernels load nodes as 2-vectors too
+    st = lib.sttopt_strain_energy(2, 4, 4, 1, 4096 + 8, ndof, ke, 8192, 16, None)
+    assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
+    st = lib.sttopt_adjoint_grad(2, 4, 4, 2, 4096, ndof, 8192, ndof + 1, ke, 12288, 16, None, 0,
+                                 None)
+    assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
+    st = lib.sttopt_stage_epilogue(4, 4, 1, 4096, 8192, 12288, 16.0, 1e-9, 1.0, 3.0, 1.0, ke,
+                                   16384 + 8, 20480, None, 24576, None, None, 28672, None)
+    assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
 
 
 def test_coarsening_factors():
