@@ -507,3 +507,174 @@ def test_mma_oracle_converges_to_closed_form_optima():
     root4 = coef**0.25
     np.testing.assert_allclose(_mma_toy(mma_ref.mmasub, True), np.sum(root4) ** (1 / 3) * root4, rtol=1e-4)
     np.testing.assert_allclose(_mma_toy(mma_ref.mmasub, False), [1.0, 1.0], atol=1e-6)
+
+
+# ------------------------------- the V-cycle as explicit dense matrices ----
+
+
+def _hat_interpolation(nc: int, k: int) -> np.ndarray:
+    """1-D prolongation on nodes: fine node i sits at coarse coordinate i / k and takes
+    the coarse nodal hat functions' values there, max(0, 1 - |i / k - j|)."""
+    fine = np.arange(nc * k + 1)[:, None] / k
+    return np.maximum(0.0, 1.0 - np.abs(fine - np.arange(nc + 1)[None, :]))
+
+
+def _dense_vcycle_operator(K0, mask0, nelx, nely, d, omega, n_smooth, gamma, max_coarse):
+    """The V-cycle preconditioner as one dense matrix, from global matrices only.
+
+    Level l: unprojected operator A_l (A_0 = the assembled K, A_{l+1} = P_l^T A_l P_l),
+    projected operator M_l A_l M_l, Jacobi diagonal D_l = diag(A_l) with 1 at fixed dofs,
+    prolongation P_l = kron(P_y, P_x, I_d) of the nodal hat functions, coarse mask =
+    the fine mask at the nodes (r ky, c kx), coarsest solve (M A M + I - M)^-1 M.
+    Smoothing x <- x + omega D^-1 (b - M A M x), n_smooth times, then x <- M x.  The
+    right-hand side is projected on entry (z = B M_0 r)."""
+    levels = []
+    A, mask, nx, ny = K0, mask0.astype(float), nelx, nely
+    while True:
+        levels.append((A, mask, nx, ny))
+        if nx * ny <= max_coarse:
+            break
+        kx = 2 if nx % 2 == 0 else (3 if nx % 3 == 0 else 1)
+        ky = 2 if ny % 2 == 0 else (3 if ny % 3 == 0 else 1)
+        if (kx, ky) == (1, 1):
+            break
+        cx, cy = nx // kx, ny // ky
+        P = np.kron(np.kron(_hat_interpolation(cy, ky), _hat_interpolation(cx, kx)), np.eye(d))
+        grid = mask.reshape(ny + 1, nx + 1, d)
+        levels[-1] = levels[-1] + (P,)
+        A, mask, nx, ny = P.T @ A @ P, grid[::ky, ::kx].reshape(-1), cx, cy
+
+    def B(i):
+        A, m = levels[i][0], np.diag(levels[i][1])
+        n = A.shape[0]
+        Ah = m @ A @ m
+        if i == len(levels) - 1:
+            return m @ np.linalg.inv(Ah + np.eye(n) - m) @ m
+        D = np.where(levels[i][1] > 0, np.diag(A), 1.0)
+        W = omega * np.diag(1.0 / D)
+        J = np.eye(n) - W @ Ah
+        P = levels[i][4]
+        mc = np.diag(levels[i + 1][1])
+        Ac = mc @ levels[i + 1][0] @ mc
+        Bc = B(i + 1)
+        # coarse correction with gamma cycles: E = sum of (I - Bc Ac)^j Bc over j < gamma
+        E = np.zeros_like(Bc)
+        Tc = np.eye(Bc.shape[0])
+        for _ in range(gamma):
+            E = E + Tc @ Bc
+            Tc = (np.eye(Bc.shape[0]) - Bc @ Ac) @ Tc
+        # pre-smoothing from 0: x = m S b, S = sum_{j < nu} J^j W
+        S = sum(np.linalg.matrix_power(J, j) for j in range(n_smooth)) @ W
+        X1 = m @ S
+        R1 = np.eye(n) - Ah @ X1  # r = R1 b
+        X2 = X1 + m @ P @ mc @ E @ mc @ P.T @ R1
+        # post-smoothing: x <- J x + W b, nu times, then x <- m x
+        Jn = np.linalg.matrix_power(J, n_smooth)
+        return m @ (Jn @ X2 + S)
+
+    return B(0) @ np.diag(levels[0][1]), len(levels)
+
+
+@pytest.mark.parametrize(
+    "nelx,nely,d,omega,n_smooth,gamma,max_coarse,reaction,nlev",
+    [
+        (8, 4, 2, 0.6, 2, 1, 8, False, 2),     # the reference's defaults, 2 levels
+        (12, 9, 2, 0.6, 2, 1, 3, False, 3),    # factor 3 in y, 3 levels
+        (12, 9, 2, 0.45, 3, 1, 3, False, 3),   # another omega and sweep count
+        (8, 6, 2, 0.6, 1, 2, 3, False, 3),     # W-cycle (gamma 2), one sweep
+        (12, 6, 1, 0.7, 2, 1, 2, True, 4),     # scalar mesh with a lumped reaction, 4 levels
+    ],
+)
+def test_port_vcycle_equals_dense_matrix_vcycle(nelx, nely, d, omega, n_smooth, gamma,
+                                                 max_coarse, reaction, nlev):
+    """mgcg_port.VCycle (element Galerkin, node-grid transfers, index_add applies,
+    Cholesky coarsest solve) == the same cycle written as products of global dense
+    matrices, to 1e-12.  A wrong omega, sweep count, smoothing order, transfer weight or
+    coarse mask changes the dense operator."""
+    rng = np.random.default_rng(7)
+    nel = nelx * nely
+    nn = (nelx + 1) * (nely + 1)
+    ndof = d * nn
+    KE = fem_direct.plane_stress_KE(NU) if d == 2 else fem_direct.diffusion_KE()
+    edof = fem_direct.element_dofs(nelx, nely, d)
+    dens = rng.uniform(0.05, 1.0, nel)
+    rea = rng.uniform(0.1, 2.0, nel) if reaction else None
+    mask = np.ones(ndof, bool)
+    grid = mask.reshape(nely + 1, nelx + 1, d)
+    if d == 2:
+        grid[:, 0, :] = False            # clamped left edge
+        grid[1, 1, 1] = False            # one fixed dof at a node off every coarse grid
+    else:
+        grid[-1, :, 0] = False           # Dirichlet bottom edge
+        grid[2, 3, 0] = False
+    K0 = fem_direct.assemble(KE, dens, edof, ndof, rea).toarray()
+    Bd, nlev_dense = _dense_vcycle_operator(K0, mask, nelx, nely, d, omega, n_smooth, gamma,
+                                            max_coarse)
+    T = torch.as_tensor
+    levels = mgcg_port.build_hierarchy(T(dens), T(edof), T(KE), T(mask), nelx, nely,
+                                       reaction=None if rea is None else T(rea),
+                                       max_coarse_elements=max_coarse)
+    assert len(levels) == nlev_dense == nlev
+    M = mgcg_port.VCycle(levels, omega=omega, n_smooth=n_smooth, gamma=gamma)
+    r = rng.standard_normal((3, ndof))
+    z = M(T(r)).numpy()
+    want = r @ Bd.T
+    np.testing.assert_allclose(z, want, rtol=0, atol=1e-12 * np.abs(want).max())
+    # the dense operator itself is the SPD preconditioner the outer CG needs
+    free = np.flatnonzero(mask)
+    Bf = Bd[np.ix_(free, free)]
+    np.testing.assert_allclose(Bf, Bf.T, atol=1e-12 * np.abs(Bf).max())
+    assert np.linalg.eigvalsh((Bf + Bf.T) / 2).min() > 0
+
+
+def test_dense_vcycle_construction_is_exact_on_one_level():
+    """The dense construction's own pin: with the coarsest level at the top, the cycle
+    is the exact inverse on the free dofs (M A M + I - M)^-1 restricted by the mask."""
+    nelx, nely = 4, 3
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(nelx, nely)
+    ndof = 2 * (nelx + 1) * (nely + 1)
+    mask = np.ones(ndof, bool)
+    mask[:2 * (nelx + 1)] = False
+    K0 = fem_direct.assemble(KE, np.full(nelx * nely, 0.7), edof, ndof).toarray()
+    Bd, nlev = _dense_vcycle_operator(K0, mask, nelx, nely, 2, 0.6, 2, 1, 100)
+    assert nlev == 1
+    free = np.flatnonzero(mask)
+    np.testing.assert_allclose(Bd[np.ix_(free, free)], np.linalg.inv(K0[np.ix_(free, free)]),
+                               rtol=1e-10, atol=1e-12)
+    # and the hat prolongation reproduces linear fields exactly
+    P = _hat_interpolation(3, 2)
+    np.testing.assert_allclose(P @ (2.0 + 0.5 * np.arange(4)), 2.0 + 0.25 * np.arange(7))
+    np.testing.assert_allclose(P.sum(1), 1.0)
+
+
+# ---------------------------------------------------- the lifted direct solve ----
+
+
+@pytest.mark.parametrize("nelx,nely", [(5, 3), (8, 6)])
+def test_lifted_solve_reproduces_linear_fields(nelx, nely):
+    """Dirichlet data of a linear field u = a + b x + c y on the whole boundary (no
+    source): the bilinear elements hold it exactly, so the lifted solve returns it at
+    every node; and 0 at the bottom, 1 at the top with free sides: u = height fraction.
+    A sign slip in F - K g, or a lift that ignores g, fails both."""
+    KE = fem_direct.diffusion_KE()
+    edof = fem_direct.element_dofs(nelx, nely, 1)
+    nn = (nelx + 1) * (nely + 1)
+    rows, cols = np.divmod(np.arange(nn), nelx + 1)
+    height = (nely - rows).astype(float)   # row 0 is the top
+    K = fem_direct.assemble(KE, np.ones(nelx * nely), edof, nn)
+    exact = 0.3 + 0.7 * cols - 1.1 * height
+    boundary = (rows == 0) | (rows == nely) | (cols == 0) | (cols == nelx)
+    u = fem_direct.lifted_solve(K, np.zeros(nn), np.where(boundary, exact, 0.0), boundary)
+    np.testing.assert_allclose(u, exact, atol=1e-12)
+    plates = (rows == 0) | (rows == nely)
+    g = np.where(rows == 0, 1.0, 0.0)
+    u = fem_direct.lifted_solve(K, np.zeros(nn), g, plates)
+    np.testing.assert_allclose(u, height / nely, atol=1e-12)
+    # with a reaction term and a source the free rows satisfy K u = F exactly
+    rea = np.random.default_rng(1).uniform(0.1, 1.0, nelx * nely)
+    Kr = fem_direct.assemble(KE, np.ones(nelx * nely), edof, nn, rea)
+    F = np.random.default_rng(2).standard_normal(nn)
+    u = fem_direct.lifted_solve(Kr, F, g, plates)
+    np.testing.assert_allclose(u[plates], g[plates])
+    np.testing.assert_allclose((Kr @ u - F)[~plates], 0.0, atol=1e-11)
