@@ -334,6 +334,25 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage,
                           double* dx, double* dt, double* scratch, void* stream);
 int64_t sttopt_stage_scratch_count(int32_t nelx, int32_t nely, int32_t nstage);
 
+/* sttopt_stage_epilogue without the scratch argument and without any allocation: the
+ * compliance partials go to a static device array of STTOPT_NOALLOC_SCRATCH_DOUBLES
+ * doubles (2 MiB, held by every GPU the library runs on until the process ends; its
+ * device address is looked up once per device, on the first call with that device
+ * current).  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  STTOPT_EINVAL when comp != NULL and
+ * sttopt_stage_scratch_count(nelx, nely, nstage) exceeds the array (1024 x 512 with up
+ * to 127 stages fits, 2048 x 1024 with up to 31).  The stream must belong to the
+ * current device.  All calls on one GPU share its array: calls that overlap on that
+ * GPU (other streams or threads) race; calls in order on one stream do not. */
+#define STTOPT_NOALLOC_SCRATCH_DOUBLES 262144
+int sttopt_stage_epilogue_noalloc(int32_t nelx, int32_t nely, int32_t nstage,
+                                  const double* x, const double* t,
+                                  const double* stage_times, double beta, double emin,
+                                  double emax, double penal, double fe, const double* ke,
+                                  const double* u, const double* density, const double* g,
+                                  double* comp, double* dx, double* dt, void* stream);
+
 /* HIP-event timing of the outer PCG's K(rho)p launches inside sttopt_mgpcg (off by
  * default; process-wide, not thread-safe).  enable resets the totals; read returns the
  * summed launch durations, the algorithmic bytes (8 (2 ndof + nel) per row still active

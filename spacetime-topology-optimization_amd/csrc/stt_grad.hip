@@ -2,6 +2,8 @@
 // (torch_solve.py:144-152), the strain energy (compliance.py:49-59), and the fused
 // space-time compliance prologue / epilogue with its hand-derived adjoint
 // (compliance.py:33-46, 137-161, 198-253; tests/reference/compliance.py:24-110).
+#include <mutex>
+
 #include "stt_kernels.h"
 
 namespace stt {
@@ -291,6 +293,42 @@ __global__ __launch_bounds__(256) void stage_adjoint_kernel(StageParams p, const
   }
 }
 
+// Device scratch of sttopt_stage_epilogue_noalloc: the per-block compliance partials
+// live in this fixed array instead of a caller buffer, so that entry point allocates
+// nothing.  Every GPU holds its own copy at its own address until the process ends.
+// In host code the array's name is only a stand-in for the symbol: kernels receive the
+// device address that hipGetSymbolAddress returns for it.
+static __device__ __align__(256) double noalloc_scratch[STTOPT_NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_noalloc_mu;
+static double* g_noalloc_ptr[kMaxDevices] = {};
+
+// Device address of noalloc_scratch on the current device, looked up on the first call
+// per device (that device current) and kept.
+static int noalloc_scratch_ptr(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_status(e, "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("sttopt_stage_epilogue_noalloc: device index beyond the supported 64");
+    return STTOPT_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (g_noalloc_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(noalloc_scratch));
+    if (e != hipSuccess) return hip_status(e, "hipGetSymbolAddress(noalloc_scratch)");
+    if (p == nullptr) {
+      set_error("sttopt_stage_epilogue_noalloc: no device address for the static scratch");
+      return STTOPT_EHIP;
+    }
+    g_noalloc_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_noalloc_ptr[dev];
+  return STTOPT_OK;
+}
+
 }  // namespace stt
 
 using namespace stt;
@@ -417,6 +455,36 @@ int sttopt_stage_epilogue(int32_t nelx, int32_t nely, int32_t nstage, const doub
     STT_CHECK_LAUNCH("stage adjoint");
   }
   return STTOPT_OK;
+}
+
+int sttopt_stage_epilogue_noalloc(int32_t nelx, int32_t nely, int32_t nstage,
+                                  const double* x, const double* t,
+                                  const double* stage_times, double beta, double emin,
+                                  double emax, double penal, double fe, const double* ke,
+                                  const double* u, const double* density, const double* g,
+                                  double* comp, double* dx, double* dt, void* stream) {
+  double* scratch = nullptr;
+  if (comp != nullptr) {
+    if (nelx <= 0 || nely <= 0 || nstage < 0) {
+      set_error("sttopt_stage_epilogue_noalloc: invalid arguments");
+      return STTOPT_EINVAL;
+    }
+    const int64_t need = sttopt_stage_scratch_count(nelx, nely, nstage);
+    if (need > STTOPT_NOALLOC_SCRATCH_DOUBLES) {
+      set_error("sttopt_stage_epilogue_noalloc: " + std::to_string(nelx) + " x " +
+                std::to_string(nely) + " with " + std::to_string(nstage) + " stages needs " +
+                std::to_string(need) + " doubles of scratch; the static buffer holds " +
+                std::to_string(STTOPT_NOALLOC_SCRATCH_DOUBLES) +
+                " (use sttopt_stage_epilogue with a caller buffer)");
+      return STTOPT_EINVAL;
+    }
+    const int st = noalloc_scratch_ptr(&scratch);
+    if (st != STTOPT_OK) return st;
+  }
+  // every partial the compliance kernel reads is written by the same call first, so
+  // the array needs no clearing between calls
+  return sttopt_stage_epilogue(nelx, nely, nstage, x, t, stage_times, beta, emin, emax,
+                               penal, fe, ke, u, density, g, comp, dx, dt, scratch, stream);
 }
 
 }  // extern "C"
