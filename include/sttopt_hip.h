@@ -281,7 +281,9 @@ int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v,
  * between convergence checks (a mark a kernel stores in mapped pinned host memory
  * when rows are still active); each is read one check late
  * (the device keeps the next iteration queued), so the solve may launch one more
- * iteration that finds no active row.  The stream is synchronized at the start, at
+ * iteration that finds no active row.  Row b of x is at x + b * x_stride (x_stride at
+ * least the level-0 dof count when rows > 1); b_stride may be 0 (one load for every
+ * row).  The stream is synchronized at the start, at
  * max_iter and at the end.  Returns STTOPT_ENOCONV when a row is still above rtol
  * after max_iter iterations (x, iters, resid are still written). */
 int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double* x,

@@ -137,7 +137,7 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
 template <int D, typename ZT>
 __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
     TileGeom g, long long n, const double* __restrict__ alpha, const double* __restrict__ beta,
-    double* __restrict__ x, double* __restrict__ p, const ZT* __restrict__ z,
+    double* __restrict__ x, long long xs, double* __restrict__ p, const ZT* __restrict__ z,
     const int* __restrict__ active, const int* __restrict__ iters, int k, int update_p) {
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
@@ -145,7 +145,7 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
   const bool dir = update_p && active[row] != 0;
   const double a = alpha[row];
   const double b = dir ? beta[row] : 0.0;
-  double* xr = x + row * n;
+  double* xr = x + row * xs;  // the caller's solution rows (stride xs)
   double* pr = p + row * n;
   const ZT* zr = z + row * n;
 #pragma unroll
@@ -694,10 +694,24 @@ static int validate(const sttopt_mg* mg) {
   return STTOPT_OK;
 }
 
-// 16 words of pinned, device-mapped, coherent host memory per host thread: word 0 for
-// synchronous readbacks, words 1-2 for the marks of the lagged convergence check.
+// Per-host-thread readback resources: 16 words of pinned, device-mapped, coherent host
+// memory (word 0 for synchronous readbacks, words 1-2 for the marks of the lagged
+// convergence check) and the two events of that check (one per pinned slot).  Created
+// on first use and released when the thread ends, so short-lived worker threads leave
+// nothing behind.
+struct ThreadReadback {
+  int* pinned = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  ~ThreadReadback() {
+    for (hipEvent_t& e : ev)
+      if (e != nullptr) (void)hipEventDestroy(e);
+    if (pinned != nullptr) (void)hipHostFree(pinned);
+  }
+};
+static thread_local ThreadReadback t_readback;
+
 static int* pinned_counter() {
-  static thread_local int* p = nullptr;
+  int*& p = t_readback.pinned;
   if (p == nullptr) {
     if (hipHostMalloc(reinterpret_cast<void**>(&p), 64,
                       hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
@@ -709,10 +723,8 @@ static int* pinned_counter() {
   return p;
 }
 
-// Two events of the lagged convergence check (one per pinned slot), created once per
-// thread.
 static hipEvent_t* lag_events() {
-  static thread_local hipEvent_t ev[2] = {nullptr, nullptr};
+  hipEvent_t* ev = t_readback.ev;
   for (int i = 0; i < 2; ++i) {
     if (ev[i] == nullptr && hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess)
       ev[i] = nullptr;
@@ -913,15 +925,15 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2],
                        fuse_now ? fused_tiles : tiles, sc.part[3], flexible, sc.active, sc.rz,
                        sc.beta, sc.counter);
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
-                    iters, k, 1);
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs, p, z,
+                    sc.active, iters, k, 1);
     STT_CHECK_LAUNCH("pcg direction");
     k_last = 0;
   }
   if (k_last > 0) {
     // the last iteration's x += alpha p (no direction update follows it)
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, p, z, sc.active,
-                    iters, k_last, 0);
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs, p, z,
+                    sc.active, iters, k_last, 0);
     STT_CHECK_LAUNCH("pcg last x update");
   }
   // n_iter: the slowest row's count
@@ -1049,6 +1061,12 @@ int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double*
                        reinterpret_cast<uintptr_t>(work)) % 16 != 0 ||
                       b_stride % 2 != 0 || x_stride % 2 != 0)) {
     set_error("sttopt_mgpcg: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  const long long ndof0 =
+      (long long)mg->d * (mg->levels[0].nelx + 1) * (long long)(mg->levels[0].nely + 1);
+  if (mg->rows > 1 && x_stride < ndof0) {
+    set_error("sttopt_mgpcg: x rows overlap (x_stride below the level-0 dof count)");
     return STTOPT_EINVAL;
   }
   if (mg->rows == 0) {

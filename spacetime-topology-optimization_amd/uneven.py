@@ -53,8 +53,10 @@ def solve_instances(instances: list[Instance], *, rtol: float = 1e-8, max_iter: 
         nelx, nely = key
         with torch.cuda.stream(stream):
             first = instances[idxs[0]]
-            ncases = first.loads.shape[0]
-            dens = torch.stack([instances[i].density for i in idxs]).repeat_interleave(ncases, dim=0)
+            # each instance's density once per load case of its own
+            ncases = [instances[i].loads.shape[0] for i in idxs]
+            dens = torch.cat([instances[i].density.reshape(1, -1).expand(nc, -1)
+                              for i, nc in zip(idxs, ncases)])
             loads = torch.cat([instances[i].loads for i in idxs])
             meta: dict = {}
             U = torch_solve.femsolve(dens, loads, first.edofMat, first.KE, first.mask, nelx, nely,
@@ -80,8 +82,10 @@ def solve_instances(instances: list[Instance], *, rtol: float = 1e-8, max_iter: 
         results = [run(key, idxs, main) for key, idxs in items]
     summary = []
     for (key, idxs), (U, ncases, meta) in zip(items, results):
-        for j, i in enumerate(idxs):
-            out[i] = U[j * ncases:(j + 1) * ncases]
+        start = 0
+        for i, nc in zip(idxs, ncases):
+            out[i] = U[start:start + nc]
+            start += nc
         summary.append(meta)
     if info is not None:
         info["buckets"] = summary

@@ -237,6 +237,57 @@ def test_warm_start_and_exact_warm_start(dev):
     assert n0 == 0 and torch.equal(U3, U)
 
 
+@pytest.mark.parametrize("cold", [True, False])
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_mgpcg_padded_solution_rows_equal_contiguous(dev, dtype, cold):
+    """sttopt_mgpcg through the C ABI with the solution rows at stride ndof + 2 (a padded
+    caller buffer) returns the contiguous call's solution bit for bit, and leaves the
+    padding alone; a stride below ndof is refused."""
+    import ctypes
+
+    from sttopt_amd import _lib, torch_fem, torch_mg
+
+    nelx, nely = 18, 12
+    s = _setup(nelx, nely, dev)
+    n = s["ndof"]
+    x = s["t"](np.stack([binary_design(nelx, nely), np.full(nelx * nely, 0.5),
+                         np.full(nelx * nely, 0.7)]))
+    rng = np.random.default_rng(3)
+    F = s["t"](rng.standard_normal((3, n))) * s["mask_t"]
+    dens = torch_fem.simp_density(x, EMIN, EMAX, PENAL)
+    levels = torch_mg.build_hierarchy(dens, s["edof_t"], s["KE_t"], s["mask_t"], nelx, nely,
+                                      **MG_KW)
+    vc = torch_mg.VCycle(levels, dtype=getattr(torch, dtype))
+    lib = _lib.load()
+    x0 = s["t"](rng.standard_normal((3, n))) * s["mask_t"]
+
+    def run(stride):
+        mg, _, work = vc._native(3)
+        buf = torch.full((3, stride), 7.0, dtype=torch.float64, device=dev)
+        if not cold:
+            buf[:, :n] = x0
+        iters = torch.zeros(3, dtype=torch.int32, device=dev)
+        resid = torch.zeros(3, dtype=torch.float64, device=dev)
+        n_iter = ctypes.c_int32(0)
+        st = lib.sttopt_mgpcg(ctypes.byref(mg), F.data_ptr(), n, buf.data_ptr(), stride, 1e-10,
+                              500, 1 if dtype == "float32" else 0, 1, 1 if cold else 0,
+                              work.data_ptr(), iters.data_ptr(), resid.data_ptr(),
+                              ctypes.byref(n_iter), _lib.stream_handle(dev))
+        _lib.check(st, "sttopt_mgpcg")
+        return buf, iters.tolist()
+
+    ref, it_ref = run(n)
+    pad, it_pad = run(n + 2)
+    assert it_ref == it_pad and max(it_ref) > 0
+    assert torch.equal(pad[:, :n], ref)
+    assert torch.all(pad[:, n:] == 7.0)
+    mg, _, work = vc._native(3)
+    st = lib.sttopt_mgpcg(ctypes.byref(mg), F.data_ptr(), n, ref.data_ptr(), n - 2, 1e-10, 500,
+                          0, 1, 1, work.data_ptr(), ref.data_ptr(), ref.data_ptr(),
+                          ctypes.byref(ctypes.c_int32(0)), _lib.stream_handle(dev))
+    assert st == _lib.EINVAL and "overlap" in _lib.last_error()
+
+
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_cold_start_equals_zero_warm_start(dev, dtype):
     """x0 = None takes the cold-start initialization (x = 0 and r = P b in one pass, x
