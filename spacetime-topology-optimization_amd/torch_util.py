@@ -1,7 +1,8 @@
-"""Sparse-matrix helpers of the density-filter path (drop-in for
-sttopt/torch_util.py:62-188): SciPy CSR to `torch.sparse_csr_tensor`, the symmetric
-tag, and `symmetric_matmul`, whose forward and backward (the same product, by symmetry)
-run in the HIP kernels of `csrc/stt_filter.hip`.  A matrix built by
+"""Boundary conversions and the sparse-matrix helpers of the density-filter path
+(drop-in for sttopt/torch_util.py:19-188): NumPy to torch, SciPy CSR to
+`torch.sparse_csr_tensor`, the symmetric tag, and `symmetric_matmul`, whose forward
+and backward (the same product, by symmetry) run in the HIP kernels of
+`csrc/stt_filter.hip`.  A matrix built by
 `filters.density_filter` is recognised and applied matrix-free as the translation-
 invariant stencil it is (`sttopt_density_filter`); any other symmetric CSR matrix
 goes through the CSR product (`sttopt_csr_matvec`).
@@ -18,6 +19,20 @@ import torch
 from torch import Tensor
 
 from . import _lib
+
+
+def to_tensor(array, device, dtype: torch.dtype | None = None) -> Tensor:
+    """`array` (a NumPy array or a tensor) as a tensor on `device`, in `dtype` when given,
+    else in its own dtype (torch_util.py:19-37)."""
+    if not isinstance(array, Tensor):
+        array = torch.as_tensor(np.asarray(array))
+    return array.to(device=device, dtype=dtype)
+
+
+def to_tensors(arrays: dict, device, dtype: torch.dtype | None = None) -> dict[str, Tensor]:
+    """`to_tensor` of every value of `arrays`, same keys (torch_util.py:40-59)."""
+    return {key: to_tensor(value, device, dtype) for key, value in arrays.items()}
+
 
 SymmetricCsr = NewType("SymmetricCsr", Tensor)
 """A `torch.sparse_csr_tensor` known to equal its own transpose (torch_util.py:127-133)."""

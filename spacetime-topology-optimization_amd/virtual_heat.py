@@ -2,8 +2,8 @@
 (drop-in for sttopt/virtual_heat.py:51-462): the diffusion-reaction solve
 `-div(chi grad u) + reaction u = F`, `u = g` on the fixed nodes; the Gauss-point
 gradient fields (`gauss_gradients`, `direction_field`), the divergence load and the
-Poisson distance fitted to a direction field; and the heat time field built from
-them.  The gradient fields, the load and their adjoints run in the HIP kernels of
+Poisson distance fitted to a direction field; and the heat and Laplace time fields
+built from them.  The gradient fields, the load and their adjoints run in the HIP kernels of
 `csrc/stt_heat.hip` (`sttopt_gauss_gradients`, `sttopt_divergence_load`,
 `sttopt_gauss_dot`, `sttopt_direction_field_backward`); the solves in the batched
 MG-PCG of `torch_solve`.
@@ -18,7 +18,9 @@ from dataclasses import dataclass, field
 from typing import NamedTuple
 
 import numpy as np
+import scipy.sparse as sp
 import torch
+from scipy.sparse.csgraph import dijkstra
 from torch import Tensor
 
 from . import _lib
@@ -32,6 +34,14 @@ DENSITY_FLOOR = 1e-6
 #: Floor on `T` in the `neg_log` map, relative to the smallest `T` on the part
 #: (virtual_heat.py:46-48).
 NEG_LOG_FLOOR = 1e-3
+
+
+class LaplaceTimeMap(str, enum.Enum):
+    """How the harmonic field maps to the time field (the values of the reference's
+    `run_config.LaplaceTimeMap`, run_config.py:487-491)."""
+
+    IDENTITY = "identity"
+    POISSON = "poisson"
 
 
 class HeatTimeMap(str, enum.Enum):
@@ -474,3 +484,105 @@ def heat_time_field(config, mesh: ScalarMesh, xPhys: Tensor, mu: Tensor,
         t = phi
     t_e = element_means(t, mesh.edof).reshape(shape)
     return TimeFieldSolution(normalize(t_e, xPhys), T, phi, t)
+
+
+# ------------------------------------------------------ Laplace time field ----
+
+
+def _node_edges(active: np.ndarray) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """The node graph through the design domain (virtual_heat.py:465-489): every
+    domain element links its four corners along its four sides (step 1) and across its
+    two diagonals (step sqrt 2).  A side shared by two domain elements is one edge.
+
+    :return: `(src, dst, step)` in both directions
+    """
+    nely, nelx = active.shape
+    er, ec = np.nonzero(active)
+    tl = er * (nelx + 1) + ec
+    bl, br, tr = tl + nelx + 1, tl + nelx + 2, tl + 1
+    pairs = [(bl, br), (tl, tr), (tl, bl), (tr, br), (bl, tr), (tl, br)]
+    lengths = [1.0, 1.0, 1.0, 1.0, np.sqrt(2.0), np.sqrt(2.0)]
+    a = np.concatenate([p[0] for p in pairs])
+    b = np.concatenate([p[1] for p in pairs])
+    w = np.concatenate([np.full(tl.size, length) for length in lengths])
+    key = np.minimum(a, b) * ((nelx + 1) * (nely + 1)) + np.maximum(a, b)
+    _, first = np.unique(key, return_index=True)
+    a, b, w = a[first], b[first], w[first]
+    return np.concatenate([a, b]), np.concatenate([b, a]), np.concatenate([w, w])
+
+
+def _plate_distance(mesh: ScalarMesh) -> Tensor:
+    """The wall arc's graph distance to the plate through the design domain, in units of
+    `l_c` (virtual_heat.py:492-507): shortest paths over `_node_edges`, exact along the
+    axes and diagonals, so on a rectangle with the plate on one edge it is the distance
+    to that edge, and a wall behind a re-entrant corner takes the path around it."""
+    active = mesh.active.detach().cpu().numpy().reshape(mesh.nely, mesh.nelx)
+    src, dst, step = _node_edges(active)
+    graph = sp.csr_matrix((step, (src, dst)), shape=(mesh.ndof, mesh.ndof))
+    plate = np.flatnonzero(mesh.plate.detach().cpu().numpy())
+    dist = dijkstra(graph, directed=True, indices=plate, min_only=True)
+    arc = mesh.arc.detach().cpu().numpy()
+    return torch.as_tensor(dist[arc] / mesh.unit_length, dtype=mesh.KE.dtype,
+                           device=mesh.arc.device)
+
+
+def distance_wall(mesh: ScalarMesh) -> Tensor:
+    """The wall design of the distance to the plate through the domain, scaled to a
+    largest value of 1/2 (virtual_heat.py:510-520)."""
+    d = _plate_distance(mesh)
+    return 0.5 * d / d.max()
+
+
+def wall_filter(n_arc: int, rmin: float) -> np.ndarray:
+    """The cone filter `max(0, rmin - |i - j|)` along the wall arc, `(n_arc, n_arc)`
+    (virtual_heat.py:523-543).
+
+    The arc meets the plate (t = 0) at positions -1 and n_arc, so the data is continued
+    past each end by odd reflection about those corners (node j reappears at -2 - j and
+    2 n_arc - j with its sign flipped): a ramp rising from the plate passes unchanged.
+    The node's own weight is max(rmin, 1), and every row is divided by the full cone's
+    sum.  rmin <= 1 leaves the data unfiltered."""
+    own = max(float(rmin), 1.0)
+    reach = int(np.ceil(rmin)) - 1  # farthest other node with a positive weight
+    cone = lambda d: np.clip(rmin - np.abs(d), 0.0, None)  # noqa: E731
+    i = np.arange(n_arc)[:, None]
+    j = np.arange(n_arc)[None, :]
+    H = np.where(i == j, own, cone(i - j))
+    H = H - cone(i - (-2 - j)) - cone(i - (2 * n_arc - j))
+    total = own + 2.0 * sum(cone(d) for d in range(1, reach + 1))
+    return H / total
+
+
+def laplace_time_field(config, mesh: ScalarMesh, xPhys: Tensor, mu: Tensor, b: Tensor,
+                       previous: TimeFieldSolution | None = None,
+                       density_exponent: float = 0.0) -> TimeFieldSolution:
+    """The time field of `div(chi grad t) = 0` over the design domain with `t = 0` on the
+    plate and `t = b` on the wall arc (virtual_heat.py:546-607); outside the domain
+    `chi = 1` (the harmonic extension).  `config` carries `chi_contrast`,
+    `chi_density_floor`, `chi_density_sensitivity`, `laplace_cg_rtol`, `time_map` (a
+    `LaplaceTimeMap` value) and `poisson_cg_rtol`.
+
+    :param density_exponent: `e` in `chi = max(xPhys**e, chi_density_floor) chi(mu)`;
+        0 leaves `chi` independent of the density
+    """
+    shape = xPhys.shape
+    chi = diffusivity(mu.flatten(), config.chi_contrast)
+    if density_exponent != 0:
+        rho = xPhys if config.chi_density_sensitivity else xPhys.detach()
+        chi = chi * torch.clamp(rho.flatten() ** density_exponent, min=config.chi_density_floor)
+    chi = torch.where(mesh.active, chi, torch.ones_like(chi))
+    g = torch.zeros(mesh.ndof, dtype=chi.dtype, device=chi.device).index_put((mesh.arc,), b)
+    fixed = mesh.plate.clone()
+    fixed[mesh.arc] = True
+    u = mesh.solve(chi, g, fixed, config.laplace_cg_rtol,
+                   x0=None if previous is None else previous.primary)
+    phi = None
+    time_map = LaplaceTimeMap(config.time_map)
+    if time_map == LaplaceTimeMap.IDENTITY:
+        t = u
+    else:
+        phi = mesh.poisson_distance(u, torch.ones_like(chi), config.poisson_cg_rtol,
+                                    x0=None if previous is None else previous.poisson)
+        t = phi
+    t_e = element_means(t, mesh.edof).reshape(shape)
+    return TimeFieldSolution(normalize(t_e, xPhys), u, phi, t)

@@ -183,3 +183,111 @@ def test_poisson_time_field_gradient_matches_finite_differences(dev):
         fd = (Lfixed(xPhys + h * dx, mu + h * dmu) - Lfixed(xPhys - h * dx, mu - h * dmu)) / (2 * h)
         ad = float((xt.grad.cpu().numpy() * dx).sum() + (mt.grad.cpu().numpy() * dmu).sum())
         assert abs(ad - fd) <= 1e-5 * max(abs(fd), 1e-8), (k, ad, fd)
+
+
+# ------------------------------------------------------- Laplace time field ----
+
+
+def _oracle_laplace(nelx, nely, active, plate, arc, xPhys, mu, b, cfg, e=0.0):
+    """The Laplace time field composed from the oracle's direct solves: chi over the
+    domain (1 outside), t = 0 on the plate, t = b on the wall arc."""
+    nn = (nelx + 1) * (nely + 1)
+    edof = fem_direct.element_dofs(nelx, nely, 1)
+    x = xPhys.ravel()
+    chi = cfg.chi_contrast ** (mu.ravel() - 0.5)
+    if e != 0:
+        chi = chi * np.maximum(x**e, cfg.chi_density_floor)
+    chi = np.where(active.ravel(), chi, 1.0)
+    K = fem_direct.assemble(fem_direct.diffusion_KE(), chi, edof, nn)
+    fixed = plate.copy()
+    fixed[arc] = True
+    g = np.zeros(nn)
+    g[arc] = b
+    u = fem_direct.lifted_solve(K, np.zeros(nn), g, fixed)
+    t = u if cfg.time_map == "identity" else heat_fields.poisson_distance(
+        u, np.ones(nelx * nely), plate, nelx, nely)
+    te = t[edof].mean(-1)
+    part = x >= x.max() / 2
+    return (te / te[part].max()).reshape(nely, nelx), u, t
+
+
+@pytest.mark.parametrize("time_map, notch, e", [("identity", False, 0.0),
+                                                ("identity", True, 3.0),
+                                                ("poisson", True, 0.0)])
+def test_laplace_time_field_matches_oracle(dev, time_map, notch, e):
+    from sttopt_amd import virtual_heat
+
+    nelx, nely = 36, 22
+    active = np.ones((nely, nelx), bool)
+    if notch:
+        active[:8, 24:] = False
+    mesh = virtual_heat.ScalarMesh.build(nelx, nely, "bottom_edge", dev, active=active)
+    rng = np.random.default_rng(4)
+    xPhys = rng.uniform(0.1, 1, (nely, nelx)) * active
+    mu = rng.uniform(0.3, 0.7, (nely, nelx))
+    wall = virtual_heat.distance_wall(mesh)
+    H = virtual_heat.wall_filter(len(mesh.arc), 3.0)
+    b = _t(H, dev) @ (wall + 0.05 * _t(rng.standard_normal(len(mesh.arc)), dev))
+    cfg = types.SimpleNamespace(chi_contrast=10.0, chi_density_floor=1e-2,
+                                chi_density_sensitivity=True, time_map=time_map,
+                                laplace_cg_rtol=1e-12, poisson_cg_rtol=1e-12)
+    sol = virtual_heat.laplace_time_field(cfg, mesh, _t(xPhys, dev), _t(mu, dev), b,
+                                          density_exponent=e)
+    want, u, t = _oracle_laplace(nelx, nely, active, mesh.plate.cpu().numpy(),
+                                 mesh.arc.cpu().numpy(), xPhys, mu, b.cpu().numpy(), cfg, e)
+    assert (sol.poisson is not None) == (time_map == "poisson")
+    np.testing.assert_allclose(sol.primary.detach().cpu().numpy(), u, atol=1e-9)
+    np.testing.assert_allclose(sol.t.detach().cpu().numpy(), t, atol=1e-8 * np.abs(t).max())
+    np.testing.assert_allclose(sol.tPhys.detach().cpu().numpy(), want, atol=1e-8)
+    # warm start from the previous solution: the same field
+    sol2 = virtual_heat.laplace_time_field(cfg, mesh, _t(xPhys, dev), _t(mu, dev), b,
+                                           previous=sol, density_exponent=e)
+    np.testing.assert_allclose(sol2.tPhys.detach().cpu().numpy(), want, atol=1e-8)
+
+
+@pytest.mark.parametrize("time_map", ["identity", "poisson"])
+def test_laplace_time_field_gradient_matches_finite_differences(dev, time_map):
+    """Autograd through the lifted solve (a real adjoint solve: the load depends on the
+    wall data through K g) and, for the Poisson map, the direction field and the second
+    solve, against central differences of the oracle along seeded directions in mu,
+    xPhys (density exponent 2) and the wall data b."""
+    from sttopt_amd import virtual_heat
+
+    nelx, nely = 18, 12
+    active = np.ones((nely, nelx), bool)
+    active[:4, 12:] = False
+    mesh = virtual_heat.ScalarMesh.build(nelx, nely, "bottom_edge", dev, active=active)
+    plate, arc = mesh.plate.cpu().numpy(), mesh.arc.cpu().numpy()
+    rng = np.random.default_rng(8)
+    xPhys = rng.uniform(0.2, 1, (nely, nelx))
+    mu = rng.uniform(0.3, 0.7, (nely, nelx))
+    b = 0.5 * virtual_heat.distance_wall(mesh).cpu().numpy() + 0.02 * rng.standard_normal(arc.size)
+    c = rng.standard_normal((nely, nelx))
+    cfg = types.SimpleNamespace(chi_contrast=10.0, chi_density_floor=1e-2,
+                                chi_density_sensitivity=True, time_map=time_map,
+                                laplace_cg_rtol=1e-13, poisson_cg_rtol=1e-13)
+    e = 2.0
+    xt = _t(xPhys, dev).requires_grad_(True)
+    mt = _t(mu, dev).requires_grad_(True)
+    bt = _t(b, dev).requires_grad_(True)
+    sol = virtual_heat.laplace_time_field(cfg, mesh, xt, mt, bt, density_exponent=e)
+    (sol.tPhys * _t(c, dev)).sum().backward()
+    edof = fem_direct.element_dofs(nelx, nely, 1)
+    _, _, t0 = _oracle_laplace(nelx, nely, active, plate, arc, xPhys, mu, b, cfg, e)
+    te0 = t0[edof].mean(-1)
+    s0 = te0[xPhys.ravel() >= xPhys.max() / 2].max()  # the normalization is held constant
+
+    def L(xv, mv, bv):
+        _, _, t = _oracle_laplace(nelx, nely, active, plate, arc, xv, mv, bv, cfg, e)
+        return ((t[edof].mean(-1).reshape(nely, nelx) / s0) * c).sum()
+
+    h = 1e-6
+    for k in range(3):
+        dx = rng.standard_normal(xPhys.shape) * 0.1
+        dm = rng.standard_normal(mu.shape)
+        db = rng.standard_normal(b.shape) * 0.1
+        fd = (L(xPhys + h * dx, mu + h * dm, b + h * db)
+              - L(xPhys - h * dx, mu - h * dm, b - h * db)) / (2 * h)
+        ad = float((xt.grad.cpu().numpy() * dx).sum() + (mt.grad.cpu().numpy() * dm).sum()
+                   + (bt.grad.cpu().numpy() * db).sum())
+        assert abs(ad - fd) <= 1e-5 * max(abs(fd), 1e-8), (k, ad, fd)
