@@ -19,16 +19,27 @@ from . import _lib
 from . import torch_util
 
 
-def _neighbor_offsets(radius: float) -> list[tuple[int, int]]:
-    """(di, dj) over the square of half-width `ceil(radius) - 1` (filters.py:22-25)."""
-    r = int(np.ceil(radius)) - 1
-    return [(di, dj) for di in range(-r, r + 1) for dj in range(-r, r + 1)]
+def filter_stencil(rmin: float) -> np.ndarray:
+    """The cone weights `max(0, rmin - |(di, dj)|)` on the square of half-width
+    `ceil(rmin) - 1` around an element, indexed `[dj + r, di + r]` (row offset, column
+    offset): the stencil `density_filter_r_kernel` applies on the device
+    (filters.py:22-64)."""
+    r = max(int(math.ceil(rmin)) - 1, 0)
+    off = np.arange(-r, r + 1, dtype=np.float64)
+    return np.clip(rmin - np.hypot(off[:, None], off[None, :]), 0.0, None)
 
 
-def _element_grid(nelx: int, nely: int) -> tuple[np.ndarray, np.ndarray]:
-    """(i, j) = (column, row) of every element in linear-index order (filters.py:28-35)."""
-    e = np.arange(nelx * nely)
-    return e % nelx, e // nelx
+def _stencil_matrix(nelx: int, nely: int, W: np.ndarray) -> sp.csr_matrix:
+    """The element-grid operator of stencil W: the sum over its nonzero offsets of
+    `W[dj, di]` times the shift to element (row + dj, column + di), a Kronecker product
+    of two one-dimensional shifts (no wrap-around at the grid edges)."""
+    r = W.shape[0] // 2
+    n = nelx * nely
+    out = sp.csr_matrix((n, n))
+    for a, b in zip(*np.nonzero(W)):
+        dj, di = int(a) - r, int(b) - r
+        out = out + W[a, b] * sp.kron(sp.eye(nely, k=dj), sp.eye(nelx, k=di), format="csr")
+    return out.tocsr()
 
 
 def density_filter(nelx: int, nely: int, rmin: float) -> tuple[sp.csr_matrix, np.ndarray]:
@@ -36,20 +47,9 @@ def density_filter(nelx: int, nely: int, rmin: float) -> tuple[sp.csr_matrix, np
     (including e1 == e2), and the row sums `Hs` (filters.py:38-64).  The returned matrix
     is tagged with its stencil, so that `torch_util.symmetric_csr_to_tensor` and
     `apply_density_filter` apply it matrix-free."""
-    i, j = _element_grid(nelx, nely)
-    n = nelx * nely
-    rows, cols, vals = [], [], []
-    for di, dj in _neighbor_offsets(rmin):
-        w = max(0.0, rmin - math.hypot(di, dj))
-        if w == 0.0:
-            continue
-        i2, j2 = i + di, j + dj
-        ok = (i2 >= 0) & (i2 < nelx) & (j2 >= 0) & (j2 < nely)
-        rows.append(np.flatnonzero(ok))
-        cols.append((j2 * nelx + i2)[ok])
-        vals.append(np.full(int(ok.sum()), w))
-    H = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                      shape=(n, n)).tocsr()
+    H = _stencil_matrix(nelx, nely, filter_stencil(rmin))
+    H.eliminate_zeros()
+    H.sort_indices()
     H._sttopt_stencil = (nelx, nely, float(rmin))
     Hs = np.asarray(H.sum(axis=1)).ravel()
     return H, Hs
@@ -80,18 +80,17 @@ def continuity_filter(nelx: int, nely: int, lrmin: float) -> sp.csr_matrix:
 
     :raises ValueError: if some element has no neighbour within `lrmin`
     """
-    H, _ = density_filter(nelx, nely, lrmin)
-    N = H.tolil()
-    N.setdiag(0.0)
-    N = N.tocsr()
+    W = filter_stencil(lrmin)
+    W[W.shape[0] // 2, W.shape[1] // 2] = 0.0  # the neighbours only
+    N = _stencil_matrix(nelx, nely, W)
     N.eliminate_zeros()
-    row_sum = np.asarray(N.sum(axis=1)).ravel()
-    if not np.all(row_sum > 0):
+    neighbour_weight = np.asarray(N.sum(axis=1)).ravel()
+    if not np.all(neighbour_weight > 0):
         raise ValueError(
             f"the continuity average of lrmin={lrmin:g} elements reaches no neighbor of some element"
         )
-    n = nelx * nely
-    return (sp.eye(n, format="csr") - sp.diags(1.0 / row_sum) @ N).tocsr()
+    average = sp.diags(1.0 / neighbour_weight) @ N
+    return (sp.identity(nelx * nely, format="csr") - average).tocsr()
 
 
 def _heaviside(x: Tensor, beta: float, eta: float, derivative: bool,

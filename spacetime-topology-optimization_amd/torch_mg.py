@@ -6,8 +6,9 @@ damped-Jacobi smoothing with equal pre/post sweeps and a dense coarsest solve, s
 the V-cycle is the same SPD preconditioner.  What moves to the device:
 
 - `build_hierarchy`: Galerkin products (`sttopt_galerkin`), coarse diagonals, the
-  dense coarsest operator (`sttopt_dense_assemble`); its Cholesky factor and
-  inverse come from torch.linalg on the device.  Coarse element matrices are kept
+  dense coarsest operator (`sttopt_dense_assemble`) and its inverse by the block
+  symmetric sweep on the fp64 matrix cores (`sttopt_spd_inverse`); the reference's
+  `chol` field is formed on first access.  Coarse element matrices are kept
   entry-major (`[row][entry][element]`) so the coarse apply reads them coalesced.
 - `VCycle.__call__`: one native call (`sttopt_vcycle`).
 - `solve` / `pcg(levels[0], b, VCycle)`: the whole MG-PCG in `sttopt_mgpcg`.
@@ -29,11 +30,11 @@ import functools
 import threading
 from dataclasses import dataclass, field
 
+import numpy as np
 import torch
 from torch import Tensor
 
 from . import _grid, _lib
-from . import fem as _fem
 from . import torch_fem
 
 #: Stop coarsening once a level has at most this many elements (torch_mg.py:47-50).
@@ -102,70 +103,102 @@ def coarsening_factors(nelx: int, nely: int) -> tuple[int, int]:
     return one(nelx), one(nely)
 
 
+@functools.lru_cache(maxsize=None)
+def _hat_matrix(nc: int, k: int, dtype, device) -> Tensor:
+    """One-dimensional prolongation on a line of nodes, `(k nc + 1, nc + 1)`: fine node i
+    lies at coarse coordinate i / k and takes the coarse nodal hat functions' values
+    there, `max(0, 1 - |i / k - j|)` (linear interpolation, exact at coarse nodes)."""
+    fine = torch.arange(k * nc + 1, dtype=torch.float64)[:, None] / k
+    coarse = torch.arange(nc + 1, dtype=torch.float64)[None, :]
+    return torch.clamp(1 - (fine - coarse).abs(), min=0).to(device=device, dtype=dtype)
+
+
 def _interp_axis(c: Tensor, k: int) -> Tensor:
-    """Linear interpolation by factor `k` along the last axis, `nc+1 -> k*nc+1` nodes."""
+    """Linear interpolation by factor `k` along the last axis, `nc+1 -> k*nc+1` nodes
+    (torch_mg.py:75-82): the hat-function matrix applied to that axis."""
     if k == 1:
         return c
-    w = torch.arange(k, dtype=c.dtype, device=c.device) / k
-    left, right = c[..., :-1, None], c[..., 1:, None]
-    inner = (left * (1 - w) + right * w).flatten(-2)
-    return torch.cat([inner, c[..., -1:]], dim=-1)
+    P = _hat_matrix(c.shape[-1] - 1, k, c.dtype, c.device)
+    return c @ P.T
 
 
 def _restrict_axis(f: Tensor, k: int) -> Tensor:
-    """Transpose of `_interp_axis` (full weighting) along the last axis."""
+    """Full weighting along the last axis, the exact transpose of `_interp_axis`
+    (torch_mg.py:85-95)."""
     if k == 1:
         return f
-    w = torch.arange(k, dtype=f.dtype, device=f.device) / k
-    blocks = f[..., :-1].unflatten(-1, (-1, k))
-    own = (blocks * (1 - w)).sum(-1)  # coarse node i from fine k*i + j, j < k
-    nxt = (blocks * w).sum(-1)  # coarse node i + 1 from the same fine nodes
-    out = torch.zeros(*f.shape[:-1], own.shape[-1] + 1, dtype=f.dtype, device=f.device)
-    out[..., :-1] += own
-    out[..., 1:] += nxt
-    out[..., -1] += f[..., -1]
-    return out
+    P = _hat_matrix((f.shape[-1] - 1) // k, k, f.dtype, f.device)
+    return f @ P
+
+
+def _along(t: Tensor, axis: int, op, k: int) -> Tensor:
+    """`op(., k)` applied along `axis` of `t` (op works on the last axis)."""
+    return op(t.movedim(axis, -1), k).movedim(-1, axis)
 
 
 def _on_node_grid(v: Tensor, nnx: int, nny: int, kx: int, ky: int, axis_op) -> Tensor:
-    """Apply `axis_op(t, k)` along x then y of a dof vector on a `(nny, nnx, d)` node grid."""
-    d = v.shape[-1] // (nnx * nny)
-    g = v.reshape(*v.shape[:-1], nny, nnx, d)
-    g = axis_op(g.transpose(-1, -2), kx).transpose(-1, -2)
-    g = axis_op(g.movedim(-3, -1), ky).movedim(-1, -3)
-    return g.reshape(*v.shape[:-1], -1)
+    """`axis_op` along x, then along y, of a dof vector laid out as an `(nny, nnx, d)`
+    node grid with `d` interleaved dofs per node (torch_mg.py:98-109)."""
+    lead = v.shape[:-1]
+    grid = v.reshape(*lead, nny, nnx, v.shape[-1] // (nnx * nny))
+    grid = _along(_along(grid, -2, axis_op, kx), -3, axis_op, ky)
+    return grid.reshape(*lead, -1)
 
 
 @functools.lru_cache(maxsize=None)
 def element_dof_map_tensor(nelx: int, nely: int, device, dofs_per_node: int = 2) -> Tensor:
-    """`fem.element_dof_map` as a cached int64 device tensor (torch_mg.py:112-127)."""
-    return torch.tensor(
-        _fem.element_dof_map(nelx, nely, dofs_per_node), dtype=torch.int64, device=device
-    )
+    """The C-order element dof map (`fem.element_dof_map`) as an int64 device tensor,
+    cached per grid and device (torch_mg.py:112-127): element (r, c) lists its corners
+    BL, BR, TR, TL, each with its `d` interleaved dofs."""
+    d = dofs_per_node
+    r = torch.arange(nely, dtype=torch.int64)[:, None]
+    c = torch.arange(nelx, dtype=torch.int64)[None, :]
+    top_left = (r * (nelx + 1) + c).reshape(-1, 1)
+    corners = top_left + torch.tensor([nelx + 1, nelx + 2, 1, 0], dtype=torch.int64)
+    dofs = d * corners[:, :, None] + torch.arange(d, dtype=torch.int64)
+    return dofs.reshape(nelx * nely, 4 * d).to(device)
 
 
 @functools.lru_cache(maxsize=None)
 def _child_elements(nelx: int, nely: int, kx: int, ky: int, device) -> Tensor:
-    """Fine elements under each coarse element, children ordered row-major in the block."""
-    e = torch.arange(nelx * nely, device=device).reshape(nely // ky, ky, nelx // kx, kx)
-    return e.permute(0, 2, 1, 3).reshape(-1, ky * kx)
+    """`(nel_coarse, kx*ky)`: the fine elements under each coarse element, children in
+    row-major order within the block (torch_mg.py:130-137).  Coarse element (R, C) holds
+    fine element (R ky + a, C kx + b) as child a kx + b."""
+    R = torch.arange(nely // ky)[:, None, None, None]
+    C = torch.arange(nelx // kx)[None, :, None, None]
+    a = torch.arange(ky)[None, None, :, None]
+    b = torch.arange(kx)[None, None, None, :]
+    fine = (R * ky + a) * nelx + C * kx + b
+    return fine.reshape(-1, ky * kx).to(device)
 
 
 @functools.lru_cache(maxsize=None)
 def child_interpolation_matrices(
     kx: int, ky: int, device=None, dtype=torch.float64, dofs_per_node: int = 2
 ) -> Tensor:
-    """`S[m]`: child element m's nodal values from its coarse element's, under the
-    same bilinear interpolation as the grid transfer, shape `(kx*ky, 4d, 4d)`
-    (torch_mg.py:140-165).  Derived by running `_interp_axis` on a one-element mesh."""
+    """`S[m]`, `(kx*ky, 4d, 4d)`: child element m's nodal values from its coarse
+    element's under bilinear interpolation (torch_mg.py:140-165).
+
+    In the coarse element's local coordinates (x right, y up, both in [0, 1]), child
+    m = a kx + b (block row a from the top, column b) has its corners BL, BR, TR, TL at
+    x in {b, b + 1} / kx and y in {ky - a - 1, ky - a} / ky; entry [p d + i, q d + j] is
+    the coarse shape function of corner q at child corner p when i == j (each dof
+    component interpolates on its own)."""
     d = dofs_per_node
-    coarse_edof = torch.tensor(_fem.element_dof_map(1, 1, d)[0])
-    fine_edof = torch.tensor(_fem.element_dof_map(kx, ky, d))
-    basis = torch.zeros(4 * d, 4 * d, dtype=torch.float64)
-    basis[torch.arange(4 * d), coarse_edof] = 1.0
-    fine = _on_node_grid(basis, 2, 2, kx, ky, _interp_axis)  # (coarse dof, fine dof)
-    S = fine[:, fine_edof].permute(1, 2, 0)
-    return S.to(device=device, dtype=dtype).contiguous()
+    corner_x = np.array([0.0, 1.0, 1.0, 0.0])  # BL, BR, TR, TL
+    corner_y = np.array([0.0, 0.0, 1.0, 1.0])
+
+    def shape(x, y):  # N_q(x, y) of the coarse corners, q in BL, BR, TR, TL order
+        return np.where(corner_x > 0, x, 1 - x) * np.where(corner_y > 0, y, 1 - y)
+
+    S = np.zeros((kx * ky, 4 * d, 4 * d))
+    for a in range(ky):
+        for b in range(kx):
+            for p in range(4):
+                x = (b + corner_x[p]) / kx
+                y = (ky - a - 1 + corner_y[p]) / ky
+                S[a * kx + b, p * d:(p + 1) * d, :] = np.kron(shape(x, y), np.eye(d))
+    return torch.as_tensor(S, dtype=dtype, device=device).contiguous()
 
 
 def _to_soa(keff: Tensor) -> Tensor:

@@ -104,6 +104,17 @@ def test_child_interpolation_matrices(kx, ky, d):
     np.testing.assert_allclose(S.sum(-1).numpy(), np.ones(S.shape[:2]), atol=1e-15)
 
 
+@pytest.mark.parametrize("nelx, nely, d", [(4, 3, 2), (5, 2, 1), (1, 1, 2)])
+def test_element_dof_map_tensor_equals_the_explicit_map(nelx, nely, d):
+    from oracle import fem_direct
+    from sttopt_amd import fem, torch_mg
+
+    t = torch_mg.element_dof_map_tensor(nelx, nely, "cpu", d)
+    assert t.dtype == torch.int64
+    np.testing.assert_array_equal(t.numpy(), fem_direct.element_dofs(nelx, nely, d))
+    np.testing.assert_array_equal(t.numpy(), fem.element_dof_map(nelx, nely, d))
+
+
 def test_child_elements_order():
     from sttopt_amd import torch_mg
 
