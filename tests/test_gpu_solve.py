@@ -199,6 +199,48 @@ def test_space_time_compliance_matches_oracle(dev):
             assert float(gt.abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("composed", [False, True])
+def test_jacobian_rows_through_the_space_time_compliance(dev, composed):
+    """`jacobian_rows` of the 1 + nStage compliances w.r.t. (x, t, an unused leaf), the
+    way the drivers call it: each row equals the oracle's hand-derived gradients of that
+    row alone (several backward passes through one retained graph), and the unused
+    leaf's block is exactly zero.  composed: through the reference composition (FemSolve
+    autograd, selected by patching `_solve_fe_batched`) instead of the fused Function."""
+    from sttopt_amd import compliance, sensitivity, workloads
+
+    p = workloads.space_time(36, 20, 3, beta=16.0, seed=4, device=dev)
+    x = p.xPhys.clone().requires_grad_(True)
+    t = p.tPhys.clone().requires_grad_(True)
+    spare = torch.ones(5, dtype=torch.float64, device=dev, requires_grad=True)
+    orig = compliance._solve_fe_batched
+    C = p.C  # the fused path takes the stencil weight; the composition needs the matrix
+    if composed:
+        compliance._solve_fe_batched = lambda *a, **k: orig(*a, **k)
+        Cs = fem_direct.gravity_matrix(p.nelx, p.nely).tocoo()
+        C = torch.sparse_coo_tensor(np.vstack([Cs.row, Cs.col]), Cs.data, Cs.shape,
+                                    device=dev).coalesce()
+    try:
+        c, stages, _ = compliance.batched_whole_and_gravity_compliance(
+            x, t, p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof, C, p.beta,
+            p.stage_times)
+        jx, jt, js = sensitivity.jacobian_rows(torch.stack([c] + list(stages)), (x, t, spare))
+    finally:
+        compliance._solve_fe_batched = orig
+    KE = fem_direct.plane_stress_KE(NU)
+    edof = fem_direct.element_dofs(p.nelx, p.nely)
+    _, dcx_ref, st_ref, _ = fem_direct.space_time(
+        p.xPhys.cpu().numpy(), p.tPhys.cpu().numpy(), p.stage_times, p.beta, KE, edof,
+        p.F.cpu().numpy(), p.freedofs.cpu().numpy(), p.C)
+    k = 1 + len(p.stage_times)
+    assert jx.shape == (k, p.nel) and jt.shape == (k, p.nel) and js.shape == (k, 5)
+    assert float(js.abs().max()) == 0.0
+    assert rel_err(jx[0].cpu().numpy(), dcx_ref) < 1e-5
+    assert float(jt[0].abs().max()) == 0.0  # the whole structure does not see t
+    for i, (_, gx_ref, gt_ref) in enumerate(st_ref, start=1):
+        assert rel_err(jx[i].cpu().numpy(), gx_ref) < 1e-5, i
+        assert rel_err(jt[i].cpu().numpy(), gt_ref) < 1e-5, i
+
+
 def test_space_time_fused_equals_composed_autograd(dev):
     """The fused path against the reference composition (torch algebra + FemSolve
     autograd), selected by patching the `_solve_fe_batched` hook."""
