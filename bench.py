@@ -91,7 +91,7 @@ def main():
         c, stages, U = compliance.batched_whole_and_gravity_compliance(
             x, t, p.KE, p.edofMat, *E, p.freedofs, p.F, p.ndof, p.C, p.beta, p.stage_times)
         (c + sum(stages)).backward()
-        return x.grad, t.grad
+        return U
 
     if args.calibration_copy:
         calibration_copy(torch, p, rows, dev)
@@ -105,7 +105,7 @@ def main():
     _lib.check(lib.sttopt_profile_outer_apply(1), "sttopt_profile_outer_apply")
     t0 = time.perf_counter()
     for i in range(args.steps):
-        step()
+        U = step()
         iters.append(dict(compliance.LAST_SOLVE_INFO))
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
@@ -123,6 +123,15 @@ def main():
     roof = apply_roofline(torch, _lib, torch_fem, p, rows, dev, live)
     print(f"# apply: {roof['achieved']:.0f} GB/s ({100 * roof['frac']:.1f}% of HBM peak)",
           file=sys.stderr, flush=True)
+    # -- the whole step against its algorithmic bytes -----------------------------
+    sb = step_bytes(p, iters[-1], rows)
+    step_gbs = sb["total"] / (ms_per_step / 1e3) / 1e9
+    roof["step"] = {"algorithmic_bytes": sb, "ms_per_step": ms_per_step,
+                    "achieved": step_gbs, "frac": step_gbs / HBM_PEAK_GBS}
+    print(f"# step: {sb['total'] / 1e9:.2f} GB algorithmic, {step_gbs:.0f} GB/s "
+          f"({100 * step_gbs / HBM_PEAK_GBS:.1f}% of HBM peak)", file=sys.stderr, flush=True)
+    resid = true_residuals(torch, p, U, E)
+    print(f"# true fp64 residual per row: max {max(resid):.2e}", file=sys.stderr, flush=True)
 
     # -- the plain fp64 one-row-at-a-time GPU path -------------------------------
     seq = None
@@ -159,6 +168,8 @@ def main():
             "preconditioner": "fp32 V-cycle (damped Jacobi 2+2, omega 0.6), fp64 outer PCG",
             "pcg_iterations": iters[-1]["n_iter"],
             "row_iterations": iters[-1]["row_iter_list"],
+            "true_relative_residual_per_row": resid,
+            "levels": iters[-1]["levels"],
             "fp64_sequential_rows_per_s": None if seq is None else seq["rows_per_s"],
             "speedup_vs_fp64_sequential": None if seq is None else value / seq["rows_per_s"],
         },
@@ -166,6 +177,99 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def step_bytes(p, info, rows):
+    """Algorithmic HBM bytes of one bench step: the compulsory traffic of every kernel
+    (each operand read once, each result written once, at its stored precision;
+    per-row scalars and reduction partials left out), summed over the kernels the step
+    launches.  Returns {"build", "per_row_iteration", "row_iterations", "iterations",
+    "gradient", "total"}.
+
+    The fp32 V-cycle with fp64 outer PCG on 2 dofs per node; level l has n_l dofs,
+    m_l nodes and e_l elements.  Per PCG iteration of a row still iterating:
+      outer apply          8 (2 n0 + e0)                      p, rho -> Ap
+      pre pass + update    8 (3 n0) + 4 (4 n0 + e0)           r, Ap, z_old, 1/D, rho32 -> r', x, r32
+      level 1 (densities)  two-sweep pass 4 (3 n1) + 4 e0, residual 4 (3 n1) + 4 e0,
+                           two post sweeps 2 (4 (4 n1) + 4 e0)
+      levels 2..L-2        (node stencils, 80 B per node) the same with 80 m_l in place
+                           of 4 e0; levels above 16384 nodes take a first sweep
+                           4 (3 n_l) and a plain sweep instead of the two-sweep pass
+      transfers            restrict 4 (n_l + n_l+1), prolongate 4 (2 n_l + n_l+1)
+      coarsest             8 n_L^2 + 4 (2 n_L)                  dense inverse, b -> x
+      post pass            8 n0 + 4 (3 n0 + n1 + e0)          r, x, e, 1/D, rho32 -> z
+      direction update     8 (4 n0) + 4 n0                    x, p, z -> x, p
+    Per solve: the first preconditioner application (the cycle without the residual
+    update, 8 n0 less) and z -> p (12 n0), the cold start (24 n0); per step: the
+    prologue 8 (2 e0 + n0 + R (e0 + n0)), the hierarchy (fp32 densities 12 e0, the
+    Galerkin element matrices 8 * 64 e_l written for levels 2.., their node stencils
+    512 e_l + 80 m_l, diagonals 8 n_l, the coarsest operator 512 e_L + 8 n_L^2 and its
+    in-place inverse 16 n_L^2, per row), the compliance 8 R (n0 + e0) and the adjoint
+    8 (R n0 + 4 e0)."""
+    lv = info["levels"]
+    nodes = [(x + 1) * (y + 1) for x, y in lv]
+    n = [2 * m for m in nodes]
+    e = [x * y for x, y in lv]
+    L = len(lv) - 1
+    e0 = e[0]
+
+    def vcycle_below_0():
+        b = 0
+        for lvl in range(1, L):
+            op = 4 * e0 if lvl == 1 else 80 * nodes[lvl]
+            fused = lvl == 1 or nodes[lvl] <= 16384
+            b += (4 * 3 * n[lvl] + op) if fused else (4 * 3 * n[lvl] + 4 * 4 * n[lvl] + op)
+            b += 4 * 3 * n[lvl] + op                       # residual
+            b += 2 * (4 * 4 * n[lvl] + op)                 # post sweeps
+            b += 4 * (n[lvl] + n[lvl + 1])                 # restrict to lvl + 1
+            b += 4 * (2 * n[lvl] + n[lvl + 1])             # prolongate from lvl + 1
+        b += 4 * (n[0] + n[1])                             # restrict from level 0
+        b += 8 * n[L] ** 2 + 4 * 2 * n[L]                  # coarsest
+        return b
+
+    pre = 8 * 3 * n[0] + 4 * (4 * n[0] + e0)
+    post = 8 * n[0] + 4 * (3 * n[0] + n[1] + e0)
+    per_iter = 8 * (2 * n[0] + e0) + pre + vcycle_below_0() + post + 8 * 4 * n[0] + 4 * n[0]
+    first = (pre - 8 * n[0]) + vcycle_below_0() + post + 12 * n[0] + 24 * n[0]
+    build = 8 * (2 * e0 + n[0] + rows * (e0 + n[0]))          # prologue
+    per_row_build = 12 * e0 + 8 * n[0]
+    for lvl in range(1, L + 1):
+        per_row_build += 8 * n[lvl]
+        if lvl >= 2:
+            per_row_build += 8 * 64 * e[lvl]
+            if lvl < L:
+                per_row_build += 512 * e[lvl] + 80 * nodes[lvl]
+    per_row_build += 512 * e[L] + 8 * n[L] ** 2 + 16 * n[L] ** 2
+    build += rows * (per_row_build + first)
+    grad = 8 * rows * (n[0] + e0) + 8 * (rows * n[0] + 4 * e0)
+    row_iters = int(sum(info["row_iter_list"]))
+    total = build + row_iters * per_iter + grad
+    return {"build": build, "per_row_iteration": per_iter, "row_iterations": row_iters,
+            "iterations": int(info["n_iter"]), "gradient": grad, "total": total}
+
+
+def true_residuals(torch, p, U, E):
+    """Per-row relative residual |P (F_row - K_row U_row)| / |P F_row| in fp64, recomputed
+    from the final U with the fp64 operator (the densities and loads rebuilt on the
+    device as the fused prologue builds them)."""
+    from sttopt_amd import compliance, torch_fem
+
+    rows = U.shape[0]
+    out = []
+    for k in range(rows):
+        if k == 0:
+            dens = torch_fem.simp_density(p.xPhys, *E).flatten()
+            F = p.F
+        else:
+            m = compliance.time_mask(p.tPhys, p.stage_times[k - 1], p.beta)
+            xt = (p.xPhys * m).flatten()
+            dens = torch_fem.simp_density(xt, *E)
+            F = torch.zeros(p.ndof, dtype=torch.float64, device=U.device)
+            F[1::2] = -(torch_sparse_gravity(torch, p) @ xt)
+        Fp = torch_fem.project(F, p.mask)
+        r = Fp - torch_fem.operator(U[k], dens, p.edofMat, p.KE, p.ndof, p.mask)
+        out.append(float(r.norm() / Fp.norm()))
+    return out
 
 
 PMC_SUMMARY = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -442,11 +546,23 @@ def torch_sparse_gravity(torch, p):
     return _C_CACHE[key]
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(p, E):
     """The oracle's torch-CPU fp64 restatement of the reference algorithm (index_add_
     matvec, einsum Galerkin, dense Cholesky, retiring PCG, alpha*U adjoint) on a
     bounded sample: the whole-structure row and one stage row of the same mesh,
-    forward solve + adjoint + element gradient, the median of 5 runs."""
+    forward solve + adjoint + element gradient, the median of 5 runs after 2 warm-ups."""
     import numpy as np
     import torch
 
@@ -466,26 +582,29 @@ def cpu_baseline(p, E):
     Fg[1::2] = -(fem_direct.gravity_matrix(nelx, nely) @ xt)
     dens = torch.as_tensor(np.stack([fem_direct.simp(x.ravel(), *E), fem_direct.simp(xt, *E)]))
     F = torch.stack([p.F.cpu(), torch.as_tensor(Fg)])
-    sample_rows, repeats = 2, 5
+    sample_rows, repeats, warmups = 2, 5, 2
     times = []
-    for _ in range(repeats):
+    for i in range(warmups + repeats):
         t0 = time.perf_counter()
         U, n, levels = mgcg_port.mg_solve(F, dens, edof, KE, mask, nelx, nely, rtol=1e-8)
         Ue = U[..., edof]
         ce = torch.einsum("rei,ij,rej->re", Ue, KE, Ue)
         g = 2 * mgcg_port.matvec(U, dens, edof, KE, p.ndof)
         mgcg_port.adjoint(levels, U, F, g, edof, KE, rtol=1e-8)
-        times.append(time.perf_counter() - t0)
+        if i >= warmups:
+            times.append(time.perf_counter() - t0)
         assert ce.shape[0] == sample_rows
     sec = float(np.median(times))
     return {
         "value": sample_rows / sec,
         "unit": "solves+grads/s",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": f"{sample_rows} of the {1 + len(p.stage_times)} rows (whole structure + one "
                   f"stage) at {nelx}x{nely}: hierarchy + MG-PCG to 1e-8 ({n} it) + adjoint + "
-                  f"gradient; median of {repeats} runs ({sum(times):.1f} s of CPU work in all)",
+                  f"gradient; median of {repeats} runs after {warmups} warm-ups "
+                  f"({sum(times):.1f} s timed)",
         "seconds": sec,
     }
 

@@ -28,7 +28,8 @@ from torch.utils.weak import WeakTensorKeyDictionary
 
 from . import _grid, _lib, torch_fem, torch_mg, torch_solve
 
-#: Iteration counts of the last fused batched solve: {"n_iter", "row_iters", "rows"}.
+#: The last fused batched solve: {"n_iter", "row_iters", "row_iter_list", "rows",
+#: "levels" (the (nelx, nely) of each multigrid level, finest first)}.
 LAST_SOLVE_INFO: dict = {}
 
 #: Coarsening stop of the fused path's hierarchy (None: torch_mg.MAX_COARSE_ELEMENTS).
@@ -237,7 +238,8 @@ class _SpaceTimeCompliance(torch.autograd.Function):
         )
         LAST_SOLVE_INFO.clear()
         LAST_SOLVE_INFO.update(n_iter=n_iter, row_iters=info["row_iters"],
-                               row_iter_list=info["row_iter_list"], rows=1 + ns)
+                               row_iter_list=info["row_iter_list"], rows=1 + ns,
+                               levels=[(lv.nelx, lv.nely) for lv in levels])
         comp = torch.empty(1 + ns, dtype=torch.float64, device=dev)
         scratch = torch.empty(lib.sttopt_stage_scratch_count(nelx, nely, ns),
                               dtype=torch.float64, device=dev)
