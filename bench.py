@@ -43,6 +43,9 @@ def parse():
     ap.add_argument("--nstage", type=int, default=NSTAGE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the PCG's second half-iteration kernel by kernel instead "
+                         "of replaying its captured graph")
     ap.add_argument("--kernels-only", action="store_true",
                     help="run only a plain copy of known size and the fp64 / fp32 K(rho)u "
                          "apply (for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE), then exit")
@@ -72,7 +75,7 @@ def main():
     from sttopt_amd import _lib, compliance, torch_fem, workloads
 
     dev = torch.device("cuda:0")
-    _lib.load()
+    _lib.load().sttopt_mgpcg_graphs(0 if args.no_graphs else 1)
     p = workloads.space_time(args.nelx, args.nely, args.nstage, beta=BETA, seed=0,
                              time_field="print", sigma=16.0, device=dev)
     rows = 1 + args.nstage

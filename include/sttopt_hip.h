@@ -291,6 +291,14 @@ int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double*
                  int32_t check_every, int32_t cold_start, double* work, int32_t* iters,
                  double* resid, int32_t* n_iter, void* stream);
 
+/* Whether sttopt_mgpcg replays the second half of each iteration (the V-cycle's coarse
+ * visit and post-smoothing, the beta finalizer and the direction update: the same
+ * launches every iteration) from a HIP graph captured once per solve (enable 1, the
+ * default) or launches it kernel by kernel (0); the results are identical.  enable < 0
+ * only queries.  Returns the previous setting; process-wide.  No counterpart in the
+ * reference (whose iteration is a host loop of torch ops, torch_fem.py:313-331). */
+int sttopt_mgpcg_graphs(int32_t enable);
+
 /* Work vectors sttopt_mgpcg needs: returns the count of doubles ([rows][ndof0] x 5). */
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg);
 

@@ -17,7 +17,7 @@ LIB_PATH = PKG_DIR / LIB_NAME
 ARCH = os.environ.get("STTOPT_OFFLOAD_ARCH", "gfx950")
 
 SOURCES = ["stt_apply.hip", "stt_mg.hip", "stt_pcg.hip", "stt_grad.hip", "stt_dense.hip",
-           "stt_smooth.hip", "stt_heat.hip",
+           "stt_smooth.hip", "stt_coarse_pair.hip", "stt_heat.hip",
            "stt_filter.hip", "stt_mma.hip"]
 
 HIPCC_FLAGS = [

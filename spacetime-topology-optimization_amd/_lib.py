@@ -225,6 +225,7 @@ _SIGNATURES = {
          c_double, _P, _P, _P, _P, _P, _P, _P, _P, _P],
     ),
     "sttopt_stage_scratch_count": (c_int64, [c_int32, c_int32, c_int32]),
+    "sttopt_mgpcg_graphs": (c_int32, [c_int32]),
     "sttopt_stage_epilogue_noalloc": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,

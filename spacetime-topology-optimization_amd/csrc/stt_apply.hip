@@ -68,33 +68,6 @@ __host__ __device__ constexpr int upper_index(int r, int c, int K) {
   return r <= c ? r * K - r * (r - 1) / 2 + (c - r) : c * K - c * (c - 1) / 2 + (r - c);
 }
 
-// N consecutive values starting 16-byte aligned (fp32: N a multiple of 4 or N = 1;
-// fp64: N even or N = 1), with vector loads.
-template <typename T, int N>
-__device__ __forceinline__ void load_record(const T* p, T (&out)[N]) {
-  constexpr int W = 16 / (int)sizeof(T);  // values per 16-byte load
-  if constexpr (N % W == 0) {
-    using V = HIP_vector_type<T, W>;
-    const V* pv = reinterpret_cast<const V*>(p);
-#pragma unroll
-    for (int q = 0; q < N / W; ++q) {
-      const V v = pv[q];
-      if constexpr (W == 4) {
-        out[4 * q] = v.x;
-        out[4 * q + 1] = v.y;
-        out[4 * q + 2] = v.z;
-        out[4 * q + 3] = v.w;
-      } else {
-        out[2 * q] = v.x;
-        out[2 * q + 1] = v.y;
-      }
-    }
-  } else {
-#pragma unroll
-    for (int q = 0; q < N; ++q) out[q] = p[q];
-  }
-}
-
 // Coarse-level stencil (SRC 1 or 2): one node per thread on 64 x 4 node tiles.
 // PKX, PKY > 0 fix the parameterized block at compile time (fully unrolled).
 template <typename T, typename TB, typename TY, int D, int SRC, int PKX = 0, int PKY = 0>

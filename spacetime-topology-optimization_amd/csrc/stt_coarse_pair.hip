@@ -1,0 +1,536 @@
+// Coarse levels of the fp32 V-cycle in two passes per level (torch_mg.py:440-471 with
+// n_smooth = 2, gamma = 1, 2 x 2 coarsening), on the assembled node stencil
+// (STTOPT_KEFF_NODE):
+//
+//   pre:  x1 = w D^-1 b,  x2 = x1 + w D^-1 (b - A x1),  r = b - A x2,  b_c = P_c R r
+//   post: xp = x2 + P_f (P e),  x3 = xp + w D^-1 (b - A xp),  x4 = x3 + w D^-1 (b - A x3)
+//
+// where the separate path takes six to seven launches per level (first sweeps,
+// residual, restriction, prolongation, two sweeps).  A workgroup stages what its tile
+// needs in LDS and evaluates the stencil on shrinking regions: the pre pass on the fine
+// nodes that feed a 32 x 4 tile of coarse nodes (65 x 9, with x2 on a one-node ring
+// around them and x1 on two), the post pass on a 64 x 8 tile of fine nodes (x3 on a
+// one-node ring, xp on two).  The intermediate fields never reach HBM; the pre pass
+// writes x2 for its own fine nodes and the restricted residual, the post pass writes x4.
+// Each thread loads the stencil rows of its nodes once, before anything else, and
+// applies them in both phases that need them.  Every value is formed with the operands
+// and in the order of the separate kernels
+// (stencil_kernel's node-stencil sum, jacobi_first's first sweep, restrict22_kernel,
+// prolong_add_kernel), so the results are the same.
+#include "stt_kernels.h"
+
+namespace stt {
+
+namespace {
+
+constexpr int kCW = 32, kCH = 4;           // pre: coarse nodes per tile
+constexpr int kPW = 2 * kCW + 5;           // pre: staged fine columns (x1 region)
+constexpr int kPH = 2 * kCH + 5;           // pre: staged fine rows
+constexpr int kTW = 64, kTH = 8;           // post: fine output nodes per tile
+constexpr int kQW = kTW + 4, kQH = kTH + 4;  // post: staged fine region (xp)
+constexpr int kEW = kTW / 2 + 3, kEH = kTH / 2 + 3;  // post: staged coarse region (e)
+// Threads per workgroup: every phase takes at most two nodes per thread, whose loads
+// are issued together (the phases are latency bound: each reads the stencil records of
+// its nodes from L2).
+constexpr int kPairThreads = 512;
+template <int N>
+constexpr int per_thread() { return (N + kPairThreads - 1) / kPairThreads; }
+
+// What one node's stencil row needs from HBM: its own record (own block's upper
+// triangle, forward blocks E, NW, N, NE), the backward neighbours' forward blocks toward
+// it, which of the eight neighbours exist, and the node's free-dof bits.  Loaded once
+// per node and kept in registers across the phases that apply the same row.
+template <int D>
+struct NodeRow {
+  static constexpr int NS = node_stencil_stride(D), B = D * D;
+  float own[NS];
+  float back[4][B];
+  bool fwd_in[4], back_in[4];
+  uint32_t free_bits;
+};
+
+constexpr int kFdr[4] = {0, 1, 1, 1}, kFdc[4] = {1, -1, 0, 1};
+
+template <int D>
+__device__ __forceinline__ void load_row(const float* S, const uint32_t* mask, int nnx,
+                                         int nny, int r, int c, NodeRow<D>& w) {
+  constexpr int NS = NodeRow<D>::NS, ND = 4, B = NodeRow<D>::B;
+  const long long node = (long long)r * nnx + c;
+  load_record<float, NS>(S + node * NS, w.own);
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    const int cc = c + kFdc[f];
+    w.fwd_in[f] = r + kFdr[f] < nny && cc >= 0 && cc < nnx;
+    const int rr = r - kFdr[f], cb = c - kFdc[f];
+    w.back_in[f] = rr >= 0 && cb >= 0 && cb < nnx;
+    if (w.back_in[f]) {
+      load_record<float, B>(S + ((long long)rr * nnx + cb) * NS + ND + f * B, w.back[f]);
+    } else {
+#pragma unroll
+      for (int q = 0; q < B; ++q) w.back[f][q] = 0.0f;
+    }
+  }
+  const uint32_t word = mask[(node * D) >> 5];
+  w.free_bits = (word >> ((node * D) & 31)) & ((1u << D) - 1u);
+}
+
+// acc = (A u)_node: the sum of stencil_kernel<..., kSrcNode> in the same order.  U: the
+// 3 x 3 neighbourhood, zero outside the grid.
+template <int D>
+__device__ __forceinline__ void apply_row(const NodeRow<D>& w, const float (&U)[3][3][D],
+                                          float (&acc)[D]) {
+  constexpr int ND = 4;
+#pragma unroll
+  for (int i = 0; i < D; ++i) acc[i] = 0.0f;
+  int idx = 0;
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+#pragma unroll
+    for (int k = i; k < D; ++k) {
+      const float v = w.own[idx++];
+      acc[i] += v * U[1][1][k];
+      if (k != i) acc[k] += v * U[1][1][i];
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (w.fwd_in[f]) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          acc[i] += w.own[ND + (f * D + i) * D + k] * U[1 + kFdr[f]][1 + kFdc[f]][k];
+      }
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    if (w.back_in[f]) {
+#pragma unroll
+      for (int i = 0; i < D; ++i) {
+#pragma unroll
+        for (int k = 0; k < D; ++k)
+          acc[i] += w.back[f][k * D + i] * U[1 - kFdr[f]][1 - kFdc[f]][k];
+      }
+    }
+  }
+}
+
+// The 3 x 3 neighbourhood of local node (lr, lc) of a staged field of width W.
+template <typename T, int D, int W>
+__device__ __forceinline__ void gather3(const T (*f)[D], int lr, int lc, T (&U)[3][3][D]) {
+#pragma unroll
+  for (int dr = 0; dr < 3; ++dr) {
+#pragma unroll
+    for (int dc = 0; dc < 3; ++dc) {
+#pragma unroll
+      for (int k = 0; k < D; ++k) U[dr][dc][k] = f[(lr + dr - 1) * W + lc + dc - 1][k];
+    }
+  }
+}
+
+__device__ __forceinline__ bool dof_free(const uint32_t* mask, long long dof) {
+  return (mask[dof >> 5] >> (dof & 31)) & 1u;
+}
+
+struct CoarsePairArgs {
+  int nnx, nny;          // fine grid nodes of this level
+  int cnnx, cnny;        // nodes of the next coarser level (2 x 2)
+  int tiles_x, tiles, total;
+  const float* S;        // node-stencil records of this level
+  long long sS;
+  const uint32_t* mask;  // this level's free dofs
+  const uint32_t* cmask; // the coarser level's free dofs (pre)
+  const float* b;        // this level's right-hand side
+  long long sb;
+  const float* invd;
+  long long sd;
+  const float* x2;       // post: the pre-smoothed x
+  long long sx2;
+  const float* ec;       // post: the coarse correction
+  long long sec;
+  float* out;            // pre: x2; post: x4
+  long long so;
+  float* bc;             // pre: the coarser level's right-hand side
+  long long sbc;
+  float omega;
+  const int* active;
+};
+
+template <int D>
+__global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePairArgs a) {
+  __shared__ float sbv[kPH * kPW][D];
+  __shared__ float sdv[kPH * kPW][D];
+  __shared__ float sx1[kPH * kPW][D];
+  __shared__ float sx2[kPH * kPW][D];
+  __shared__ float srr[kPH * kPW][D];
+  const int t = xcd_remap(blockIdx.x, a.total);
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
+  if (a.active != nullptr && a.active[row] == 0) return;  // whole block
+  const int ty = tt / a.tiles_x, tx = tt - ty * a.tiles_x;
+  const int I0 = ty * kCH, J0 = tx * kCW;     // first coarse node of the tile
+  // fine node at local (0, 0): the residual is needed on fine rows 2 I0 - 1 ..
+  // 2 (I0 + kCH) - 1 (local 2 .. kPH - 3), x2 one node beyond, x1 two
+  const int fr0 = 2 * I0 - 3, fc0 = 2 * J0 - 3;
+  const float* b = a.b + row * a.sb;
+  const float* dinv = a.invd + row * a.sd;
+  const float* S = a.S + row * a.sS;
+  // The stencil rows of this thread's nodes one node inside the staged region (phase
+  // x2; the residual's nodes are a subset, taken by the same threads), loaded before
+  // anything else so that every HBM read of the workgroup is in flight at once.
+  constexpr int NJ = per_thread<kPH * kPW>();
+  NodeRow<D> w[NJ];
+  bool inner[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    const int lr = i / kPW, lc = i % kPW;
+    const int fr = fr0 + lr, fc = fc0 + lc;
+    inner[j] = i < kPH * kPW && lr >= 1 && lr <= kPH - 2 && lc >= 1 && lc <= kPW - 2 &&
+               fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx;
+    if (inner[j]) load_row<D>(S, a.mask, a.nnx, a.nny, fr, fc, w[j]);
+  }
+  // x1 = w D^-1 b on the whole staged region (jacobi_first's expression; 0 outside)
+#pragma unroll
+  for (int j = 0; j < per_thread<kPH * kPW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kPH * kPW) break;
+    const int fr = fr0 + i / kPW, fc = fc0 + i % kPW;
+    Vec<float, D> bv, dv;
+#pragma unroll
+    for (int k = 0; k < D; ++k) bv.v[k] = dv.v[k] = 0.0f;
+    if (fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx) {
+      const long long node = (long long)fr * a.nnx + fc;
+      bv = load_node<float, D>(b, node);
+      dv = load_node<float, D>(dinv, node);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      sbv[i][k] = bv.v[k];
+      sdv[i][k] = dv.v[k];
+      sx1[i][k] = a.omega * dv.v[k] * bv.v[k];
+    }
+  }
+  __syncthreads();
+  // x2 = x1 + w D^-1 (b - A x1) one node inside the staged region; x2 of the tile's own
+  // fine nodes (those whose coarse node, by integer halving, is in the tile) to HBM
+#pragma unroll
+  for (int j = 0; j < per_thread<kPH * kPW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kPH * kPW) break;
+    const int lr = i / kPW, lc = i % kPW;
+    if (lr < 1 || lr > kPH - 2 || lc < 1 || lc > kPW - 2) continue;
+    const int fr = fr0 + lr, fc = fc0 + lc;
+    Vec<float, D> xo;
+#pragma unroll
+    for (int k = 0; k < D; ++k) xo.v[k] = 0.0f;
+    if (inner[j]) {
+      float U[3][3][D], acc[D];
+      gather3<float, D, kPW>(sx1, lr, lc, U);
+      apply_row<D>(w[j], U, acc);
+      const long long node = (long long)fr * a.nnx + fc;
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        xo.v[k] = ((w[j].free_bits >> k) & 1u)
+                      ? U[1][1][k] + a.omega * sdv[i][k] * (sbv[i][k] - acc[k])
+                      : 0.0f;
+      if (fr >= 2 * I0 && fr < 2 * I0 + 2 * kCH && fc >= 2 * J0 && fc < 2 * J0 + 2 * kCW)
+        store_node<float, D>(a.out + row * a.so, node, xo);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) sx2[i][k] = xo.v[k];
+  }
+  __syncthreads();
+  // r = b - A x2 on the fine nodes that feed the tile's coarse nodes
+#pragma unroll
+  for (int j = 0; j < per_thread<kPH * kPW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kPH * kPW) break;
+    const int lr = i / kPW, lc = i % kPW;
+    if (lr < 2 || lr > kPH - 3 || lc < 2 || lc > kPW - 3) continue;
+    const int fr = fr0 + lr, fc = fc0 + lc;
+    float ro[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) ro[k] = 0.0f;
+    if (inner[j]) {
+      float U[3][3][D], acc[D];
+      gather3<float, D, kPW>(sx2, lr, lc, U);
+      apply_row<D>(w[j], U, acc);
+#pragma unroll
+      for (int k = 0; k < D; ++k) ro[k] = ((w[j].free_bits >> k) & 1u) ? sbv[i][k] - acc[k] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) srr[i][k] = ro[k];
+  }
+  __syncthreads();
+  // full weighting to the tile's coarse nodes, in restrict22_kernel's order
+  for (int i = threadIdx.x; i < kCW * kCH; i += kPairThreads) {
+    const int I = I0 + i / kCW, J = J0 + i % kCW;
+    if (I >= a.cnny || J >= a.cnnx) continue;
+    float acc[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) acc[k] = 0.0f;
+    const float wh = 1.0f / 2.0f, w1 = 2.0f / 2.0f;
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      const int R = 2 * I + dy;
+      if (R < 0 || R >= a.nny) continue;
+      const float wy = float(2 - (dy < 0 ? -dy : dy)) / 2.0f;
+      const int lr = R - fr0, lc = 2 * J - fc0;
+      float racc[D];
+#pragma unroll
+      for (int k = 0; k < D; ++k) racc[k] = 0.0f;
+      if (J > 0) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) racc[k] += wh * srr[lr * kPW + lc - 1][k];
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) racc[k] += w1 * srr[lr * kPW + lc][k];
+      if (2 * J + 1 < a.nnx) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) racc[k] += wh * srr[lr * kPW + lc + 1][k];
+      }
+#pragma unroll
+      for (int k = 0; k < D; ++k) acc[k] += wy * racc[k];
+    }
+    const long long cnode = (long long)I * a.cnnx + J;
+    Vec<float, D> o;
+#pragma unroll
+    for (int k = 0; k < D; ++k) o.v[k] = dof_free(a.cmask, cnode * D + k) ? acc[k] : 0.0f;
+    store_node<float, D>(a.bc + row * a.sbc, cnode, o);
+  }
+}
+
+template <int D>
+__global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePairArgs a) {
+  __shared__ float sbv[kQH * kQW][D];
+  __shared__ float sdv[kQH * kQW][D];
+  __shared__ float sxp[kQH * kQW][D];
+  __shared__ float sx3[kQH * kQW][D];
+  __shared__ float sec[kEH * kEW][D];
+  const int t = xcd_remap(blockIdx.x, a.total);
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
+  if (a.active != nullptr && a.active[row] == 0) return;  // whole block
+  const int ty = tt / a.tiles_x, tx = tt - ty * a.tiles_x;
+  const int r0 = ty * kTH, c0 = tx * kTW;          // first output node of the tile
+  const int fr0 = r0 - 2, fc0 = c0 - 2;            // fine node at local (0, 0)
+  const int er0 = r0 / 2 - 1, ec0 = c0 / 2 - 1;    // coarse node at local (0, 0)
+  const float* b = a.b + row * a.sb;
+  const float* dinv = a.invd + row * a.sd;
+  const float* x2 = a.x2 + row * a.sx2;
+  const float* e = a.ec + row * a.sec;
+  const float* S = a.S + row * a.sS;
+  // the stencil rows of this thread's nodes one node around the tile (phase x3; the
+  // tile's own nodes, phase x4, are a subset taken by the same threads)
+  constexpr int NJ = per_thread<kQH * kQW>();
+  NodeRow<D> w[NJ];
+  bool inner[NJ];
+#pragma unroll
+  for (int j = 0; j < NJ; ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    const int lr = i / kQW, lc = i % kQW;
+    const int fr = fr0 + lr, fc = fc0 + lc;
+    inner[j] = i < kQH * kQW && lr >= 1 && lr <= kQH - 2 && lc >= 1 && lc <= kQW - 2 &&
+               fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx;
+    if (inner[j]) load_row<D>(S, a.mask, a.nnx, a.nny, fr, fc, w[j]);
+  }
+  for (int i = threadIdx.x; i < kEH * kEW; i += kPairThreads) {
+    const int cr = er0 + i / kEW, cc = ec0 + i % kEW;
+    Vec<float, D> v;
+#pragma unroll
+    for (int k = 0; k < D; ++k) v.v[k] = 0.0f;
+    if (cr >= 0 && cr < a.cnny && cc >= 0 && cc < a.cnnx)
+      v = load_node<float, D>(e, (long long)cr * a.cnnx + cc);
+#pragma unroll
+    for (int k = 0; k < D; ++k) sec[i][k] = v.v[k];
+  }
+  Vec<float, D> xv[per_thread<kQH * kQW>()];
+#pragma unroll
+  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kQH * kQW) break;
+    const int fr = fr0 + i / kQW, fc = fc0 + i % kQW;
+    Vec<float, D> bv, dv;
+#pragma unroll
+    for (int k = 0; k < D; ++k) bv.v[k] = dv.v[k] = xv[j].v[k] = 0.0f;
+    if (fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx) {
+      const long long node = (long long)fr * a.nnx + fc;
+      bv = load_node<float, D>(b, node);
+      dv = load_node<float, D>(dinv, node);
+      xv[j] = load_node<float, D>(x2, node);
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+      sbv[i][k] = bv.v[k];
+      sdv[i][k] = dv.v[k];
+    }
+  }
+  __syncthreads();
+  // xp = x2 + P e at the free dofs (prolong_add_kernel's expression)
+#pragma unroll
+  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kQH * kQW) break;
+    const int fr = fr0 + i / kQW, fc = fc0 + i % kQW;
+    Vec<float, D> v = xv[j];
+    if (fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx) {
+      const long long node = (long long)fr * a.nnx + fc;
+      const uint32_t word = a.mask[(node * D) >> 5];
+      const uint32_t bits = (word >> ((node * D) & 31)) & ((1u << D) - 1u);
+      if (bits != 0) {
+        const int I0 = fr / 2, fy = fr - 2 * I0;
+        const int J0 = fc / 2, fx = fc - 2 * J0;
+        const float wy1 = float(fy) / 2.0f, wy0 = 1.0f - wy1;
+        const float wx1 = float(fx) / 2.0f, wx0 = 1.0f - wx1;
+        const int l00 = (I0 - er0) * kEW + (J0 - ec0);
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+          const float a00 = sec[l00][k];
+          const float a01 = fx > 0 ? sec[l00 + 1][k] : 0.0f;
+          const float a10 = fy > 0 ? sec[l00 + kEW][k] : 0.0f;
+          const float a11 = (fx > 0 && fy > 0) ? sec[l00 + kEW + 1][k] : 0.0f;
+          const float top = wx0 * a00 + wx1 * a01;
+          const float bot = wx0 * a10 + wx1 * a11;
+          if ((bits >> k) & 1u) v.v[k] += wy0 * top + wy1 * bot;
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) sxp[i][k] = v.v[k];
+  }
+  __syncthreads();
+  // x3 one node around the tile, then x4 on it
+#pragma unroll
+  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kQH * kQW) break;
+    const int lr = i / kQW, lc = i % kQW;
+    if (lr < 1 || lr > kQH - 2 || lc < 1 || lc > kQW - 2) continue;
+    const int fr = fr0 + lr, fc = fc0 + lc;
+    float xo[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) xo[k] = 0.0f;
+    if (inner[j]) {
+      float U[3][3][D], acc[D];
+      gather3<float, D, kQW>(sxp, lr, lc, U);
+      apply_row<D>(w[j], U, acc);
+#pragma unroll
+      for (int k = 0; k < D; ++k)
+        xo[k] = ((w[j].free_bits >> k) & 1u)
+                    ? U[1][1][k] + a.omega * sdv[i][k] * (sbv[i][k] - acc[k])
+                    : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) sx3[i][k] = xo[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
+    const int i = threadIdx.x + j * kPairThreads;
+    if (i >= kQH * kQW) break;
+    const int lr = i / kQW, lc = i % kQW;
+    if (lr < 2 || lr > kQH - 3 || lc < 2 || lc > kQW - 3) continue;
+    const int fr = fr0 + lr, fc = fc0 + lc;
+    if (!inner[j]) continue;  // outside the grid
+    float U[3][3][D], acc[D];
+    gather3<float, D, kQW>(sx3, lr, lc, U);
+    apply_row<D>(w[j], U, acc);
+    const long long node = (long long)fr * a.nnx + fc;
+    Vec<float, D> xo;
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+      xo.v[k] = ((w[j].free_bits >> k) & 1u)
+                    ? U[1][1][k] + a.omega * sdv[i][k] * (sbv[i][k] - acc[k])
+                    : 0.0f;
+    store_node<float, D>(a.out + row * a.so, node, xo);
+  }
+}
+
+int check_pair_level(const OpDesc& op, const char* what) {
+  if (op.keff == nullptr || !op.keff_node || op.mask == nullptr || (op.d != 1 && op.d != 2) ||
+      op.nelx % 2 != 0 || op.nely % 2 != 0) {
+    set_error(std::string(what) +
+              ": needs a node-stencil level with a free-dof mask, coarsened 2 x 2");
+    return STTOPT_EINVAL;
+  }
+  return STTOPT_OK;
+}
+
+}  // namespace
+
+int coarse_pair_pre(const OpDesc& op, int rows, const float* b, long long sb,
+                    const float* invd, long long sd, float* x2, long long sx2,
+                    const uint32_t* cmask, float* bc, long long sbc, double omega,
+                    const int* active, hipStream_t s) {
+  if (int e = check_pair_level(op, "coarse_pair_pre")) return e;
+  if (rows <= 0) return STTOPT_OK;
+  CoarsePairArgs a{};
+  a.nnx = op.nnx();
+  a.nny = op.nny();
+  a.cnnx = op.nelx / 2 + 1;
+  a.cnny = op.nely / 2 + 1;
+  a.tiles_x = ceil_div(a.cnnx, kCW);
+  a.tiles = a.tiles_x * ceil_div(a.cnny, kCH);
+  a.total = a.tiles * rows;
+  a.S = static_cast<const float*>(op.keff);
+  a.sS = op.sk;
+  a.mask = op.mask;
+  a.cmask = cmask;
+  a.b = b;
+  a.sb = sb;
+  a.invd = invd;
+  a.sd = sd;
+  a.out = x2;
+  a.so = sx2;
+  a.bc = bc;
+  a.sbc = sbc;
+  a.omega = static_cast<float>(omega);
+  a.active = active;
+  if (op.d == 2)
+    hipLaunchKernelGGL(coarse_pair_pre_kernel<2>, dim3(a.total), dim3(kPairThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(coarse_pair_pre_kernel<1>, dim3(a.total), dim3(kPairThreads), 0, s, a);
+  STT_CHECK_LAUNCH("coarse_pair_pre_kernel");
+  return STTOPT_OK;
+}
+
+int coarse_pair_post(const OpDesc& op, int rows, const float* x2, long long sx2,
+                     const float* ec, long long sec, const float* b, long long sb,
+                     const float* invd, long long sd, float* x4, long long sx4, double omega,
+                     const int* active, hipStream_t s) {
+  if (int e = check_pair_level(op, "coarse_pair_post")) return e;
+  if (rows <= 0) return STTOPT_OK;
+  CoarsePairArgs a{};
+  a.nnx = op.nnx();
+  a.nny = op.nny();
+  a.cnnx = op.nelx / 2 + 1;
+  a.cnny = op.nely / 2 + 1;
+  a.tiles_x = ceil_div(a.nnx, kTW);
+  a.tiles = a.tiles_x * ceil_div(a.nny, kTH);
+  a.total = a.tiles * rows;
+  a.S = static_cast<const float*>(op.keff);
+  a.sS = op.sk;
+  a.mask = op.mask;
+  a.b = b;
+  a.sb = sb;
+  a.invd = invd;
+  a.sd = sd;
+  a.x2 = x2;
+  a.sx2 = sx2;
+  a.ec = ec;
+  a.sec = sec;
+  a.out = x4;
+  a.so = sx4;
+  a.omega = static_cast<float>(omega);
+  a.active = active;
+  if (op.d == 2)
+    hipLaunchKernelGGL(coarse_pair_post_kernel<2>, dim3(a.total), dim3(kPairThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(coarse_pair_post_kernel<1>, dim3(a.total), dim3(kPairThreads), 0, s, a);
+  STT_CHECK_LAUNCH("coarse_pair_post_kernel");
+  return STTOPT_OK;
+}
+
+}  // namespace stt

@@ -10,6 +10,33 @@ namespace stt {
 // start 16-byte aligned in fp32).
 __host__ __device__ constexpr int node_stencil_stride(int d) { return 4 + 4 * d * d; }
 
+// N consecutive values starting 16-byte aligned (fp32: N a multiple of 4 or N = 1;
+// fp64: N even or N = 1), with vector loads.
+template <typename T, int N>
+__device__ __forceinline__ void load_record(const T* p, T (&out)[N]) {
+  constexpr int W = 16 / (int)sizeof(T);  // values per 16-byte load
+  if constexpr (N % W == 0) {
+    using V = HIP_vector_type<T, W>;
+    const V* pv = reinterpret_cast<const V*>(p);
+#pragma unroll
+    for (int q = 0; q < N / W; ++q) {
+      const V v = pv[q];
+      if constexpr (W == 4) {
+        out[4 * q] = v.x;
+        out[4 * q + 1] = v.y;
+        out[4 * q + 2] = v.z;
+        out[4 * q + 3] = v.w;
+      } else {
+        out[2 * q] = v.x;
+        out[2 * q + 1] = v.y;
+      }
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < N; ++q) out[q] = p[q];
+  }
+}
+
 // kSmoothFirst (level 0 only): two damped-Jacobi sweeps from x = 0 in one pass, the
 // first one (x0 = w D^-1 b) formed while staging the tile; also copies b into bcopy in
 // the stencil precision when given.
@@ -121,6 +148,21 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
                      const float* ec, long long sec, int cnnx, const double* b, long long sb,
                      const float* invd, long long sd, ZT* z, long long sz, double omega,
                      const int* active, double* partial, hipStream_t s);
+
+// Coarse levels of the fp32 V-cycle (node-stencil operator, 2 x 2 coarsening to the
+// next level, two sweeps) in two passes.  pre: x2 = two sweeps from zero on b (written
+// to x2) and the restricted residual of x2 on the coarser level (written to bc, masked
+// by cmask).  post: x4 = two sweeps from x2 + P ec (written to x4, which must not alias
+// x2).  Same results as the separate first sweeps, residual, restriction, prolongation
+// and sweeps.
+int coarse_pair_pre(const OpDesc& op, int rows, const float* b, long long sb,
+                    const float* invd, long long sd, float* x2, long long sx2,
+                    const uint32_t* cmask, float* bc, long long sbc, double omega,
+                    const int* active, hipStream_t s);
+int coarse_pair_post(const OpDesc& op, int rows, const float* x2, long long sx2,
+                     const float* ec, long long sec, const float* b, long long sb,
+                     const float* invd, long long sd, float* x4, long long sx4, double omega,
+                     const int* active, hipStream_t s);
 
 // Copy one int from device memory on stream s and wait for it (pinned readback word).
 int read_device_int(const int* dev, hipStream_t s, int* out);

@@ -132,15 +132,17 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
   }
 }
 
-// For the rows that took iteration k (iters == k): x += alpha p, then, for those still
-// active and when update_p, p = z + beta p.
+// For the rows that took iteration k (iters == k; k = *kdev when kdev is given): x +=
+// alpha p, then, for those still active and when update_p, p = z + beta p.
 template <int D, typename ZT>
 __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
     TileGeom g, long long n, const double* __restrict__ alpha, const double* __restrict__ beta,
     double* __restrict__ x, long long xs, double* __restrict__ p, const ZT* __restrict__ z,
-    const int* __restrict__ active, const int* __restrict__ iters, int k, int update_p) {
+    const int* __restrict__ active, const int* __restrict__ iters, int k, int update_p,
+    const int* __restrict__ kdev) {
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
+  if (kdev != nullptr) k = *kdev;  // the iteration fin_alpha advanced it to
   if (iters[row] != k || c >= g.nnx) return;
   const bool dir = update_p && active[row] != 0;
   const double a = alpha[row];
@@ -296,11 +298,15 @@ __global__ __launch_bounds__(kBlock) void fin_rz_kernel(int tiles, const double*
   if (threadIdx.x == 0) rz[row] = s;
 }
 
+// kdev: the device's copy of the PCG iteration number, advanced here once per
+// iteration (read by the direction update, which may run from a captured graph).
 __global__ __launch_bounds__(kBlock) void fin_alpha_kernel(int tiles, const double* ppap,
                                                            const int* active,
-                                                           const double* rz, double* alpha) {
+                                                           const double* rz, double* alpha,
+                                                           int* kdev) {
   __shared__ double red[kBlock / kWave];
   const int row = blockIdx.x;
+  if (row == 0 && threadIdx.x == 0) kdev[0] += 1;
   if (active[row] == 0) return;
   const double pap = row_sum(ppap, tiles, row, red);
   if (threadIdx.x == 0) alpha[row] = safe_div(rz[row], pap);
@@ -443,6 +449,25 @@ struct VCycle {
     T* x = buf(L.vx);
     T* alt = buf(L.vx2);
     const int ns = mg->n_smooth;
+    if constexpr (sizeof(T) == sizeof(float)) {
+      // node-stencil levels at 2 x 2 with two sweeps: pre-smoothing, residual and
+      // restriction in one pass, prolongation and post-smoothing in another
+      const sttopt_level& C = mg->levels[l + 1];
+      if (op[l].keff_node && L.kx == 2 && L.ky == 2 && ns == 2 && mg->gamma == 1 &&
+          L.mask != nullptr && (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0) {
+        const long long nc = n[l + 1];
+        if (int e = coarse_pair_pre(op[l], rows, b, nl, invd(l), sd(l), x, nl, C.mask,
+                                    buf(C.vb), nc, mg->omega, active, s))
+          return e;
+        T* ec = nullptr;
+        if (int e = cycle(l + 1, buf(C.vb), &ec)) return e;
+        if (int e = coarse_pair_post(op[l], rows, x, nl, ec, nc, b, nl, invd(l), sd(l), alt,
+                                     nl, mg->omega, active, s))
+          return e;
+        *out = alt;
+        return STTOPT_OK;
+      }
+    }
     // the first two sweeps in one pass (the first one, from zero, formed where the
     // second reads it): on the LDS tile of the fp32 level 1 at 2 x 2, and on the
     // one-node-per-thread kernel for small levels, where the launch it saves costs
@@ -778,6 +803,74 @@ static int prof_events(size_t pairs) {
   return STTOPT_OK;
 }
 
+// Process-wide switch of the PCG's captured second half-iteration (on by default).
+static int g_pcg_graphs = 1;
+
+// Executable graphs of the PCG's second half-iteration, one per residual-buffer
+// parity, kept per host thread and refreshed in place (hipGraphExecUpdate) by each
+// solve's capture; destroyed when the thread ends.  A solve synchronizes its stream
+// before it returns, so no launch of a kept graph is in flight when the next solve on
+// the thread updates it.
+// The graphs are recorded on a private stream of the thread (a caller's stream may be
+// the null stream, which cannot be captured); recording runs nothing, and the graph is
+// then launched on the solve's stream.
+struct ThreadGraphs {
+  hipGraphExec_t exec[2] = {nullptr, nullptr};
+  hipStream_t cap = nullptr;
+  int capture_stream(hipStream_t* out) {
+    if (cap == nullptr) {
+      hipError_t e = hipStreamCreateWithFlags(&cap, hipStreamNonBlocking);
+      if (e != hipSuccess) {
+        cap = nullptr;
+        return hip_status(e, "hipStreamCreateWithFlags(capture stream)");
+      }
+    }
+    *out = cap;
+    return STTOPT_OK;
+  }
+  ~ThreadGraphs() {
+    for (hipGraphExec_t& g : exec)
+      if (g != nullptr) (void)hipGraphExecDestroy(g);
+    if (cap != nullptr) (void)hipStreamDestroy(cap);
+  }
+};
+static thread_local ThreadGraphs t_graphs;
+
+// Captures the launches `body` makes on stream s into *slot: an update of the graph
+// already there when the topology matches, else a new instantiation.
+template <typename Body>
+static int capture_graph(hipStream_t s, Body body, hipGraphExec_t* slot) {
+  hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) return hip_status(e, "hipStreamBeginCapture");
+  const int st = body();
+  hipGraph_t graph = nullptr;
+  e = hipStreamEndCapture(s, &graph);
+  if (st != STTOPT_OK) {
+    if (graph != nullptr) (void)hipGraphDestroy(graph);
+    return st;
+  }
+  if (e != hipSuccess) return hip_status(e, "hipStreamEndCapture");
+  if (*slot != nullptr) {
+    hipGraphNode_t bad = nullptr;
+    hipGraphExecUpdateResult res;
+    if (hipGraphExecUpdate(*slot, graph, &bad, &res) == hipSuccess &&
+        res == hipGraphExecUpdateSuccess) {
+      (void)hipGraphDestroy(graph);
+      return STTOPT_OK;
+    }
+    (void)hipGetLastError();
+    (void)hipGraphExecDestroy(*slot);
+    *slot = nullptr;
+  }
+  e = hipGraphInstantiate(slot, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    *slot = nullptr;
+    return hip_status(e, "hipGraphInstantiate");
+  }
+  return STTOPT_OK;
+}
+
 template <typename T>
 static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double* x,
                       long long xs, double rtol, int max_iter, int flexible, int check_every,
@@ -820,7 +913,8 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     STT_TILE_LAUNCH(dot_kernel, double, g, s, n, r, n, static_cast<const double*>(nullptr), 0,
                     op.mask, nullptr, sc.part[1]);
   }
-  if (int e = fill<int>(1, 1, 0, sc.counter, 1, nullptr, s)) return e;
+  // counter[0]: active rows; counter[1]: the device's iteration number
+  if (int e = fill<int>(2, 1, 0, sc.counter, 2, nullptr, s)) return e;
   hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4],
                      cold ? sc.part[4] : sc.part[1], rtol, sc.bsafe, resid, sc.active, iters,
                      sc.counter);
@@ -856,6 +950,37 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   int check_ids[2] = {0, 0};  // the id each pinned slot is expected to hold
   bool done = false;
   int k_last = 0;  // iteration whose x += alpha p is still pending
+  int* kdev = sc.counter + 1;  // the device's iteration number (fin_alpha advances it)
+  // The residual buffers: the fused update of iteration k reads rbuf[(k - 1) & 1] and
+  // writes rbuf[k & 1]; without it the residual stays in rbuf[0].
+  double* rbuf[2] = {r, r2};
+  // The second half of an iteration -- the coarse visit and post-smoothing (or the
+  // whole cycle), fin_beta and the direction update -- is the same launch sequence every
+  // iteration up to the residual buffer, so it is captured once per buffer parity into
+  // a HIP graph and replayed (about 25 launches in one submission).
+  // Launches on stream st (the solve's stream, or the capture stream).
+  auto rest = [&](int parity, hipStream_t st) -> int {
+    struct Restore {
+      VCycle<T>& v;
+      hipStream_t s0;
+      ~Restore() { v.s = s0; }
+    } restore{vc, vc.s};
+    vc.s = st;
+    if (fused) {
+      if (int e = vc.pair_rest(rbuf[parity], n, z, n, sc.part[2])) return e;
+    } else if (int e = vc.run(rbuf[0], n, z, n, sc.part[2], Ap)) {
+      return e;
+    }
+    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, st, rz_tiles, sc.part[2],
+                       fused ? fused_tiles : tiles, sc.part[3], flexible, sc.active, sc.rz,
+                       sc.beta, sc.counter);
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, st, n, sc.alpha, sc.beta, x, xs, p, z,
+                    sc.active, iters, 0, 1, static_cast<const int*>(kdev));
+    STT_CHECK_LAUNCH("pcg direction");
+    return STTOPT_OK;
+  };
+  const bool graphs = g_pcg_graphs && max_iter > 1;
+  bool captured[2] = {false, false};
   for (int k = 1; k <= max_iter; ++k) {
     const size_t pi = g_apply_prof.iter_of.size();
     if (g_apply_prof.on) {
@@ -871,14 +996,16 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       g_apply_prof.iter_of.push_back(k);
     }
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
-                       sc.rz, sc.alpha);
+                       sc.rz, sc.alpha, kdev);
     const bool fuse_now = fused && k < max_iter;
+    const int parity = fused ? (k & 1) : 0;
+    double* rcur = fused ? rbuf[(k - 1) & 1] : rbuf[0];  // the residual entering iteration k
     if (fuse_now) {
-      if (int e = vc.pair_pre_update(r, Ap, sc.alpha, r2, sc.part[1],
+      if (int e = vc.pair_pre_update(rcur, Ap, sc.alpha, rbuf[k & 1], sc.part[1],
                                      flexible ? z : static_cast<T*>(nullptr), sc.part[3]))
         return e;
     } else {
-      STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, r, Ap,
+      STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, rcur, Ap,
                       flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
                       sc.part[3]);
     }
@@ -916,24 +1043,26 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       }
       ++checks;
     }
-    if (fuse_now) {
-      if (int e = vc.pair_rest(r2, n, z, n, sc.part[2])) return e;
-      std::swap(r, r2);
-    } else if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) {
+    if (graphs) {
+      if (!captured[parity]) {
+        hipStream_t cap = nullptr;
+        if (int e = t_graphs.capture_stream(&cap)) return e;
+        if (int e = capture_graph(cap, [&] { return rest(parity, cap); },
+                                  &t_graphs.exec[parity]))
+          return e;
+        captured[parity] = true;
+      }
+      hipError_t e = hipGraphLaunch(t_graphs.exec[parity], s);
+      if (e != hipSuccess) return hip_status(e, "hipGraphLaunch");
+    } else if (int e = rest(parity, s)) {
       return e;
     }
-    hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2],
-                       fuse_now ? fused_tiles : tiles, sc.part[3], flexible, sc.active, sc.rz,
-                       sc.beta, sc.counter);
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs, p, z,
-                    sc.active, iters, k, 1);
-    STT_CHECK_LAUNCH("pcg direction");
     k_last = 0;
   }
   if (k_last > 0) {
     // the last iteration's x += alpha p (no direction update follows it)
     STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs, p, z,
-                    sc.active, iters, k_last, 0);
+                    sc.active, iters, k_last, 0, static_cast<const int*>(nullptr));
     STT_CHECK_LAUNCH("pcg last x update");
   }
   // n_iter: the slowest row's count
@@ -987,6 +1116,12 @@ int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {
   const long long n = (long long)(mg->levels[0].nelx + 1) * (mg->levels[0].nely + 1) * mg->d;
   // r, z, p, Ap and the second residual buffer of the fused update
   return 5LL * mg->rows * n;
+}
+
+int sttopt_mgpcg_graphs(int32_t enable) {
+  const int prev = g_pcg_graphs;
+  if (enable >= 0) g_pcg_graphs = enable != 0;
+  return prev;
 }
 
 int sttopt_profile_outer_apply(int32_t enable) {

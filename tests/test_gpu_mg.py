@@ -289,6 +289,58 @@ def test_mgpcg_padded_solution_rows_equal_contiguous(dev, dtype, cold):
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
+def test_mgpcg_graph_replay_equals_kernel_by_kernel(dev, dtype):
+    """The captured second half-iteration (replayed every iteration) against the same
+    launches issued one by one: stopped after k = 1..6 iterations and run to
+    convergence, twice over (the second solve refreshes the kept graphs), with eager work
+    and synchronizations between the solves; x, the per-row iteration counts and
+    residuals are bit-identical every time."""
+    import ctypes
+
+    from sttopt_amd import _lib, torch_fem, torch_mg
+
+    nelx, nely = 24, 16
+    s = _setup(nelx, nely, dev)
+    n = s["ndof"]
+    x = s["t"](np.stack([binary_design(nelx, nely), np.full(nelx * nely, 0.5),
+                         np.full(nelx * nely, 0.7)]))
+    rng = np.random.default_rng(12)
+    F = s["t"](rng.standard_normal((3, n))) * s["mask_t"]
+    dens = torch_fem.simp_density(x, EMIN, EMAX, PENAL)
+    levels = torch_mg.build_hierarchy(dens, s["edof_t"], s["KE_t"], s["mask_t"], nelx, nely,
+                                      **MG_KW)
+    vc = torch_mg.VCycle(levels, dtype=getattr(torch, dtype))
+    lib = _lib.load()
+
+    def run(graphs, max_iter):
+        prev = lib.sttopt_mgpcg_graphs(int(graphs))
+        try:
+            mg, _, work = vc._native(3)
+            xo = torch.empty(3, n, dtype=torch.float64, device=dev)
+            iters = torch.zeros(3, dtype=torch.int32, device=dev)
+            resid = torch.zeros(3, dtype=torch.float64, device=dev)
+            st = lib.sttopt_mgpcg(ctypes.byref(mg), F.data_ptr(), n, xo.data_ptr(), n, 1e-10,
+                                  max_iter, 1 if dtype == "float32" else 0, 1, 1,
+                                  work.data_ptr(), iters.data_ptr(), resid.data_ptr(),
+                                  ctypes.byref(ctypes.c_int32(0)), _lib.stream_handle(dev))
+        finally:
+            lib.sttopt_mgpcg_graphs(prev)
+        assert st in (_lib.OK, _lib.ENOCONV), _lib.last_error()
+        torch.cuda.synchronize(dev)
+        return st, xo.clone(), iters.tolist(), resid.clone()
+
+    assert lib.sttopt_mgpcg_graphs(-1) == 1  # on by default
+    for repeat in range(2):
+        for max_iter in (1, 2, 3, 4, 5, 6, 500):
+            a = run(True, max_iter)
+            _ = (F * 2).sum().item()  # eager work and a synchronization between the solves
+            b = run(False, max_iter)
+            assert a[0] == b[0] and a[2] == b[2], (repeat, max_iter)
+            assert torch.equal(a[1], b[1]) and torch.equal(a[3], b[3]), (repeat, max_iter)
+        assert a[0] == _lib.OK and max(a[2]) > 6
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
 def test_cold_start_equals_zero_warm_start(dev, dtype):
     """x0 = None takes the cold-start initialization (x = 0 and r = P b in one pass, x
     not read); a warm start from zeros takes the projection and residual apply.  Both
@@ -505,15 +557,19 @@ def test_fp32_vcycle_node_stencil_coarse_levels_equal_full(dev):
 @pytest.mark.parametrize("nelx, nely, reaction, coarse",
                          [(64, 48, False, None), (64, 48, True, None), (124, 28, False, None),
                           (122, 26, False, None), (16, 12, True, 2), (4, 2, False, 2),
-                          (64, 48, False, 20), (64, 48, True, 20)])
+                          (64, 48, False, 20), (64, 48, True, 20), (272, 136, False, 300),
+                          (264, 392, True, 300)])
 def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coarse):
     """The fp32 V-cycle's fused passes equal one pass per sweep bit for bit: level 0's
     two-sweep passes (pre: two sweeps and the residual; post: prolongation and two
     sweeps) on grids whose node counts fall on and next to the 62 x 14 output tiles
     (125 = 2 * 62 + 1, 29 = 2 * 14 + 1, 123, 27), and the first two sweeps in one pass
     on the coarse levels (level 1 on the LDS tile, deeper levels, reached with a small
-    coarsest level, on the one-node-per-thread kernel), with and without a lumped
-    reaction diagonal."""
+    coarsest level, on the one-node-per-thread kernel), the two-pass node-stencil levels
+    (pre-smoothing with residual and restriction, prolongation with post-smoothing: on
+    levels 2 and 3 at 64 x 48 with 20 coarsest elements, on grids of several partial
+    tiles at 272 x 136 and 264 x 392, and beside levels coarsened 2 x 1 and 3 x 1), with
+    and without a lumped reaction diagonal."""
     from sttopt_amd import torch_fem, torch_mg
 
     s = _setup(nelx, nely, dev)
@@ -534,6 +590,31 @@ def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coars
     assert float(z_sep.abs().max()) > 0
     diff = float((z_pair - z_sep).abs().max())
     print(f"{nelx}x{nely} reaction={reaction}: max |z_pair - z_sep| = {diff:.3e}")
+    assert torch.equal(z_pair, z_sep)
+
+
+def test_scalar_two_pass_coarse_levels_equal_separate_sweeps(dev):
+    """The same on a scalar (one dof per node) mesh with a lumped reaction: the
+    node-stencil levels' two passes against one pass per sweep, bit for bit."""
+    from sttopt_amd import torch_fem, torch_mg
+
+    nelx, nely = 96, 64
+    nn = (nelx + 1) * (nely + 1)
+    mask = np.ones(nn, bool)
+    mask[-(nelx + 1):] = False  # bottom edge fixed
+    rng = np.random.default_rng(21)
+    t = lambda a, dt=torch.float64: torch.as_tensor(np.asarray(a), dtype=dt, device=dev)  # noqa: E731
+    edof = t(fem_direct.element_dofs(nelx, nely, 1), torch.int64)
+    KE = t(fem_direct.diffusion_KE())
+    dens = t(rng.uniform(0.01, 1, (2, nelx * nely)))
+    levels = torch_mg.build_hierarchy(dens, edof, KE, torch.as_tensor(mask, device=dev), nelx,
+                                      nely, reaction=t(rng.uniform(0, 1e-2, (2, nelx * nely))),
+                                      max_coarse_elements=24)
+    assert len(levels) >= 4
+    r = torch_fem.project(t(rng.standard_normal((2, nn))), torch.as_tensor(mask, device=dev))
+    z_pair = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    z_sep = torch_mg.VCycle(levels, dtype=torch.float32, separate_sweeps=True)(r)
+    assert float(z_sep.abs().max()) > 0
     assert torch.equal(z_pair, z_sep)
 
 
