@@ -58,9 +58,16 @@ extern "C" {
 #define STTOPT_KEFF_UPPER 1
 #define STTOPT_KEFF_NODE 2
 
-/* sttopt_level.flags (level 0 only) */
-#define STTOPT_SEPARATE_SWEEPS 1 /* one pass per smoothing sweep at every level instead
-                                    of the fused passes (same results; for testing) */
+/* sttopt_level.flags */
+#define STTOPT_SEPARATE_SWEEPS 1 /* level 0: one pass per smoothing sweep at every level
+                                    instead of the fused passes (same results; for
+                                    testing) */
+#define STTOPT_LEVEL_SEPARATE 2  /* a node-stencil level of the fp32 cycle: one pass per
+                                    sweep, residual and transfer at this level instead of
+                                    its two passes (same results) */
+#define STTOPT_TILE_VARIANT_SHIFT 8 /* bits 8-11: the tile schedule of this level's two
+                                       passes (0: 32 x 4 coarse / 64 x 8 fine nodes per
+                                       512-thread workgroup, 1: 16 x 4 / 32 x 8 per 256) */
 
 /* operator modes for sttopt_apply */
 #define STTOPT_MODE_MATVEC 0   /* y = K u (no projection)                 */
@@ -139,7 +146,8 @@ typedef struct sttopt_level {
   const void* keff;
   int64_t keff_stride;
   int32_t keff_layout;    /* STTOPT_KEFF_FULL, STTOPT_KEFF_UPPER or STTOPT_KEFF_NODE */
-  int32_t flags;          /* level 0: STTOPT_SEPARATE_SWEEPS or 0; others: 0      */
+  int32_t flags;          /* STTOPT_SEPARATE_SWEEPS (level 0), STTOPT_LEVEL_SEPARATE,
+                             tile variant << STTOPT_TILE_VARIANT_SHIFT; 0 default   */
   /* coarser levels, alternative to keff: keff_E = sum over the pkx x pky level-0
    * elements g under E of density0_g * ptable[g], ptable: (pkx*pky) x (4d)^2 in the
    * V-cycle precision (device); the V-cycle then reads only level 0's density      */

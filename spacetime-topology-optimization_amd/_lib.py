@@ -21,6 +21,8 @@ LIB_PATH = _build.LIB_PATH
 OK, EINVAL, EHIP, ENOCONV, ENOTSPD = 0, -1, -2, -3, -4
 KEFF_FULL, KEFF_UPPER, KEFF_NODE = 0, 1, 2
 SEPARATE_SWEEPS = 1  # sttopt_level.flags of level 0
+LEVEL_SEPARATE = 2  # sttopt_level.flags: one pass per sweep at this level
+TILE_VARIANT_SHIFT = 8  # sttopt_level.flags bits 8-11: the two passes' tile schedule
 F64, F32 = 0, 1
 MODE_MATVEC, MODE_OPERATOR, MODE_OPERATOR_PROJECTED = 0, 1, 2
 MAX_LEVELS = 16

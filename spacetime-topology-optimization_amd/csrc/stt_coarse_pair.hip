@@ -23,18 +23,22 @@ namespace stt {
 
 namespace {
 
-constexpr int kCW = 32, kCH = 4;           // pre: coarse nodes per tile
-constexpr int kPW = 2 * kCW + 5;           // pre: staged fine columns (x1 region)
-constexpr int kPH = 2 * kCH + 5;           // pre: staged fine rows
-constexpr int kTW = 64, kTH = 8;           // post: fine output nodes per tile
-constexpr int kQW = kTW + 4, kQH = kTH + 4;  // post: staged fine region (xp)
-constexpr int kEW = kTW / 2 + 3, kEH = kTH / 2 + 3;  // post: staged coarse region (e)
-// Threads per workgroup: every phase takes at most two nodes per thread, whose loads
-// are issued together (the phases are latency bound: each reads the stencil records of
-// its nodes from L2).
-constexpr int kPairThreads = 512;
-template <int N>
-constexpr int per_thread() { return (N + kPairThreads - 1) / kPairThreads; }
+// Tile configurations (the autotuned schedule of a level): the pre pass covers
+// CW x CH coarse nodes (staging 2 CW + 5 by 2 CH + 5 fine nodes), the post pass TW x TH
+// fine nodes (staging TW + 4 by TH + 4), with NT threads per workgroup.  Every phase
+// takes at most two nodes per thread, whose loads are issued together (the phases are
+// latency bound: each reads the stencil rows of its nodes from L2).
+template <int CW_, int CH_, int TW_, int TH_, int NT_>
+struct PairCfg {
+  static constexpr int CW = CW_, CH = CH_, TW = TW_, TH = TH_, NT = NT_;
+  static constexpr int PW = 2 * CW + 5, PH = 2 * CH + 5;      // pre: staged fine region
+  static constexpr int QW = TW + 4, QH = TH + 4;              // post: staged fine region
+  static constexpr int EW = TW / 2 + 3, EH = TH / 2 + 3;      // post: staged coarse region
+  template <int N>
+  static constexpr int per_thread() { return (N + NT - 1) / NT; }
+};
+using PairWide = PairCfg<32, 4, 64, 8, 512>;    // variant 0
+using PairNarrow = PairCfg<16, 4, 32, 8, 256>;  // variant 1: twice the workgroups
 
 // What one node's stencil row needs from HBM: its own record (own block's upper
 // triangle, forward blocks E, NW, N, NE), the backward neighbours' forward blocks toward
@@ -157,21 +161,21 @@ struct CoarsePairArgs {
   const int* active;
 };
 
-template <int D>
-__global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePairArgs a) {
-  __shared__ float sbv[kPH * kPW][D];
-  __shared__ float sdv[kPH * kPW][D];
-  __shared__ float sx1[kPH * kPW][D];
-  __shared__ float sx2[kPH * kPW][D];
-  __shared__ float srr[kPH * kPW][D];
-  const int t = xcd_remap(blockIdx.x, a.total);
+template <int D, typename C>
+__device__ __forceinline__ void pair_pre_body(const CoarsePairArgs& a, int bid) {
+  __shared__ float sbv[C::PH * C::PW][D];
+  __shared__ float sdv[C::PH * C::PW][D];
+  __shared__ float sx1[C::PH * C::PW][D];
+  __shared__ float sx2[C::PH * C::PW][D];
+  __shared__ float srr[C::PH * C::PW][D];
+  const int t = xcd_remap(bid, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block
   const int ty = tt / a.tiles_x, tx = tt - ty * a.tiles_x;
-  const int I0 = ty * kCH, J0 = tx * kCW;     // first coarse node of the tile
+  const int I0 = ty * C::CH, J0 = tx * C::CW;     // first coarse node of the tile
   // fine node at local (0, 0): the residual is needed on fine rows 2 I0 - 1 ..
-  // 2 (I0 + kCH) - 1 (local 2 .. kPH - 3), x2 one node beyond, x1 two
+  // 2 (I0 + C::CH) - 1 (local 2 .. C::PH - 3), x2 one node beyond, x1 two
   const int fr0 = 2 * I0 - 3, fc0 = 2 * J0 - 3;
   const float* b = a.b + row * a.sb;
   const float* dinv = a.invd + row * a.sd;
@@ -179,24 +183,24 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
   // The stencil rows of this thread's nodes one node inside the staged region (phase
   // x2; the residual's nodes are a subset, taken by the same threads), loaded before
   // anything else so that every HBM read of the workgroup is in flight at once.
-  constexpr int NJ = per_thread<kPH * kPW>();
+  constexpr int NJ = C::template per_thread<C::PH * C::PW>();
   NodeRow<D> w[NJ];
   bool inner[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    const int lr = i / kPW, lc = i % kPW;
+    const int i = threadIdx.x + j * C::NT;
+    const int lr = i / C::PW, lc = i % C::PW;
     const int fr = fr0 + lr, fc = fc0 + lc;
-    inner[j] = i < kPH * kPW && lr >= 1 && lr <= kPH - 2 && lc >= 1 && lc <= kPW - 2 &&
+    inner[j] = i < C::PH * C::PW && lr >= 1 && lr <= C::PH - 2 && lc >= 1 && lc <= C::PW - 2 &&
                fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx;
     if (inner[j]) load_row<D>(S, a.mask, a.nnx, a.nny, fr, fc, w[j]);
   }
   // x1 = w D^-1 b on the whole staged region (jacobi_first's expression; 0 outside)
 #pragma unroll
-  for (int j = 0; j < per_thread<kPH * kPW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kPH * kPW) break;
-    const int fr = fr0 + i / kPW, fc = fc0 + i % kPW;
+  for (int j = 0; j < C::template per_thread<C::PH * C::PW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::PH * C::PW) break;
+    const int fr = fr0 + i / C::PW, fc = fc0 + i % C::PW;
     Vec<float, D> bv, dv;
 #pragma unroll
     for (int k = 0; k < D; ++k) bv.v[k] = dv.v[k] = 0.0f;
@@ -216,18 +220,18 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
   // x2 = x1 + w D^-1 (b - A x1) one node inside the staged region; x2 of the tile's own
   // fine nodes (those whose coarse node, by integer halving, is in the tile) to HBM
 #pragma unroll
-  for (int j = 0; j < per_thread<kPH * kPW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kPH * kPW) break;
-    const int lr = i / kPW, lc = i % kPW;
-    if (lr < 1 || lr > kPH - 2 || lc < 1 || lc > kPW - 2) continue;
+  for (int j = 0; j < C::template per_thread<C::PH * C::PW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::PH * C::PW) break;
+    const int lr = i / C::PW, lc = i % C::PW;
+    if (lr < 1 || lr > C::PH - 2 || lc < 1 || lc > C::PW - 2) continue;
     const int fr = fr0 + lr, fc = fc0 + lc;
     Vec<float, D> xo;
 #pragma unroll
     for (int k = 0; k < D; ++k) xo.v[k] = 0.0f;
     if (inner[j]) {
       float U[3][3][D], acc[D];
-      gather3<float, D, kPW>(sx1, lr, lc, U);
+      gather3<float, D, C::PW>(sx1, lr, lc, U);
       apply_row<D>(w[j], U, acc);
       const long long node = (long long)fr * a.nnx + fc;
 #pragma unroll
@@ -235,7 +239,7 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
         xo.v[k] = ((w[j].free_bits >> k) & 1u)
                       ? U[1][1][k] + a.omega * sdv[i][k] * (sbv[i][k] - acc[k])
                       : 0.0f;
-      if (fr >= 2 * I0 && fr < 2 * I0 + 2 * kCH && fc >= 2 * J0 && fc < 2 * J0 + 2 * kCW)
+      if (fr >= 2 * I0 && fr < 2 * I0 + 2 * C::CH && fc >= 2 * J0 && fc < 2 * J0 + 2 * C::CW)
         store_node<float, D>(a.out + row * a.so, node, xo);
     }
 #pragma unroll
@@ -244,18 +248,18 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
   __syncthreads();
   // r = b - A x2 on the fine nodes that feed the tile's coarse nodes
 #pragma unroll
-  for (int j = 0; j < per_thread<kPH * kPW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kPH * kPW) break;
-    const int lr = i / kPW, lc = i % kPW;
-    if (lr < 2 || lr > kPH - 3 || lc < 2 || lc > kPW - 3) continue;
+  for (int j = 0; j < C::template per_thread<C::PH * C::PW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::PH * C::PW) break;
+    const int lr = i / C::PW, lc = i % C::PW;
+    if (lr < 2 || lr > C::PH - 3 || lc < 2 || lc > C::PW - 3) continue;
     const int fr = fr0 + lr, fc = fc0 + lc;
     float ro[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) ro[k] = 0.0f;
     if (inner[j]) {
       float U[3][3][D], acc[D];
-      gather3<float, D, kPW>(sx2, lr, lc, U);
+      gather3<float, D, C::PW>(sx2, lr, lc, U);
       apply_row<D>(w[j], U, acc);
 #pragma unroll
       for (int k = 0; k < D; ++k) ro[k] = ((w[j].free_bits >> k) & 1u) ? sbv[i][k] - acc[k] : 0.0f;
@@ -265,8 +269,8 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
   }
   __syncthreads();
   // full weighting to the tile's coarse nodes, in restrict22_kernel's order
-  for (int i = threadIdx.x; i < kCW * kCH; i += kPairThreads) {
-    const int I = I0 + i / kCW, J = J0 + i % kCW;
+  for (int i = threadIdx.x; i < C::CW * C::CH; i += C::NT) {
+    const int I = I0 + i / C::CW, J = J0 + i % C::CW;
     if (I >= a.cnny || J >= a.cnnx) continue;
     float acc[D];
 #pragma unroll
@@ -283,13 +287,13 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
       for (int k = 0; k < D; ++k) racc[k] = 0.0f;
       if (J > 0) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) racc[k] += wh * srr[lr * kPW + lc - 1][k];
+        for (int k = 0; k < D; ++k) racc[k] += wh * srr[lr * C::PW + lc - 1][k];
       }
 #pragma unroll
-      for (int k = 0; k < D; ++k) racc[k] += w1 * srr[lr * kPW + lc][k];
+      for (int k = 0; k < D; ++k) racc[k] += w1 * srr[lr * C::PW + lc][k];
       if (2 * J + 1 < a.nnx) {
 #pragma unroll
-        for (int k = 0; k < D; ++k) racc[k] += wh * srr[lr * kPW + lc + 1][k];
+        for (int k = 0; k < D; ++k) racc[k] += wh * srr[lr * C::PW + lc + 1][k];
       }
 #pragma unroll
       for (int k = 0; k < D; ++k) acc[k] += wy * racc[k];
@@ -302,19 +306,19 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_pre_kernel(CoarsePai
   }
 }
 
-template <int D>
-__global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePairArgs a) {
-  __shared__ float sbv[kQH * kQW][D];
-  __shared__ float sdv[kQH * kQW][D];
-  __shared__ float sxp[kQH * kQW][D];
-  __shared__ float sx3[kQH * kQW][D];
-  __shared__ float sec[kEH * kEW][D];
-  const int t = xcd_remap(blockIdx.x, a.total);
+template <int D, typename C>
+__device__ __forceinline__ void pair_post_body(const CoarsePairArgs& a, int bid) {
+  __shared__ float sbv[C::QH * C::QW][D];
+  __shared__ float sdv[C::QH * C::QW][D];
+  __shared__ float sxp[C::QH * C::QW][D];
+  __shared__ float sx3[C::QH * C::QW][D];
+  __shared__ float sec[C::EH * C::EW][D];
+  const int t = xcd_remap(bid, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block
   const int ty = tt / a.tiles_x, tx = tt - ty * a.tiles_x;
-  const int r0 = ty * kTH, c0 = tx * kTW;          // first output node of the tile
+  const int r0 = ty * C::TH, c0 = tx * C::TW;          // first output node of the tile
   const int fr0 = r0 - 2, fc0 = c0 - 2;            // fine node at local (0, 0)
   const int er0 = r0 / 2 - 1, ec0 = c0 / 2 - 1;    // coarse node at local (0, 0)
   const float* b = a.b + row * a.sb;
@@ -324,20 +328,20 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
   const float* S = a.S + row * a.sS;
   // the stencil rows of this thread's nodes one node around the tile (phase x3; the
   // tile's own nodes, phase x4, are a subset taken by the same threads)
-  constexpr int NJ = per_thread<kQH * kQW>();
+  constexpr int NJ = C::template per_thread<C::QH * C::QW>();
   NodeRow<D> w[NJ];
   bool inner[NJ];
 #pragma unroll
   for (int j = 0; j < NJ; ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    const int lr = i / kQW, lc = i % kQW;
+    const int i = threadIdx.x + j * C::NT;
+    const int lr = i / C::QW, lc = i % C::QW;
     const int fr = fr0 + lr, fc = fc0 + lc;
-    inner[j] = i < kQH * kQW && lr >= 1 && lr <= kQH - 2 && lc >= 1 && lc <= kQW - 2 &&
+    inner[j] = i < C::QH * C::QW && lr >= 1 && lr <= C::QH - 2 && lc >= 1 && lc <= C::QW - 2 &&
                fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx;
     if (inner[j]) load_row<D>(S, a.mask, a.nnx, a.nny, fr, fc, w[j]);
   }
-  for (int i = threadIdx.x; i < kEH * kEW; i += kPairThreads) {
-    const int cr = er0 + i / kEW, cc = ec0 + i % kEW;
+  for (int i = threadIdx.x; i < C::EH * C::EW; i += C::NT) {
+    const int cr = er0 + i / C::EW, cc = ec0 + i % C::EW;
     Vec<float, D> v;
 #pragma unroll
     for (int k = 0; k < D; ++k) v.v[k] = 0.0f;
@@ -346,12 +350,12 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
 #pragma unroll
     for (int k = 0; k < D; ++k) sec[i][k] = v.v[k];
   }
-  Vec<float, D> xv[per_thread<kQH * kQW>()];
+  Vec<float, D> xv[C::template per_thread<C::QH * C::QW>()];
 #pragma unroll
-  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kQH * kQW) break;
-    const int fr = fr0 + i / kQW, fc = fc0 + i % kQW;
+  for (int j = 0; j < C::template per_thread<C::QH * C::QW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::QH * C::QW) break;
+    const int fr = fr0 + i / C::QW, fc = fc0 + i % C::QW;
     Vec<float, D> bv, dv;
 #pragma unroll
     for (int k = 0; k < D; ++k) bv.v[k] = dv.v[k] = xv[j].v[k] = 0.0f;
@@ -370,10 +374,10 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
   __syncthreads();
   // xp = x2 + P e at the free dofs (prolong_add_kernel's expression)
 #pragma unroll
-  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kQH * kQW) break;
-    const int fr = fr0 + i / kQW, fc = fc0 + i % kQW;
+  for (int j = 0; j < C::template per_thread<C::QH * C::QW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::QH * C::QW) break;
+    const int fr = fr0 + i / C::QW, fc = fc0 + i % C::QW;
     Vec<float, D> v = xv[j];
     if (fr >= 0 && fr < a.nny && fc >= 0 && fc < a.nnx) {
       const long long node = (long long)fr * a.nnx + fc;
@@ -384,13 +388,13 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
         const int J0 = fc / 2, fx = fc - 2 * J0;
         const float wy1 = float(fy) / 2.0f, wy0 = 1.0f - wy1;
         const float wx1 = float(fx) / 2.0f, wx0 = 1.0f - wx1;
-        const int l00 = (I0 - er0) * kEW + (J0 - ec0);
+        const int l00 = (I0 - er0) * C::EW + (J0 - ec0);
 #pragma unroll
         for (int k = 0; k < D; ++k) {
           const float a00 = sec[l00][k];
           const float a01 = fx > 0 ? sec[l00 + 1][k] : 0.0f;
-          const float a10 = fy > 0 ? sec[l00 + kEW][k] : 0.0f;
-          const float a11 = (fx > 0 && fy > 0) ? sec[l00 + kEW + 1][k] : 0.0f;
+          const float a10 = fy > 0 ? sec[l00 + C::EW][k] : 0.0f;
+          const float a11 = (fx > 0 && fy > 0) ? sec[l00 + C::EW + 1][k] : 0.0f;
           const float top = wx0 * a00 + wx1 * a01;
           const float bot = wx0 * a10 + wx1 * a11;
           if ((bits >> k) & 1u) v.v[k] += wy0 * top + wy1 * bot;
@@ -403,18 +407,18 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
   __syncthreads();
   // x3 one node around the tile, then x4 on it
 #pragma unroll
-  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kQH * kQW) break;
-    const int lr = i / kQW, lc = i % kQW;
-    if (lr < 1 || lr > kQH - 2 || lc < 1 || lc > kQW - 2) continue;
+  for (int j = 0; j < C::template per_thread<C::QH * C::QW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::QH * C::QW) break;
+    const int lr = i / C::QW, lc = i % C::QW;
+    if (lr < 1 || lr > C::QH - 2 || lc < 1 || lc > C::QW - 2) continue;
     const int fr = fr0 + lr, fc = fc0 + lc;
     float xo[D];
 #pragma unroll
     for (int k = 0; k < D; ++k) xo[k] = 0.0f;
     if (inner[j]) {
       float U[3][3][D], acc[D];
-      gather3<float, D, kQW>(sxp, lr, lc, U);
+      gather3<float, D, C::QW>(sxp, lr, lc, U);
       apply_row<D>(w[j], U, acc);
 #pragma unroll
       for (int k = 0; k < D; ++k)
@@ -427,15 +431,15 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
   }
   __syncthreads();
 #pragma unroll
-  for (int j = 0; j < per_thread<kQH * kQW>(); ++j) {
-    const int i = threadIdx.x + j * kPairThreads;
-    if (i >= kQH * kQW) break;
-    const int lr = i / kQW, lc = i % kQW;
-    if (lr < 2 || lr > kQH - 3 || lc < 2 || lc > kQW - 3) continue;
+  for (int j = 0; j < C::template per_thread<C::QH * C::QW>(); ++j) {
+    const int i = threadIdx.x + j * C::NT;
+    if (i >= C::QH * C::QW) break;
+    const int lr = i / C::QW, lc = i % C::QW;
+    if (lr < 2 || lr > C::QH - 3 || lc < 2 || lc > C::QW - 3) continue;
     const int fr = fr0 + lr, fc = fc0 + lc;
     if (!inner[j]) continue;  // outside the grid
     float U[3][3][D], acc[D];
-    gather3<float, D, kQW>(sx3, lr, lc, U);
+    gather3<float, D, C::QW>(sx3, lr, lc, U);
     apply_row<D>(w[j], U, acc);
     const long long node = (long long)fr * a.nnx + fc;
     Vec<float, D> xo;
@@ -448,6 +452,33 @@ __global__ __launch_bounds__(kPairThreads) void coarse_pair_post_kernel(CoarsePa
   }
 }
 
+// One launch per level (the fine grid of one hierarchy), or one launch over the same
+// level of several hierarchies: group i takes workgroups first[i] .. first[i + 1] - 1.
+constexpr int kMaxPairGroups = 8;
+struct PairGroups {
+  int n;
+  int first[kMaxPairGroups + 1];
+  CoarsePairArgs a[kMaxPairGroups];
+};
+
+__device__ __forceinline__ int group_of(const PairGroups& g, int b) {
+  int i = 0;
+  while (i + 1 < g.n && b >= g.first[i + 1]) ++i;
+  return i;
+}
+
+template <int D, typename C>
+__global__ __launch_bounds__(C::NT) void coarse_pair_pre_kernel(PairGroups g) {
+  const int i = group_of(g, blockIdx.x);
+  pair_pre_body<D, C>(g.a[i], blockIdx.x - g.first[i]);
+}
+
+template <int D, typename C>
+__global__ __launch_bounds__(C::NT) void coarse_pair_post_kernel(PairGroups g) {
+  const int i = group_of(g, blockIdx.x);
+  pair_post_body<D, C>(g.a[i], blockIdx.x - g.first[i]);
+}
+
 int check_pair_level(const OpDesc& op, const char* what) {
   if (op.keff == nullptr || !op.keff_node || op.mask == nullptr || (op.d != 1 && op.d != 2) ||
       op.nelx % 2 != 0 || op.nely % 2 != 0) {
@@ -458,79 +489,130 @@ int check_pair_level(const OpDesc& op, const char* what) {
   return STTOPT_OK;
 }
 
-}  // namespace
-
-int coarse_pair_pre(const OpDesc& op, int rows, const float* b, long long sb,
-                    const float* invd, long long sd, float* x2, long long sx2,
-                    const uint32_t* cmask, float* bc, long long sbc, double omega,
-                    const int* active, hipStream_t s) {
-  if (int e = check_pair_level(op, "coarse_pair_pre")) return e;
-  if (rows <= 0) return STTOPT_OK;
+template <typename C>
+CoarsePairArgs pair_args(const PairLevel& L, bool pre) {
+  const OpDesc& op = L.op;
   CoarsePairArgs a{};
   a.nnx = op.nnx();
   a.nny = op.nny();
   a.cnnx = op.nelx / 2 + 1;
   a.cnny = op.nely / 2 + 1;
-  a.tiles_x = ceil_div(a.cnnx, kCW);
-  a.tiles = a.tiles_x * ceil_div(a.cnny, kCH);
-  a.total = a.tiles * rows;
+  if (pre) {
+    a.tiles_x = ceil_div(a.cnnx, C::CW);
+    a.tiles = a.tiles_x * ceil_div(a.cnny, C::CH);
+  } else {
+    a.tiles_x = ceil_div(a.nnx, C::TW);
+    a.tiles = a.tiles_x * ceil_div(a.nny, C::TH);
+  }
+  a.total = a.tiles * L.rows;
   a.S = static_cast<const float*>(op.keff);
   a.sS = op.sk;
   a.mask = op.mask;
-  a.cmask = cmask;
-  a.b = b;
-  a.sb = sb;
-  a.invd = invd;
-  a.sd = sd;
-  a.out = x2;
-  a.so = sx2;
-  a.bc = bc;
-  a.sbc = sbc;
-  a.omega = static_cast<float>(omega);
-  a.active = active;
-  if (op.d == 2)
-    hipLaunchKernelGGL(coarse_pair_pre_kernel<2>, dim3(a.total), dim3(kPairThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL(coarse_pair_pre_kernel<1>, dim3(a.total), dim3(kPairThreads), 0, s, a);
-  STT_CHECK_LAUNCH("coarse_pair_pre_kernel");
+  a.cmask = L.cmask;
+  a.b = L.b;
+  a.sb = L.sb;
+  a.invd = L.invd;
+  a.sd = L.sd;
+  a.x2 = L.x2;
+  a.sx2 = L.sx2;
+  a.ec = L.ec;
+  a.sec = L.sec;
+  a.out = pre ? const_cast<float*>(L.x2) : L.x4;
+  a.so = pre ? L.sx2 : L.sx4;
+  a.bc = L.bc;
+  a.sbc = L.sbc;
+  a.omega = static_cast<float>(L.omega);
+  a.active = L.active;
+  return a;
+}
+
+template <typename C>
+int launch_pair(const PairLevel* lv, int n, bool pre, hipStream_t s) {
+  PairGroups g{};
+  int total = 0, d = 0;
+  for (int i = 0; i < n; ++i) {
+    if (int e = check_pair_level(lv[i].op, pre ? "coarse_pair_pre" : "coarse_pair_post"))
+      return e;
+    if (i > 0 && lv[i].op.d != d) {
+      set_error("coarse pair pass: levels of one launch need the same dofs per node");
+      return STTOPT_EINVAL;
+    }
+    d = lv[i].op.d;
+    if (lv[i].rows <= 0) continue;
+    g.a[g.n] = pair_args<C>(lv[i], pre);
+    g.first[g.n] = total;
+    total += g.a[g.n].total;
+    ++g.n;
+  }
+  g.first[g.n] = total;
+  if (total == 0) return STTOPT_OK;
+  if (pre) {
+    if (d == 2)
+      hipLaunchKernelGGL((coarse_pair_pre_kernel<2, C>), dim3(total), dim3(C::NT), 0, s, g);
+    else
+      hipLaunchKernelGGL((coarse_pair_pre_kernel<1, C>), dim3(total), dim3(C::NT), 0, s, g);
+  } else {
+    if (d == 2)
+      hipLaunchKernelGGL((coarse_pair_post_kernel<2, C>), dim3(total), dim3(C::NT), 0, s, g);
+    else
+      hipLaunchKernelGGL((coarse_pair_post_kernel<1, C>), dim3(total), dim3(C::NT), 0, s, g);
+  }
+  STT_CHECK_LAUNCH(pre ? "coarse_pair_pre_kernel" : "coarse_pair_post_kernel");
   return STTOPT_OK;
+}
+
+}  // namespace
+
+int coarse_pair_groups(const PairLevel* lv, int n, bool pre, int variant, hipStream_t s) {
+  if (n < 1 || n > kMaxPairGroups) {
+    set_error("coarse pair pass: 1 to 8 levels per launch");
+    return STTOPT_EINVAL;
+  }
+  if (variant == 1) return launch_pair<PairNarrow>(lv, n, pre, s);
+  return launch_pair<PairWide>(lv, n, pre, s);
+}
+
+int coarse_pair_pre(const OpDesc& op, int rows, const float* b, long long sb,
+                    const float* invd, long long sd, float* x2, long long sx2,
+                    const uint32_t* cmask, float* bc, long long sbc, double omega,
+                    const int* active, hipStream_t s, int variant) {
+  PairLevel L{};
+  L.op = op;
+  L.rows = rows;
+  L.b = b;
+  L.sb = sb;
+  L.invd = invd;
+  L.sd = sd;
+  L.x2 = x2;
+  L.sx2 = sx2;
+  L.cmask = cmask;
+  L.bc = bc;
+  L.sbc = sbc;
+  L.omega = omega;
+  L.active = active;
+  return coarse_pair_groups(&L, 1, true, variant, s);
 }
 
 int coarse_pair_post(const OpDesc& op, int rows, const float* x2, long long sx2,
                      const float* ec, long long sec, const float* b, long long sb,
                      const float* invd, long long sd, float* x4, long long sx4, double omega,
-                     const int* active, hipStream_t s) {
-  if (int e = check_pair_level(op, "coarse_pair_post")) return e;
-  if (rows <= 0) return STTOPT_OK;
-  CoarsePairArgs a{};
-  a.nnx = op.nnx();
-  a.nny = op.nny();
-  a.cnnx = op.nelx / 2 + 1;
-  a.cnny = op.nely / 2 + 1;
-  a.tiles_x = ceil_div(a.nnx, kTW);
-  a.tiles = a.tiles_x * ceil_div(a.nny, kTH);
-  a.total = a.tiles * rows;
-  a.S = static_cast<const float*>(op.keff);
-  a.sS = op.sk;
-  a.mask = op.mask;
-  a.b = b;
-  a.sb = sb;
-  a.invd = invd;
-  a.sd = sd;
-  a.x2 = x2;
-  a.sx2 = sx2;
-  a.ec = ec;
-  a.sec = sec;
-  a.out = x4;
-  a.so = sx4;
-  a.omega = static_cast<float>(omega);
-  a.active = active;
-  if (op.d == 2)
-    hipLaunchKernelGGL(coarse_pair_post_kernel<2>, dim3(a.total), dim3(kPairThreads), 0, s, a);
-  else
-    hipLaunchKernelGGL(coarse_pair_post_kernel<1>, dim3(a.total), dim3(kPairThreads), 0, s, a);
-  STT_CHECK_LAUNCH("coarse_pair_post_kernel");
-  return STTOPT_OK;
+                     const int* active, hipStream_t s, int variant) {
+  PairLevel L{};
+  L.op = op;
+  L.rows = rows;
+  L.b = b;
+  L.sb = sb;
+  L.invd = invd;
+  L.sd = sd;
+  L.x2 = x2;
+  L.sx2 = sx2;
+  L.ec = ec;
+  L.sec = sec;
+  L.x4 = x4;
+  L.sx4 = sx4;
+  L.omega = omega;
+  L.active = active;
+  return coarse_pair_groups(&L, 1, false, variant, s);
 }
 
 }  // namespace stt

@@ -154,15 +154,40 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
 // to x2) and the restricted residual of x2 on the coarser level (written to bc, masked
 // by cmask).  post: x4 = two sweeps from x2 + P ec (written to x4, which must not alias
 // x2).  Same results as the separate first sweeps, residual, restriction, prolongation
-// and sweeps.
+// and sweeps.  variant: the tile schedule (0: 32 x 4 coarse / 64 x 8 fine nodes per
+// 512-thread workgroup; 1: 16 x 4 / 32 x 8 per 256 threads).
 int coarse_pair_pre(const OpDesc& op, int rows, const float* b, long long sb,
                     const float* invd, long long sd, float* x2, long long sx2,
                     const uint32_t* cmask, float* bc, long long sbc, double omega,
-                    const int* active, hipStream_t s);
+                    const int* active, hipStream_t s, int variant = 0);
 int coarse_pair_post(const OpDesc& op, int rows, const float* x2, long long sx2,
                      const float* ec, long long sec, const float* b, long long sb,
                      const float* invd, long long sd, float* x4, long long sx4, double omega,
-                     const int* active, hipStream_t s);
+                     const int* active, hipStream_t s, int variant = 0);
+
+// The same passes over one level of each of up to 8 hierarchies in one launch (the
+// uneven batch): lv[i] describes hierarchy i's level, its rows and vectors (pre: b,
+// invd, x2 out, cmask, bc out; post: x2, ec, b, invd, x4 out).
+struct PairLevel {
+  OpDesc op;
+  int rows;
+  const float* b;
+  long long sb;
+  const float* invd;
+  long long sd;
+  const float* x2;
+  long long sx2;
+  const uint32_t* cmask;
+  float* bc;
+  long long sbc;
+  const float* ec;
+  long long sec;
+  float* x4;
+  long long sx4;
+  double omega;
+  const int* active;
+};
+int coarse_pair_groups(const PairLevel* lv, int n, bool pre, int variant, hipStream_t s);
 
 // Copy one int from device memory on stream s and wait for it (pinned readback word).
 int read_device_int(const int* dev, hipStream_t s, int* out);

@@ -454,15 +454,17 @@ struct VCycle {
       // restriction in one pass, prolongation and post-smoothing in another
       const sttopt_level& C = mg->levels[l + 1];
       if (op[l].keff_node && L.kx == 2 && L.ky == 2 && ns == 2 && mg->gamma == 1 &&
-          L.mask != nullptr && (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0) {
+          L.mask != nullptr && (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0 &&
+          (L.flags & STTOPT_LEVEL_SEPARATE) == 0) {
         const long long nc = n[l + 1];
+        const int variant = (L.flags >> STTOPT_TILE_VARIANT_SHIFT) & 15;
         if (int e = coarse_pair_pre(op[l], rows, b, nl, invd(l), sd(l), x, nl, C.mask,
-                                    buf(C.vb), nc, mg->omega, active, s))
+                                    buf(C.vb), nc, mg->omega, active, s, variant))
           return e;
         T* ec = nullptr;
         if (int e = cycle(l + 1, buf(C.vb), &ec)) return e;
         if (int e = coarse_pair_post(op[l], rows, x, nl, ec, nc, b, nl, invd(l), sd(l), alt,
-                                     nl, mg->omega, active, s))
+                                     nl, mg->omega, active, s, variant))
           return e;
         *out = alt;
         return STTOPT_OK;

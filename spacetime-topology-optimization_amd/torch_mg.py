@@ -27,6 +27,8 @@ import collections
 import ctypes
 import dataclasses
 import functools
+import json
+import os
 import threading
 from dataclasses import dataclass, field
 
@@ -57,6 +59,24 @@ PARAM_MAX_CHILDREN = 4
 #: coarse element runs over its block, so very deep levels lack parallelism); deeper
 #: ones from the level above.
 GALERKIN_MAX_BLOCK = 64
+
+
+#: Time each node-stencil level's smoother schedule the first time a hierarchy shape is
+#: seen (`autotune`) and use the fastest: two passes per level with the wide or the
+#: narrow tile, or one pass per sweep.  All three give the same results.
+AUTOTUNE = True
+#: The schedules `autotune` picks between, by their flag values (sttopt_level.flags).
+SCHEDULES = {
+    "two_pass_wide": 0,
+    "two_pass_narrow": 1 << _lib.TILE_VARIANT_SHIFT,
+    "separate": _lib.LEVEL_SEPARATE,
+}
+#: Choices made so far: shape key -> {"levels": {level: schedule}, "us": {level:
+#: {schedule: microseconds per V-cycle}}}.  Written as JSON to the file named by the
+#: environment variable STTOPT_AUTOTUNE_FILE after each new choice, when it is set.
+AUTOTUNE_CHOICES: dict = {}
+#: A schedule name of `SCHEDULES` to use at every tunable level instead of timing.
+SCHEDULE_OVERRIDE: str | None = None
 
 
 #: Per-position Galerkin tables of the coarsening sequences built so far.  They depend
@@ -680,6 +700,12 @@ class VCycle:
         mg.scratch = scratch.data_ptr()
         work = torch.empty(_lib.load().sttopt_mgpcg_work_count(ctypes.byref(mg)),
                            dtype=torch.float64, device=dev)
+        if SCHEDULE_OVERRIDE is not None and not self.separate_sweeps:
+            for l in _tunable_levels(self):
+                mg.levels[l].flags = SCHEDULES[SCHEDULE_OVERRIDE]
+        elif AUTOTUNE and not self.separate_sweeps:
+            for l, flags in autotune(self, mg, rows).items():
+                mg.levels[l].flags = flags
         entry = (mg, keep, work)
         self._work[rows] = entry
         return entry
@@ -707,6 +733,80 @@ class VCycle:
         if apply_A is not self.levels[0] or b.dtype != torch.float64:
             return None
         return native_mgpcg(self, b, rtol=rtol, max_iter=max_iter, x0=x0, info=info)
+
+
+def _tunable_levels(vc: VCycle) -> list[int]:
+    """Levels whose smoother schedule has a choice: node-stencil levels of the fp32 cycle
+    coarsened 2 x 2 with two sweeps and one coarse visit."""
+    if vc.dtype != torch.float32 or vc.n_smooth != 2 or vc.gamma != 1:
+        return []
+    out = []
+    for l, lv in enumerate(vc.levels[:-1]):
+        if l == 0 or (lv.kx, lv.ky) != (2, 2):
+            continue
+        data = lv.vdata(torch.float32)
+        if data.get("keff_layout") == _lib.KEFF_NODE:
+            out.append(l)
+    return out
+
+
+def autotune(vc: VCycle, mg, rows: int, trials: int = 10, repeats: int = 3) -> dict[int, int]:
+    """The schedule flags of each tunable level of `vc` (`SCHEDULES`), chosen by timing
+    V-cycles on a seeded residual with HIP events the first time this hierarchy shape
+    and row count are seen: coarsest tunable level first, each with the coarser choices
+    already in place; per level the candidates take turns, `repeats` rounds of `trials`
+    cycles, and each keeps its fastest round.  Later hierarchies of the same shape reuse
+    the choice."""
+    levels = _tunable_levels(vc)
+    if not levels:
+        return {}
+    key = f"d{vc.levels[0].d}_rows{rows}_" + "_".join(f"{lv.nelx}x{lv.nely}" for lv in vc.levels)
+    hit = AUTOTUNE_CHOICES.get(key)
+    if hit is not None:
+        return {int(l): SCHEDULES[name] for l, name in hit["levels"].items()}
+    lv0 = vc.levels[0]
+    dev = lv0.mask.device
+    g = torch.Generator(device="cpu").manual_seed(0)
+    r = torch.randn(rows, lv0.ndof, generator=g, dtype=torch.float64).to(dev)
+    r = r * lv0.mask.to(torch.float64)
+    z = torch.empty_like(r)
+    lib = _lib.load()
+    stream = torch.cuda.current_stream(dev)
+
+    def cycles(n: int) -> None:
+        for _ in range(n):
+            _lib.check(lib.sttopt_vcycle(ctypes.byref(mg), r.data_ptr(), lv0.ndof, z.data_ptr(),
+                                         lv0.ndof, stream.cuda_stream), "sttopt_vcycle")
+
+    def time_cycles() -> float:
+        start = torch.cuda.Event(enable_timing=True)
+        end = torch.cuda.Event(enable_timing=True)
+        start.record(stream)
+        cycles(trials)
+        end.record(stream)
+        end.synchronize()
+        return start.elapsed_time(end) * 1e3 / trials
+
+    cycles(3)  # warm
+    chosen: dict[int, str] = {}
+    timings: dict[int, dict[str, float]] = {}
+    for l in reversed(levels):
+        us = {name: float("inf") for name in SCHEDULES}
+        for _ in range(repeats):
+            for name, flags in SCHEDULES.items():
+                mg.levels[l].flags = flags
+                us[name] = min(us[name], time_cycles())
+        best = min(us, key=us.get)
+        mg.levels[l].flags = SCHEDULES[best]
+        chosen[l] = best
+        timings[l] = us
+    AUTOTUNE_CHOICES[key] = {"levels": {str(l): n for l, n in sorted(chosen.items())},
+                             "us": {str(l): t for l, t in sorted(timings.items())}}
+    path = os.environ.get("STTOPT_AUTOTUNE_FILE")
+    if path:
+        with open(path, "w") as fh:
+            json.dump(AUTOTUNE_CHOICES, fh, indent=1, sort_keys=True)
+    return {l: SCHEDULES[n] for l, n in chosen.items()}
 
 
 def native_mgpcg(vcycle: VCycle, b: Tensor, *, rtol: float, max_iter: int,

@@ -559,7 +559,8 @@ def test_fp32_vcycle_node_stencil_coarse_levels_equal_full(dev):
                           (122, 26, False, None), (16, 12, True, 2), (4, 2, False, 2),
                           (64, 48, False, 20), (64, 48, True, 20), (272, 136, False, 300),
                           (264, 392, True, 300)])
-def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coarse):
+@pytest.mark.parametrize("schedule", ["two_pass_wide", "two_pass_narrow"])
+def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coarse, schedule):
     """The fp32 V-cycle's fused passes equal one pass per sweep bit for bit: level 0's
     two-sweep passes (pre: two sweeps and the residual; post: prolongation and two
     sweeps) on grids whose node counts fall on and next to the 62 x 14 output tiles
@@ -585,7 +586,12 @@ def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coars
     if reaction:
         assert levels[0].reaction_diag is not None
     r = torch_fem.project(s["t"](rng.standard_normal((3, s["ndof"]))), s["mask_t"])
-    z_pair = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    old = torch_mg.SCHEDULE_OVERRIDE
+    torch_mg.SCHEDULE_OVERRIDE = schedule  # every node-stencil level in two passes
+    try:
+        z_pair = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    finally:
+        torch_mg.SCHEDULE_OVERRIDE = old
     z_sep = torch_mg.VCycle(levels, dtype=torch.float32, separate_sweeps=True)(r)
     assert float(z_sep.abs().max()) > 0
     diff = float((z_pair - z_sep).abs().max())
@@ -612,10 +618,54 @@ def test_scalar_two_pass_coarse_levels_equal_separate_sweeps(dev):
                                       max_coarse_elements=24)
     assert len(levels) >= 4
     r = torch_fem.project(t(rng.standard_normal((2, nn))), torch.as_tensor(mask, device=dev))
-    z_pair = torch_mg.VCycle(levels, dtype=torch.float32)(r)
     z_sep = torch_mg.VCycle(levels, dtype=torch.float32, separate_sweeps=True)(r)
     assert float(z_sep.abs().max()) > 0
-    assert torch.equal(z_pair, z_sep)
+    old = torch_mg.SCHEDULE_OVERRIDE
+    try:
+        for schedule in ("two_pass_wide", "two_pass_narrow"):
+            torch_mg.SCHEDULE_OVERRIDE = schedule
+            z_pair = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+            assert torch.equal(z_pair, z_sep), schedule
+    finally:
+        torch_mg.SCHEDULE_OVERRIDE = old
+
+
+def test_autotuner_records_a_schedule_per_level_and_keeps_results(dev, tmp_path,
+                                                                  monkeypatch):
+    """The autotuner times the three schedules on each node-stencil level the first time
+    a hierarchy shape is seen, records its choice (also in the JSON file named by
+    STTOPT_AUTOTUNE_FILE), reuses it for the next hierarchy of that shape, and the cycle
+    with the chosen schedules equals the one-pass-per-sweep cycle bit for bit."""
+    import json
+
+    from sttopt_amd import torch_fem, torch_mg
+
+    nelx, nely = 128, 96
+    s = _setup(nelx, nely, dev)
+    rng = np.random.default_rng(31)
+    x = rng.uniform(0.05, 1, (2, nelx * nely))
+    levels = _hip_hierarchy(s, nelx, nely, x, max_coarse_elements=48)
+    path = tmp_path / "tune.json"
+    monkeypatch.setenv("STTOPT_AUTOTUNE_FILE", str(path))
+    monkeypatch.setattr(torch_mg, "AUTOTUNE_CHOICES", {})
+    monkeypatch.setattr(torch_mg, "AUTOTUNE", True)
+    monkeypatch.setattr(torch_mg, "SCHEDULE_OVERRIDE", None)
+    r = torch_fem.project(s["t"](rng.standard_normal((2, s["ndof"]))), s["mask_t"])
+    z = torch_mg.VCycle(levels, dtype=torch.float32)(r)
+    tunable = torch_mg._tunable_levels(torch_mg.VCycle(levels, dtype=torch.float32))
+    assert len(tunable) >= 2
+    (key, rec), = torch_mg.AUTOTUNE_CHOICES.items()
+    assert sorted(int(l) for l in rec["levels"]) == tunable
+    for l, name in rec["levels"].items():
+        assert name in torch_mg.SCHEDULES
+        assert set(rec["us"][l]) == set(torch_mg.SCHEDULES)
+        assert rec["us"][l][name] == min(rec["us"][l].values())
+    assert json.loads(path.read_text()) == torch_mg.AUTOTUNE_CHOICES
+    levels2 = _hip_hierarchy(s, nelx, nely, x[::-1].copy(), max_coarse_elements=48)
+    torch_mg.VCycle(levels2, dtype=torch.float32)(r)
+    assert list(torch_mg.AUTOTUNE_CHOICES) == [key]  # reused, not timed again
+    z_sep = torch_mg.VCycle(levels, dtype=torch.float32, separate_sweeps=True)(r)
+    assert torch.equal(z, z_sep)
 
 
 def test_mgpcg_lagged_convergence_check_any_interval(dev):
