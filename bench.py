@@ -43,6 +43,8 @@ def parse():
     ap.add_argument("--nstage", type=int, default=NSTAGE)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-sequential", action="store_true")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven-batch line (64 instances of four grids)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the PCG's second half-iteration kernel by kernel instead "
                          "of replaying its captured graph")
@@ -143,6 +145,14 @@ def main():
         print(f"# fp64 one-row-at-a-time: {seq['rows_per_s']:.2f} rows/s", file=sys.stderr,
               flush=True)
 
+    # -- the uneven batch: 64 instances of four grids x 2 load cases ----------------
+    uneven_line = None
+    if not args.no_uneven:
+        uneven_line = uneven_throughput(torch, dev)
+        print(f"# uneven batch: {uneven_line['solves_per_s']:.1f} solves+grads/s in one batched "
+              f"MG-PCG, {uneven_line['per_grid_calls_solves_per_s']:.1f} with one call per grid",
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(p, E)
@@ -178,6 +188,7 @@ def main():
         },
         "roofline": roof,
         "cpu_baseline": cpu,
+        "uneven_batch": uneven_line,
     }
     print(json.dumps(out), flush=True)
 
@@ -486,6 +497,85 @@ def apply_roofline(torch, _lib, torch_fem, p, rows, dev, live, launches=50):
         "traffic": pmc_traffic(),
         "algorithmic_bytes_per_launch": live_bytes / live_n,
         "avg_launch_us": live_sec / live_n * 1e6,
+    }
+
+
+def uneven_workload(torch, dev, n_instances=64):
+    """The uneven batch as two callables that solve it with compliance gradients:
+    `batched(info=None)` (one batched MG-PCG, `uneven.solve_instances`) and `per_grid()`
+    (one `femsolve` per grid)."""
+    from sttopt_amd import compliance, fem, torch_fem, torch_solve, uneven, workloads
+
+    specs = workloads.uneven_batch(n_instances, seed=0)
+    KE = torch.as_tensor(fem.plane_stress_KE(0.3), dtype=torch.float64, device=dev)
+    base = []
+    for s in specs:
+        mask = torch.zeros(s["loads"].shape[1], dtype=torch.bool, device=dev)
+        mask[torch.as_tensor(s["freedofs"], device=dev)] = True
+        edof = torch.as_tensor(fem.element_dof_map(s["nelx"], s["nely"]), device=dev)
+        base.append((s, mask, edof, torch.as_tensor(s["loads"], dtype=torch.float64, device=dev)))
+
+    def batched(info=None):
+        xs, insts = [], []
+        for s, mask, edof, F in base:
+            x = torch.as_tensor(s["x"], dtype=torch.float64, device=dev).requires_grad_(True)
+            xs.append(x)
+            insts.append(uneven.Instance(s["nelx"], s["nely"],
+                                         torch_fem.simp_density(x.flatten(), 1e-9, 1.0, 3.0),
+                                         F, mask, edof, KE))
+        Us = uneven.solve_instances(insts, info=info)
+        total = 0
+        for inst, U in zip(insts, Us):
+            total = total + torch.sum(inst.density * compliance._StrainEnergy.apply(
+                U, KE, inst.nelx, inst.nely))
+        total.backward()
+
+    def per_grid():
+        grids = {}
+        for s, mask, edof, F in base:
+            grids.setdefault((s["nelx"], s["nely"]), []).append((s, mask, edof, F))
+        for (nelx, nely), items in grids.items():
+            x = torch.stack([torch.as_tensor(s["x"], dtype=torch.float64, device=dev).flatten()
+                             for s, _, _, _ in items]).requires_grad_(True)
+            dens = torch_fem.simp_density(x, 1e-9, 1.0, 3.0).repeat_interleave(2, dim=0)
+            F = torch.cat([f for _, _, _, f in items])
+            U = torch_solve.femsolve(dens, F, items[0][2], KE, items[0][1], nelx, nely)
+            ce = compliance._StrainEnergy.apply(U, KE, nelx, nely)
+            torch.sum(dens * ce).backward()
+
+    return batched, per_grid
+
+
+def uneven_throughput(torch, dev, n_instances=64, reps=3):
+    """Solves with compliance gradients per second on the uneven batch: `n_instances`
+    cantilever instances with grids drawn from {64x32, 128x64, 256x128, 512x256}, two
+    load cases each (a top-edge traction and a seeded point load), seeded smooth densities
+    at volume fraction 0.5; rtol 1e-8, fp32 V-cycles.  Timed: the whole batch in one
+    batched MG-PCG (`uneven.solve_instances`) with its batched adjoint, against the same
+    solves as one `femsolve` per grid (one grid per call, as the reference batches)."""
+    batched, per_grid = uneven_workload(torch, dev, n_instances)
+    out = {}
+    for name, fn in (("batched", batched), ("per_grid", per_grid)):
+        fn()  # warm (hierarchy shapes, autotuned schedules)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        out[name] = (time.perf_counter() - t0) / reps
+    info = {}
+    batched(info)
+    solves = 2 * n_instances
+    return {
+        "workload": f"{n_instances} cantilever instances, grids 64x32..512x256, 2 load cases "
+                    "each, rtol 1e-8, fp32 V-cycle, compliance gradients",
+        "groups": info.get("groups"),
+        "solves": solves,
+        "solves_per_s": solves / out["batched"],
+        "ms_per_batch": 1e3 * out["batched"],
+        "per_grid_calls_solves_per_s": solves / out["per_grid"],
+        "forward_iterations": info.get("forward_n_iter"),
+        "adjoint_iterations": info.get("backward_n_iter"),
     }
 
 

@@ -299,6 +299,35 @@ int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double*
                  int32_t check_every, int32_t cold_start, double* work, int32_t* iters,
                  double* resid, int32_t* n_iter, void* stream);
 
+/* One hierarchy of an uneven batch for sttopt_mgpcg_batch: the hierarchy (mg, with its
+ * rows), right-hand sides b, solutions x (warm start on entry unless cold_start), the
+ * sttopt_mgpcg_work_count(mg) doubles of work, and per-row iteration counts and final
+ * relative residuals. */
+typedef struct sttopt_batch_group {
+  const sttopt_mg* mg;
+  const double* b;
+  int64_t b_stride;
+  double* x;
+  int64_t x_stride;
+  double* work;
+  int32_t* iters;
+  double* resid;
+} sttopt_batch_group;
+
+/* One batched MG-PCG over hierarchies of different grids (instances of uneven size): the
+ * solve of sttopt_mgpcg for every group at once, each pass of an iteration one launch
+ * over all groups (up to 4 per launch), each group with its own workgroups and
+ * arguments, rows retiring on their own.  Every group needs the fp32 two-sweep cycle
+ * (vdtype F32, n_smooth 2, gamma 1, flexible beta) coarsened 2 x 2 down to its
+ * coarsest level, level 1 density-parameterized and deeper levels node stencils
+ * (STTOPT_EINVAL otherwise), and the same dofs per node and omega.  Results equal
+ * sttopt_mgpcg on each group alone.  n_iter: the slowest row's count over all groups;
+ * STTOPT_ENOCONV as sttopt_mgpcg.  No counterpart in the reference, which solves one
+ * grid per call (torch_fem.py:62-83 takes one element dof map). */
+int sttopt_mgpcg_batch(const sttopt_batch_group* groups, int32_t ngroups, double rtol,
+                       int32_t max_iter, int32_t check_every, int32_t cold_start,
+                       int32_t* n_iter, void* stream);
+
 /* Whether sttopt_mgpcg replays the second half of each iteration (the V-cycle's coarse
  * visit and post-smoothing, the beta finalizer and the direction update: the same
  * launches every iteration) from a HIP graph captured once per solve (enable 1, the

@@ -166,6 +166,7 @@ _SIGNATURES = {
          _P, _P, _P],
     ),
     "sttopt_mgpcg_work_count": (c_int64, [_P]),
+    "sttopt_mgpcg_batch": (c_int32, [_P, c_int32, c_double, c_int32, c_int32, c_int32, _P, _P]),
     "sttopt_adjoint_grad": (
         c_int32,
         [c_int32, c_int32, c_int32, c_int32, _P, c_int64, _P, c_int64, _P, _P, c_int64, _P,

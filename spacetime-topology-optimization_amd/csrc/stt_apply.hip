@@ -296,7 +296,8 @@ constexpr int kHaloX = kTileX + 2, kHaloY = kTileY + 2;
 constexpr int kElemX = kTileX + 1, kElemY = kTileY + 1;
 
 template <typename T, typename TB, typename TY, int D, int SRC, int MODE, int DOT>
-__global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB, TY> a) {
+__device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& a, const int bid,
+                                                  const int karg_off) {
   constexpr int K = 4 * D;
   constexpr int NR = kRowsPerThread;
   // SRC kSrcDensity: level 0 (density * KE).  kSrcParam: level 1 at 2 x 2 coarsening,
@@ -318,7 +319,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   constexpr int kMW = kTileX * D / 32 + 1;
   constexpr bool kOutMask = MODE != kMatvec;
   __shared__ uint32_t sMW[kOutMask ? kTileY : 1][kOutMask ? kMW : 1];
-  const int t = xcd_remap(blockIdx.x, a.total);
+  const int t = xcd_remap(bid, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
@@ -557,7 +558,7 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   using ConstPtr = const __attribute__((address_space(4))) T*;
   const ConstPtr kargs_ke = reinterpret_cast<ConstPtr>(
       (__attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
-      offsetof(Args, ke));
+      karg_off + offsetof(Args, ke));  // karg_off: where `a` sits in the argument segment
   // level 1: the per-position tables, read the same way (constant address space, one
   // element position's 4 x (D x 4D) rows live at a time)
   const ConstPtr ptab_c =
@@ -719,6 +720,30 @@ __global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB,
   }
 }
 
+template <typename T, typename TB, typename TY, int D, int SRC, int MODE, int DOT>
+__global__ __launch_bounds__(kBlock) void stencil_tile_kernel(StencilArgs<T, TB, TY> a) {
+  stencil_tile_body<T, TB, TY, D, SRC, MODE, DOT>(a, blockIdx.x, 0);
+}
+
+// The same tile pass over the level of several hierarchies in one launch (the uneven
+// batch): hierarchy i takes workgroups first[i] .. first[i + 1] - 1.
+template <typename T, typename TB, typename TY>
+struct StencilGroups {
+  int n;
+  int first[kMaxBatchGroups + 1];
+  StencilArgs<T, TB, TY> a[kMaxBatchGroups];
+};
+
+template <typename T, typename TB, typename TY, int D, int SRC, int MODE, int DOT>
+__global__ __launch_bounds__(kBlock) void stencil_tile_groups_kernel(StencilGroups<T, TB, TY> g) {
+  int i = 0;
+  while (i + 1 < g.n && (int)blockIdx.x >= g.first[i + 1]) ++i;
+  using G = StencilGroups<T, TB, TY>;
+  stencil_tile_body<T, TB, TY, D, SRC, MODE, DOT>(
+      g.a[i], blockIdx.x - g.first[i],
+      (int)(offsetof(G, a) + i * sizeof(StencilArgs<T, TB, TY>)));
+}
+
 template <typename T, typename TB, typename TY, int D, int SRC>
 static int launch_tile(const StencilArgs<T, TB, TY>& a, dim3 grid, hipStream_t s) {
   const dim3 block(kBlock);
@@ -751,13 +776,14 @@ static int launch_tile(const StencilArgs<T, TB, TY>& a, dim3 grid, hipStream_t s
   return STTOPT_OK;
 }
 
+// The kernel arguments of one stencil pass, and its source kind and tile flag.
 template <typename T, typename TB, typename TY>
-int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
-            long long su, const TB* b, long long sb, const T* invd, long long sd,
-            TY* y, long long sy, double omega, const int* active, double* partial,
-            hipStream_t s, T* bcopy, long long sbc) {
-  if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
-  StencilArgs<T, TB, TY> a;
+static int stencil_args(const OpDesc& op, int rows, int mode, bool proj_in, int dot,
+                        const T* u, long long su, const TB* b, long long sb, const T* invd,
+                        long long sd, TY* y, long long sy, double omega, const int* active,
+                        double* partial, T* bcopy, long long sbc, StencilArgs<T, TB, TY>& a,
+                        int& src_out, bool& param_tile_out) {
+  a = StencilArgs<T, TB, TY>{};
   a.nelx = op.nelx;
   a.nely = op.nely;
   a.nnx = op.nnx();
@@ -835,6 +861,25 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     set_error("stencil: parameterized level needs the fine density and a matching block");
     return STTOPT_EINVAL;
   }
+  src_out = src;
+  param_tile_out = param_tile;
+  return STTOPT_OK;
+}
+
+template <typename T, typename TB, typename TY>
+int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T* u,
+            long long su, const TB* b, long long sb, const T* invd, long long sd,
+            TY* y, long long sy, double omega, const int* active, double* partial,
+            hipStream_t s, T* bcopy, long long sbc) {
+  if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
+  StencilArgs<T, TB, TY> a;
+  int src = 0;
+  bool param_tile = false;
+  if (int e = stencil_args<T, TB, TY>(op, rows, mode, proj_in, dot, u, su, b, sb, invd, sd, y,
+                                      sy, omega, active, partial, bcopy, sbc, a, src,
+                                      param_tile))
+    return e;
+  const bool p22 = op.pkx == 2 && op.pky == 2;
   dim3 grid((unsigned)a.total), block(kBlock);
 #define STT_STENCIL(D, SRC) \
   hipLaunchKernelGGL((stencil_kernel<T, TB, TY, D, SRC>), grid, block, 0, s, a)
@@ -878,6 +923,65 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
   STT_CHECK_LAUNCH("stencil_kernel");
   return STTOPT_OK;
 }
+
+template <typename T>
+int stencil_tile_groups(const StencilCall<T>* calls, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchGroups) {
+    set_error("stencil groups: 1 to 4 hierarchies per launch");
+    return STTOPT_EINVAL;
+  }
+  StencilGroups<T, T, T> g{};
+  int total = 0, src = -1, d = 0, mode = -1, dot = -1;
+  for (int i = 0; i < n; ++i) {
+    const StencilCall<T>& c = calls[i];
+    if (c.rows <= 0) continue;
+    StencilArgs<T, T, T> a;
+    int csrc = 0;
+    bool tile = false;
+    if (int e = stencil_args<T, T, T>(c.op, c.rows, c.mode, false, c.dot, c.u, c.su, c.b, c.sb,
+                                      c.invd, c.sd, c.y, c.sy, c.omega, c.active, c.partial,
+                                      nullptr, 0, a, csrc, tile))
+      return e;
+    if (!(csrc == kSrcDensity || tile) ||
+        (src >= 0 && (csrc != src || c.op.d != d || c.mode != mode || c.dot != dot))) {
+      set_error("stencil groups: one tile pass (same source, mode and dofs per node) per "
+                "launch");
+      return STTOPT_EINVAL;
+    }
+    src = csrc;
+    d = c.op.d;
+    mode = c.mode;
+    dot = c.dot;
+    g.a[g.n] = a;
+    g.first[g.n] = total;
+    total += a.total;
+    ++g.n;
+  }
+  g.first[g.n] = total;
+  if (total == 0) return STTOPT_OK;
+  const dim3 grid(total), block(kBlock);
+#define STT_G(D, SRC, MODE, DOT) \
+  hipLaunchKernelGGL((stencil_tile_groups_kernel<T, T, T, D, SRC, MODE, DOT>), grid, block, 0, s, g)
+  const int key = (src == kSrcDensity ? 0 : 1) * 1000 + d * 100 + mode * 10 + dot;
+  switch (key) {
+    case 0 * 1000 + 2 * 100 + kOperator * 10 + kDotU: STT_G(2, kSrcDensity, kOperator, kDotU); break;
+    case 0 * 1000 + 1 * 100 + kOperator * 10 + kDotU: STT_G(1, kSrcDensity, kOperator, kDotU); break;
+    case 1 * 1000 + 2 * 100 + kSmoothFirst * 10 + kNoDot: STT_G(2, kSrcParam, kSmoothFirst, kNoDot); break;
+    case 1 * 1000 + 2 * 100 + kResid * 10 + kNoDot: STT_G(2, kSrcParam, kResid, kNoDot); break;
+    case 1 * 1000 + 2 * 100 + kSmooth * 10 + kNoDot: STT_G(2, kSrcParam, kSmooth, kNoDot); break;
+    case 1 * 1000 + 1 * 100 + kSmoothFirst * 10 + kNoDot: STT_G(1, kSrcParam, kSmoothFirst, kNoDot); break;
+    case 1 * 1000 + 1 * 100 + kResid * 10 + kNoDot: STT_G(1, kSrcParam, kResid, kNoDot); break;
+    case 1 * 1000 + 1 * 100 + kSmooth * 10 + kNoDot: STT_G(1, kSrcParam, kSmooth, kNoDot); break;
+    default:
+      set_error("stencil groups: unsupported pass");
+      return STTOPT_EINVAL;
+  }
+#undef STT_G
+  STT_CHECK_LAUNCH("stencil_tile_groups_kernel");
+  return STTOPT_OK;
+}
+template int stencil_tile_groups<double>(const StencilCall<double>*, int, hipStream_t);
+template int stencil_tile_groups<float>(const StencilCall<float>*, int, hipStream_t);
 
 #define STT_STENCIL_INST(T, TB, TY)                                                      \
   template int stencil<T, TB, TY>(const OpDesc&, int, int, bool, int, const T*, long long,  \

@@ -87,6 +87,33 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
             TY* y, long long sy, double omega, const int* active, double* partial,
             hipStream_t s, T* bcopy = nullptr, long long sbc = 0);
 
+// ---- one launch over the same level of several hierarchies (the uneven batch) ----
+// Each kernel below runs the same pass on up to kMaxBatchGroups hierarchies of different
+// grids in one launch: hierarchy i takes a contiguous range of workgroups, with its own
+// arguments; the arithmetic is that of the single-hierarchy pass.
+constexpr int kMaxBatchGroups = 4;
+
+// A tile stencil pass (level 0 with density * KE, or the parameterized 2 x 2 level 1 in
+// fp32): the arguments of `stencil` with TB = TY = T and no input projection.
+template <typename T>
+struct StencilCall {
+  OpDesc op;
+  int rows, mode, dot;
+  const T* u;
+  long long su;
+  const T* b;
+  long long sb;
+  const T* invd;
+  long long sd;
+  T* y;
+  long long sy;
+  double omega;
+  const int* active;
+  double* partial;
+};
+template <typename T>
+int stencil_tile_groups(const StencilCall<T>* calls, int n, hipStream_t s);
+
 // Node-centric diagonal of an operator (+ rdiag).  out_mode: 0 raw diagonal,
 // 1 diagonal with 1 at fixed dofs, 2 inverse at free dofs and 0 at fixed.
 template <typename TO>
@@ -188,6 +215,84 @@ struct PairLevel {
   const int* active;
 };
 int coarse_pair_groups(const PairLevel* lv, int n, bool pre, int variant, hipStream_t s);
+
+// The level-0 passes of several hierarchies in one launch (arguments of smooth_pair_pre
+// and smooth_pair_post; all calls of a launch with the same dofs per node, reaction
+// diagonal presence and, for pre, residual update presence).
+struct PairPreCall {
+  OpDesc op;
+  int rows;
+  const double* b;
+  long long sb;
+  const float* invd;
+  long long sd;
+  float* x;
+  long long sx;
+  float* r;
+  long long sr;
+  double omega;
+  const int* active;
+  const double* ap;
+  long long sap;
+  const double* alpha;
+  double* rnew;
+  long long srn;
+  double* rr;
+  const float* zold;
+  long long szo;
+  double* rzo;
+};
+template <typename ZT>
+struct PairPostCall {
+  OpDesc op;
+  int rows;
+  const float* x;
+  long long sx;
+  const float* ec;
+  long long sec;
+  int cnnx;
+  const double* b;
+  long long sb;
+  const float* invd;
+  long long sd;
+  ZT* z;
+  long long sz;
+  double omega;
+  const int* active;
+  double* partial;
+};
+int smooth_pair_pre_groups(const PairPreCall* calls, int n, hipStream_t s);
+int smooth_pair_post_groups(const PairPostCall<float>* calls, int n, hipStream_t s);
+
+// 2 x 2 grid transfers of several hierarchies in one launch (fp32): restriction
+// coarse = P_c (R src) with mask the coarse free dofs, or prolongation dst += P_f (P src)
+// with mask the fine free dofs; cnelx x cnely is the coarse grid.
+struct TransferCall {
+  int d, cnelx, cnely, rows;
+  const float* src;
+  long long ss;
+  const uint32_t* mask;
+  float* dst;
+  long long sd;
+  const int* active;
+};
+int transfer22_groups(const TransferCall* calls, int n, bool restrict_, hipStream_t s);
+
+// The coarsest dense solves x = P (cinv b) of several hierarchies in one launch (fp32
+// vectors, even n).
+struct CoarseSolveCall {
+  long long n;
+  int rows;
+  const double* cinv;
+  long long scinv;
+  const float* b;
+  long long sb;
+  const uint32_t* mask;
+  float* x;
+  long long sx;
+  const int* active;
+};
+int coarse_solve_groups(const CoarseSolveCall* calls, int n, hipStream_t s);
 
 // Copy one int from device memory on stream s and wait for it (pinned readback word).
 int read_device_int(const int* dev, hipStream_t s, int* out);

@@ -239,18 +239,17 @@ __device__ __forceinline__ T lane_from_left(T v) {
 }
 
 template <typename T, int D>
-__global__ __launch_bounds__(256) void restrict22_kernel(int cnelx, int cnely,
-                                                         const T* __restrict__ fine,
-                                                         long long sf,
-                                                         const uint32_t* __restrict__ cmask,
-                                                         T* __restrict__ coarse, long long sc,
-                                                         const int* __restrict__ active) {
-  const int row = blockIdx.y;
+__device__ __forceinline__ void restrict22_body(int cnelx, int cnely, const T* __restrict__ fine,
+                                                long long sf,
+                                                const uint32_t* __restrict__ cmask,
+                                                T* __restrict__ coarse, long long sc,
+                                                const int* __restrict__ active, const int bx,
+                                                const int row) {
   if (active != nullptr && active[row] == 0) return;  // whole block
   const int cnnx = cnelx + 1, cnny = cnely + 1;
   const int fnnx = 2 * cnelx + 1, fnny = 2 * cnely + 1;
   const long long cn = (long long)cnnx * cnny;
-  const long long cnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long cnode = (long long)bx * blockDim.x + threadIdx.x;
   const bool valid = cnode < cn;  // every lane takes part in the shifts
   const long long cnc = valid ? cnode : cn - 1;
   const int I = (int)(cnc / cnnx), J = (int)(cnc - (long long)I * cnnx);
@@ -305,18 +304,17 @@ __global__ __launch_bounds__(256) void restrict22_kernel(int cnelx, int cnely,
 }
 
 template <typename T, int D, int KX, int KY>
-__global__ __launch_bounds__(256) void prolong_add_kernel(int cnelx, int cnely, int kx_,
-                                                          int ky_, const T* __restrict__ coarse,
-                                                          long long sc,
-                                                          const uint32_t* __restrict__ fmask,
-                                                          T* __restrict__ fine, long long sf,
-                                                          const int* __restrict__ active) {
-  const int row = blockIdx.y;
+__device__ __forceinline__ void prolong_add_body(int cnelx, int cnely, int kx_, int ky_,
+                                                 const T* __restrict__ coarse, long long sc,
+                                                 const uint32_t* __restrict__ fmask,
+                                                 T* __restrict__ fine, long long sf,
+                                                 const int* __restrict__ active, const int bx,
+                                                 const int row) {
   if (active != nullptr && active[row] == 0) return;
   const int kx = KX > 0 ? KX : kx_, ky = KY > 0 ? KY : ky_;
   const int cnnx = cnelx + 1;
   const int fnnx = cnelx * kx + 1, fnny = cnely * ky + 1;
-  const long long fnode = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long fnode = (long long)bx * blockDim.x + threadIdx.x;
   if (fnode >= (long long)fnnx * fnny) return;
   const uint32_t word = fmask != nullptr ? fmask[(fnode * D) >> 5] : ~0u;
   const uint32_t bits = (word >> ((fnode * D) & 31)) & ((1u << D) - 1u);
@@ -344,6 +342,100 @@ __global__ __launch_bounds__(256) void prolong_add_kernel(int cnelx, int cnely, 
     if ((bits >> k) & 1u) v.v[k] += wy0 * top + wy1 * bot;
   }
   store_node<T, D>(fr, fnode, v);
+}
+
+template <typename T, int D>
+__global__ __launch_bounds__(256) void restrict22_kernel(int cnelx, int cnely,
+                                                         const T* __restrict__ fine,
+                                                         long long sf,
+                                                         const uint32_t* __restrict__ cmask,
+                                                         T* __restrict__ coarse, long long sc,
+                                                         const int* __restrict__ active) {
+  restrict22_body<T, D>(cnelx, cnely, fine, sf, cmask, coarse, sc, active, blockIdx.x,
+                        blockIdx.y);
+}
+
+template <typename T, int D, int KX, int KY>
+__global__ __launch_bounds__(256) void prolong_add_kernel(int cnelx, int cnely, int kx_,
+                                                          int ky_, const T* __restrict__ coarse,
+                                                          long long sc,
+                                                          const uint32_t* __restrict__ fmask,
+                                                          T* __restrict__ fine, long long sf,
+                                                          const int* __restrict__ active) {
+  prolong_add_body<T, D, KX, KY>(cnelx, cnely, kx_, ky_, coarse, sc, fmask, fine, sf, active,
+                                 blockIdx.x, blockIdx.y);
+}
+
+// 2 x 2 transfers of several hierarchies in one launch: hierarchy i takes workgroups
+// first[i] .. first[i + 1] - 1, bx_count per row.
+struct TransferGroups {
+  int n;
+  int first[kMaxBatchGroups + 1];
+  TransferCall a[kMaxBatchGroups];
+  int bx_count[kMaxBatchGroups];
+};
+
+__device__ __forceinline__ int transfer_group(const int* first, int n, int b) {
+  int i = 0;
+  while (i + 1 < n && b >= first[i + 1]) ++i;
+  return i;
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void restrict22_groups_kernel(TransferGroups g) {
+  const int i = transfer_group(g.first, g.n, blockIdx.x);
+  const int b = blockIdx.x - g.first[i], bx = b % g.bx_count[i], row = b / g.bx_count[i];
+  const TransferCall& c = g.a[i];
+  restrict22_body<float, D>(c.cnelx, c.cnely, c.src, c.ss, c.mask, c.dst, c.sd, c.active, bx,
+                            row);
+}
+
+template <int D>
+__global__ __launch_bounds__(256) void prolong22_groups_kernel(TransferGroups g) {
+  const int i = transfer_group(g.first, g.n, blockIdx.x);
+  const int b = blockIdx.x - g.first[i], bx = b % g.bx_count[i], row = b / g.bx_count[i];
+  const TransferCall& c = g.a[i];
+  prolong_add_body<float, D, 2, 2>(c.cnelx, c.cnely, 2, 2, c.src, c.ss, c.mask, c.dst, c.sd,
+                                   c.active, bx, row);
+}
+
+int transfer22_groups(const TransferCall* calls, int n, bool restrict_, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchGroups) {
+    set_error("transfer groups: 1 to 4 hierarchies per launch");
+    return STTOPT_EINVAL;
+  }
+  TransferGroups g{};
+  int total = 0, d = 0;
+  for (int i = 0; i < n; ++i) {
+    const TransferCall& c = calls[i];
+    if (c.rows <= 0) continue;
+    if ((c.d != 1 && c.d != 2) || (g.n > 0 && c.d != d) ||
+        (c.d == 2 && !(pair_aligned(c.src, c.ss, sizeof(float)) &&
+                       pair_aligned(c.dst, c.sd, sizeof(float))))) {
+      set_error("transfer groups: one dofs-per-node per launch, node-aligned vectors");
+      return STTOPT_EINVAL;
+    }
+    d = c.d;
+    // restriction: one thread per coarse node; prolongation: per fine node
+    const long long nodes = restrict_ ? (long long)(c.cnelx + 1) * (c.cnely + 1)
+                                      : (long long)(2 * c.cnelx + 1) * (2 * c.cnely + 1);
+    g.a[g.n] = c;
+    g.bx_count[g.n] = ceil_div(nodes, 256);
+    g.first[g.n] = total;
+    total += g.bx_count[g.n] * c.rows;
+    ++g.n;
+  }
+  g.first[g.n] = total;
+  if (total == 0) return STTOPT_OK;
+  if (restrict_) {
+    if (d == 2) hipLaunchKernelGGL(restrict22_groups_kernel<2>, dim3(total), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL(restrict22_groups_kernel<1>, dim3(total), dim3(256), 0, s, g);
+  } else {
+    if (d == 2) hipLaunchKernelGGL(prolong22_groups_kernel<2>, dim3(total), dim3(256), 0, s, g);
+    else hipLaunchKernelGGL(prolong22_groups_kernel<1>, dim3(total), dim3(256), 0, s, g);
+  }
+  STT_CHECK_LAUNCH("transfer22_groups_kernel");
+  return STTOPT_OK;
 }
 
 template <typename T>
@@ -424,14 +516,13 @@ template int prolong_add<float>(int, int, int, int, int, int, const float*, long
 // inverse (coalesced) and reduce in fp64.
 
 template <typename T, bool PAIRS>
-__global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const double* cinv,
-                                                           long long scinv, const T* b,
-                                                           long long sb, const uint32_t* mask,
-                                                           T* x, long long sx,
-                                                           const int* active) {
-  const int row = blockIdx.y;
+__device__ __forceinline__ void coarse_solve_body(long long n, const double* cinv,
+                                                  long long scinv, const T* b, long long sb,
+                                                  const uint32_t* mask, T* x, long long sx,
+                                                  const int* active, const int bx,
+                                                  const int row) {
   if (active != nullptr && active[row] == 0) return;
-  const long long i = (long long)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const long long i = (long long)bx * (blockDim.x / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x & (kWave - 1);
   if (i >= n) return;  // uniform per wave
   const double* Ai = cinv + row * scinv + i * n;
@@ -452,6 +543,62 @@ __global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const do
   }
   acc = wave_sum(acc);
   if (lane == 0) x[row * sx + i] = mask_free(mask, i) ? static_cast<T>(acc) : T(0);
+}
+
+template <typename T, bool PAIRS>
+__global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const double* cinv,
+                                                           long long scinv, const T* b,
+                                                           long long sb, const uint32_t* mask,
+                                                           T* x, long long sx,
+                                                           const int* active) {
+  coarse_solve_body<T, PAIRS>(n, cinv, scinv, b, sb, mask, x, sx, active, blockIdx.x,
+                              blockIdx.y);
+}
+
+// The coarsest solves of several hierarchies in one launch (each at its own coarsest
+// level): hierarchy i takes workgroups first[i] .. first[i + 1] - 1, bx_count per row.
+struct CoarseSolveGroups {
+  int n;
+  int first[kMaxBatchGroups + 1];
+  CoarseSolveCall a[kMaxBatchGroups];
+  int bx_count[kMaxBatchGroups];
+};
+
+__global__ __launch_bounds__(256) void coarse_solve_groups_kernel(CoarseSolveGroups g) {
+  int i = 0;
+  while (i + 1 < g.n && (int)blockIdx.x >= g.first[i + 1]) ++i;
+  const int b = blockIdx.x - g.first[i], bx = b % g.bx_count[i], row = b / g.bx_count[i];
+  const CoarseSolveCall& c = g.a[i];
+  coarse_solve_body<float, true>(c.n, c.cinv, c.scinv, c.b, c.sb, c.mask, c.x, c.sx, c.active,
+                                 bx, row);
+}
+
+int coarse_solve_groups(const CoarseSolveCall* calls, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchGroups) {
+    set_error("coarse solve groups: 1 to 4 hierarchies per launch");
+    return STTOPT_EINVAL;
+  }
+  CoarseSolveGroups g{};
+  int total = 0;
+  for (int i = 0; i < n; ++i) {
+    const CoarseSolveCall& c = calls[i];
+    if (c.rows <= 0 || c.n <= 0) continue;
+    if (c.cinv == nullptr || c.n % 2 != 0 || c.scinv % 2 != 0 ||
+        (reinterpret_cast<uintptr_t>(c.cinv) & 15) != 0) {
+      set_error("coarse solve groups: even-sized inverses with 16-byte aligned rows");
+      return STTOPT_EINVAL;
+    }
+    g.a[g.n] = c;
+    g.bx_count[g.n] = ceil_div(c.n, 4);
+    g.first[g.n] = total;
+    total += g.bx_count[g.n] * c.rows;
+    ++g.n;
+  }
+  g.first[g.n] = total;
+  if (total == 0) return STTOPT_OK;
+  hipLaunchKernelGGL(coarse_solve_groups_kernel, dim3(total), dim3(256), 0, s, g);
+  STT_CHECK_LAUNCH("coarse_solve_groups_kernel");
+  return STTOPT_OK;
 }
 
 template <typename T>

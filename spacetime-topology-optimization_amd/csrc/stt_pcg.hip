@@ -11,6 +11,10 @@
 #include <utility>
 #include <vector>
 
+#include <algorithm>
+#include <memory>
+#include <vector>
+
 #include "stt_kernels.h"
 
 namespace stt {
@@ -1098,6 +1102,670 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   return STTOPT_OK;
 }
 
+// ------------------------------------------------------------ uneven batch ----
+// One batched MG-PCG over hierarchies of different grids (sttopt_mgpcg_batch): every
+// pass of an iteration is one launch over all hierarchies (kMaxBatchGroups per launch),
+// each hierarchy taking its own range of workgroups with its own arguments; the V-cycle
+// visits level l of every hierarchy that has it in the same launch, and the coarsest
+// solves of all hierarchies (at their own depths) in one launch.  Rows converge and
+// retire per row, as in sttopt_mgpcg, whose arithmetic every pass repeats.
+
+// Finalizers and the direction update over the rows of several hierarchies: workgroup
+// b is row b - first[i] of hierarchy i.
+struct FinCall {
+  int tiles, tiles_o, flexible, k;
+  const double* p0;  // alpha: p.Ap; resid: r.r; beta: r.z
+  const double* p1;  // beta: r.z_old
+  const double* bsafe;
+  const int* active_c;
+  int* active;
+  double *rz, *alpha, *beta, *resid;
+  int *iters, *counter, *kdev;
+};
+struct FinGroups {
+  int n;
+  int first[kMaxBatchGroups + 1];
+  FinCall a[kMaxBatchGroups];
+};
+
+__device__ __forceinline__ int fin_group(const FinGroups& g, int b) {
+  int i = 0;
+  while (i + 1 < g.n && b >= g.first[i + 1]) ++i;
+  return i;
+}
+
+__global__ __launch_bounds__(kBlock) void batch_alpha_kernel(FinGroups g) {
+  __shared__ double red[kBlock / kWave];
+  const int i = fin_group(g, blockIdx.x);
+  const FinCall& c = g.a[i];
+  const int row = blockIdx.x - g.first[i];
+  if (row == 0 && threadIdx.x == 0) c.kdev[0] += 1;
+  if (c.active[row] == 0) return;
+  const double pap = row_sum(c.p0, c.tiles, row, red);
+  if (threadIdx.x == 0) c.alpha[row] = safe_div(c.rz[row], pap);
+}
+
+__global__ __launch_bounds__(kBlock) void batch_resid_kernel(FinGroups g, double rtol,
+                                                             int* host_flag, int check_id) {
+  __shared__ double red[kBlock / kWave];
+  const int i = fin_group(g, blockIdx.x);
+  const FinCall& c = g.a[i];
+  const int row = blockIdx.x - g.first[i];
+  if (c.active[row] == 0) return;
+  const double rr = row_sum(c.p0, c.tiles, row, red);
+  if (threadIdx.x == 0) {
+    const double rel = sqrt(rr) / c.bsafe[row];
+    c.resid[row] = rel;
+    c.iters[row] = c.k;
+    if (rel <= rtol) {
+      c.active[row] = 0;
+    } else {
+      atomicAdd(c.counter, 1);
+      if (host_flag != nullptr)
+        __hip_atomic_store(host_flag, check_id, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void batch_beta_kernel(FinGroups g) {
+  __shared__ double red[kBlock / kWave];
+  const int i = fin_group(g, blockIdx.x);
+  const FinCall& c = g.a[i];
+  const int row = blockIdx.x - g.first[i];
+  if (row == 0 && threadIdx.x == 0) *c.counter = 0;
+  if (c.active[row] == 0) return;
+  const double rzn = row_sum(c.p0, c.tiles, row, red);
+  double rzo = 0.0;
+  if (c.flexible) {
+    __syncthreads();
+    rzo = row_sum(c.p1, c.tiles_o, row, red);
+  }
+  if (threadIdx.x == 0) {
+    c.beta[row] = safe_div(rzn - rzo, c.rz[row]);
+    c.rz[row] = rzn;
+  }
+}
+
+// x += alpha p and p = z + beta p for the rows that took iteration *kdev (the body of
+// pcg_pupdate_kernel with update_p, tile origin from the group-local workgroup).
+struct PupdateCall {
+  TileGeom g;
+  long long n, xs;
+  const double *alpha, *beta;
+  double *x, *p;
+  const float* z;
+  const int *active, *iters, *kdev;
+};
+struct PupdateGroups {
+  int n;
+  int first[kMaxBatchGroups + 1];
+  PupdateCall a[kMaxBatchGroups];
+};
+
+template <int D>
+__global__ __launch_bounds__(kBlock) void batch_pupdate_kernel(PupdateGroups gs) {
+  int gi = 0;
+  while (gi + 1 < gs.n && (int)blockIdx.x >= gs.first[gi + 1]) ++gi;
+  const PupdateCall& c = gs.a[gi];
+  const TileGeom& g = c.g;
+  const int t = xcd_remap(blockIdx.x - gs.first[gi], g.total);
+  int row, tt;
+  row_tile(t, g.tiles, g.total, row, tt);
+  const int by = tt / g.ntx, bx = tt - by * g.ntx;
+  const int cc = bx * kTileX + (threadIdx.x & (kTileX - 1));
+  const int r0 = by * kTileY + (threadIdx.x / kTileX) * kRowsPerThread;
+  const int k = *c.kdev;
+  if (c.iters[row] != k || cc >= g.nnx) return;
+  const bool dir = c.active[row] != 0;
+  const double a = c.alpha[row];
+  const double b = dir ? c.beta[row] : 0.0;
+  double* xr = c.x + row * c.xs;
+  double* pr = c.p + row * c.n;
+  const float* zr = c.z + row * c.n;
+#pragma unroll
+  for (int j = 0; j < kRowsPerThread; ++j) {
+    if (r0 + j < g.nny) {
+      const long long node = (long long)(r0 + j) * g.nnx + cc;
+      Vec<double, D> pv = load_node<double, D>(pr, node);
+      Vec<double, D> xv = load_node<double, D>(xr, node);
+#pragma unroll
+      for (int q = 0; q < D; ++q) xv.v[q] += a * pv.v[q];
+      store_node<double, D>(xr, node, xv);
+      if (dir) {
+        const Vec<float, D> zv = load_node<float, D>(zr, node);
+#pragma unroll
+        for (int q = 0; q < D; ++q) pv.v[q] = (double)zv.v[q] + b * pv.v[q];
+        store_node<double, D>(pr, node, pv);
+      }
+    }
+  }
+}
+
+// One hierarchy of the batch.
+struct BatchGroup {
+  const sttopt_mg* mg = nullptr;
+  int rows = 0, L = 0;
+  OpDesc op;  // the fp64 outer operator
+  long long n = 0, xs = 0, bs = 0;
+  int tiles = 0, ptiles = 0;
+  Scratch sc{};
+  double *x = nullptr, *p = nullptr, *Ap = nullptr;
+  double* rbuf[2] = {nullptr, nullptr};
+  float* z = nullptr;
+  const double* b = nullptr;
+  int* iters = nullptr;
+  double* resid = nullptr;
+  VCycle<float>* vc = nullptr;
+  TileGeom g{};
+
+  float* lbuf(int l, void* sttopt_level::*f) const {
+    return static_cast<float*>(mg->levels[l].*f);
+  }
+  // where the V-cycle leaves level l's correction: vx at the coarsest level and at the
+  // tile-smoothed level 1, vx2 at a node-stencil level (its post pass writes there)
+  float* result(int l) const { return (l == L - 1 || l == 1) ? lbuf(l, &sttopt_level::vx) : lbuf(l, &sttopt_level::vx2); }
+};
+
+// Calls fn(chunk of at most kMaxBatchGroups entries) over a list.
+template <typename C, typename F>
+static int in_chunks(const std::vector<C>& calls, F fn) {
+  for (size_t i = 0; i < calls.size(); i += kMaxBatchGroups) {
+    const int m = (int)std::min<size_t>(kMaxBatchGroups, calls.size() - i);
+    if (int e = fn(calls.data() + i, m)) return e;
+  }
+  return STTOPT_OK;
+}
+
+static int launch_fin(int which, const std::vector<FinCall>& calls, const std::vector<int>& rows,
+                      double rtol, int* host_flag, int check_id, hipStream_t s) {
+  for (size_t i0 = 0; i0 < calls.size(); i0 += kMaxBatchGroups) {
+    FinGroups g{};
+    int total = 0;
+    for (size_t i = i0; i < calls.size() && g.n < kMaxBatchGroups; ++i) {
+      g.a[g.n] = calls[i];
+      g.first[g.n] = total;
+      total += rows[i];
+      ++g.n;
+    }
+    g.first[g.n] = total;
+    if (which == 0)
+      hipLaunchKernelGGL(batch_alpha_kernel, dim3(total), dim3(kBlock), 0, s, g);
+    else if (which == 1)
+      hipLaunchKernelGGL(batch_resid_kernel, dim3(total), dim3(kBlock), 0, s, g, rtol,
+                         host_flag, check_id);
+    else
+      hipLaunchKernelGGL(batch_beta_kernel, dim3(total), dim3(kBlock), 0, s, g);
+    STT_CHECK_LAUNCH("batch finalizer");
+  }
+  return STTOPT_OK;
+}
+
+static int batch_validate(const BatchGroup& G, int d) {
+  const sttopt_mg* mg = G.mg;
+  if (mg->vdtype != STTOPT_F32 || mg->n_smooth != 2 || mg->gamma != 1 || mg->nlevels < 2 ||
+      mg->d != d || !G.vc->pair()) {
+    set_error("sttopt_mgpcg_batch: every hierarchy needs the fp32 two-sweep cycle coarsened "
+              "2 x 2 from level 0, at least two levels and the same dofs per node");
+    return STTOPT_EINVAL;
+  }
+  for (int l = 1; l < mg->nlevels - 1; ++l) {
+    const sttopt_level& Lv = mg->levels[l];
+    const OpDesc& o = G.vc->op[l];
+    const bool ok = Lv.kx == 2 && Lv.ky == 2 &&
+                    ((l == 1 && o.ptab != nullptr && o.pkx == 2 && o.pky == 2) ||
+                     (l >= 2 && o.keff_node));
+    if (!ok) {
+      set_error("sttopt_mgpcg_batch: level 1 must be the density-parameterized 2 x 2 level "
+                "and deeper levels node stencils, each coarsened 2 x 2");
+      return STTOPT_EINVAL;
+    }
+  }
+  return STTOPT_OK;
+}
+
+// The second half of an iteration for every hierarchy: the coarse visit below level 0,
+// the level-0 post-smoothing, the beta finalizer and the direction update.
+static int batch_rest(std::vector<BatchGroup>& G, int parity, double omega, hipStream_t s) {
+  const int ng = (int)G.size();
+  int Lmax = 0;
+  for (auto& g : G) Lmax = std::max(Lmax, g.L);
+  // restriction from level 0
+  {
+    std::vector<TransferCall> c;
+    for (auto& g : G) {
+      const sttopt_level& L1 = g.mg->levels[1];
+      c.push_back({g.mg->d, L1.nelx, L1.nely, g.rows, g.lbuf(0, &sttopt_level::vr), g.n,
+                   L1.mask, g.lbuf(1, &sttopt_level::vb), g.vc->n[1], g.sc.active});
+    }
+    if (int e = in_chunks(c, [&](const TransferCall* p, int m) {
+          return transfer22_groups(p, m, true, s);
+        }))
+      return e;
+  }
+  // down: level l of every hierarchy that has l above its coarsest level
+  for (int l = 1; l <= Lmax - 2; ++l) {
+    if (l == 1) {
+      std::vector<StencilCall<float>> first, resid;
+      std::vector<TransferCall> down;
+      for (auto& g : G) {
+        if (g.L - 1 <= 1) continue;
+        const VCycle<float>& v = *g.vc;
+        const long long n1 = v.n[1];
+        float* vb = g.lbuf(1, &sttopt_level::vb);
+        float* vx = g.lbuf(1, &sttopt_level::vx);
+        float* vr = g.lbuf(1, &sttopt_level::vr);
+        first.push_back({v.op[1], g.rows, kSmoothFirst, kNoDot, nullptr, 0, vb, n1, v.invd(1),
+                         v.sd(1), vx, n1, omega, g.sc.active, nullptr});
+        resid.push_back({v.op[1], g.rows, kResid, kNoDot, vx, n1, vb, n1, nullptr, 0, vr, n1,
+                         0.0, g.sc.active, nullptr});
+        const sttopt_level& L2 = g.mg->levels[2];
+        down.push_back({g.mg->d, L2.nelx, L2.nely, g.rows, vr, n1, L2.mask,
+                        g.lbuf(2, &sttopt_level::vb), v.n[2], g.sc.active});
+      }
+      if (first.empty()) continue;
+      auto st = [&](const StencilCall<float>* p, int m) { return stencil_tile_groups<float>(p, m, s); };
+      if (int e = in_chunks(first, st)) return e;
+      if (int e = in_chunks(resid, st)) return e;
+      if (int e = in_chunks(down, [&](const TransferCall* p, int m) {
+            return transfer22_groups(p, m, true, s);
+          }))
+        return e;
+    } else {
+      std::vector<PairLevel> pre;
+      int variant = -1;
+      for (auto& g : G) {
+        if (g.L - 1 <= l) continue;
+        const VCycle<float>& v = *g.vc;
+        PairLevel P{};
+        P.op = v.op[l];
+        P.rows = g.rows;
+        P.b = g.lbuf(l, &sttopt_level::vb);
+        P.sb = v.n[l];
+        P.invd = v.invd(l);
+        P.sd = v.sd(l);
+        P.x2 = g.lbuf(l, &sttopt_level::vx);
+        P.sx2 = v.n[l];
+        P.cmask = g.mg->levels[l + 1].mask;
+        P.bc = g.lbuf(l + 1, &sttopt_level::vb);
+        P.sbc = v.n[l + 1];
+        P.omega = omega;
+        P.active = g.sc.active;
+        pre.push_back(P);
+        if (variant < 0) variant = (g.mg->levels[l].flags >> STTOPT_TILE_VARIANT_SHIFT) & 15;
+      }
+      if (pre.empty()) continue;
+      if (int e = in_chunks(pre, [&](const PairLevel* p, int m) {
+            return coarse_pair_groups(p, m, true, variant, s);
+          }))
+        return e;
+    }
+  }
+  // the coarsest solves, each hierarchy at its own depth
+  {
+    std::vector<CoarseSolveCall> c;
+    for (auto& g : G) {
+      const int lc = g.L - 1;
+      const sttopt_level& Lc = g.mg->levels[lc];
+      const long long nc = g.vc->n[lc];
+      c.push_back({nc, g.rows, Lc.cinv, Lc.cinv_stride, g.lbuf(lc, &sttopt_level::vb), nc,
+                   Lc.mask, g.lbuf(lc, &sttopt_level::vx), nc, g.sc.active});
+    }
+    if (int e = in_chunks(c, [&](const CoarseSolveCall* p, int m) {
+          return coarse_solve_groups(p, m, s);
+        }))
+      return e;
+  }
+  // up
+  for (int l = Lmax - 2; l >= 1; --l) {
+    if (l >= 2) {
+      std::vector<PairLevel> post;
+      int variant = -1;
+      for (auto& g : G) {
+        if (g.L - 1 <= l) continue;
+        const VCycle<float>& v = *g.vc;
+        PairLevel P{};
+        P.op = v.op[l];
+        P.rows = g.rows;
+        P.x2 = g.lbuf(l, &sttopt_level::vx);
+        P.sx2 = v.n[l];
+        P.ec = g.result(l + 1);
+        P.sec = v.n[l + 1];
+        P.b = g.lbuf(l, &sttopt_level::vb);
+        P.sb = v.n[l];
+        P.invd = v.invd(l);
+        P.sd = v.sd(l);
+        P.x4 = g.lbuf(l, &sttopt_level::vx2);
+        P.sx4 = v.n[l];
+        P.omega = omega;
+        P.active = g.sc.active;
+        post.push_back(P);
+        if (variant < 0) variant = (g.mg->levels[l].flags >> STTOPT_TILE_VARIANT_SHIFT) & 15;
+      }
+      if (post.empty()) continue;
+      if (int e = in_chunks(post, [&](const PairLevel* p, int m) {
+            return coarse_pair_groups(p, m, false, variant, s);
+          }))
+        return e;
+    } else {
+      std::vector<TransferCall> up;
+      std::vector<StencilCall<float>> sw1, sw2;
+      for (auto& g : G) {
+        if (g.L - 1 <= 1) continue;
+        const VCycle<float>& v = *g.vc;
+        const long long n1 = v.n[1];
+        const sttopt_level& L2 = g.mg->levels[2];
+        float* vb = g.lbuf(1, &sttopt_level::vb);
+        float* vx = g.lbuf(1, &sttopt_level::vx);
+        float* vx2 = g.lbuf(1, &sttopt_level::vx2);
+        up.push_back({g.mg->d, L2.nelx, L2.nely, g.rows, g.result(2), v.n[2],
+                      g.mg->levels[1].mask, vx, n1, g.sc.active});
+        sw1.push_back({v.op[1], g.rows, kSmooth, kNoDot, vx, n1, vb, n1, v.invd(1), v.sd(1),
+                       vx2, n1, omega, g.sc.active, nullptr});
+        sw2.push_back({v.op[1], g.rows, kSmooth, kNoDot, vx2, n1, vb, n1, v.invd(1), v.sd(1),
+                       vx, n1, omega, g.sc.active, nullptr});
+      }
+      if (up.empty()) continue;
+      if (int e = in_chunks(up, [&](const TransferCall* p, int m) {
+            return transfer22_groups(p, m, false, s);
+          }))
+        return e;
+      auto st = [&](const StencilCall<float>* p, int m) { return stencil_tile_groups<float>(p, m, s); };
+      if (int e = in_chunks(sw1, st)) return e;
+      if (int e = in_chunks(sw2, st)) return e;
+    }
+  }
+  // level-0 post-smoothing: z and the r.z partials
+  {
+    std::vector<PairPostCall<float>> c;
+    for (auto& g : G) {
+      const VCycle<float>& v = *g.vc;
+      c.push_back({v.op[0], g.rows, g.lbuf(0, &sttopt_level::vx), g.n, g.result(1), v.n[1],
+                   g.mg->levels[1].nelx + 1, g.rbuf[parity], g.n, v.invd(0), v.sd(0), g.z,
+                   g.n, omega, g.sc.active, g.sc.part[2]});
+    }
+    if (int e = in_chunks(c, [&](const PairPostCall<float>* p, int m) {
+          return smooth_pair_post_groups(p, m, s);
+        }))
+      return e;
+  }
+  // beta, then the direction update
+  std::vector<FinCall> fb;
+  std::vector<int> rows;
+  std::vector<PupdateCall> pu;
+  for (auto& g : G) {
+    FinCall f{};
+    f.tiles = g.ptiles;
+    f.tiles_o = g.ptiles;
+    f.flexible = 1;
+    f.p0 = g.sc.part[2];
+    f.p1 = g.sc.part[3];
+    f.active = g.sc.active;
+    f.rz = g.sc.rz;
+    f.beta = g.sc.beta;
+    f.counter = g.sc.counter;
+    fb.push_back(f);
+    rows.push_back(g.rows);
+    pu.push_back({g.g, g.n, g.xs, g.sc.alpha, g.sc.beta, g.x, g.p, g.z, g.sc.active, g.iters,
+                  g.sc.counter + 1});
+  }
+  if (int e = launch_fin(2, fb, rows, 0.0, nullptr, 0, s)) return e;
+  for (size_t i0 = 0; i0 < pu.size(); i0 += kMaxBatchGroups) {
+    PupdateGroups pg{};
+    int total = 0;
+    for (size_t i = i0; i < pu.size() && pg.n < kMaxBatchGroups; ++i) {
+      pg.a[pg.n] = pu[i];
+      pg.first[pg.n] = total;
+      total += pu[i].g.total;
+      ++pg.n;
+    }
+    pg.first[pg.n] = total;
+    if (G[0].mg->d == 2)
+      hipLaunchKernelGGL(batch_pupdate_kernel<2>, dim3(total), dim3(kBlock), 0, s, pg);
+    else
+      hipLaunchKernelGGL(batch_pupdate_kernel<1>, dim3(total), dim3(kBlock), 0, s, pg);
+    STT_CHECK_LAUNCH("batch_pupdate_kernel");
+  }
+  (void)ng;
+  return STTOPT_OK;
+}
+
+static thread_local ThreadGraphs t_batch_graphs;
+
+static int mgpcg_batch_impl(const sttopt_batch_group* in, int ng, double rtol, int max_iter,
+                            int check_every, int cold, int* n_iter, hipStream_t s) {
+  std::vector<BatchGroup> G(ng);
+  std::vector<std::unique_ptr<VCycle<float>>> vcs;
+  const int d = in[0].mg->d;
+  const double omega = in[0].mg->omega;
+  for (int i = 0; i < ng; ++i) {
+    BatchGroup& g = G[i];
+    const sttopt_mg* mg = in[i].mg;
+    g.mg = mg;
+    g.rows = mg->rows;
+    g.L = mg->nlevels;
+    g.op.d = mg->d;
+    g.op.nelx = mg->levels[0].nelx;
+    g.op.nely = mg->levels[0].nely;
+    g.op.mask = mg->levels[0].mask;
+    g.op.rho = mg->density64;
+    g.op.sr = mg->density64_stride;
+    g.op.rdiag = mg->rdiag64;
+    g.op.srd = mg->rdiag64_stride;
+    for (int q = 0; q < 64; ++q) g.op.ke[q] = mg->ke[q];
+    g.n = g.op.ndof();
+    g.tiles = g.op.tiles();
+    g.ptiles = pair_tiles(g.op);
+    g.sc = carve(mg->scratch, g.rows, partial_slots(g.op));
+    double* w = in[i].work;
+    g.rbuf[0] = w;
+    g.z = reinterpret_cast<float*>(w + g.rows * g.n);
+    g.p = w + 2 * g.rows * g.n;
+    g.Ap = w + 3 * g.rows * g.n;
+    g.rbuf[1] = w + 4 * g.rows * g.n;
+    g.x = in[i].x;
+    g.xs = in[i].x_stride;
+    g.b = in[i].b;
+    g.bs = in[i].b_stride;
+    g.iters = in[i].iters;
+    g.resid = in[i].resid;
+    g.g = TileGeom{g.op.nnx(), g.op.nny(), g.op.ntx(), g.tiles, g.tiles * g.rows, d};
+    vcs.emplace_back(new VCycle<float>(mg, s, g.sc.active));
+    g.vc = vcs.back().get();
+    if (int e = batch_validate(g, d)) return e;
+    if (mg->omega != omega) {
+      set_error("sttopt_mgpcg_batch: one omega for all hierarchies");
+      return STTOPT_EINVAL;
+    }
+  }
+  *n_iter = 0;
+  const dim3 tblock(kBlock);
+  // initial residuals, per hierarchy (once per solve)
+  int remaining = 0;
+  for (auto& g : G) {
+    const TileGeom& tg = g.g;
+    if (cold) {
+      STT_TILE_LAUNCH(cold_init_kernel, double, tg, s, g.n, g.b, g.bs, g.op.mask, g.x, g.xs,
+                      g.rbuf[0], g.sc.part[4]);
+    } else {
+      if (int e = project_f64(g.n, g.rows, g.op.mask, g.x, g.xs, s)) return e;
+      if (int e = stencil<double, double, double>(g.op, g.rows, kResid, false, kNoDot, g.x,
+                                                  g.xs, g.b, g.bs, nullptr, 0, g.rbuf[0], g.n,
+                                                  0.0, nullptr, nullptr, s))
+        return e;
+      STT_TILE_LAUNCH(dot_kernel, double, tg, s, g.n, g.b, g.bs,
+                      static_cast<const double*>(nullptr), 0, g.op.mask, nullptr, g.sc.part[4]);
+      STT_TILE_LAUNCH(dot_kernel, double, tg, s, g.n, g.rbuf[0], g.n,
+                      static_cast<const double*>(nullptr), 0, g.op.mask, nullptr, g.sc.part[1]);
+    }
+    if (int e = fill<int>(2, 1, 0, g.sc.counter, 2, nullptr, s)) return e;
+    hipLaunchKernelGGL(fin_init_kernel, dim3(g.rows), tblock, 0, s, g.tiles, g.sc.part[4],
+                       cold ? g.sc.part[4] : g.sc.part[1], rtol, g.sc.bsafe, g.resid,
+                       g.sc.active, g.iters, g.sc.counter);
+    STT_CHECK_LAUNCH("batch init");
+  }
+  for (auto& g : G) {
+    int rem = 0;
+    if (int e = read_device_int(g.sc.counter, s, &rem)) return e;
+    remaining += rem;
+  }
+  if (remaining == 0) return STTOPT_OK;
+  for (auto& g : G) {
+    if (int e = g.vc->run(g.rbuf[0], g.n, g.z, g.n, g.sc.part[2], g.Ap)) return e;
+    hipLaunchKernelGGL(fin_rz_kernel, dim3(g.rows), tblock, 0, s, g.vc->rz_tiles(),
+                       g.sc.part[2], g.sc.active, g.sc.rz, g.sc.counter);
+    if (int e = convert<float, double>(g.n, g.rows, g.z, g.n, g.p, g.n, g.sc.active, s))
+      return e;
+    STT_CHECK_LAUNCH("batch first preconditioner");
+  }
+  if (check_every < 1) check_every = 1;
+  int* lag_host = pinned_counter();
+  hipEvent_t* lag_ev = lag_events();
+  int* lag_dev = nullptr;
+  if (lag_host == nullptr || lag_ev == nullptr ||
+      hipHostGetDevicePointer(reinterpret_cast<void**>(&lag_dev), lag_host, 0) != hipSuccess) {
+    set_error("sttopt_mgpcg_batch: pinned readback words or events unavailable");
+    return STTOPT_EHIP;
+  }
+  std::vector<int> rows;
+  for (auto& g : G) rows.push_back(g.rows);
+  int checks = 0;
+  int check_ids[2] = {0, 0};
+  bool done = false;
+  int k_last = 0;
+  bool captured[2] = {false, false};
+  const bool graphs = g_pcg_graphs && max_iter > 1;
+  for (int k = 1; k <= max_iter; ++k) {
+    // outer apply with the p.Ap partials, alpha
+    std::vector<StencilCall<double>> ap;
+    std::vector<FinCall> fa, fr;
+    for (auto& g : G) {
+      ap.push_back({g.op, g.rows, kOperator, kDotU, g.p, g.n, nullptr, 0, nullptr, 0, g.Ap,
+                    g.n, 0.0, g.sc.active, g.sc.part[0]});
+      FinCall f{};
+      f.tiles = g.tiles;
+      f.p0 = g.sc.part[0];
+      f.active = g.sc.active;
+      f.rz = g.sc.rz;
+      f.alpha = g.sc.alpha;
+      f.kdev = g.sc.counter + 1;
+      fa.push_back(f);
+    }
+    if (int e = in_chunks(ap, [&](const StencilCall<double>* p, int m) {
+          return stencil_tile_groups<double>(p, m, s);
+        }))
+      return e;
+    if (int e = launch_fin(0, fa, rows, 0.0, nullptr, 0, s)) return e;
+    const bool last = k == max_iter;
+    // the residual update (fused into level 0's pre-smoothing except at the last
+    // iteration)
+    if (!last) {
+      std::vector<PairPreCall> pre;
+      for (auto& g : G) {
+        const VCycle<float>& v = *g.vc;
+        pre.push_back({v.op[0], g.rows, g.rbuf[(k - 1) & 1], g.n, v.invd(0), v.sd(0),
+                       g.lbuf(0, &sttopt_level::vx), g.n, g.lbuf(0, &sttopt_level::vr), g.n,
+                       omega, g.sc.active, g.Ap, g.n, g.sc.alpha, g.rbuf[k & 1], g.n,
+                       g.sc.part[1], g.z, g.n, g.sc.part[3]});
+      }
+      if (int e = in_chunks(pre, [&](const PairPreCall* p, int m) {
+            return smooth_pair_pre_groups(p, m, s);
+          }))
+        return e;
+    } else {
+      for (auto& g : G) {
+        STT_TILE_LAUNCH(pcg_update_kernel, float, g.g, s, g.n, g.sc.alpha, g.rbuf[(k - 1) & 1],
+                        g.Ap, g.z, g.sc.active, g.sc.part[1], g.sc.part[3]);
+      }
+    }
+    const bool check = !last && k % check_every == 0;
+    const int cur = checks & 1, prev = cur ^ 1;
+    if (check) check_ids[cur] = next_check_id();
+    for (auto& g : G) {
+      FinCall f{};
+      f.tiles = last ? g.tiles : g.ptiles;
+      f.k = k;
+      f.p0 = g.sc.part[1];
+      f.bsafe = g.sc.bsafe;
+      f.resid = g.resid;
+      f.active = g.sc.active;
+      f.iters = g.iters;
+      f.counter = g.sc.counter;
+      fr.push_back(f);
+    }
+    if (int e = launch_fin(1, fr, rows, rtol, check ? lag_dev + 1 + cur : nullptr,
+                           check ? check_ids[cur] : 0, s))
+      return e;
+    k_last = k;
+    if (last) {
+      remaining = 0;
+      for (auto& g : G) {
+        int rem = 0;
+        if (int e = read_device_int(g.sc.counter, s, &rem)) return e;
+        remaining += rem;
+      }
+      done = remaining == 0;
+      break;
+    }
+    if (check) {
+      hipError_t e = hipEventRecord(lag_ev[cur], s);
+      if (e != hipSuccess) return hip_status(e, "hipEventRecord");
+      if (checks > 0) {
+        e = hipEventSynchronize(lag_ev[prev]);
+        if (e != hipSuccess) return hip_status(e, "hipEventSynchronize");
+        if (reinterpret_cast<volatile int*>(lag_host)[1 + prev] != check_ids[prev]) {
+          done = true;
+          k_last = 0;
+          break;
+        }
+      }
+      ++checks;
+    }
+    const int parity = k & 1;
+    if (graphs) {
+      if (!captured[parity]) {
+        hipStream_t cap = nullptr;
+        if (int e = t_batch_graphs.capture_stream(&cap)) return e;
+        if (int e = capture_graph(cap, [&] {
+              for (auto& g : G) g.vc->s = cap;
+              const int st = batch_rest(G, parity, omega, cap);
+              for (auto& g : G) g.vc->s = s;
+              return st;
+            }, &t_batch_graphs.exec[parity]))
+          return e;
+        captured[parity] = true;
+      }
+      hipError_t e = hipGraphLaunch(t_batch_graphs.exec[parity], s);
+      if (e != hipSuccess) return hip_status(e, "hipGraphLaunch");
+    } else if (int e = batch_rest(G, parity, omega, s)) {
+      return e;
+    }
+    k_last = 0;
+  }
+  if (k_last > 0) {
+    for (auto& g : G) {
+      STT_TILE_LAUNCH(pcg_pupdate_kernel, float, g.g, s, g.n, g.sc.alpha, g.sc.beta, g.x, g.xs,
+                      g.p, g.z, g.sc.active, g.iters, k_last, 0, static_cast<const int*>(nullptr));
+      STT_CHECK_LAUNCH("batch last x update");
+    }
+  }
+  int mx = 0;
+  for (auto& g : G) {
+    std::vector<int> h(g.rows);
+    hipError_t e = hipMemcpyAsync(h.data(), g.iters, sizeof(int) * g.rows,
+                                  hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(iters)");
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+    for (int v : h) mx = std::max(mx, v);
+  }
+  *n_iter = done ? mx : max_iter;
+  if (!done) {
+    set_error("sttopt_mgpcg_batch: rows above rtol after max_iter iterations");
+    return STTOPT_ENOCONV;
+  }
+  return STTOPT_OK;
+}
+
 }  // namespace stt
 
 using namespace stt;
@@ -1216,6 +1884,36 @@ int sttopt_mgpcg(const sttopt_mg* mg, const double* b, int64_t b_stride, double*
                               check_every, cold_start, work, iters, resid, n_iter, s);
   return mgpcg_impl<float>(mg, b, b_stride, x, x_stride, rtol, max_iter, flexible,
                            check_every, cold_start, work, iters, resid, n_iter, s);
+}
+
+int sttopt_mgpcg_batch(const sttopt_batch_group* groups, int32_t ngroups, double rtol,
+                       int32_t max_iter, int32_t check_every, int32_t cold_start,
+                       int32_t* n_iter, void* stream) {
+  if (groups == nullptr || ngroups < 1 || n_iter == nullptr || max_iter < 0) {
+    set_error("sttopt_mgpcg_batch: no groups, null n_iter or negative max_iter");
+    return STTOPT_EINVAL;
+  }
+  for (int i = 0; i < ngroups; ++i) {
+    const sttopt_batch_group& g = groups[i];
+    if (int e = validate(g.mg)) return e;
+    if (g.b == nullptr || g.x == nullptr || g.work == nullptr || g.iters == nullptr ||
+        g.resid == nullptr || g.mg->scratch == nullptr || g.mg->density64 == nullptr ||
+        g.mg->rows < 1 || g.mg->d != groups[0].mg->d) {
+      set_error("sttopt_mgpcg_batch: a group with null vectors, no rows or other dofs per node");
+      return STTOPT_EINVAL;
+    }
+    const long long ndof0 = (long long)g.mg->d * (g.mg->levels[0].nelx + 1) *
+                            (long long)(g.mg->levels[0].nely + 1);
+    if ((g.mg->d == 2 && ((reinterpret_cast<uintptr_t>(g.b) | reinterpret_cast<uintptr_t>(g.x) |
+                           reinterpret_cast<uintptr_t>(g.work)) % 16 != 0 ||
+                          g.b_stride % 2 != 0 || g.x_stride % 2 != 0)) ||
+        (g.mg->rows > 1 && g.x_stride < ndof0)) {
+      set_error("sttopt_mgpcg_batch: node-aligned vectors, even strides, non-overlapping x rows");
+      return STTOPT_EINVAL;
+    }
+  }
+  return mgpcg_batch_impl(groups, ngroups, rtol, max_iter, check_every, cold_start, n_iter,
+                          (hipStream_t)stream);
 }
 
 }  // extern "C"

@@ -139,7 +139,8 @@ __device__ __forceinline__ Vec<T, D> zvec() {
 // RD: the operator carries a lumped reaction diagonal.  UPD (pre only): the pass also
 // takes the PCG's residual update r <- r - alpha Ap.
 template <typename ZT, int D, bool POST, bool RD, bool UPD>
-__device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
+__device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a, const int bid,
+                                                 const int karg_off) {
   static_assert(!(POST && UPD), "the residual update belongs to the pre pass");
   constexpr int NR = kRowsPerThread;
   // input of the first stencil, and its output with a one-node border (sX).  In the
@@ -156,7 +157,7 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
   __shared__ Vec<float, D> sB[POST ? 1 : kTileY][POST ? 1 : kTileX];
   __shared__ Vec<float, D> sI[POST ? 1 : kTileY][POST ? 1 : kTileX];
   __shared__ double red[kWaves];
-  const int t = xcd_remap(blockIdx.x, a.total);
+  const int t = xcd_remap(bid, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
@@ -409,7 +410,7 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
   using Args = PairArgs<ZT>;
   const KePtr ke_args = reinterpret_cast<KePtr>(
       (__attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr() +
-      offsetof(Args, ke));
+      karg_off + offsetof(Args, ke));  // karg_off: where `a` sits in the argument segment
   float R[NR + 1][2];
 #pragma unroll
   for (int r = 0; r < NR + 1; ++r) {
@@ -529,11 +530,40 @@ __device__ __forceinline__ void smooth_pair_body(const PairArgs<ZT>& a) {
 template <int D, bool RD, bool UPD>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
 void smooth_pair_pre_kernel(PairArgs<float> a) {
-  smooth_pair_body<float, D, false, RD, UPD>(a);
+  smooth_pair_body<float, D, false, RD, UPD>(a, blockIdx.x, 0);
 }
 template <typename ZT, int D, bool RD>
 __global__ __launch_bounds__(kBlock) void smooth_pair_post_kernel(PairArgs<ZT> a) {
-  smooth_pair_body<ZT, D, true, RD, false>(a);
+  smooth_pair_body<ZT, D, true, RD, false>(a, blockIdx.x, 0);
+}
+
+// The same passes over level 0 of several hierarchies in one launch: hierarchy i takes
+// workgroups first[i] .. first[i + 1] - 1.
+template <typename ZT>
+struct PairArgGroups {
+  int n;
+  int first[kMaxBatchGroups + 1];
+  PairArgs<ZT> a[kMaxBatchGroups];
+};
+__device__ __forceinline__ int pair_group(const int* first, int n, int b) {
+  int i = 0;
+  while (i + 1 < n && b >= first[i + 1]) ++i;
+  return i;
+}
+template <int D, bool RD, bool UPD>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+void smooth_pair_pre_groups_kernel(PairArgGroups<float> g) {
+  const int i = pair_group(g.first, g.n, blockIdx.x);
+  smooth_pair_body<float, D, false, RD, UPD>(
+      g.a[i], blockIdx.x - g.first[i],
+      (int)(offsetof(PairArgGroups<float>, a) + i * sizeof(PairArgs<float>)));
+}
+template <typename ZT, int D, bool RD>
+__global__ __launch_bounds__(kBlock) void smooth_pair_post_groups_kernel(PairArgGroups<ZT> g) {
+  const int i = pair_group(g.first, g.n, blockIdx.x);
+  smooth_pair_body<ZT, D, true, RD, false>(
+      g.a[i], blockIdx.x - g.first[i],
+      (int)(offsetof(PairArgGroups<ZT>, a) + i * sizeof(PairArgs<ZT>)));
 }
 
 template <typename ZT>
@@ -589,6 +619,89 @@ int pair_tiles(const OpDesc& op) {
   return ceil_div(op.nnx(), kOutX) * ceil_div(op.nny(), kOutY);
 }
 
+static int pre_args(const PairPreCall& c, PairArgs<float>& a) {
+  const OpDesc& op = c.op;
+  if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) || !fits_u32(op)) {
+    set_error("smooth_pair_pre: level-0 density operator with a free-dof mask, d = 1 or 2, "
+              "rows below 4 GiB required");
+    return STTOPT_EINVAL;
+  }
+  const bool upd = c.ap != nullptr;
+  if (upd && (c.alpha == nullptr || c.rnew == nullptr || c.rr == nullptr || c.rnew == c.b ||
+              (c.zold != nullptr && c.rzo == nullptr))) {
+    set_error("smooth_pair_pre: the residual update needs alpha, a separate output and the "
+              "partial sums");
+    return STTOPT_EINVAL;
+  }
+  if (op.d == 2 && !(pair_aligned(c.b, c.sb, sizeof(double)) &&
+                     pair_aligned(c.invd, c.sd, sizeof(float)) &&
+                     pair_aligned(c.x, c.sx, sizeof(float)) && pair_aligned(c.r, c.sr, sizeof(float)) &&
+                     pair_aligned(op.rdiag, op.srd, sizeof(float)) &&
+                     pair_aligned(c.ap, c.sap, sizeof(double)) &&
+                     pair_aligned(c.rnew, c.srn, sizeof(double)) &&
+                     pair_aligned(c.zold, c.szo, sizeof(float)))) {
+    set_error("smooth_pair_pre: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  a = pair_args<float>(op, c.rows);
+  a.b = c.b;
+  a.sb = c.sb;
+  a.invd = c.invd;
+  a.sd = c.sd;
+  a.out = c.x;
+  a.so = c.sx;
+  a.res = c.r;
+  a.sres = c.sr;
+  a.omega = static_cast<float>(c.omega);
+  a.active = c.active;
+  if (upd) {
+    a.ap = c.ap;
+    a.sap = c.sap;
+    a.alpha = c.alpha;
+    a.rnew = c.rnew;
+    a.srn = c.srn;
+    a.rr = c.rr;
+    a.zold = c.zold;
+    a.szo = c.szo;
+    a.rzo = c.rzo;
+  }
+  return STTOPT_OK;
+}
+
+template <typename ZT>
+static int post_args(const PairPostCall<ZT>& c, PairArgs<ZT>& a) {
+  const OpDesc& op = c.op;
+  if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) ||
+      op.nelx % 2 != 0 || op.nely % 2 != 0 || c.cnnx != op.nelx / 2 + 1 || !fits_u32(op)) {
+    set_error("smooth_pair_post: level-0 density operator over a 2 x 2 coarsening required");
+    return STTOPT_EINVAL;
+  }
+  if (op.d == 2 && !(pair_aligned(c.b, c.sb, sizeof(double)) &&
+                     pair_aligned(c.invd, c.sd, sizeof(float)) &&
+                     pair_aligned(c.x, c.sx, sizeof(float)) && pair_aligned(c.ec, c.sec, sizeof(float)) &&
+                     pair_aligned(c.z, c.sz, sizeof(ZT)) &&
+                     pair_aligned(op.rdiag, op.srd, sizeof(float)))) {
+    set_error("smooth_pair_post: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  a = pair_args<ZT>(op, c.rows);
+  a.cnnx = c.cnnx;
+  a.x = c.x;
+  a.sx = c.sx;
+  a.ec = c.ec;
+  a.sec = c.sec;
+  a.b = c.b;
+  a.sb = c.sb;
+  a.invd = c.invd;
+  a.sd = c.sd;
+  a.out = c.z;
+  a.so = c.sz;
+  a.omega = static_cast<float>(c.omega);
+  a.active = c.active;
+  a.partial = c.partial;
+  return STTOPT_OK;
+}
+
 int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
                     const float* invd, long long sd, float* x, long long sx, float* r,
                     long long sr, double omega, const int* active, hipStream_t s,
@@ -596,52 +709,12 @@ int smooth_pair_pre(const OpDesc& op, int rows, const double* b, long long sb,
                     long long srn, double* rr, const float* zold, long long szo,
                     double* rzo) {
   if (rows <= 0) return STTOPT_OK;
-  if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) || !fits_u32(op)) {
-    set_error("smooth_pair_pre: level-0 density operator with a free-dof mask, d = 1 or 2, "
-              "rows below 4 GiB required");
-    return STTOPT_EINVAL;
-  }
-  const bool upd = ap != nullptr;
-  if (upd && (alpha == nullptr || rnew == nullptr || rr == nullptr || rnew == b ||
-              (zold != nullptr && rzo == nullptr))) {
-    set_error("smooth_pair_pre: the residual update needs alpha, a separate output and the "
-              "partial sums");
-    return STTOPT_EINVAL;
-  }
-  if (op.d == 2 && !(pair_aligned(b, sb, sizeof(double)) && pair_aligned(invd, sd, sizeof(float)) &&
-                     pair_aligned(x, sx, sizeof(float)) && pair_aligned(r, sr, sizeof(float)) &&
-                     pair_aligned(op.rdiag, op.srd, sizeof(float)) &&
-                     pair_aligned(ap, sap, sizeof(double)) &&
-                     pair_aligned(rnew, srn, sizeof(double)) &&
-                     pair_aligned(zold, szo, sizeof(float)))) {
-    set_error("smooth_pair_pre: d = 2 vectors need node-aligned pointers and even row strides");
-    return STTOPT_EINVAL;
-  }
-  PairArgs<float> a = pair_args<float>(op, rows);
-  a.b = b;
-  a.sb = sb;
-  a.invd = invd;
-  a.sd = sd;
-  a.out = x;
-  a.so = sx;
-  a.res = r;
-  a.sres = sr;
-  a.omega = static_cast<float>(omega);
-  a.active = active;
-  if (upd) {
-    a.ap = ap;
-    a.sap = sap;
-    a.alpha = alpha;
-    a.rnew = rnew;
-    a.srn = srn;
-    a.rr = rr;
-    a.zold = zold;
-    a.szo = szo;
-    a.rzo = rzo;
-    launch_pre<true>(a, op.d, s);
-  } else {
-    launch_pre<false>(a, op.d, s);
-  }
+  const PairPreCall c{op, rows, b, sb, invd, sd, x, sx, r, sr, omega, active, ap, sap, alpha,
+                      rnew, srn, rr, zold, szo, rzo};
+  PairArgs<float> a;
+  if (int e = pre_args(c, a)) return e;
+  if (ap != nullptr) launch_pre<true>(a, op.d, s);
+  else launch_pre<false>(a, op.d, s);
   STT_CHECK_LAUNCH("smooth_pair_pre_kernel");
   return STTOPT_OK;
 }
@@ -652,35 +725,91 @@ int smooth_pair_post(const OpDesc& op, int rows, const float* x, long long sx,
                      const float* invd, long long sd, ZT* z, long long sz, double omega,
                      const int* active, double* partial, hipStream_t s) {
   if (rows <= 0) return STTOPT_OK;
-  if (op.rho == nullptr || op.mask == nullptr || (op.d != 1 && op.d != 2) ||
-      op.nelx % 2 != 0 || op.nely % 2 != 0 || cnnx != op.nelx / 2 + 1 || !fits_u32(op)) {
-    set_error("smooth_pair_post: level-0 density operator over a 2 x 2 coarsening required");
-    return STTOPT_EINVAL;
-  }
-  if (op.d == 2 && !(pair_aligned(b, sb, sizeof(double)) && pair_aligned(invd, sd, sizeof(float)) &&
-                     pair_aligned(x, sx, sizeof(float)) && pair_aligned(ec, sec, sizeof(float)) &&
-                     pair_aligned(z, sz, sizeof(ZT)) &&
-                     pair_aligned(op.rdiag, op.srd, sizeof(float)))) {
-    set_error("smooth_pair_post: d = 2 vectors need node-aligned pointers and even row strides");
-    return STTOPT_EINVAL;
-  }
-  PairArgs<ZT> a = pair_args<ZT>(op, rows);
-  a.cnnx = cnnx;
-  a.x = x;
-  a.sx = sx;
-  a.ec = ec;
-  a.sec = sec;
-  a.b = b;
-  a.sb = sb;
-  a.invd = invd;
-  a.sd = sd;
-  a.out = z;
-  a.so = sz;
-  a.omega = static_cast<float>(omega);
-  a.active = active;
-  a.partial = partial;
+  const PairPostCall<ZT> c{op, rows, x, sx, ec, sec, cnnx, b, sb, invd, sd, z, sz, omega,
+                           active, partial};
+  PairArgs<ZT> a;
+  if (int e = post_args<ZT>(c, a)) return e;
   launch_post<ZT>(a, op.d, s);
   STT_CHECK_LAUNCH("smooth_pair_post_kernel");
+  return STTOPT_OK;
+}
+
+int smooth_pair_pre_groups(const PairPreCall* calls, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchGroups) {
+    set_error("smooth_pair_pre groups: 1 to 4 hierarchies per launch");
+    return STTOPT_EINVAL;
+  }
+  PairArgGroups<float> g{};
+  int total = 0, d = 0, rd = -1, upd = -1;
+  for (int i = 0; i < n; ++i) {
+    if (calls[i].rows <= 0) continue;
+    PairArgs<float> a;
+    if (int e = pre_args(calls[i], a)) return e;
+    const int crd = a.rdiag != nullptr, cupd = calls[i].ap != nullptr;
+    if (rd >= 0 && (calls[i].op.d != d || crd != rd || cupd != upd)) {
+      set_error("smooth_pair_pre groups: one kind of pass per launch");
+      return STTOPT_EINVAL;
+    }
+    d = calls[i].op.d;
+    rd = crd;
+    upd = cupd;
+    g.a[g.n] = a;
+    g.first[g.n] = total;
+    total += a.total;
+    ++g.n;
+  }
+  g.first[g.n] = total;
+  if (total == 0) return STTOPT_OK;
+  const dim3 grid(total), block(kBlock);
+#define STT_PRE(D, RD, UPD) \
+  hipLaunchKernelGGL((smooth_pair_pre_groups_kernel<D, RD, UPD>), grid, block, 0, s, g)
+  if (d == 2) {
+    if (rd) { if (upd) STT_PRE(2, true, true); else STT_PRE(2, true, false); }
+    else { if (upd) STT_PRE(2, false, true); else STT_PRE(2, false, false); }
+  } else {
+    if (rd) { if (upd) STT_PRE(1, true, true); else STT_PRE(1, true, false); }
+    else { if (upd) STT_PRE(1, false, true); else STT_PRE(1, false, false); }
+  }
+#undef STT_PRE
+  STT_CHECK_LAUNCH("smooth_pair_pre_groups_kernel");
+  return STTOPT_OK;
+}
+
+int smooth_pair_post_groups(const PairPostCall<float>* calls, int n, hipStream_t s) {
+  if (n < 1 || n > kMaxBatchGroups) {
+    set_error("smooth_pair_post groups: 1 to 4 hierarchies per launch");
+    return STTOPT_EINVAL;
+  }
+  PairArgGroups<float> g{};
+  int total = 0, d = 0, rd = -1;
+  for (int i = 0; i < n; ++i) {
+    if (calls[i].rows <= 0) continue;
+    PairArgs<float> a;
+    if (int e = post_args<float>(calls[i], a)) return e;
+    const int crd = a.rdiag != nullptr;
+    if (rd >= 0 && (calls[i].op.d != d || crd != rd)) {
+      set_error("smooth_pair_post groups: one kind of pass per launch");
+      return STTOPT_EINVAL;
+    }
+    d = calls[i].op.d;
+    rd = crd;
+    g.a[g.n] = a;
+    g.first[g.n] = total;
+    total += a.total;
+    ++g.n;
+  }
+  g.first[g.n] = total;
+  if (total == 0) return STTOPT_OK;
+  const dim3 grid(total), block(kBlock);
+#define STT_POST(D, RD) \
+  hipLaunchKernelGGL((smooth_pair_post_groups_kernel<float, D, RD>), grid, block, 0, s, g)
+  if (d == 2) {
+    if (rd) STT_POST(2, true); else STT_POST(2, false);
+  } else {
+    if (rd) STT_POST(1, true); else STT_POST(1, false);
+  }
+#undef STT_POST
+  STT_CHECK_LAUNCH("smooth_pair_post_groups_kernel");
   return STTOPT_OK;
 }
 
