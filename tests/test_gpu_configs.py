@@ -314,6 +314,41 @@ def test_c4_uneven_batch_matches_oracle_and_single_grid_solves(dev):
 # ------------------------------------------------------------------ C5 -------
 
 
+def test_c5_recipe_at_reduced_size_vs_oracle(dev):
+    """The C5 recipe at 256x128 (an eighth of its edge): density clip(0.45 + 0.05 N),
+    bottom clamped, self-weight plus a -y unit load at the top-right node on the
+    whole-structure row, 16 stages at beta 32 of a bottom-up print-order time field
+    (sigma 4 elements, C5's 32 scaled with the grid), fp32 V-cycle inside the fp64 PCG at
+    rtol 1e-8, the reference's default hierarchy (coarsest 32x16): every row's U and
+    compliance and both gradients against the direct oracle."""
+    from sttopt_amd import compliance, workloads
+
+    p = workloads.space_time(256, 128, 16, beta=32.0, seed=0, mean=0.45, std=0.05,
+                             point_load=True, time_field="print", sigma=4.0, device=dev)
+    x = p.xPhys.clone().requires_grad_(True)
+    t = p.tPhys.clone().requires_grad_(True)
+    c, stages, U = compliance.batched_whole_and_gravity_compliance(
+        x, t, p.KE, p.edofMat, EMIN, EMAX, PENAL, p.freedofs, p.F, p.ndof, p.C, p.beta,
+        p.stage_times)
+    info = dict(compliance.LAST_SOLVE_INFO)
+    (c + sum(stages)).backward()
+    c_ref, dcx_ref, st_ref, U_ref = fem_direct.space_time(
+        p.xPhys.cpu().numpy(), p.tPhys.cpu().numpy(), p.stage_times, p.beta,
+        fem_direct.plane_stress_KE(NU), fem_direct.element_dofs(p.nelx, p.nely),
+        p.F.cpu().numpy(), p.freedofs.cpu().numpy(), p.C)
+    eu = max(rel_err(U[k].detach().cpu().numpy(), U_ref[k]) for k in range(U_ref.shape[0]))
+    ec = max([abs(float(c) - c_ref) / abs(c_ref)]
+             + [abs(float(stages[k]) - s[0]) / abs(s[0]) for k, s in enumerate(st_ref)])
+    ex = rel_err(x.grad.cpu().numpy().ravel(), dcx_ref + sum(s[1] for s in st_ref))
+    et = rel_err(t.grad.cpu().numpy().ravel(), sum(s[2] for s in st_ref))
+    print(f"C5 recipe 256x128: levels {info['levels']}, iterations per row "
+          f"{info['row_iter_list']}, max rel err U {eu:.2e}, c {ec:.2e}, dc/drho {ex:.2e}, "
+          f"dc/dt {et:.2e}")
+    assert len(info["levels"]) == 4
+    assert eu < 1e-6 and ec < 1e-6
+    assert ex < 1e-5 and et < 1e-5
+
+
 def test_c5_large_space_time_step_size_independent_checks(dev):
     """2048x1024 (4.2M dofs, 7 levels), 16 stages (beta 32) of a bottom-up print-order
     time field, self-weight + a top-right point load, fp32 V-cycle: true fp64 residual
