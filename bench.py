@@ -150,7 +150,9 @@ def main():
     if not args.no_uneven:
         uneven_line = uneven_throughput(torch, dev)
         print(f"# uneven batch: {uneven_line['solves_per_s']:.1f} solves+grads/s in one batched "
-              f"MG-PCG, {uneven_line['per_grid_calls_solves_per_s']:.1f} with one call per grid",
+              f"MG-PCG, {uneven_line['per_grid_calls_solves_per_s']:.1f} with one call per grid; "
+              f"forward solves alone {uneven_line['forward_solves_only']['batched_ms']:.2f} ms "
+              f"batched, {uneven_line['forward_solves_only']['per_grid_calls_ms']:.2f} ms per grid",
               file=sys.stderr, flush=True)
 
     cpu = None
@@ -433,17 +435,18 @@ def next_rows(torch, _lib, nelx, nely, rows, dev, launches=50, rmin=3.0):
 
 
 def pmc_traffic():
-    """HBM bytes per launch of the outer PCG's fp64 apply from the committed counter
-    summary: FETCH_SIZE and WRITE_SIZE (KiB per dispatch, averaged over that kernel's
-    dispatches in one bench step), each scaled by the ratio of the bytes a plain copy
-    moves to what the same counter reads for it.  None when the summary is absent."""
+    """HBM bytes per launch of the outer PCG's fused fp64 apply from the committed counter
+    summary: FETCH_SIZE and WRITE_SIZE (KiB per launch, summed over that kernel's
+    dispatches in one bench step and divided by its launches that still had rows), each
+    scaled by the ratio of the bytes a plain copy moves to what the same counter reads
+    for it.  None when the summary is absent."""
     try:
         with open(PMC_SUMMARY) as fh:
             kv = dict(line.split(":", 1) for line in fh if line.startswith("pmc."))
         kv = {k.strip(): float(v) for k, v in kv.items()}
         fetch = kv["pmc.apply_f64.fetch_kib"] * kv["pmc.copy.read_bytes"] / kv["pmc.copy.fetch_kib"]
         write = kv["pmc.apply_f64.write_kib"] * kv["pmc.copy.write_bytes"] / kv["pmc.copy.write_kib"]
-        return fetch + write
+        return fetch + write  # per launch with rows, as the live figure counts launches
     except (OSError, KeyError, ValueError, ZeroDivisionError):
         return None
 
@@ -462,11 +465,14 @@ def _time_apply(torch, _lib, args, stream, launches):
 
 
 def apply_roofline(torch, _lib, torch_fem, p, rows, dev, live, launches=50):
-    """The fp64 K(rho)p apply of the outer PCG against its algorithmic bytes
-    8 * (2 ndof + nel) per row.  `live`: (seconds, bytes, launches) from HIP events
-    around each of its launches in the timed steps (bytes counted for the rows still
-    active in each launch's iteration) -- the reported figure.  Also timed in a
-    separate loop on all rows: the same launch and the public operator() with its input
+    """The fp64 K(rho)p apply of the outer PCG against its algorithmic bytes.  `live`:
+    (seconds, bytes, launches) from HIP events around each of its launches in the timed
+    steps -- the reported figure.  These are the launches from the second PCG iteration
+    on, which carry the direction update (p_new = z + beta p_old, x += alpha p_old): per
+    row still iterating, p_old, z (fp32), x and rho read and p_new, x and Ap written,
+    8 (5 ndof + nel) + 4 ndof bytes; per row that retired the iteration before, 24 ndof
+    (x += alpha p_old only).  Also timed in a separate loop on all rows: the plain apply
+    (8 (2 ndof + nel) bytes per row) and the public operator() with its input
     projection."""
     args, keep, stream = _apply_args(torch, _lib, p, rows, dev, torch.float64, _lib.F64)
     sec = _time_apply(torch, _lib, args, stream, launches)
@@ -479,9 +485,10 @@ def apply_roofline(torch, _lib, torch_fem, p, rows, dev, live, launches=50):
     live_sec, live_bytes, live_n = live
     achieved = live_bytes / live_sec / 1e9
     return {
-        "kernel": "stencil_tile_kernel<double,double,double,2,OPERATOR,DOT> (fp64 K(rho)p of "
-                  "the outer PCG, projected input and output), HIP events around each launch "
-                  "in the timed steps",
+        "kernel": "stencil_tile_kernel<double,double,double,2,OPERATOR_DIR,DOT> (fp64 "
+                  "K(rho)p of the outer PCG with the direction update p = z + beta p, x += "
+                  "alpha p fused in, PCG iterations 2 on), HIP events around each launch in "
+                  "the timed steps",
         "launches_in_timed_steps": int(live_n),
         "all_rows_loop": {"avg_launch_us": sec * 1e6, "achieved": nbytes / sec / 1e9,
                           "frac": nbytes / sec / 1e9 / HBM_PEAK_GBS,
@@ -543,6 +550,35 @@ def uneven_workload(torch, dev, n_instances=64):
             ce = compliance._StrainEnergy.apply(U, KE, nelx, nely)
             torch.sum(dens * ce).backward()
 
+    def solver_only():
+        """The same forward solves with the hierarchies built once, untimed: returns
+        `(batched, per_grid)` callables that only solve (`uneven.batch_solve` over all
+        grids against `torch_mg.native_mgpcg` once per grid)."""
+        from sttopt_amd import torch_mg
+
+        grids = {}
+        for s, mask, edof, F in base:
+            grids.setdefault((s["nelx"], s["nely"]), []).append((s, mask, edof, F))
+        vcs, rhs = [], []
+        for (nelx, nely), items in grids.items():
+            x = torch.stack([torch.as_tensor(s["x"], dtype=torch.float64, device=dev).flatten()
+                             for s, _, _, _ in items])
+            dens = torch_fem.simp_density(x, 1e-9, 1.0, 3.0).repeat_interleave(2, dim=0)
+            levels = torch_mg.build_hierarchy(dens, items[0][2], KE, items[0][1], nelx, nely)
+            vcs.append(torch_mg.VCycle(levels, dtype=torch.float32))
+            F = torch.cat([f for _, _, _, f in items])
+            rhs.append(torch_fem.project(F, items[0][1]))
+
+        def one_launch_per_pass():
+            uneven.batch_solve(vcs, rhs, rtol=1e-8, max_iter=500)
+
+        def one_call_per_grid():
+            for vc, b in zip(vcs, rhs):
+                torch_mg.native_mgpcg(vc, b, rtol=1e-8, max_iter=500)
+
+        return one_launch_per_pass, one_call_per_grid
+
+    batched.solver_only = solver_only
     return batched, per_grid
 
 
@@ -554,8 +590,10 @@ def uneven_throughput(torch, dev, n_instances=64, reps=3):
     batched MG-PCG (`uneven.solve_instances`) with its batched adjoint, against the same
     solves as one `femsolve` per grid (one grid per call, as the reference batches)."""
     batched, per_grid = uneven_workload(torch, dev, n_instances)
+    solve_batched, solve_per_grid = batched.solver_only()
     out = {}
-    for name, fn in (("batched", batched), ("per_grid", per_grid)):
+    for name, fn in (("batched", batched), ("per_grid", per_grid),
+                     ("solve_batched", solve_batched), ("solve_per_grid", solve_per_grid)):
         fn()  # warm (hierarchy shapes, autotuned schedules)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -574,6 +612,11 @@ def uneven_throughput(torch, dev, n_instances=64, reps=3):
         "solves_per_s": solves / out["batched"],
         "ms_per_batch": 1e3 * out["batched"],
         "per_grid_calls_solves_per_s": solves / out["per_grid"],
+        "forward_solves_only": {
+            "note": "hierarchies built once, untimed; forward MG-PCG only",
+            "batched_ms": 1e3 * out["solve_batched"],
+            "per_grid_calls_ms": 1e3 * out["solve_per_grid"],
+        },
         "forward_iterations": info.get("forward_n_iter"),
         "adjoint_iterations": info.get("backward_n_iter"),
     }

@@ -336,7 +336,7 @@ int sttopt_mgpcg_batch(const sttopt_batch_group* groups, int32_t ngroups, double
  * reference (whose iteration is a host loop of torch ops, torch_fem.py:313-331). */
 int sttopt_mgpcg_graphs(int32_t enable);
 
-/* Work vectors sttopt_mgpcg needs: returns the count of doubles ([rows][ndof0] x 5). */
+/* Work vectors sttopt_mgpcg needs: returns the count of doubles ([rows][ndof0] x 6). */
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg);
 
 /* ---------------------------------------------------------- adjoints -------- */
@@ -401,10 +401,12 @@ int sttopt_stage_epilogue_noalloc(int32_t nelx, int32_t nely, int32_t nstage,
                                   double* comp, double* dx, double* dt, void* stream);
 
 /* HIP-event timing of the outer PCG's K(rho)p launches inside sttopt_mgpcg (off by
- * default; process-wide, not thread-safe).  enable resets the totals; read returns the
- * summed launch durations, the algorithmic bytes (8 (2 ndof + nel) per row still active
- * in each launch's iteration) and the launch count.  Measurement only: no counterpart
- * in the reference. */
+ * default; process-wide, not thread-safe): from iteration 2 on, the apply that carries
+ * the direction update (iteration 1's plain apply is not counted).  enable resets the
+ * totals; read returns the summed launch durations, the algorithmic bytes (per row still
+ * active in the launch's iteration 8 (5 ndof + nel) + s ndof, z in s-byte words; 24 ndof
+ * per row that retired the iteration before) and the launch count.  Measurement only: no
+ * counterpart in the reference. */
 int sttopt_profile_outer_apply(int32_t enable);
 int sttopt_profile_outer_apply_read(double* seconds, double* bytes, int64_t* launches);
 

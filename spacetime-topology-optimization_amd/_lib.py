@@ -325,14 +325,16 @@ _KE_HOST_LOCK = threading.Lock()
 
 
 def ke_array(KE: torch.Tensor) -> ctypes.Array:
-    """The element matrix as a host double[64] (row-major, zero padded).  Cached per KE
-    tensor and version (reading a device KE back synchronizes the stream); an entry
-    keeps its KE alive, so the id cannot be reused while cached.  Callers only read
-    the array."""
-    key = (id(KE), KE._version)
+    """The element matrix as a host double[64] (row-major, zero padded).  Cached per
+    storage, layout and version (reading a device KE back synchronizes the stream), so
+    `KE.detach()` and other aliases of the same memory hit the same entry; an entry
+    keeps its KE, and with it the memory, alive, so the address cannot be reused while
+    cached.  Callers only read the array."""
+    key = (str(KE.device), KE.data_ptr(), tuple(KE.shape), tuple(KE.stride()), KE.dtype,
+           KE._version)
     with _KE_HOST_LOCK:
         hit = _KE_HOST.get(key)
-        if hit is not None and hit[0] is KE:
+        if hit is not None:
             return hit[1]
     flat = KE.detach().to("cpu", torch.float64).reshape(-1).tolist()
     if len(flat) not in (16, 64):

@@ -13,6 +13,8 @@
 #include <cstdio>
 #include <mutex>
 
+#include <hip/hip_ext.h>
+
 #include "stt_kernels.h"
 
 namespace stt {
@@ -54,6 +56,17 @@ struct StencilArgs {
   long long spr;
   T* bcopy;
   long long sbc;
+  // kOperatorDir: u holds p_old; the staged (and applied) vector is p_new = z + beta p_old;
+  // the tile's own nodes write p_new to dp and, for rows with iters == dk, take
+  // x += alpha p_old
+  double* dp;
+  const void* dz;
+  double* dx;
+  long long sdx;
+  const double* dalpha;
+  const double* dbeta;
+  const int* diters;
+  int dk, dz_f32;
   KE<T> ke;
 };
 
@@ -319,17 +332,74 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
   constexpr int kMW = kTileX * D / 32 + 1;
   constexpr bool kOutMask = MODE != kMatvec;
   __shared__ uint32_t sMW[kOutMask ? kTileY : 1][kOutMask ? kMW : 1];
+  constexpr bool kDir = MODE == kOperatorDir;
   const int t = xcd_remap(bid, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
-  if (a.active != nullptr && a.active[row] == 0) return;  // whole block: uniform
   const int by = tt / a.ntx, bx = tt - by * a.ntx;
   const int x0 = bx * kTileX, y0 = by * kTileY;
   const int lane = threadIdx.x & (kWave - 1);
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);  // wave-uniform
   const T* __restrict__ u = a.u + row * a.su;
+  if (a.active != nullptr && a.active[row] == 0) {  // whole block: uniform
+    if constexpr (kDir) {
+      // a row that retired in iteration dk still takes that iteration's x += alpha p
+      if (a.diters[row] == a.dk) {
+        const double al = a.dalpha[row];
+        double* xr = a.dx + row * a.sdx;
+        const int gx = x0 + lane;
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          const int gy = y0 + wave * NR + j;
+          if (gy < a.nny && gx < a.nnx) {
+            const long long node = (long long)gy * a.nnx + gx;
+            const Vec<double, D> pv = load_node<double, D>(u, node);
+            Vec<double, D> xv = load_node<double, D>(xr, node);
+#pragma unroll
+            for (int q = 0; q < D; ++q) xv.v[q] += al * pv.v[q];
+            store_node<double, D>(xr, node, xv);
+          }
+        }
+      }
+    }
+    return;
+  }
   const T* __restrict__ rho = kParam ? a.prho + row * a.spr : a.rho + row * a.sr;
-  auto stage_node = [&](long long node) -> Vec<T, D> { return load_node<T, D>(u, node); };
+  // kOperatorDir: beta and alpha of this row, and whether it takes x += alpha p_old
+  const double dbe = kDir ? a.dbeta[row] : 0.0;
+  const double dal = kDir ? a.dalpha[row] : 0.0;
+  const bool dxup = kDir && a.diters[row] == a.dk;
+  auto stage_node = [&](long long node, bool owned) -> Vec<T, D> {
+    if constexpr (kDir) {
+      // p_new = z + beta p_old, the expression of the unfused direction update
+      const Vec<double, D> pv = load_node<double, D>(u, node);
+      Vec<double, D> zv;
+      if (a.dz_f32) {
+        const Vec<float, D> zf =
+            load_node<float, D>(static_cast<const float*>(a.dz) + row * a.su, node);
+#pragma unroll
+        for (int q = 0; q < D; ++q) zv.v[q] = (double)zf.v[q];
+      } else {
+        zv = load_node<double, D>(static_cast<const double*>(a.dz) + row * a.su, node);
+      }
+      Vec<double, D> pn;
+#pragma unroll
+      for (int q = 0; q < D; ++q) pn.v[q] = zv.v[q] + dbe * pv.v[q];
+      if (owned) {
+        store_node<double, D>(a.dp + row * a.su, node, pn);
+        if (dxup) {
+          double* xr = a.dx + row * a.sdx;
+          Vec<double, D> xv = load_node<double, D>(xr, node);
+#pragma unroll
+          for (int q = 0; q < D; ++q) xv.v[q] += dal * pv.v[q];
+          store_node<double, D>(xr, node, xv);
+        }
+      }
+      return pn;
+    } else {
+      return load_node<T, D>(u, node);
+    }
+  };
   // Stage the tile.  Wavefront w loads halo rows w, w + 4, ...; lane l loads column
   // x0 - 1 + l of each, so column validity is per lane and row validity per wavefront.
   // The two trailing node columns and the trailing element column of the halo are
@@ -365,7 +435,9 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
         wb[n] = load_node<TB, D>(a.b + row * a.sb, node);
         wd[n] = load_node<T, D>(a.invd + row * a.sd, node);
       } else {
-        wu[n] = stage_node(node);
+        // the tile's own nodes: rows 1 .. kTileY, lanes 1 .. 63 (column x0 + 63 is a
+        // trailing column)
+        wu[n] = stage_node(node, i >= 1 && i <= kTileY && lane >= 1);
       }
       if (a.proj_in) wm[n] = a.mask[(node * D) >> 5];
     }
@@ -408,7 +480,7 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
       tb = load_node<TB, D>(a.b + row * a.sb, node);
       td = load_node<T, D>(a.invd + row * a.sd, node);
     } else {
-      tu = stage_node(node);
+      tu = stage_node(node, tk == kTileX && ti >= 1 && ti <= kTileY);
     }
   }
   Vec<T, 2> tr2;
@@ -699,7 +771,7 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
         const bool fr = (word >> ((node * D + ii) & 31)) & 1u;
         T out = T(0);
         if (fr) {
-          if constexpr (MODE == kMatvec || MODE == kOperator) {
+          if constexpr (MODE == kMatvec || MODE == kOperator || MODE == kOperatorDir) {
             out = acc[j][ii];
           } else if constexpr (MODE == kResid) {
             out = static_cast<T>(bv.v[ii]) - acc[j][ii];
@@ -921,6 +993,62 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     return STTOPT_EINVAL;
   }
   STT_CHECK_LAUNCH("stencil_kernel");
+  return STTOPT_OK;
+}
+
+int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
+                 const void* z, int z_f32, double* x, long long xs, const double* alpha,
+                 const double* beta, const int* iters, int k, double* Ap,
+                 const int* active, double* partial, hipStream_t s, hipEvent_t start,
+                 hipEvent_t stop) {
+  if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
+  const long long n = op.ndof();
+  StencilArgs<double, double, double> a;
+  int src = 0;
+  bool param_tile = false;
+  if (int e = stencil_args<double, double, double>(op, rows, kOperator, false, kDotU, p_old, n,
+                                                   nullptr, 0, nullptr, 0, Ap, n, 0.0, active,
+                                                   partial, nullptr, 0, a, src, param_tile))
+    return e;
+  if (src != kSrcDensity || active == nullptr || p_new == nullptr || z == nullptr ||
+      x == nullptr || alpha == nullptr || beta == nullptr || iters == nullptr ||
+      p_new == p_old) {
+    set_error("operator_dir: level-0 density operator with separate p buffers, z, x, "
+              "alpha, beta, iteration counts and active flags");
+    return STTOPT_EINVAL;
+  }
+  if (op.d == 2 && (!pair_aligned(p_new, n, sizeof(double)) ||
+                    !pair_aligned(x, xs, sizeof(double)) ||
+                    !pair_aligned(z, n, z_f32 ? sizeof(float) : sizeof(double)))) {
+    set_error("operator_dir: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  a.mode = kOperatorDir;
+  a.dp = p_new;
+  a.dz = z;
+  a.dz_f32 = z_f32 ? 1 : 0;
+  a.dx = x;
+  a.sdx = xs;
+  a.dalpha = alpha;
+  a.dbeta = beta;
+  a.diters = iters;
+  a.dk = k;
+  const dim3 grid((unsigned)a.total), block(kBlock);
+  auto launch = [&](auto kernel) {
+    if (start != nullptr || stop != nullptr)
+      hipExtLaunchKernelGGL(kernel, grid, block, 0, s, start, stop, 0, a);
+    else
+      hipLaunchKernelGGL(kernel, grid, block, 0, s, a);
+  };
+  if (op.d == 2)
+    launch(stencil_tile_kernel<double, double, double, 2, kSrcDensity, kOperatorDir, kDotU>);
+  else if (op.d == 1)
+    launch(stencil_tile_kernel<double, double, double, 1, kSrcDensity, kOperatorDir, kDotU>);
+  else {
+    set_error("operator_dir: d must be 1 or 2");
+    return STTOPT_EINVAL;
+  }
+  STT_CHECK_LAUNCH("operator_dir");
   return STTOPT_OK;
 }
 

@@ -10,12 +10,14 @@
 //   A_KK <- -D
 //
 // After every block has been swept the matrix holds -A^-1; the last step writes with
-// the sign flipped.  The matrix stays symmetric throughout, so the update is computed
-// on the lower block triangle only and each tile is written to both halves.
+// the sign flipped.  The matrix stays symmetric throughout, so only its lower block
+// triangle is kept up to date: the update computes and writes the lower tiles, and the
+// last step also writes each tile's mirror, leaving the whole inverse.
 //
 // Per pivot block (64 wide): sweep_panel_kernel forms P = D R from the row panel
-// R = A_K:, and sweep_update_kernel writes every tile from A, R, P and D -- it reads no
-// tile that another workgroup of the same launch writes.  D of the first block comes from
+// R = A_K: (its tiles right of the diagonal read as the transposes of the lower tiles
+// below it), and sweep_update_kernel writes every lower tile from A, R, P and D -- it
+// reads no tile that another workgroup of the same launch writes.  D of the first block comes from
 // sweep_pivot_kernel; the update launch of step K inverts the updated diagonal block
 // K + 1 in the workgroup that writes it (dispatched first), so the next pivot block is
 // ready when step K ends.
@@ -185,6 +187,8 @@ __device__ __forceinline__ void gemm_tile_kmajor(const double* __restrict__ X, l
 }
 
 // R = A[K rows, column block J] (copy) and P = D R for column block J = blockIdx.x.
+// Only the lower block triangle of A is current: a block J > K of the panel is the
+// transpose of the lower tile (J, K), transposed through LDS (coalesced on both sides).
 __global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
                                                           const double* __restrict__ a,
                                                           long long sa,
@@ -193,19 +197,39 @@ __global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
                                                           double* __restrict__ P) {
   __shared__ __align__(16) double sX[kKC * kSB];
   __shared__ __align__(16) double sY[kKC * kSB];
+  __shared__ double sT[kSB][kSB + 1];
   const int row = blockIdx.y;
   const int j0 = blockIdx.x * kSB;
   const int bj = min(kSB, n - j0);
-  const double* Ak = a + row * sa + (long long)k0 * n + j0;  // row k of the panel: Ak + k n
+  const double* A = a + row * sa;
   double* Rr = R + (long long)row * kSB * n + j0;
   double* Pr = P + (long long)row * kSB * n + j0;
-  for (int e = threadIdx.x; e < kSB * kSB; e += 256) {
-    const int k = e / kSB, c = e & (kSB - 1);
-    if (k < bk && c < bj) Rr[(long long)k * n + c] = Ak[(long long)k * n + c];
+  // the panel block as a k-major operand (row k at Y + k n)
+  const double* Y;
+  if (j0 > k0) {
+    // R(k, c) = A(j0 + c, k0 + k): rows of the lower tile into LDS, then rows of R out
+    for (int e = threadIdx.x; e < kSB * kSB; e += 256) {
+      const int c = e / kSB, k = e & (kSB - 1);
+      sT[c][k] = (k < bk && c < bj) ? A[(long long)(j0 + c) * n + (k0 + k)] : 0.0;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < kSB * kSB; e += 256) {
+      const int k = e / kSB, c = e & (kSB - 1);
+      if (k < bk && c < bj) Rr[(long long)k * n + c] = sT[c][k];
+    }
+    __syncthreads();  // the block's writes of R are visible to its own reads below
+    Y = Rr;
+  } else {
+    const double* Ak = A + (long long)k0 * n + j0;  // row k of the panel: Ak + k n
+    for (int e = threadIdx.x; e < kSB * kSB; e += 256) {
+      const int k = e / kSB, c = e & (kSB - 1);
+      if (k < bk && c < bj) Rr[(long long)k * n + c] = Ak[(long long)k * n + c];
+    }
+    Y = Ak;
   }
   double out[16] = {};
   // P(i, j) = sum_k D(i, k) R(k, j); D is symmetric, so its rows serve as X(k, i)
-  gemm_tile_kmajor(D + (long long)row * kSB * kSB, kSB, bk, Ak, n, bj, bk, out, sX, sY);
+  gemm_tile_kmajor(D + (long long)row * kSB * kSB, kSB, bk, Y, n, bj, bk, out, sX, sY);
 #pragma unroll
   for (int q = 0; q < 16; ++q) {
     const int i = tile_row(q), j = tile_col(q);
@@ -213,11 +237,11 @@ __global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
   }
 }
 
-// Lower-triangle tile t = (I, J), I >= J, of matrix blockIdx.x: every tile of the
-// matrix is written once per sweep step (the tile and, off the diagonal, its mirror
-// through an LDS transpose).  A workgroup reads only its own tile of A, and no other
+// Lower-triangle tile t = (I, J), I >= J, of matrix blockIdx.x: every lower tile is
+// written once per sweep step, and in the last step also, off the diagonal, its mirror
+// (through an LDS transpose).  A workgroup reads only its own tile of A, and no other
 // workgroup writes that tile or reads the mirror.
-__global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0, int bk,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void sweep_update_kernel(int n, int kb, int k0, int bk,
                                                            double* __restrict__ a, long long sa,
                                                            const double* __restrict__ D,
                                                            const double* __restrict__ R,
@@ -304,8 +328,8 @@ __global__ __launch_bounds__(256) void sweep_update_kernel(int n, int kb, int k0
       Dr2[i * kSB + jj] = (i < bi && jj < bj) ? -m[q] : 0.0;
     }
   }
-  if (I != J) {
-    // mirror tile (J, I): element (r, c) = out(c, r), written row by row
+  if (I != J && sign < 0.0) {
+    // the last step: mirror tile (J, I), element (r, c) = out(c, r), written row by row
     __syncthreads();  // the product's last chunk is consumed before sT overwrites it
 #pragma unroll
     for (int q = 0; q < 16; ++q) sT[tile_row(q)][tile_col(q)] = out[q];

@@ -40,7 +40,16 @@ __device__ __forceinline__ void load_record(const T* p, T (&out)[N]) {
 // kSmoothFirst (level 0 only): two damped-Jacobi sweeps from x = 0 in one pass, the
 // first one (x0 = w D^-1 b) formed while staging the tile; also copies b into bcopy in
 // the stencil precision when given.
-enum StencilMode { kMatvec = 0, kOperator = 1, kSmooth = 2, kResid = 3, kSmoothFirst = 4 };
+// kOperatorDir: the PCG's direction update fused into the level-0 fp64 operator (the
+// staged vector is p_new = z + beta p_old; see operator_dir).
+enum StencilMode {
+  kMatvec = 0,
+  kOperator = 1,
+  kSmooth = 2,
+  kResid = 3,
+  kSmoothFirst = 4,
+  kOperatorDir = 5
+};
 enum DotKind { kNoDot = 0, kDotU = 1, kDotB = 2 };
 
 // One level's operator, type-erased.  Exactly one of rho (level 0: density * KE)
@@ -86,6 +95,19 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
             long long su, const TB* b, long long sb, const T* invd, long long sd,
             TY* y, long long sy, double omega, const int* active, double* partial,
             hipStream_t s, T* bcopy = nullptr, long long sbc = 0);
+
+// One PCG step's direction update fused into the level-0 fp64 apply (d = 1 or 2,
+// density operator): for the rows with iters == k, x += alpha p_old at every node; for
+// the active rows also p_new = z + beta p_old (z fp32 when z_f32, else fp64), Ap = K
+// p_new at free dofs and the per-tile p_new . Ap partials.  p_old and p_new are
+// different buffers (each tile reads p_old on its halo), all rows with stride n.
+// start / stop (optional): events that the launch itself records at the kernel's start
+// and end (no marker of their own in the stream).
+int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
+                 const void* z, int z_f32, double* x, long long xs, const double* alpha,
+                 const double* beta, const int* iters, int k, double* Ap,
+                 const int* active, double* partial, hipStream_t s,
+                 hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
 
 // ---- one launch over the same level of several hierarchies (the uneven batch) ----
 // Each kernel below runs the same pass on up to kMaxBatchGroups hierarchies of different

@@ -15,6 +15,8 @@
 #include <memory>
 #include <vector>
 
+#include <hip/hip_ext.h>
+
 #include "stt_kernels.h"
 
 namespace stt {
@@ -901,6 +903,9 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   double* p = work + 2 * rows * n;
   double* Ap = work + 3 * rows * n;
   double* r2 = work + 4 * rows * n;  // the other residual buffer of the fused update
+  // the search direction of iteration k is in pbuf[k & 1]: the fused direction update
+  // (operator_dir) reads p_k on every tile's halo while writing p_{k+1}
+  double* pbuf[2] = {work + 5 * rows * n, p};
   TileGeom g{op.nnx(), op.nny(), op.ntx(), tiles, tiles * rows, mg->d};
   const dim3 tblock(kBlock), rgrid(rows);
   *n_iter = 0;
@@ -934,7 +939,7 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
   hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2], sc.active,
                      sc.rz, sc.counter);
-  if (int e = convert<T, double>(n, rows, z, n, p, n, sc.active, s)) return e;
+  if (int e = convert<T, double>(n, rows, z, n, pbuf[1], n, sc.active, s)) return e;
   STT_CHECK_LAUNCH("pcg first preconditioner");
 
   // The fp32 pair cycle takes the residual update r <- r - alpha Ap in its pre-smoothing
@@ -961,9 +966,10 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   // writes rbuf[k & 1]; without it the residual stays in rbuf[0].
   double* rbuf[2] = {r, r2};
   // The second half of an iteration -- the coarse visit and post-smoothing (or the
-  // whole cycle), fin_beta and the direction update -- is the same launch sequence every
-  // iteration up to the residual buffer, so it is captured once per buffer parity into
-  // a HIP graph and replayed (about 25 launches in one submission).
+  // whole cycle) and fin_beta -- is the same launch sequence every iteration up to the
+  // residual buffer, so it is captured once per buffer parity into a HIP graph and
+  // replayed (about 25 launches in one submission).  The direction update p = z + beta p
+  // and the x += alpha p it carries run in the next iteration's apply (operator_dir).
   // Launches on stream st (the solve's stream, or the capture stream).
   auto rest = [&](int parity, hipStream_t st) -> int {
     struct Restore {
@@ -980,26 +986,31 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     hipLaunchKernelGGL(fin_beta_kernel, rgrid, tblock, 0, st, rz_tiles, sc.part[2],
                        fused ? fused_tiles : tiles, sc.part[3], flexible, sc.active, sc.rz,
                        sc.beta, sc.counter);
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, st, n, sc.alpha, sc.beta, x, xs, p, z,
-                    sc.active, iters, 0, 1, static_cast<const int*>(kdev));
-    STT_CHECK_LAUNCH("pcg direction");
+    STT_CHECK_LAUNCH("pcg beta");
     return STTOPT_OK;
   };
   const bool graphs = g_pcg_graphs && max_iter > 1;
   bool captured[2] = {false, false};
   for (int k = 1; k <= max_iter; ++k) {
-    const size_t pi = g_apply_prof.iter_of.size();
-    if (g_apply_prof.on) {
-      if (int e = prof_events(pi + 1)) return e;
-      hipEventRecord(g_apply_prof.pool[2 * pi], s);
-    }
-    if (int e = stencil<double, double, double>(op, rows, kOperator, false, kDotU, p, n,
-                                                (const double*)nullptr, 0, nullptr, 0, Ap, n,
-                                                0.0, sc.active, sc.part[0], s))
-      return e;
-    if (g_apply_prof.on) {
-      hipEventRecord(g_apply_prof.pool[2 * pi + 1], s);
-      g_apply_prof.iter_of.push_back(k);
+    if (k == 1) {
+      if (int e = stencil<double, double, double>(op, rows, kOperator, false, kDotU, pbuf[1],
+                                                  n, (const double*)nullptr, 0, nullptr, 0, Ap,
+                                                  n, 0.0, sc.active, sc.part[0], s))
+        return e;
+    } else {
+      // the profiled launches record their events as part of the dispatch
+      hipEvent_t t0 = nullptr, t1 = nullptr;
+      if (g_apply_prof.on) {
+        const size_t pi = g_apply_prof.iter_of.size();
+        if (int e = prof_events(pi + 1)) return e;
+        t0 = g_apply_prof.pool[2 * pi];
+        t1 = g_apply_prof.pool[2 * pi + 1];
+        g_apply_prof.iter_of.push_back(k);
+      }
+      if (int e = operator_dir(op, rows, pbuf[(k - 1) & 1], pbuf[k & 1], z,
+                               sizeof(T) == sizeof(float), x, xs, sc.alpha, sc.beta, iters,
+                               k - 1, Ap, sc.active, sc.part[0], s, t0, t1))
+        return e;
     }
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
                        sc.rz, sc.alpha, kdev);
@@ -1019,9 +1030,13 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
     const bool check = k < max_iter && k % check_every == 0;
     const int cur = checks & 1, prev = cur ^ 1;
     if (check) check_ids[cur] = next_check_id();
-    hipLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, fuse_now ? fused_tiles : tiles,
-                       sc.part[1], rtol, k, sc.bsafe, resid, sc.active, iters, sc.counter,
-                       check ? lag_dev + 1 + cur : nullptr, check ? check_ids[cur] : 0);
+    // a check's event is recorded by the fin_resid launch itself (no marker of its own
+    // between the launches)
+    hipExtLaunchKernelGGL(fin_resid_kernel, rgrid, tblock, 0, s, nullptr,
+                          check ? lag_ev[cur] : nullptr, 0, fuse_now ? fused_tiles : tiles,
+                          (const double*)sc.part[1], rtol, k, (const double*)sc.bsafe, resid,
+                          sc.active, iters, sc.counter, check ? lag_dev + 1 + cur : nullptr,
+                          check ? check_ids[cur] : 0);
     STT_CHECK_LAUNCH("pcg update");
     k_last = k;
     if (k == max_iter) {
@@ -1034,12 +1049,11 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
       // wait; the host reads the previous check's mark, by which time this iteration
       // is queued behind it, so the device never drains.  An unmarked check means
       // every row converged there: this iteration's launches found no active row, and
-      // the converged rows' last x += alpha p was in that iteration's direction update.
-      // Checks alternate between two pinned slots and events.
-      hipError_t e = hipEventRecord(lag_ev[cur], s);
-      if (e != hipSuccess) return hip_status(e, "hipEventRecord");
+      // the converged rows' last x += alpha p was in this iteration's fused apply (the
+      // direction update of the iteration after the check).  Checks alternate between
+      // two pinned slots and events.
       if (checks > 0) {
-        e = hipEventSynchronize(lag_ev[prev]);
+        hipError_t e = hipEventSynchronize(lag_ev[prev]);
         if (e != hipSuccess) return hip_status(e, "hipEventSynchronize");
         if (reinterpret_cast<volatile int*>(lag_host)[1 + prev] != check_ids[prev]) {
           done = true;
@@ -1067,8 +1081,9 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   }
   if (k_last > 0) {
     // the last iteration's x += alpha p (no direction update follows it)
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs, p, z,
-                    sc.active, iters, k_last, 0, static_cast<const int*>(nullptr));
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs,
+                    pbuf[k_last & 1], z, sc.active, iters, k_last, 0,
+                    static_cast<const int*>(nullptr));
     STT_CHECK_LAUNCH("pcg last x update");
   }
   // n_iter: the slowest row's count
@@ -1080,16 +1095,26 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   int mx = 0;
   for (int i = 0; i < rows; ++i) mx = h[i] > mx ? h[i] : mx;
   if (g_apply_prof.on && !g_apply_prof.iter_of.empty()) {
-    const double per_row = 8.0 * (2.0 * (double)n + (double)op.nel());
+    // the fused launches (iteration 2 on; iteration 1's plain apply is not counted):
+    // algorithmic bytes per row still iterating: p_old, z, x and rho read, p_new, x and Ap
+    // written; per row that retired in the iteration before: p_old and x read, x written
+    const double nd = (double)n, ne = (double)op.nel();
+    const double fused = 8.0 * (5.0 * nd + ne) + (double)sizeof(T) * nd;
+    const double retiring = 8.0 * 3.0 * nd;
     for (size_t i = 0; i < g_apply_prof.iter_of.size(); ++i) {
       float ms = 0.0f;
       e = hipEventElapsedTime(&ms, g_apply_prof.pool[2 * i], g_apply_prof.pool[2 * i + 1]);
       if (e != hipSuccess) return hip_status(e, "hipEventElapsedTime");
-      int act = 0;
-      for (int r = 0; r < rows; ++r) act += h[r] >= g_apply_prof.iter_of[i] ? 1 : 0;
+      const int k = g_apply_prof.iter_of[i];
+      if (k == 1) continue;
+      int act = 0, ret = 0;
+      for (int r = 0; r < rows; ++r) {
+        act += h[r] >= k ? 1 : 0;
+        ret += h[r] == k - 1 ? 1 : 0;
+      }
       if (act == 0) continue;  // the lagged check's launch after convergence: no rows
       g_apply_prof.seconds += ms * 1e-3;
-      g_apply_prof.bytes += per_row * act;
+      g_apply_prof.bytes += fused * act + retiring * ret;
       g_apply_prof.launches += 1;
     }
     g_apply_prof.iter_of.clear();
@@ -1784,8 +1809,9 @@ int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg) {
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {
   if (mg == nullptr || mg->nlevels < 1) return 0;
   const long long n = (long long)(mg->levels[0].nelx + 1) * (mg->levels[0].nely + 1) * mg->d;
-  // r, z, p, Ap and the second residual buffer of the fused update
-  return 5LL * mg->rows * n;
+  // r, z, p, Ap, the second residual buffer of the fused update and the second search
+  // direction buffer of the fused direction update
+  return 6LL * mg->rows * n;
 }
 
 int sttopt_mgpcg_graphs(int32_t enable) {

@@ -836,8 +836,9 @@ def native_mgpcg(vcycle: VCycle, b: Tensor, *, rtol: float, max_iter: int,
         iters.data_ptr(), resid.data_ptr(), ctypes.byref(n_iter), _lib.stream_handle(dev),
     )
     if info is not None:
-        info["row_iters"] = int(iters.sum())
-        info["row_iter_list"] = iters.tolist()
+        per_row = iters.tolist()  # one read back
+        info["row_iters"] = int(sum(per_row))
+        info["row_iter_list"] = per_row
     if status == _lib.ENOCONV:
         raise torch_fem.CGConvergenceError(resid.reshape(batch) if batch else resid[0], int(max_iter), rtol)
     _lib.check(status, "sttopt_mgpcg")
