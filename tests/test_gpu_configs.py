@@ -399,3 +399,43 @@ def test_c5_large_space_time_step_size_independent_checks(dev):
     print(f"C5: fused vs composed gradient rel L2 {ex:.2e} (x), {et:.2e} (t)")
     assert ex < 1e-5 and et < 1e-5
     assert sp is not None
+
+
+def test_uneven_instances_with_different_case_counts_and_masks(dev):
+    """Instances of one grid with 1, 2 and 3 load cases and two different free-dof masks,
+    plus one of another grid: each instance's U (from one batched MG-PCG) equals its own
+    single-instance femsolve to 1e-9, and each instance's density gradient of
+    sum(F . U) matches the single-instance gradient."""
+    from sttopt_amd import torch_fem, torch_solve, uneven, workloads
+
+    specs = workloads.uneven_batch(6, seed=3, sizes=[(64, 32), (128, 64)])
+    KE = torch.tensor(fem_direct.plane_stress_KE(NU), dtype=torch.float64, device=dev)
+    cases = [1, 2, 3, 2, 1, 2]
+    insts, xs, Fs = [], [], []
+    for i, (s, nc) in enumerate(zip(specs, cases)):
+        ndof = s["loads"].shape[1]
+        free = s["freedofs"] if i % 2 == 0 else s["freedofs"][2:]  # a second mask
+        mask = torch.zeros(ndof, dtype=torch.bool, device=dev)
+        mask[torch.as_tensor(free, device=dev)] = True
+        loads = np.concatenate([s["loads"], s["loads"][:1] * 0.5])[:nc]
+        F = torch.tensor(loads, dtype=torch.float64, device=dev)
+        x = torch.tensor(s["x"], dtype=torch.float64, device=dev).requires_grad_(True)
+        xs.append(x)
+        Fs.append(F)
+        insts.append(uneven.Instance(
+            s["nelx"], s["nely"], torch_fem.simp_density(x.flatten(), EMIN, EMAX, PENAL), F,
+            mask, torch.tensor(fem_direct.element_dofs(s["nelx"], s["nely"]), device=dev), KE))
+    info = {}
+    Us = uneven.solve_instances(insts, rtol=1e-10, info=info)
+    print("uneven groups:", info["groups"])
+    assert info["batched"]
+    sum(torch.sum(F * U) for F, U in zip(Fs, Us)).backward()
+    for inst, U, x, F, nc in zip(insts, Us, xs, Fs, cases):
+        assert U.shape == (nc, F.shape[1])
+        x1 = x.detach().clone().requires_grad_(True)
+        d1 = torch_fem.simp_density(x1.flatten(), EMIN, EMAX, PENAL)
+        U1 = torch_solve.femsolve(d1.expand(nc, -1), F, inst.edofMat, KE, inst.mask, inst.nelx,
+                                  inst.nely, rtol=1e-10)
+        torch.sum(F * U1).backward()
+        assert rel_err(U.detach().cpu().numpy(), U1.detach().cpu().numpy()) < 1e-9
+        assert rel_err(x.grad.cpu().numpy(), x1.grad.cpu().numpy()) < 1e-8
