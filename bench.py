@@ -508,9 +508,10 @@ def apply_roofline(torch, _lib, torch_fem, p, rows, dev, live, launches=50):
 
 
 def uneven_workload(torch, dev, n_instances=64):
-    """The uneven batch as two callables that solve it with compliance gradients:
-    `batched(info=None)` (one batched MG-PCG, `uneven.solve_instances`) and `per_grid()`
-    (one `femsolve` per grid)."""
+    """The uneven batch as two callables that solve it with compliance gradients, both
+    forming the densities and the compliance per grid: `batched(info=None)` (the
+    instances, rows of their grid's density, in one batched MG-PCG,
+    `uneven.solve_instances`) and `per_grid()` (one `femsolve` per grid)."""
     from sttopt_amd import compliance, fem, torch_fem, torch_solve, uneven, workloads
 
     specs = workloads.uneven_batch(n_instances, seed=0)
@@ -523,18 +524,26 @@ def uneven_workload(torch, dev, n_instances=64):
         base.append((s, mask, edof, torch.as_tensor(s["loads"], dtype=torch.float64, device=dev)))
 
     def batched(info=None):
-        xs, insts = [], []
+        # as per_grid below: the densities and the compliance formed per grid (one
+        # stacked tensor each); the instances are rows of those, all solved together
+        grids = {}
         for s, mask, edof, F in base:
-            x = torch.as_tensor(s["x"], dtype=torch.float64, device=dev).requires_grad_(True)
-            xs.append(x)
-            insts.append(uneven.Instance(s["nelx"], s["nely"],
-                                         torch_fem.simp_density(x.flatten(), 1e-9, 1.0, 3.0),
-                                         F, mask, edof, KE))
+            grids.setdefault((s["nelx"], s["nely"]), []).append((s, mask, edof, F))
+        insts, dens_of = [], []
+        for (nelx, nely), items in grids.items():
+            x = torch.stack([torch.as_tensor(s["x"], dtype=torch.float64, device=dev).flatten()
+                             for s, _, _, _ in items]).requires_grad_(True)
+            dens = torch_fem.simp_density(x, 1e-9, 1.0, 3.0)
+            dens_of.append((nelx, nely, dens, len(items)))
+            for i, (s, mask, edof, F) in enumerate(items):
+                insts.append(uneven.Instance(nelx, nely, dens[i], F, mask, edof, KE))
         Us = uneven.solve_instances(insts, info=info)
-        total = 0
-        for inst, U in zip(insts, Us):
-            total = total + torch.sum(inst.density * compliance._StrainEnergy.apply(
-                U, KE, inst.nelx, inst.nely))
+        total, start = 0, 0
+        for nelx, nely, dens, n in dens_of:
+            U = torch.cat(Us[start:start + n])
+            start += n
+            ce = compliance._StrainEnergy.apply(U, KE, nelx, nely)
+            total = total + torch.sum(dens.repeat_interleave(2, dim=0) * ce)
         total.backward()
 
     def per_grid():
