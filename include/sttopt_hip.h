@@ -222,6 +222,18 @@ int sttopt_galerkin(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t k
                     int64_t fine_keff_stride, const double* s, double* coarse_keff,
                     int64_t coarse_keff_stride, void* stream);
 
+/* Jacobi diagonal (fp64, 1 at fixed dofs) of the nelx x nely level whose element
+ * matrices are sum over the kx x ky level-0 elements g under each element of
+ * rho_g G_pos(g) (kx ky <= 4): density is the level-0 field (rows, (kx nelx)(ky nely)),
+ * table_diag a host array of the ky x kx positions' diagonals (4d each, position-major),
+ * mask the level's free-dof bits.  Lets that level skip forming its element matrices.
+ * No counterpart in the reference (whose diagonal comes from the assembled coarse
+ * matrices, torch_mg.py:377-378). */
+int sttopt_param_diagonal(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t ky,
+                          int32_t rows, const double* density, int64_t density_stride,
+                          const double* table_diag, const uint32_t* mask, double* out,
+                          int64_t out_stride, void* stream);
+
 /* Jacobi diagonal of a keff level: 1/diag at free dofs and 0 at fixed (inv != 0),
  * or diag with 1 at fixed dofs (inv == 0).  Replaces torch_mg.py:377-378. */
 int sttopt_keff_diagonal(int32_t dtype_out, int32_t d, int32_t nelx, int32_t nely,

@@ -135,6 +135,11 @@ _SIGNATURES = {
         [c_int32, c_int32, c_int32, c_int32, c_int32, _P, c_int64, _P, c_int32, _P, c_int64,
          _P],
     ),
+    "sttopt_param_diagonal": (
+        c_int32,
+        [c_int32, c_int32, c_int32, c_int32, c_int32, c_int32, _P, c_int64, _P, _P, _P,
+         c_int64, _P],
+    ),
     "sttopt_invert_diagonal": (
         c_int32, [c_int32, c_int64, c_int32, _P, c_int64, _P, _P, c_int64, _P]
     ),

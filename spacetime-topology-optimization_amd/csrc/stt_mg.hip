@@ -832,6 +832,88 @@ int sttopt_keff_diagonal(int32_t dtype_out, int32_t d, int32_t nelx, int32_t nel
 }  // extern "C"
 
 namespace stt {
+
+// Per-position diagonals of the level-0 element matrices under one element of a level
+// applied from the level-0 densities: v[(gy kx + gx) 4d + a], at most 4 positions.
+struct ParamDiag {
+  double v[4 * 8];
+};
+
+// Jacobi diagonal of the level (nelx x nely elements, kx x ky level-0 elements under
+// each) whose element matrices are sum_g rho_g G_pos(g): one thread per node, the four
+// elements around it in a fixed order, fp64; 1 at fixed dofs.
+template <int D>
+__global__ __launch_bounds__(256) void param_diagonal_kernel(int nelx, int nely, int kx, int ky,
+                                                             const double* rho, long long sr,
+                                                             ParamDiag pd, const uint32_t* mask,
+                                                             double* out, long long so) {
+  constexpr int K = 4 * D;
+  const int nnx = nelx + 1, nny = nely + 1;
+  const long long nnodes = (long long)nnx * nny;
+  const long long node = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int row = blockIdx.y;
+  if (node >= nnodes) return;
+  const int r = (int)(node / nnx), c = (int)(node - (long long)r * nnx);
+  const long long fnelx = (long long)nelx * kx;
+  const double* rr = rho + row * sr;
+  double acc[D];
+#pragma unroll
+  for (int i = 0; i < D; ++i) acc[i] = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int er = r - 1 + (q >> 1), ec = c - 1 + (q & 1);
+    if (er < 0 || er >= nely || ec < 0 || ec >= nelx) continue;
+    constexpr int locs[4] = {1, 0, 2, 3};  // the node's corner of that element
+    const int loc = locs[q];
+    for (int gy = 0; gy < ky; ++gy) {
+      for (int gx = 0; gx < kx; ++gx) {
+        const double rg = rr[(long long)(ky * er + gy) * fnelx + (long long)kx * ec + gx];
+        const double* t = pd.v + (gy * kx + gx) * K + loc * D;
+#pragma unroll
+        for (int i = 0; i < D; ++i) acc[i] += rg * t[i];
+      }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    const long long dof = node * D + i;
+    out[row * so + dof] = mask_free(mask, dof) ? acc[i] : 1.0;
+  }
+}
+
+}  // namespace stt
+
+extern "C" {
+
+int sttopt_param_diagonal(int32_t d, int32_t nelx, int32_t nely, int32_t kx, int32_t ky,
+                          int32_t rows, const double* density, int64_t density_stride,
+                          const double* table_diag, const uint32_t* mask, double* out,
+                          int64_t out_stride, void* stream) {
+  if ((d != 1 && d != 2) || nelx <= 0 || nely <= 0 || kx < 1 || ky < 1 || kx * ky > 4 ||
+      rows < 0 || density == nullptr || table_diag == nullptr || mask == nullptr ||
+      out == nullptr) {
+    set_error("sttopt_param_diagonal: invalid arguments (d 1 or 2, at most 4 positions)");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  ParamDiag pd{};
+  for (int i = 0; i < kx * ky * 4 * d; ++i) pd.v[i] = table_diag[i];
+  const long long nnodes = (long long)(nelx + 1) * (nely + 1);
+  const dim3 grid((unsigned)ceil_div(nnodes, 256), (unsigned)rows), block(256);
+  hipStream_t s = (hipStream_t)stream;
+  if (d == 2)
+    hipLaunchKernelGGL(param_diagonal_kernel<2>, grid, block, 0, s, nelx, nely, kx, ky, density,
+                       (long long)density_stride, pd, mask, out, (long long)out_stride);
+  else
+    hipLaunchKernelGGL(param_diagonal_kernel<1>, grid, block, 0, s, nelx, nely, kx, ky, density,
+                       (long long)density_stride, pd, mask, out, (long long)out_stride);
+  STT_CHECK_LAUNCH("param_diagonal_kernel");
+  return STTOPT_OK;
+}
+
+}  // extern "C"
+
+namespace stt {
 template <typename TO>
 __global__ __launch_bounds__(256) void invert_diag_kernel(long long n, const double* diag,
                                                           long long sd, const uint32_t* mask,

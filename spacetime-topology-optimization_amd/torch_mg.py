@@ -31,6 +31,7 @@ import json
 import os
 import threading
 from dataclasses import dataclass, field
+from typing import Callable
 
 import numpy as np
 import torch
@@ -270,7 +271,9 @@ def _select_rows(t: Tensor | None, rows: Tensor, core_ndim: int) -> Tensor | Non
 class _Level:
     """One grid of the hierarchy; calling it applies `A(v) = P(K P v)` on that grid
     (torch_mg.py:211-266).  Level 0 carries `density` (and `reaction_diag`); coarser
-    levels carry `keff`, a `(*batch, nel, k, k)` view of the entry-major storage."""
+    levels carry `keff`, a `(*batch, nel, k, k)` view of the entry-major storage.  A
+    level that the fp32 V-cycle applies from the level-0 densities forms that storage
+    only on first access (`keff`, the fp64 cycle, `__call__`, `select`)."""
 
     nelx: int
     nely: int
@@ -280,7 +283,7 @@ class _Level:
     edofMat: Tensor
     KE: Tensor | None = None
     density: Tensor | None = None
-    keff: Tensor | None = None
+    _keff: Tensor | None = field(default=None, repr=False)
     reaction_diag: Tensor | None = None
     kx: int = 1
     ky: int = 1
@@ -297,6 +300,25 @@ class _Level:
     # ((pky*pkx, k*k) fp64) and the block size (pkx, pky); None with a reaction term
     _ptab: Tensor | None = field(default=None, repr=False)
     _pblock: tuple = field(default=(1, 1), repr=False)
+    # builds `_soa` on first access (levels formed lazily), else None
+    _soa_make: Callable[[], Tensor] | None = field(default=None, repr=False)
+
+    @property
+    def keff(self) -> Tensor | None:
+        """The element matrices, `(*batch, nel, k, k)`; None on level 0."""
+        self._materialize()
+        return self._keff
+
+    @property
+    def coarse(self) -> bool:
+        """Whether the level is applied from element matrices (every level but 0)."""
+        return self._keff is not None or self._soa_make is not None
+
+    def _materialize(self) -> None:
+        if self._soa_make is not None:
+            self._soa = self._soa_make()
+            self._soa_make = None
+            self._keff = _soa_view(self._soa, 4 * self.d, self._dbatch)
 
     @property
     def d(self) -> int:
@@ -321,10 +343,11 @@ class _Level:
         return _grid.mask_bits(self.mask)
 
     def __call__(self, v: Tensor) -> Tensor:
-        if self.keff is None:
+        if not self.coarse:
             return torch_fem.operator(
                 v, self.density, self.edofMat, self.KE, self.ndof, self.mask, self.reaction_diag
             )
+        self._materialize()
         dev = _lib.require_device(v)
         batch = _grid.batch_shape(v.shape[:-1], self._dbatch)
         rows = _grid.numel(batch)
@@ -346,6 +369,7 @@ class _Level:
 
     def select(self, rows: Tensor) -> "_Level":
         """This level restricted to batch rows `rows` (shared tensors pass through)."""
+        self._materialize()
         soa = self._soa
         dbatch = self._dbatch
         if len(dbatch) == 1:
@@ -359,7 +383,7 @@ class _Level:
             diag=_select_rows(self.diag, rows, 1),
             density=_select_rows(self.density, rows, 1),
             reaction_diag=_select_rows(self.reaction_diag, rows, 1),
-            keff=None if soa is None else _soa_view(soa, k, dbatch),
+            _keff=None if soa is None else _soa_view(soa, k, dbatch),
             _dense=_select_rows(self._dense, rows, 2),
             _chol=_select_rows(self._chol, rows, 2),
             _soa=soa,
@@ -386,13 +410,14 @@ class _Level:
             "sttopt_invert_diagonal",
         )
         out["invdiag"] = inv
-        if self.keff is None:
+        if not self.coarse:
             out["density"] = self.density.detach().to(dtype).reshape(rows, -1).contiguous()
             if self.reaction_diag is not None:
                 out["rdiag"] = self.reaction_diag.detach().to(dtype).reshape(rows, -1).contiguous()
         elif self._ptab is not None and self._pblock[0] * self._pblock[1] <= PARAM_MAX_CHILDREN:
             out["ptable"] = self._ptab.to(dtype).contiguous()
         else:
+            self._materialize()
             if dtype == torch.float64:
                 out["keff"] = self._soa
             else:
@@ -424,7 +449,8 @@ def _assemble_dense(level: _Level, rows: int) -> Tensor:
     n = level.ndof
     dense = torch.empty(rows, n, n, dtype=torch.float64, device=dev)
     keep = None
-    if level.keff is not None:
+    if level.coarse:
+        level._materialize()
         args = (level._soa.data_ptr(), level._soa.stride(0), None, 0, None, 0, None)
     else:
         den = level.density.detach().to(torch.float64).reshape(rows, -1).contiguous()
@@ -475,6 +501,37 @@ def _dense_factor(level: _Level, rows: int) -> None:
     level._chol = None
 
 
+_PTAB_DIAG_HOST: dict = {}
+
+
+def _param_diagonal(den64: Tensor, ptab: Tensor, cmask: Tensor, nelx: int, nely: int,
+                    kx: int, ky: int, d: int) -> Tensor:
+    """Jacobi diagonal `(rows, ndof_c)` fp64 of the level whose element matrices are
+    sum_g rho_g ptab[gy, gx] over the kx x ky level-0 elements under each element
+    (`sttopt_param_diagonal`; 1 at fixed dofs, as `sttopt_keff_diagonal` gives it).  The
+    positions' diagonals are read back once per cached table."""
+    rows = den64.shape[0]
+    cnelx, cnely = nelx // kx, nely // ky
+    hit = _PTAB_DIAG_HOST.get(id(ptab))
+    if hit is None or hit[0] is not ptab:
+        diag = torch.diagonal(ptab, dim1=-2, dim2=-1).reshape(-1).to("cpu", torch.float64)
+        host = (ctypes.c_double * diag.numel())(*diag.tolist())
+        hit = _PTAB_DIAG_HOST[id(ptab)] = (ptab, host)  # keeps ptab, so its id stays unique
+        while len(_PTAB_DIAG_HOST) > _TABLES_MAX:
+            _PTAB_DIAG_HOST.pop(next(iter(_PTAB_DIAG_HOST)))
+    out = torch.empty(rows, d * (cnelx + 1) * (cnely + 1), dtype=torch.float64,
+                      device=den64.device)
+    _lib.check(
+        _lib.load().sttopt_param_diagonal(
+            d, cnelx, cnely, kx, ky, rows, den64.data_ptr(), den64.stride(0), hit[1],
+            _grid.mask_bits(cmask).data_ptr(), out.data_ptr(), out.stride(0),
+            _lib.stream_handle(den64.device),
+        ),
+        "sttopt_param_diagonal",
+    )
+    return out
+
+
 def build_hierarchy(
     density: Tensor,
     edofMat: Tensor,
@@ -511,9 +568,7 @@ def build_hierarchy(
                KE=KE, density=density.detach(), reaction_diag=reaction_diag, _dbatch=dbatch)
     ]
     lib = _lib.load()
-    stream = _lib.stream_handle(dev)
     k = 4 * d
-    soa = None
     ptab = None  # (Ky, Kx, k, k) per-position matrices of the current coarsest level
     seq: tuple = ()  # the coarsening factors so far (the tables' cache key)
     while True:
@@ -523,13 +578,13 @@ def build_hierarchy(
         kx, ky = coarsening_factors(top.nelx, top.nely)
         if (kx, ky) == (1, 1):
             break
+        first = len(levels) == 1
         S = child_interpolation_matrices(kx, ky, dev, torch.float64, d)
         cnelx, cnely = top.nelx // kx, top.nely // ky
-        csoa = torch.empty(rows, k * k, cnelx * cnely, dtype=torch.float64, device=dev)
         # per-position matrices of the new level: level-0 element (gy, gx) of the block
         # under one coarse element contributes density * ptab[gy, gx]
         seq = seq + ((kx, ky),)
-        if soa is None:
+        if first:
             ptab = _cached_table(
                 KE, ("ptab", seq, str(dev), d),
                 lambda: torch.einsum("mpi,pq,mqj->mij", S, KE64, S).reshape(ky, kx, k, k))
@@ -544,54 +599,77 @@ def build_hierarchy(
                 return nxt.reshape(ky * Ky, kx * Kx, k, k)
 
             ptab = _cached_table(KE, ("ptab", seq, str(dev), d), deeper)
-        if soa is None or (react64 is None and ptab is not None
-                           and ptab.shape[0] * ptab.shape[1] <= GALERKIN_MAX_BLOCK):
-            # density form: straight from the level-0 densities (any depth)
+        density_form = first or (react64 is None and ptab is not None
+                                 and ptab.shape[0] * ptab.shape[1] <= GALERKIN_MAX_BLOCK)
+        if density_form:
+            # straight from the level-0 densities (any depth)
             GM = None
-            if soa is None and react64 is not None:
+            if first and react64 is not None:
                 GM = _cached_table(
                     KE, ("reaction", (kx, ky), str(dev), d),
                     lambda: torch.einsum("mpi,pq,mqj->mij", S,
                                          torch.eye(k, dtype=torch.float64, device=dev) / 4,
                                          S).contiguous())
             table = ptab.reshape(-1, k * k).contiguous()
-            _lib.check(
-                lib.sttopt_galerkin(
-                    d, nelx, nely, ptab.shape[1], ptab.shape[0], rows, den64.data_ptr(), nel,
-                    _lib.ptr(react64), nel, table.data_ptr(), _lib.ptr(GM), None, 0, None,
-                    csoa.data_ptr(), csoa.stride(0), stream,
-                ),
-                "sttopt_galerkin",
-            )
+
+            def make(table=table, GM=GM, bx=ptab.shape[1], by=ptab.shape[0],
+                     n_el=cnelx * cnely) -> Tensor:
+                out = torch.empty(rows, k * k, n_el, dtype=torch.float64, device=dev)
+                _lib.check(
+                    lib.sttopt_galerkin(
+                        d, nelx, nely, bx, by, rows, den64.data_ptr(), nel, _lib.ptr(react64),
+                        nel, table.data_ptr(), _lib.ptr(GM), None, 0, None, out.data_ptr(),
+                        out.stride(0), _lib.stream_handle(dev),
+                    ),
+                    "sttopt_galerkin",
+                )
+                return out
         else:
-            _lib.check(
-                lib.sttopt_galerkin(
-                    d, top.nelx, top.nely, kx, ky, rows, None, 0, None, 0, None, None,
-                    soa.data_ptr(), soa.stride(0), S.data_ptr(), csoa.data_ptr(),
-                    csoa.stride(0), stream,
-                ),
-                "sttopt_galerkin",
-            )
+            top._materialize()
+            fine = top._soa
+
+            def make(fine=fine, S=S, fnelx=top.nelx, fnely=top.nely, kx=kx, ky=ky,
+                     n_el=cnelx * cnely) -> Tensor:
+                out = torch.empty(rows, k * k, n_el, dtype=torch.float64, device=dev)
+                _lib.check(
+                    lib.sttopt_galerkin(
+                        d, fnelx, fnely, kx, ky, rows, None, 0, None, 0, None, None,
+                        fine.data_ptr(), fine.stride(0), S.data_ptr(), out.data_ptr(),
+                        out.stride(0), _lib.stream_handle(dev),
+                    ),
+                    "sttopt_galerkin",
+                )
+                return out
         if react64 is not None:
             ptab = None  # the reaction term has no per-position form below level 1
-        soa = csoa
         top.kx, top.ky = kx, ky
         cndof = d * (cnelx + 1) * (cnely + 1)
         cmask = top.mask.reshape(top.nely + 1, top.nelx + 1, d)[::ky, ::kx].reshape(-1).contiguous()
-        cdiag = torch.empty(rows, cndof, dtype=torch.float64, device=dev)
-        _lib.check(
-            lib.sttopt_keff_diagonal(
-                _lib.F64, d, cnelx, cnely, rows, soa.data_ptr(), soa.stride(0),
-                _grid.mask_bits(cmask).data_ptr(), 0, cdiag.data_ptr(), cndof, stream,
-            ),
-            "sttopt_keff_diagonal",
-        )
+        # the first level of an fp32 cycle is applied from the level-0 densities
+        # (PARAM_MAX_CHILDREN): its element matrices are formed only if something
+        # reads them, and its diagonal comes from the per-position tables
+        lazy = first and ptab is not None and kx * ky <= PARAM_MAX_CHILDREN
+        if lazy:
+            soa = None
+            cdiag = _param_diagonal(den64, ptab, cmask, nelx, nely, kx, ky, d)
+        else:
+            soa = make()
+            cdiag = torch.empty(rows, cndof, dtype=torch.float64, device=dev)
+            _lib.check(
+                lib.sttopt_keff_diagonal(
+                    _lib.F64, d, cnelx, cnely, rows, soa.data_ptr(), soa.stride(0),
+                    _grid.mask_bits(cmask).data_ptr(), 0, cdiag.data_ptr(), cndof,
+                    _lib.stream_handle(dev),
+                ),
+                "sttopt_keff_diagonal",
+            )
         levels.append(
             _Level(
                 nelx=cnelx, nely=cnely, ndof=cndof, mask=cmask,
                 diag=cdiag.reshape(*dbatch, cndof).to(density.dtype),
                 edofMat=element_dof_map_tensor(cnelx, cnely, dev, d),
-                keff=_soa_view(soa, k, dbatch), _soa=soa, _dbatch=dbatch,
+                _keff=None if soa is None else _soa_view(soa, k, dbatch), _soa=soa,
+                _soa_make=make if lazy else None, _dbatch=dbatch,
                 _ptab=None if ptab is None else ptab.reshape(-1, k * k).contiguous(),
                 _pblock=(1, 1) if ptab is None else (ptab.shape[1], ptab.shape[0]),
             )
