@@ -794,3 +794,39 @@ def test_spd_inverse_reports_indefinite(dev):
     A[1, 66, 66] = -1.0  # second matrix, second pivot block
     st, _ = _spd_inverse(A, dev)
     assert st == _lib.ENOTSPD and "not positive definite" in _lib.last_error()
+
+
+@pytest.mark.parametrize("d, kx, ky", [(2, 2, 2), (2, 2, 1), (2, 1, 2), (1, 2, 2)])
+def test_param_diagonal_equals_the_diagonal_of_the_explicit_galerkin_matrices(dev, d, kx, ky):
+    """sttopt_param_diagonal (the level's diagonal from the per-position tables and the
+    level-0 densities) == sttopt_keff_diagonal of the explicit element matrices
+    sum_g rho_g S_g^T KE S_g, on 3 rows of random densities with fixed dofs, 1 at the
+    fixed dofs; and a lazily built first level forms the same matrices on access."""
+    from sttopt_amd import _grid, _lib, torch_mg
+
+    nelx, nely = 12, 8
+    cnelx, cnely = nelx // kx, nely // ky
+    rows = 3
+    KE = torch.as_tensor(fem_direct.plane_stress_KE(NU) if d == 2 else fem_direct.diffusion_KE(),
+                         dtype=torch.float64, device=dev)
+    k = 4 * d
+    S = torch_mg.child_interpolation_matrices(kx, ky, dev, torch.float64, d)
+    ptab = torch.einsum("mpi,pq,mqj->mij", S, KE, S).reshape(ky, kx, k, k)
+    rng = np.random.default_rng(5)
+    den = torch.as_tensor(rng.uniform(1e-3, 1.0, (rows, nelx * nely)), device=dev)
+    cndof = d * (cnelx + 1) * (cnely + 1)
+    cmask = torch.ones(cndof, dtype=torch.bool, device=dev)
+    cmask[:d * (cnelx + 1)] = False                  # the top row of nodes fixed
+    cmask[-1] = False
+    got = torch_mg._param_diagonal(den, ptab, cmask, nelx, nely, kx, ky, d)
+    # explicit matrices: block (ey, ex) covers level-0 rows ky ey + gy, columns kx ex + gx
+    blocks = den.reshape(rows, cnely, ky, cnelx, kx)
+    keff = torch.einsum("ryaxb,abij->ryxij", blocks, ptab).reshape(rows, cnelx * cnely, k, k)
+    soa = keff.permute(0, 2, 3, 1).reshape(rows, k * k, cnelx * cnely).contiguous()
+    want = torch.empty(rows, cndof, dtype=torch.float64, device=dev)
+    _lib.check(_lib.load().sttopt_keff_diagonal(
+        _lib.F64, d, cnelx, cnely, rows, soa.data_ptr(), soa.stride(0),
+        _grid.mask_bits(cmask).data_ptr(), 0, want.data_ptr(), cndof,
+        _lib.stream_handle(dev)), "sttopt_keff_diagonal")
+    np.testing.assert_allclose(got.cpu().numpy(), want.cpu().numpy(), rtol=1e-13, atol=0)
+    assert bool((got[:, ~cmask] == 1.0).all())
