@@ -63,6 +63,16 @@ def test_library_reports_errors_without_touching_the_gpu():
     st = lib.sttopt_stage_epilogue(4, 4, 1, 4096, 8192, 12288, 16.0, 1e-9, 1.0, 3.0, 1.0, ke,
                                    16384 + 8, 20480, None, 24576, None, None, 28672, None)
     assert st == _lib.EINVAL and "node-aligned" in _lib.last_error()
+    # the table-derived coarse diagonal takes at most 4 positions and d = 1 or 2
+    table = (ctypes.c_double * 64)()
+    st = lib.sttopt_param_diagonal(2, 4, 4, 3, 2, 1, 4096, 16, table, 8192, 12288, 50, None)
+    assert st == _lib.EINVAL and "at most 4 positions" in _lib.last_error()
+    st = lib.sttopt_param_diagonal(3, 4, 4, 2, 2, 1, 4096, 16, table, 8192, 12288, 50, None)
+    assert st == _lib.EINVAL
+    # the batched MG-PCG needs at least one group
+    n_iter = ctypes.c_int32(0)
+    st = lib.sttopt_mgpcg_batch(4096, 0, 1e-8, 10, 1, 1, ctypes.byref(n_iter), None)
+    assert st == _lib.EINVAL and "no groups" in _lib.last_error()
 
 
 def test_coarsening_factors():
