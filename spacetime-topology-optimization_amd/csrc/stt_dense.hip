@@ -241,14 +241,12 @@ __global__ __launch_bounds__(256) void sweep_panel_kernel(int n, int k0, int bk,
 // written once per sweep step, and in the last step also, off the diagonal, its mirror
 // (through an LDS transpose).  A workgroup reads only its own tile of A, and no other
 // workgroup writes that tile or reads the mirror.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void sweep_update_kernel(int n, int kb, int k0, int bk,
-                                                           double* __restrict__ a, long long sa,
-                                                           const double* __restrict__ D,
-                                                           const double* __restrict__ R,
-                                                           const double* __restrict__ P,
-                                                           double sign, int ntile,
-                                                           double* __restrict__ Dnext,
-                                                           int* __restrict__ bad) {
+// Four waves per SIMD (the LDS limit; without the hint the compiler settles for three).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void
+sweep_update_kernel(int n, int kb, int k0, int bk, double* __restrict__ a, long long sa,
+                    const double* __restrict__ D, const double* __restrict__ R,
+                    const double* __restrict__ P, double sign, int ntile,
+                    double* __restrict__ Dnext, int* __restrict__ bad) {
   // the k-chunks of the product and, afterwards, the transpose of the mirror tile
   __shared__ __align__(16) double lds[kSB * (kSB + 1)];
   double* sX = lds;
