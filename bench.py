@@ -65,7 +65,7 @@ def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (rows shard "
-              "across GPUs without communication; see DESIGN.md). Run with --gpus 1.",
+              "across GPUs without communication). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
 
