@@ -64,7 +64,7 @@ def parse():
 def main():
     args = parse()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (rows shard "
+        print("bench.py measures one GPU: the solver has no multi-GPU path (rows shard "
               "across GPUs without communication). Run with --gpus 1.",
               file=sys.stderr)
         sys.exit(2)
