@@ -270,6 +270,15 @@ int64_t sttopt_spd_inverse_work_count(int32_t n, int32_t rows);
 int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, double* work,
                        void* stream);
 
+/* sttopt_spd_inverse of banded matrices: every entry (i, j) with |i - j| > band must be
+ * zero (a Q4 grid's assembled operator with nnx nodes per row and d dofs per node has
+ * band = d (nnx + 1) + d - 1).  The sweep then skips the tiles that stay zero until it
+ * reaches them.  band >= n - 1 is the dense sweep.  Same work block, result and errors
+ * as sttopt_spd_inverse; the inverse itself is dense.
+ * Replaces the same reference step (torch_mg.py:269-290, 448-451). */
+int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
+                            int64_t a_stride, double* work, void* stream);
+
 /* Grid transfers on the node grid (torch_mg._on_node_grid with _restrict_axis /
  * _interp_axis, torch_mg.py:75-109).  restrict: coarse = P_c (R fine);
  * prolong_add: fine += P_f (P coarse).  Masks may be NULL. */

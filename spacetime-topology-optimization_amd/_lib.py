@@ -148,6 +148,7 @@ _SIGNATURES = {
     "sttopt_node_stencil": (c_int32, [c_int32, c_int32, c_int32, c_int32, c_int32, _P, c_int64,
                                       _P, c_int64, _P]),
     "sttopt_spd_inverse": (c_int32, [c_int32, c_int32, _P, c_int64, _P, _P]),
+    "sttopt_spd_inverse_band": (c_int32, [c_int32, c_int32, c_int32, _P, c_int64, _P, _P]),
     "sttopt_dense_assemble": (
         c_int32,
         [c_int32, c_int32, c_int32, c_int32, _P, c_int64, _P, c_int64, _P, c_int64, _P, _P,

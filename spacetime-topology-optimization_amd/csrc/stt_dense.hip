@@ -14,6 +14,14 @@
 // triangle is kept up to date: the update computes and writes the lower tiles, and the
 // last step also writes each tile's mirror, leaving the whole inverse.
 //
+// A banded matrix (entries zero beyond a half-bandwidth b) keeps every tile more than
+// c = (b + 63) / 64 blocks from the diagonal at zero until the sweep reaches it: at step K
+// the tiles (I, J) with max(I, J) > K + c see A_IK = 0 or A_KJ = 0, so neither their
+// update nor the pivot row and column change them.  Step K therefore works on the
+// leading (K + c + 1) blocks only (the coarsest operator of a Q4 grid with nnx nodes
+// per row has b = d (nnx + 1) + d - 1, two blocks at 33 x 17 nodes: about half the
+// tiles of the full sweep).
+//
 // Per pivot block (64 wide): sweep_panel_kernel forms P = D R from the row panel
 // R = A_K: (its tiles right of the diagonal read as the transposes of the lower tiles
 // below it), and sweep_update_kernel writes every lower tile from A, R, P and D -- it
@@ -357,9 +365,9 @@ int64_t sttopt_spd_inverse_work_count(int32_t n, int32_t rows) {
   return 2LL * rows * kSB * kSB + 2LL * rows * kSB * n + 1;
 }
 
-int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, double* work,
-                       void* stream) {
-  if (n <= 0 || rows < 0 || a == nullptr || work == nullptr ||
+int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
+                            int64_t a_stride, double* work, void* stream) {
+  if (n <= 0 || rows < 0 || band < 0 || a == nullptr || work == nullptr ||
       (rows > 1 && a_stride < (int64_t)n * n)) {
     set_error("sttopt_spd_inverse: invalid arguments");
     return STTOPT_EINVAL;
@@ -369,14 +377,18 @@ int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, dou
   SweepWork w = carve_sweep(work, n, rows);
   if (int e = fill<int>(1, 1, 0, w.bad, 1, nullptr, s)) return e;
   const int nb = (n + kSB - 1) / kSB;
-  const int ntile = nb * (nb + 1) / 2;
+  // blocks off the diagonal that the band reaches; at least one, so that step K always
+  // updates diagonal block K + 1, whose workgroup inverts it for the next step
+  const int reach = max(1, (int)((min(band, n - 1) + (int64_t)kSB - 1) / kSB));
   hipLaunchKernelGGL(sweep_pivot_kernel, dim3(rows), dim3(256), 0, s, n, 0,
                      n < kSB ? n : kSB, a, a_stride, w.D[0], w.bad);
   for (int kb = 0; kb < nb; ++kb) {
     const int k0 = kb * kSB, bk = n - k0 < kSB ? n - k0 : kSB;
+    const int lead = min(nb, kb + reach + 1);  // blocks this step changes
+    const int ntile = lead * (lead + 1) / 2;
     double* Dk = w.D[kb & 1];
     double* Dn = kb + 1 < nb ? w.D[(kb + 1) & 1] : nullptr;
-    hipLaunchKernelGGL(sweep_panel_kernel, dim3(nb, rows), dim3(256), 0, s, n, k0, bk, a,
+    hipLaunchKernelGGL(sweep_panel_kernel, dim3(lead, rows), dim3(256), 0, s, n, k0, bk, a,
                        a_stride, Dk, w.R, w.P);
     hipLaunchKernelGGL(sweep_update_kernel, dim3(rows, ntile), dim3(256), 0, s, n, kb, k0,
                        bk, a, a_stride, Dk, w.R, w.P, kb == nb - 1 ? -1.0 : 1.0, ntile, Dn,
@@ -390,6 +402,11 @@ int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, dou
     return STTOPT_ENOTSPD;
   }
   return STTOPT_OK;
+}
+
+int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, double* work,
+                       void* stream) {
+  return sttopt_spd_inverse_band(n, n > 0 ? n - 1 : 0, rows, a, a_stride, work, stream);
 }
 
 }  // extern "C"

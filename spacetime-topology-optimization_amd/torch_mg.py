@@ -488,8 +488,11 @@ def _dense_factor(level: _Level, rows: int) -> None:
     lib = _lib.load()
     work = torch.empty(lib.sttopt_spd_inverse_work_count(n, rows), dtype=torch.float64,
                        device=dev)
-    status = lib.sttopt_spd_inverse(n, rows, inv.data_ptr(), n * n, work.data_ptr(),
-                                    _lib.stream_handle(dev))
+    # the operator couples a node with its eight grid neighbours only: half-bandwidth
+    # d (nnx + 1) + d - 1 in the node-major dof numbering
+    band = level.d * (level.nelx + 2) + level.d - 1
+    status = lib.sttopt_spd_inverse_band(n, band, rows, inv.data_ptr(), n * n,
+                                         work.data_ptr(), _lib.stream_handle(dev))
     if status == _lib.ENOTSPD:
         raise torch.linalg.LinAlgError(
             "linalg.cholesky: The factorization could not be completed because the input is "

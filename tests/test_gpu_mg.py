@@ -787,6 +787,73 @@ def test_spd_inverse_ill_conditioned(dev):
     assert err < 1e3 * cond * 2.2e-16
 
 
+def _spd_inverse_band(a, band, dev):
+    """sttopt_spd_inverse_band on a copy of `a`; returns (status, inverse)."""
+    from sttopt_amd import _lib
+
+    lib = _lib.load()
+    rows, n, _ = a.shape
+    out = a.to(dev).contiguous().clone()
+    work = torch.empty(lib.sttopt_spd_inverse_work_count(n, rows), dtype=torch.float64, device=dev)
+    st = lib.sttopt_spd_inverse_band(n, band, rows, out.data_ptr(), n * n, work.data_ptr(),
+                                     _lib.stream_handle(dev))
+    return st, out.cpu()
+
+
+def _banded_spd(n, band, rows, seed):
+    """Random SPD matrices with zeros beyond `band` (diagonally dominant)."""
+    g = torch.Generator().manual_seed(seed)
+    A = torch.randn(rows, n, n, generator=g, dtype=torch.float64)
+    A = A + A.transpose(1, 2)
+    i = torch.arange(n)
+    A[:, (i[:, None] - i[None, :]).abs() > band] = 0.0
+    A += (3.0 * band + 8.0) * torch.eye(n, dtype=torch.float64)
+    return A
+
+
+@pytest.mark.parametrize("n, band, rows", [(1122, 69, 3), (1122, 64, 1), (1122, 65, 1),
+                                           (300, 0, 2), (300, 1, 1), (200, 130, 2),
+                                           (70, 69, 1), (129, 5, 2)])
+def test_spd_inverse_band_matches_lapack(dev, n, band, rows):
+    """The band-limited sweep (steps skip the tiles that are still zero) == LAPACK's
+    inverse, and == the dense sweep bit for bit (it skips only exact-zero updates)."""
+    A = _banded_spd(n, band, rows, seed=n + band)
+    st, inv = _spd_inverse_band(A, band, dev)
+    assert st == 0
+    ref = torch.linalg.inv(A)
+    err = float((inv - ref).norm() / ref.norm())
+    print(f"n {n} band {band}: rel err {err:.2e}")
+    assert err < 1e-13
+    st, dense = _spd_inverse(A, dev)
+    assert st == 0
+    assert torch.equal(inv, dense)
+
+
+def test_spd_inverse_band_of_the_coarsest_operator(dev):
+    """The coarsest operator of a 32 x 16 grid (half-bandwidth 2 (33 + 1) + 1 = 69, as
+    the hierarchy passes it) inverted with the band equals the dense sweep."""
+    from sttopt_amd import torch_mg
+
+    nelx, nely, rows = 32, 16, 2
+    rng = np.random.default_rng(5)
+    den = torch.tensor(np.clip(0.4 + 0.3 * rng.standard_normal((rows, nelx * nely)), 1e-3, 1.0),
+                       dtype=torch.float64, device=dev) ** 3
+    s = _setup(nelx, nely, dev)
+    levels = torch_mg.build_hierarchy(den, s["edof_t"], s["KE_t"], s["mask_t"], nelx, nely,
+                                      max_coarse_elements=700)
+    assert len(levels) == 1 and levels[0].coarse is False
+    A = torch_mg._assemble_dense(levels[0], rows).cpu()
+    band = 2 * (nelx + 2) + 1
+    i = torch.arange(A.shape[1])
+    assert float(A[:, (i[:, None] - i[None, :]).abs() > band].abs().max()) == 0.0
+    assert float(A[:, (i[:, None] - i[None, :]).abs() == band].abs().max()) > 0.0
+    st, inv = _spd_inverse_band(A, band, dev)
+    assert st == 0
+    st, dense = _spd_inverse(A, dev)
+    assert st == 0
+    assert torch.equal(inv, dense)
+
+
 def test_spd_inverse_reports_indefinite(dev):
     from sttopt_amd import _lib
 
