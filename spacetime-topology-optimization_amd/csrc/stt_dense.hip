@@ -52,37 +52,105 @@ static SweepWork carve_sweep(double* work, int n, int rows) {
   return w;
 }
 
-// Scalar sweep of a bk x bk block held in registers by a 256-thread block: thread t
-// holds column j = t % 64 of rows t / 64 + 4 q (q < 16), padded with the identity past
-// bk.  Each pivot step publishes the pivot column through LDS (col: two buffers, one
-// barrier per step).  On return m holds -(block)^-1; the result is false if a pivot was
-// not positive (or not finite).  Every thread sees the same pivots.
-__device__ bool sweep_block_registers(double (&m)[16], int bk, double (*col)[kSB]) {
-  const int j = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
+constexpr int kPB = 4;  // pivots swept per barrier in the register sweep
+
+// Sweep of a bk x bk block held in registers by a 256-thread block: thread t holds
+// column j = t % 64 of rows t / 64 + 4 q (q < 16), padded with the identity past bk.
+// On return m holds -(block)^-1; the result is false if a pivot was not positive (or
+// not finite).  Every thread sees the same pivots.
+// The pivots are swept kPB at a time (the block form of the sweep, as between the
+// 64-wide blocks): the lanes of pivot set P publish their columns through LDS (col:
+// two buffers of 64 x kPB, one barrier per set), every thread sweeps the kPB x kPB
+// block M = A_PP in registers (M <- -A_PP^-1, pivot by pivot), and then, with
+// u = -M v where v = A_Pj off P and v = -e_j on it,
+//   A_ij <- keep A_ij - A_iP u   (i not in P; keep = 0 in the columns of P)
+//   A_ij <- u_(i - p0)           (i in P)
+// The barrier, the LDS round trip and the divisions are on the critical path of every
+// sweep launch (one wave per SIMD, nothing to hide their latency), hence the sets.
+// Sweeping the identity padding past bk leaves the real entries unchanged.
+__device__ bool sweep_block_registers(double (&m)[16], int bk, double (*col)[kSB][kPB]) {
+  const int j = threadIdx.x & (kSB - 1);
+  const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / kSB);
   bool bad_pivot = false;
-  for (int p = 0; p < bk; ++p) {
-    double* c = col[p & 1];
-    if (j == p) {
+  for (int p0 = 0; p0 < bk; p0 += kPB) {
+    double (*c)[kPB] = col[(p0 / kPB) & 1];
+    const int jj = j - p0;
+    const bool colp = jj >= 0 && jj < kPB;
+    if (colp) {
 #pragma unroll
-      for (int q = 0; q < 16; ++q) c[g + 4 * q] = m[q];
+      for (int q = 0; q < 16; ++q) c[g + 4 * q][jj] = m[q];
     }
     __syncthreads();
-    const double piv = c[p];
-    const bool ok = piv > 0.0 && isfinite(piv);
-    bad_pivot |= !ok;
-    const double d = ok ? 1.0 / piv : 0.0;
-    const double cj = c[j];
-    const double dcj = d * cj;
-    const bool colp = j == p;
+    double M[kPB][kPB], v[kPB];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      const int i = g + 4 * q;
-      const double ci = c[i];
-      // off the pivot row and column: m - ci d cj; pivot column: ci d; pivot row: cj d;
-      // pivot: -d (selects, no divergent branches)
-      double v = colp ? ci * d : m[q] - ci * dcj;
-      if (i == p) v = colp ? -d : dcj;  // i == p is uniform across the wavefront
-      m[q] = v;
+    for (int a = 0; a < kPB; ++a) {
+#pragma unroll
+      for (int b = 0; b < kPB; ++b) M[a][b] = c[p0 + a][b];
+      v[a] = colp ? (a == jj ? -1.0 : 0.0) : c[j][a];  // A_Pj = A_jP (symmetric)
+    }
+#pragma unroll
+    for (int a = 0; a < kPB; ++a) {
+      const double piv = M[a][a];
+      const bool ok = piv > 0.0 && isfinite(piv);
+      bad_pivot |= !ok;
+      const double d = ok ? 1.0 / piv : 0.0;
+#pragma unroll
+      for (int r = 0; r < kPB; ++r) {
+        if (r == a) continue;
+        const double dra = d * M[r][a];
+#pragma unroll
+        for (int t = 0; t < kPB; ++t) {
+          if (t != a) M[r][t] = fma(-dra, M[a][t], M[r][t]);
+        }
+      }
+#pragma unroll
+      for (int r = 0; r < kPB; ++r) {
+        if (r != a) {
+          M[r][a] *= d;
+          M[a][r] = M[r][a];
+        }
+      }
+      M[a][a] = -d;
+    }
+    double u[kPB];
+#pragma unroll
+    for (int a = 0; a < kPB; ++a) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < kPB; ++b) acc = fma(-M[a][b], v[b], acc);
+      u[a] = acc;
+    }
+    const double keep = colp ? 0.0 : 1.0;
+    // The rows' LDS reads in two halves of eight rows: their offset (zero) passes
+    // through an empty asm that takes u, then the first half's last result, so the
+    // compiler cannot issue all 64 reads ahead of the block's sweep (that would take the
+    // look-ahead workgroup of the update launch past four waves per SIMD).
+    int hoff = 0;
+    asm volatile("" : "+v"(hoff) : "v"(u[kPB - 1]));
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      if (h == 1) asm volatile("" : "+v"(hoff) : "v"(m[7]));
+#pragma unroll
+      for (int qq = 0; qq < 8; ++qq) {
+        const int q = 8 * h + qq;
+        const double* ci = c[g + 4 * q + hoff];
+        double t = m[q] * keep;
+#pragma unroll
+        for (int a = 0; a < kPB; ++a) t = fma(-ci[a], u[a], t);
+        m[q] = t;
+      }
+    }
+    // rows of P: row p0 + a sits in wave (p0 + a) % 4 at q = (p0 + a) / 4
+#pragma unroll
+    for (int a = 0; a < kPB; ++a) {
+      const int r = p0 + a;
+      if ((r & 3) == g) {
+        const int qp = r >> 2;
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+          if (q == qp) m[q] = u[a];
+        }
+      }
     }
   }
   return !bad_pivot;
@@ -93,7 +161,7 @@ __global__ __launch_bounds__(256) void sweep_pivot_kernel(int n, int k0, int bk,
                                                           const double* __restrict__ a,
                                                           long long sa, double* __restrict__ D,
                                                           int* __restrict__ bad) {
-  __shared__ double col[2][kSB];
+  __shared__ double col[2][kSB][kPB];
   const int row = blockIdx.x;
   const double* A = a + row * sa;
   const int j = threadIdx.x & (kSB - 1), g = threadIdx.x / kSB;
@@ -260,7 +328,7 @@ sweep_update_kernel(int n, int kb, int k0, int bk, double* __restrict__ a, long 
   double* sX = lds;
   double* sY = lds + kKC * kSB;
   double (*sT)[kSB + 1] = reinterpret_cast<double (*)[kSB + 1]>(lds);
-  __shared__ double col[2][kSB];
+  __shared__ double col[2][kSB][kPB];
   // grid (rows, tiles): the matrices are the fast dimension, so tile slot 0 of every
   // matrix is dispatched first; that slot is the next pivot block's diagonal tile,
   // whose workgroup also inverts the block
@@ -333,6 +401,7 @@ sweep_update_kernel(int n, int kb, int k0, int bk, double* __restrict__ a, long 
       const int i = g + 4 * q;
       Dr2[i * kSB + jj] = (i < bi && jj < bj) ? -m[q] : 0.0;
     }
+    return;  // a diagonal tile: no mirror to write
   }
   if (I != J && sign < 0.0) {
     // the last step: mirror tile (J, I), element (r, c) = out(c, r), written row by row
