@@ -67,6 +67,10 @@ struct StencilArgs {
   const double* dbeta;
   const int* diters;
   int dk, dz_f32;
+  // kSmoothProlong: the coarse correction (row stride spe) on a grid of pcnnx nodes per row
+  const T* pe;
+  long long spe;
+  int pcnnx;
   KE<T> ke;
 };
 
@@ -333,6 +337,12 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
   constexpr bool kOutMask = MODE != kMatvec;
   __shared__ uint32_t sMW[kOutMask ? kTileY : 1][kOutMask ? kMW : 1];
   constexpr bool kDir = MODE == kOperatorDir;
+  constexpr bool kProl = MODE == kSmoothProlong;
+  // kSmoothProlong: the coarse correction at the coarse nodes under the staged halo
+  // (kPY x kPX from (y0 / 2 - 1, x0 / 2 - 1); x0 and y0 are even)
+  constexpr int kPY = kHaloY / 2 + 1, kPX = kHaloX / 2 + 1;
+  constexpr int kPE = (kPY * kPX + kBlock - 1) / kBlock;
+  __shared__ Vec<T, D> sE[kProl ? kPY : 1][kProl ? kPX : 1];
   const int t = xcd_remap(bid, a.total);
   int row, tt;
   row_tile(t, a.tiles, a.total, row, tt);
@@ -408,6 +418,20 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
   constexpr int kUR = (kHaloY + kWaves - 1) / kWaves;
   constexpr int kRR = (kRows + kWaves - 1) / kWaves;
   constexpr int kTailU = 2 * kHaloY, kTail = kTailU + kRows;
+  const int pcy0 = y0 / 2 - 1, pcx0 = x0 / 2 - 1;
+  Vec<T, D> pev[kProl ? kPE : 1];
+  if constexpr (kProl) {
+    const int cnny = a.nely / 2 + 1;
+    const T* pe = a.pe + row * a.spe;
+#pragma unroll
+    for (int m = 0; m < kPE; ++m) {
+      const int e = threadIdx.x + m * kBlock, cy = pcy0 + e / kPX, ccx = pcx0 + e % kPX;
+#pragma unroll
+      for (int c = 0; c < D; ++c) pev[m].v[c] = T(0);
+      if (e < kPY * kPX && cy >= 0 && cy < cnny && ccx >= 0 && ccx < a.pcnnx)
+        pev[m] = load_node<T, D>(pe, (long long)cy * a.pcnnx + ccx);
+    }
+  }
   const int cx = x0 - 1 + lane;
   const bool cnode = cx >= 0 && cx < a.nnx;
   const bool celem = cx >= 0 && cx < a.nelx;
@@ -439,7 +463,7 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
         // trailing column)
         wu[n] = stage_node(node, i >= 1 && i <= kTileY && lane >= 1);
       }
-      if (a.proj_in) wm[n] = a.mask[(node * D) >> 5];
+      if (a.proj_in || (kProl && a.mask != nullptr)) wm[n] = a.mask[(node * D) >> 5];
     }
   }
   Vec<T, 2> wr2[kParam ? kRR : 1];
@@ -474,6 +498,7 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
     tb.v[c] = TB(0);
     td.v[c] = T(0);
   }
+  uint32_t tm = ~0u;  // kSmoothProlong: the trailing node's free-dof word
   if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx) {
     const long long node = (long long)tgy * a.nnx + tgx;
     if constexpr (kFirst) {
@@ -482,6 +507,7 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
     } else {
       tu = stage_node(node, tk == kTileX && ti >= 1 && ti <= kTileY);
     }
+    if (kProl && a.mask != nullptr) tm = a.mask[(node * D) >> 5];
   }
   Vec<T, 2> tr2;
   tr2.v[0] = tr2.v[1] = T(0);
@@ -517,11 +543,48 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
     }
     if (tix < kTailU) tu = finish(tb, td, ti, tk, tgy, tgx);
   }
+  if constexpr (kProl) {
+    // xp = x + P e at the free dofs of the staged nodes, the expression of prolong_add
+    // (bilinear weights 1, 0 at even and 1/2, 1/2 at odd fine positions)
+#pragma unroll
+    for (int m = 0; m < kPE; ++m) {
+      const int e = threadIdx.x + m * kBlock;
+      if (e < kPY * kPX) sE[e / kPX][e % kPX] = pev[m];
+    }
+    __syncthreads();
+    auto prolong = [&](Vec<T, D>& v, uint32_t word, int gy, int gx) {
+      const long long node = (long long)gy * a.nnx + gx;
+      const uint32_t bits = (word >> ((node * D) & 31)) & ((1u << D) - 1u);
+      const int fy = gy & 1, fx = gx & 1;
+      const T wy1 = fy ? T(0.5) : T(0), wy0 = T(1) - wy1;
+      const T wx1 = fx ? T(0.5) : T(0), wx0 = T(1) - wx1;
+      const int ly = (gy >> 1) - pcy0, lx = (gx >> 1) - pcx0;
+      const Vec<T, D> c00 = sE[ly][lx];
+      Vec<T, D> c01, c10, c11;
+#pragma unroll
+      for (int k = 0; k < D; ++k) c01.v[k] = c10.v[k] = c11.v[k] = T(0);
+      if (fx) c01 = sE[ly][lx + 1];
+      if (fy) c10 = sE[ly + 1][lx];
+      if (fx && fy) c11 = sE[ly + 1][lx + 1];
+#pragma unroll
+      for (int k = 0; k < D; ++k) {
+        const T top = wx0 * c00.v[k] + wx1 * c01.v[k];
+        const T bot = wx0 * c10.v[k] + wx1 * c11.v[k];
+        if ((bits >> k) & 1u) v.v[k] += wy0 * top + wy1 * bot;
+      }
+    };
+#pragma unroll
+    for (int n = 0; n < kUR; ++n) {
+      const int i = wave + n * kWaves, gy = y0 - 1 + i;
+      if (i < kHaloY && gy >= 0 && gy < a.nny && cnode) prolong(wu[n], wm[n], gy, cx);
+    }
+    if (tix < kTailU && tgy >= 0 && tgy < a.nny && tgx < a.nnx) prolong(tu, tm, tgy, tgx);
+  }
   // Epilogue operands of this thread's output nodes (b, D^-1) are loaded now, before
   // the barrier, so that the epilogue does not wait for them; the output rows' free-dof
   // words are loaded by the first kTileY * kMW threads (one word each) and pass through
   // LDS.
-  constexpr bool kSweep = MODE == kSmooth;  // kSmoothFirst reads them from LDS
+  constexpr bool kSweep = MODE == kSmooth || kProl;  // kSmoothFirst reads them from LDS
   constexpr bool kNeedB = kSweep || MODE == kResid;
   const bool has_mask = kOutMask && a.mask != nullptr;
   uint32_t mw = ~0u;
@@ -993,6 +1056,51 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
     return STTOPT_EINVAL;
   }
   STT_CHECK_LAUNCH("stencil_kernel");
+  return STTOPT_OK;
+}
+
+int smooth_prolong(const OpDesc& op, int rows, const float* x, long long sx, const float* e,
+                   long long se, int cnnx, int cnny, const float* b, long long sb,
+                   const float* invd, long long sd, float* y, long long sy, double omega,
+                   const int* active, hipStream_t s) {
+  if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
+  if (op.ptab == nullptr || op.pkx != 2 || op.pky != 2 || op.nelx % 2 != 0 ||
+      op.nely % 2 != 0 || cnnx != op.nelx / 2 + 1 || cnny != op.nely / 2 + 1 ||
+      e == nullptr || b == nullptr ||
+      invd == nullptr || x == y) {
+    set_error("smooth_prolong: needs the parameterized 2 x 2 level, its coarse grid, b, "
+              "D^-1 and separate input and output");
+    return STTOPT_EINVAL;
+  }
+  if (op.d == 2 && !pair_aligned(e, se, sizeof(float))) {
+    set_error("stencil: d = 2 vectors need node-aligned pointers and even row strides");
+    return STTOPT_EINVAL;
+  }
+  StencilArgs<float, float, float> a;
+  int src = 0;
+  bool param_tile = false;
+  if (int err = stencil_args<float, float, float>(op, rows, kSmooth, false, kNoDot, x, sx, b,
+                                                  sb, invd, sd, y, sy, omega, active, nullptr,
+                                                  nullptr, 0, a, src, param_tile))
+    return err;
+  if (!param_tile) {
+    set_error("smooth_prolong: the level does not run on LDS tiles");
+    return STTOPT_EINVAL;
+  }
+  a.mode = kSmoothProlong;
+  a.pe = e;
+  a.spe = se;
+  a.pcnnx = cnnx;
+  dim3 grid((unsigned)a.total), block(kBlock);
+  if (op.d == 2)
+    hipLaunchKernelGGL((stencil_tile_kernel<float, float, float, 2, kSrcParam, kSmoothProlong,
+                                            kNoDot>),
+                       grid, block, 0, s, a);
+  else
+    hipLaunchKernelGGL((stencil_tile_kernel<float, float, float, 1, kSrcParam, kSmoothProlong,
+                                            kNoDot>),
+                       grid, block, 0, s, a);
+  STT_CHECK_LAUNCH("smooth_prolong");
   return STTOPT_OK;
 }
 

@@ -42,13 +42,16 @@ __device__ __forceinline__ void load_record(const T* p, T (&out)[N]) {
 // the stencil precision when given.
 // kOperatorDir: the PCG's direction update fused into the level-0 fp64 operator (the
 // staged vector is p_new = z + beta p_old; see operator_dir).
+// kSmoothProlong (the fp32 parameterized level 1 at 2 x 2): a damped-Jacobi sweep from
+// x + P e, the coarse correction e prolongated while staging (see smooth_prolong).
 enum StencilMode {
   kMatvec = 0,
   kOperator = 1,
   kSmooth = 2,
   kResid = 3,
   kSmoothFirst = 4,
-  kOperatorDir = 5
+  kOperatorDir = 5,
+  kSmoothProlong = 6
 };
 enum DotKind { kNoDot = 0, kDotU = 1, kDotB = 2 };
 
@@ -108,6 +111,16 @@ int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
                  const double* beta, const int* iters, int k, double* Ap,
                  const int* active, double* partial, hipStream_t s,
                  hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+
+// One damped-Jacobi sweep y = xp + w D^-1 (b - A xp) from xp = x + P e (P: the bilinear
+// 2 x 2 prolongation of the coarse correction e, added at the free dofs, the
+// arithmetic of prolong_add), on the fp32 level applied from the level-0 densities at
+// 2 x 2 (op.ptab with pkx = pky = 2, float).  cnnx x cnny: e's grid, which must be this
+// level's at 2 x 2.  x is read, not updated; y must not overlap x.
+int smooth_prolong(const OpDesc& op, int rows, const float* x, long long sx, const float* e,
+                   long long se, int cnnx, int cnny, const float* b, long long sb,
+                   const float* invd, long long sd, float* y, long long sy, double omega,
+                   const int* active, hipStream_t s);
 
 // ---- one launch over the same level of several hierarchies (the uneven batch) ----
 // Each kernel below runs the same pass on up to kMaxBatchGroups hierarchies of different

@@ -501,6 +501,25 @@ struct VCycle {
     } else if (int e = fill<T>(nl, rows, T(0), x, nl, active, s)) {
       return e;
     }
+    if constexpr (sizeof(T) == sizeof(float)) {
+      // the prolongation folded into the first post sweep (its staging adds P e) on the
+      // LDS tile of the fp32 level 1 at 2 x 2 over a 2 x 2 coarsening
+      if (tile_level && ns >= 1 && mg->gamma == 1 && L.kx == 2 && L.ky == 2 &&
+          (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) == 0) {
+        const sttopt_level& C = mg->levels[l + 1];
+        T* ec = nullptr;
+        if (int e = coarse_visit(l, x, b, nl, &ec)) return e;
+        if (int e = smooth_prolong(op[l], rows, x, nl, ec, n[l + 1], C.nelx + 1, C.nely + 1,
+                                   b, nl, invd(l), sd(l), alt, nl, mg->omega, active, s))
+          return e;
+        T* t = x;
+        x = alt;
+        alt = t;
+        if (int e = sweeps(l, ns - 1, x, alt, b, nl)) return e;
+        *out = x;
+        return STTOPT_OK;
+      }
+    }
     if (int e = coarse_correct(l, x, b, nl)) return e;
     if (int e = sweeps(l, ns, x, alt, b, nl)) return e;
     *out = x;
@@ -509,6 +528,18 @@ struct VCycle {
 
   // x += P (cycle(R (b - A x))), with gamma coarse visits.
   int coarse_correct(int l, T* x, const T* b, long long sb) {
+    const sttopt_level& L = mg->levels[l];
+    const sttopt_level& C = mg->levels[l + 1];
+    const long long nl = n[l], nc = n[l + 1];
+    T* ec = nullptr;
+    if (int e = coarse_visit(l, x, b, sb, &ec)) return e;
+    return prolong_add<T>(mg->d, C.nelx, C.nely, L.kx, L.ky, rows, ec, nc, L.mask, x, nl,
+                          active, s);
+  }
+
+  // *ec = cycle(R (b - A x)) on level l + 1, with gamma coarse visits (the correction
+  // that coarse_correct prolongates).
+  int coarse_visit(int l, const T* x, const T* b, long long sb, T** ec_out) {
     const sttopt_level& L = mg->levels[l];
     const sttopt_level& C = mg->levels[l + 1];
     const long long nl = n[l], nc = n[l + 1];
@@ -536,8 +567,8 @@ struct VCycle {
       }
       ec = acc;
     }
-    return prolong_add<T>(mg->d, cnelx, cnely, L.kx, L.ky, rows, ec, nc, L.mask, x, nl,
-                          active, s);
+    *ec_out = ec;
+    return STTOPT_OK;
   }
 
   // The pair cycle split at the PCG's convergence check.  pair_pre_update: the residual

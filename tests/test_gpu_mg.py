@@ -564,7 +564,8 @@ def test_two_sweep_passes_equal_separate_sweeps(dev, nelx, nely, reaction, coars
     """The fp32 V-cycle's fused passes equal one pass per sweep bit for bit: level 0's
     two-sweep passes (pre: two sweeps and the residual; post: prolongation and two
     sweeps) on grids whose node counts fall on and next to the 62 x 14 output tiles
-    (125 = 2 * 62 + 1, 29 = 2 * 14 + 1, 123, 27), and the first two sweeps in one pass
+    (125 = 2 * 62 + 1, 29 = 2 * 14 + 1, 123, 27), level 1's prolongation folded into its
+    first post sweep, and the first two sweeps in one pass
     on the coarse levels (level 1 on the LDS tile, deeper levels, reached with a small
     coarsest level, on the one-node-per-thread kernel), the two-pass node-stencil levels
     (pre-smoothing with residual and restriction, prolongation with post-smoothing: on
