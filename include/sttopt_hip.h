@@ -357,6 +357,15 @@ int sttopt_mgpcg_batch(const sttopt_batch_group* groups, int32_t ngroups, double
  * reference (whose iteration is a host loop of torch ops, torch_fem.py:313-331). */
 int sttopt_mgpcg_graphs(int32_t enable);
 
+/* How many of the fp32 V-cycle's two-pass node-stencil levels above the coarsest run,
+ * with the coarsest solve, as one persistent launch per cycle (0 to 3; 0: a launch per
+ * pass).  The launch takes the passes' tiles as one work list, phase after phase, with
+ * per-row completion counters in the sttopt_mg scratch (zeroed by sttopt_mgpcg and
+ * sttopt_vcycle, left at zero by the launch) and bounded waits; the results are
+ * identical.  Process-wide; STTOPT_EINVAL outside 0 to 3.  No counterpart in the
+ * reference (whose cycle recurses level by level in torch ops, torch_mg.py:440-471). */
+int sttopt_vcycle_bottom(int32_t max_levels);
+
 /* Work vectors sttopt_mgpcg needs: returns the count of doubles ([rows][ndof0] x 6). */
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg);
 

@@ -235,6 +235,7 @@ _SIGNATURES = {
     ),
     "sttopt_stage_scratch_count": (c_int64, [c_int32, c_int32, c_int32]),
     "sttopt_mgpcg_graphs": (c_int32, [c_int32]),
+    "sttopt_vcycle_bottom": (c_int32, [c_int32]),
     "sttopt_stage_epilogue_noalloc": (
         c_int32,
         [c_int32, c_int32, c_int32, _P, _P, _P, c_double, c_double, c_double, c_double,

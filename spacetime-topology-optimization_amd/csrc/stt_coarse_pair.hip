@@ -161,16 +161,14 @@ struct CoarsePairArgs {
   const int* active;
 };
 
+// The pre pass of tile tt of batch row `row` (the whole workgroup).
 template <int D, typename C>
-__device__ __forceinline__ void pair_pre_body(const CoarsePairArgs& a, int bid) {
+__device__ __forceinline__ void pair_pre_tile(const CoarsePairArgs& a, int row, int tt) {
   __shared__ float sbv[C::PH * C::PW][D];
   __shared__ float sdv[C::PH * C::PW][D];
   __shared__ float sx1[C::PH * C::PW][D];
   __shared__ float sx2[C::PH * C::PW][D];
   __shared__ float srr[C::PH * C::PW][D];
-  const int t = xcd_remap(bid, a.total);
-  int row, tt;
-  row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block
   const int ty = tt / a.tiles_x, tx = tt - ty * a.tiles_x;
   const int I0 = ty * C::CH, J0 = tx * C::CW;     // first coarse node of the tile
@@ -306,16 +304,14 @@ __device__ __forceinline__ void pair_pre_body(const CoarsePairArgs& a, int bid) 
   }
 }
 
+// The post pass of tile tt of batch row `row` (the whole workgroup).
 template <int D, typename C>
-__device__ __forceinline__ void pair_post_body(const CoarsePairArgs& a, int bid) {
+__device__ __forceinline__ void pair_post_tile(const CoarsePairArgs& a, int row, int tt) {
   __shared__ float sbv[C::QH * C::QW][D];
   __shared__ float sdv[C::QH * C::QW][D];
   __shared__ float sxp[C::QH * C::QW][D];
   __shared__ float sx3[C::QH * C::QW][D];
   __shared__ float sec[C::EH * C::EW][D];
-  const int t = xcd_remap(bid, a.total);
-  int row, tt;
-  row_tile(t, a.tiles, a.total, row, tt);
   if (a.active != nullptr && a.active[row] == 0) return;  // whole block
   const int ty = tt / a.tiles_x, tx = tt - ty * a.tiles_x;
   const int r0 = ty * C::TH, c0 = tx * C::TW;          // first output node of the tile
@@ -454,6 +450,23 @@ __device__ __forceinline__ void pair_post_body(const CoarsePairArgs& a, int bid)
 
 // One launch per level (the fine grid of one hierarchy), or one launch over the same
 // level of several hierarchies: group i takes workgroups first[i] .. first[i + 1] - 1.
+// Launch index bid of a pass over rows x tiles: its row and tile (XCD-aware order).
+template <int D, typename C>
+__device__ __forceinline__ void pair_pre_body(const CoarsePairArgs& a, int bid) {
+  const int t = xcd_remap(bid, a.total);
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
+  pair_pre_tile<D, C>(a, row, tt);
+}
+
+template <int D, typename C>
+__device__ __forceinline__ void pair_post_body(const CoarsePairArgs& a, int bid) {
+  const int t = xcd_remap(bid, a.total);
+  int row, tt;
+  row_tile(t, a.tiles, a.total, row, tt);
+  pair_post_tile<D, C>(a, row, tt);
+}
+
 constexpr int kMaxPairGroups = 8;
 struct PairGroups {
   int n;
@@ -561,7 +574,201 @@ int launch_pair(const PairLevel* lv, int n, bool pre, hipStream_t s) {
   return STTOPT_OK;
 }
 
+// ---- the bottom of the cycle in one persistent launch (vcycle_bottom) ----
+constexpr int kBottomPhases = 2 * kMaxBottomLevels + 1;
+// a wait gives up after this many ticks of the 100 MHz wall clock (0.2 s; a wait of a
+// working launch lasts microseconds)
+constexpr unsigned long long kBottomWaitTicks = 20000000ull;
+
+struct BottomArgs {
+  int nlev, rows, nphase;
+  int per_row[kBottomPhases];           // items per row of each phase
+  unsigned first[kBottomPhases + 1];    // first item of each phase
+  CoarsePairArgs pre[kMaxBottomLevels], post[kMaxBottomLevels];
+  long long n;
+  const double* cinv;
+  long long scinv;
+  const float* cb;
+  long long scb;
+  const uint32_t* cmask;
+  float* cx;
+  long long scx;
+  const int* active;
+  unsigned* queue;  // [0] next item, [1] workgroups that have left the loop
+  unsigned* done;   // [phase][row] items done
+  int* err;
+};
+
+// Wave 0 of a workgroup (all its lanes, so that no branch of the work loop diverges
+// within a wave): poll *c until it reaches `need` (giving up after kBottomWaitTicks,
+// with *err set, and at once when an earlier wait has given up), then acquire: the
+// stores its producers made before their release are visible to the loads that follow.
+__device__ __forceinline__ void bottom_wait(const unsigned* c, unsigned need, int* err) {
+  const long long t0 = wall_clock64();
+  while (__builtin_amdgcn_readfirstlane(
+             __hip_atomic_load(c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) < need) {
+    if (__builtin_amdgcn_readfirstlane(
+            __hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) != 0)
+      break;
+    if ((unsigned long long)(wall_clock64() - t0) > kBottomWaitTicks) {
+      atomicExch(err, 1);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(2);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // stale L2 lines dropped
+}
+
+// Wave 0 (all lanes): add one to *c (lane 0's add; the others add zero) and return the
+// value before the add, the same in every lane.
+__device__ __forceinline__ unsigned bottom_take(unsigned* c) {
+  const unsigned v = atomicAdd(c, (threadIdx.x & (kWave - 1)) == 0 ? 1u : 0u);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+// Items are taken in list order, and an item depends only on items of the previous
+// phase of its row, all earlier in the list and held by workgroups that are running:
+// the earliest unfinished item always waits on nothing, so every wait ends.  Every
+// branch of the loop is uniform per wavefront (the item and the phase are read as
+// uniform, and the counter updates are made by all of wave 0): with a branch on thread 0
+// before the loop's back edge the compiler ran the next item's barriers for the other
+// lanes before thread 0 had counted the item and taken the next one, so the workgroup
+// repeated its item without end.
+template <int D>
+__global__ __launch_bounds__(PairNarrow::NT) void vcycle_bottom_kernel(BottomArgs a) {
+  __shared__ unsigned s_item;
+  __shared__ unsigned s_last;
+  const unsigned total = a.first[a.nphase];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  for (;;) {
+    if (wave == 0) s_item = bottom_take(a.queue);
+    __syncthreads();
+    const unsigned item = __builtin_amdgcn_readfirstlane(s_item);
+    if (item >= total) break;
+    int p = 0;
+    while (item >= a.first[p + 1]) ++p;
+    const int local = (int)(item - a.first[p]);
+    const int row = local / a.per_row[p], tile = local - row * a.per_row[p];
+    if (p > 0) {
+      if (wave == 0)
+        bottom_wait(a.done + (p - 1) * a.rows + row, (unsigned)a.per_row[p - 1], a.err);
+      __syncthreads();
+    }
+    if (p < a.nlev) {
+      pair_pre_tile<D, PairNarrow>(a.pre[p], row, tile);
+    } else if (p == a.nlev) {
+      coarse_solve_body<float, true>(a.n, a.cinv, a.scinv, a.cb, a.scb, a.cmask, a.cx, a.scx,
+                                     a.active, tile, row);
+    } else {
+      pair_post_tile<D, PairNarrow>(a.post[a.nphase - 1 - p], row, tile);
+    }
+    // every store of the item has completed (and its LDS is free for the next item);
+    // then release them and count the item
+    __syncthreads();
+    if (wave == 0) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // the L2 written back
+      (void)bottom_take(a.done + p * a.rows + row);
+    }
+  }
+  // the last workgroup to leave the loop zeroes the counters for the next launch (the
+  // fence completes this workgroup's counter updates before its exit is counted, so no
+  // update lands after the reset)
+  if (wave == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    s_last = bottom_take(a.queue + 1) == gridDim.x - 1 ? 1u : 0u;
+  }
+  __syncthreads();
+  if (s_last != 0) {
+    const int nd = a.nphase * a.rows;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x)
+      __hip_atomic_store(a.done + i, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 2)
+      __hip_atomic_store(a.queue + threadIdx.x, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Workgroups of the persistent launch: as many as fit on the device at once.
+int bottom_grid(int d) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  static thread_local int cached_dev = -1, cached[2] = {0, 0};
+  if (cached_dev != dev) {
+    cached_dev = dev;
+    cached[0] = cached[1] = 0;
+  }
+  int& g = cached[d - 1];
+  if (g == 0) {
+    int cus = 0, per_cu = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    hipError_t e = d == 2 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                &per_cu, vcycle_bottom_kernel<2>, PairNarrow::NT, 0)
+                          : hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                                &per_cu, vcycle_bottom_kernel<1>, PairNarrow::NT, 0);
+    if (e != hipSuccess || cus <= 0 || per_cu <= 0) return 256;
+    g = cus * per_cu;
+  }
+  return g;
+}
+
 }  // namespace
+
+long long bottom_sync_words(int rows) { return 2 + (long long)kBottomPhases * rows; }
+
+int vcycle_bottom(const BottomCall& c, unsigned* sync, int* err, hipStream_t s) {
+  if (c.nlev < 1 || c.nlev > kMaxBottomLevels || sync == nullptr || err == nullptr ||
+      c.cinv == nullptr || c.cb == nullptr || c.cx == nullptr || c.n <= 0 || c.n % 2 != 0 ||
+      c.scinv % 2 != 0 || (reinterpret_cast<uintptr_t>(c.cinv) & 15) != 0) {
+    set_error("vcycle_bottom: 1 to 3 pair levels and an even-sized, aligned coarsest inverse");
+    return STTOPT_EINVAL;
+  }
+  const int d = c.lv[0].op.d, rows = c.lv[0].rows;
+  BottomArgs a{};
+  a.nlev = c.nlev;
+  a.rows = rows;
+  a.nphase = 2 * c.nlev + 1;
+  for (int l = 0; l < c.nlev; ++l) {
+    if (int e = check_pair_level(c.lv[l].op, "vcycle_bottom")) return e;
+    if (c.lv[l].op.d != d || c.lv[l].rows != rows) {
+      set_error("vcycle_bottom: the levels need the same dofs per node and rows");
+      return STTOPT_EINVAL;
+    }
+    a.pre[l] = pair_args<PairNarrow>(c.lv[l], true);
+    a.post[l] = pair_args<PairNarrow>(c.lv[l], false);
+    a.per_row[l] = a.pre[l].tiles;
+    a.per_row[a.nphase - 1 - l] = a.post[l].tiles;
+  }
+  if (rows <= 0) return STTOPT_OK;
+  a.per_row[c.nlev] = (int)ceil_div(c.n, (long long)(PairNarrow::NT / kWave));
+  long long total = 0;
+  for (int p = 0; p < a.nphase; ++p) {
+    a.first[p] = (unsigned)total;
+    total += (long long)a.per_row[p] * rows;
+  }
+  if (total >= 0x7fffffffLL) {
+    set_error("vcycle_bottom: too many items");
+    return STTOPT_EINVAL;
+  }
+  a.first[a.nphase] = (unsigned)total;
+  a.n = c.n;
+  a.cinv = c.cinv;
+  a.scinv = c.scinv;
+  a.cb = c.cb;
+  a.scb = c.scb;
+  a.cmask = c.cmask;
+  a.cx = c.cx;
+  a.scx = c.scx;
+  a.active = c.lv[0].active;
+  a.queue = sync;
+  a.done = sync + 2;
+  a.err = err;
+  const int grid = (int)(total < bottom_grid(d) ? total : bottom_grid(d));
+  if (d == 2)
+    hipLaunchKernelGGL(vcycle_bottom_kernel<2>, dim3(grid), dim3(PairNarrow::NT), 0, s, a);
+  else
+    hipLaunchKernelGGL(vcycle_bottom_kernel<1>, dim3(grid), dim3(PairNarrow::NT), 0, s, a);
+  STT_CHECK_LAUNCH("vcycle_bottom_kernel");
+  return STTOPT_OK;
+}
 
 int coarse_pair_groups(const PairLevel* lv, int n, bool pre, int variant, hipStream_t s) {
   if (n < 1 || n > kMaxPairGroups) {

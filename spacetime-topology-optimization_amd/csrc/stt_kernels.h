@@ -122,6 +122,39 @@ int smooth_prolong(const OpDesc& op, int rows, const float* x, long long sx, con
                    const float* invd, long long sd, float* y, long long sy, double omega,
                    const int* active, hipStream_t s);
 
+// Coarsest solve x = cinv b, output dofs bx * (waves per workgroup) + wave of one row:
+// one wavefront per output dof, the 64 lanes stream one row of the dense inverse
+// (coalesced) and reduce in fp64.
+template <typename T, bool PAIRS>
+__device__ __forceinline__ void coarse_solve_body(long long n, const double* cinv,
+                                                  long long scinv, const T* b, long long sb,
+                                                  const uint32_t* mask, T* x, long long sx,
+                                                  const int* active, const int bx,
+                                                  const int row) {
+  if (active != nullptr && active[row] == 0) return;
+  const long long i = (long long)bx * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (i >= n) return;  // uniform per wave
+  const double* Ai = cinv + row * scinv + i * n;
+  const T* bb = b + row * sb;
+  double acc = 0.0;
+  if (PAIRS) {
+    // 16-byte loads (n even and 16-byte aligned rows), several in flight per lane
+    const long long n2 = n >> 1;
+    const double2* A2 = reinterpret_cast<const double2*>(Ai);
+#pragma unroll 8
+    for (long long j = lane; j < n2; j += kWave) {
+      const double2 av = A2[j];
+      acc += av.x * (double)bb[2 * j] + av.y * (double)bb[2 * j + 1];
+    }
+  } else {
+#pragma unroll 8
+    for (long long j = lane; j < n; j += kWave) acc += Ai[j] * (double)bb[j];
+  }
+  acc = wave_sum(acc);
+  if (lane == 0) x[row * sx + i] = mask_free(mask, i) ? static_cast<T>(acc) : T(0);
+}
+
 // ---- one launch over the same level of several hierarchies (the uneven batch) ----
 // Each kernel below runs the same pass on up to kMaxBatchGroups hierarchies of different
 // grids in one launch: hierarchy i takes a contiguous range of workgroups, with its own
@@ -250,6 +283,33 @@ struct PairLevel {
   const int* active;
 };
 int coarse_pair_groups(const PairLevel* lv, int n, bool pre, int variant, hipStream_t s);
+
+// The bottom of the fp32 V-cycle in one persistent launch: the two-pass node-stencil
+// levels lv[0 .. nlev - 1] (finest first, each 2 x 2 over the next; lv[i].bc is
+// lv[i + 1].b and the coarsest level's right-hand side cb), the coarsest solve
+// cx = cinv cb, and the post passes back up (lv[i].ec is lv[i + 1].x4, or cx).  The
+// passes' tiles form one work list, phase by phase (pre passes, coarsest solve, post
+// passes) and row by row; workgroups take items in order, and an item waits (bounded)
+// until every item of the previous phase of its row is done, so no launch boundary
+// separates the phases.  The arithmetic is that of the separate passes (the same
+// bodies), so results are bit-identical.  sync: bottom_sync_words(rows) words that are
+// zero before the launch (the launch leaves them at zero); *err is set if a wait gave
+// up (the results are then not valid).
+constexpr int kMaxBottomLevels = 3;
+struct BottomCall {
+  PairLevel lv[kMaxBottomLevels];
+  int nlev;
+  long long n;  // coarsest level dofs (even, 16-byte aligned rows)
+  const double* cinv;
+  long long scinv;
+  const float* cb;
+  long long scb;
+  const uint32_t* cmask;
+  float* cx;
+  long long scx;
+};
+long long bottom_sync_words(int rows);
+int vcycle_bottom(const BottomCall& c, unsigned* sync, int* err, hipStream_t s);
 
 // The level-0 passes of several hierarchies in one launch (arguments of smooth_pair_pre
 // and smooth_pair_post; all calls of a launch with the same dofs per node, reaction

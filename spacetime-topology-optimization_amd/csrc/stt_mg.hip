@@ -512,38 +512,7 @@ template int prolong_add<float>(int, int, int, int, int, int, const float*, long
                                 const uint32_t*, float*, long long, const int*, hipStream_t);
 
 // ---------------------------------------------------------- coarse solve ----
-// One wavefront per output dof: the 64 lanes stream one row of the dense
-// inverse (coalesced) and reduce in fp64.
-
-template <typename T, bool PAIRS>
-__device__ __forceinline__ void coarse_solve_body(long long n, const double* cinv,
-                                                  long long scinv, const T* b, long long sb,
-                                                  const uint32_t* mask, T* x, long long sx,
-                                                  const int* active, const int bx,
-                                                  const int row) {
-  if (active != nullptr && active[row] == 0) return;
-  const long long i = (long long)bx * (blockDim.x / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (i >= n) return;  // uniform per wave
-  const double* Ai = cinv + row * scinv + i * n;
-  const T* bb = b + row * sb;
-  double acc = 0.0;
-  if (PAIRS) {
-    // 16-byte loads (n even and 16-byte aligned rows), several in flight per lane
-    const long long n2 = n >> 1;
-    const double2* A2 = reinterpret_cast<const double2*>(Ai);
-#pragma unroll 8
-    for (long long j = lane; j < n2; j += kWave) {
-      const double2 av = A2[j];
-      acc += av.x * (double)bb[2 * j] + av.y * (double)bb[2 * j + 1];
-    }
-  } else {
-#pragma unroll 8
-    for (long long j = lane; j < n; j += kWave) acc += Ai[j] * (double)bb[j];
-  }
-  acc = wave_sum(acc);
-  if (lane == 0) x[row * sx + i] = mask_free(mask, i) ? static_cast<T>(acc) : T(0);
-}
+// (coarse_solve_body: stt_kernels.h)
 
 template <typename T, bool PAIRS>
 __global__ __launch_bounds__(256) void coarse_solve_kernel(long long n, const double* cinv,

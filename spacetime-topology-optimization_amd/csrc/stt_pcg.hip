@@ -31,6 +31,8 @@ struct Scratch {
   double* beta;     // [rows]
   int* active;      // [rows]
   int* counter;     // [1]
+  unsigned* bottom;  // the persistent bottom pass's counters (bottom_sync_words(rows))
+  int* bottom_err;   // [1] set when a wait of that pass gave up
 };
 
 static long long align256(long long b) { return (b + 255) & ~255LL; }
@@ -41,6 +43,7 @@ static long long scratch_bytes(int rows, int tiles) {
   b += 4 * align256(8LL * rows);
   b += align256(4LL * rows);
   b += align256(16);
+  b += align256(4LL * (bottom_sync_words(rows) + 1));
   return b;
 }
 
@@ -69,8 +72,22 @@ static Scratch carve(void* base, int rows, int tiles) {
   s.active = reinterpret_cast<int*>(p);
   p += align256(4LL * rows);
   s.counter = reinterpret_cast<int*>(p);
+  p += align256(16);
+  s.bottom = reinterpret_cast<unsigned*>(p);
+  s.bottom_err = reinterpret_cast<int*>(s.bottom + bottom_sync_words(rows));
   return s;
 }
+
+// Zero the persistent bottom pass's counters and error word (before a solve or cycle).
+static int clear_bottom(const Scratch& sc, int rows, hipStream_t s) {
+  const hipError_t e =
+      hipMemsetAsync(sc.bottom, 0, 4 * (size_t)(bottom_sync_words(rows) + 1), s);
+  return e == hipSuccess ? STTOPT_OK : hip_status(e, "hipMemsetAsync(bottom counters)");
+}
+
+// Levels of the V-cycle's bottom that run as one persistent launch (vcycle_bottom): at
+// most this many two-pass levels above the coarsest; 0 runs them launch by launch.
+static int g_bottom_levels = 0;
 
 // --------------------------------------------------------- tile kernels -----
 // Elementwise PCG kernels walk the same 64 x 16 node tiles as the level-0 stencil
@@ -373,6 +390,11 @@ struct VCycle {
   const sttopt_mg* mg;
   hipStream_t s;
   const int* active;
+  // the persistent bottom pass's counters and error word (null: launch by launch), and
+  // whether a cycle has used it
+  unsigned* bsync = nullptr;
+  int* berr = nullptr;
+  bool bottom_used = false;
   int rows;
   OpDesc op[STTOPT_MAX_LEVELS];
   long long n[STTOPT_MAX_LEVELS];
@@ -440,10 +462,82 @@ struct VCycle {
     return STTOPT_OK;
   }
 
+  // The number of two-pass node-stencil levels from level l to the one above the
+  // coarsest when the rest of the cycle from l can run as one persistent launch
+  // (vcycle_bottom), else 0.
+  int bottom_levels(int l) const {
+    if constexpr (sizeof(T) != sizeof(float)) {
+      return 0;
+    } else {
+      const int last = mg->nlevels - 1, nb = last - l;
+      if (bsync == nullptr || berr == nullptr || nb < 1 || nb > g_bottom_levels ||
+          nb > kMaxBottomLevels || mg->n_smooth != 2 || mg->gamma != 1 ||
+          (mg->levels[0].flags & STTOPT_SEPARATE_SWEEPS) != 0)
+        return 0;
+      for (int i = l; i < last; ++i) {
+        const sttopt_level& Li = mg->levels[i];
+        if (!op[i].keff_node || Li.kx != 2 || Li.ky != 2 || Li.mask == nullptr ||
+            (Li.flags & STTOPT_LEVEL_SEPARATE) != 0)
+          return 0;
+      }
+      const sttopt_level& C = mg->levels[last];
+      if (C.cinv == nullptr || n[last] % 2 != 0 || C.cinv_stride % 2 != 0 ||
+          (reinterpret_cast<uintptr_t>(C.cinv) & 15) != 0)
+        return 0;
+      return nb;
+    }
+  }
+
   // Cycle on level l >= 1 with right-hand side b; *out receives the result buffer.
   int cycle(int l, const T* b, T** out) {
     const sttopt_level& L = mg->levels[l];
     const long long nl = n[l];
+    if constexpr (sizeof(T) == sizeof(float)) {
+      // the two passes of levels l .. last - 1 and the coarsest solve in one launch: the
+      // buffers of the launch-by-launch path below (x2 in vx, the post result in vx2,
+      // the restricted residuals in the coarser levels' vb, the coarsest result in vx)
+      if (const int nb = bottom_levels(l)) {
+        const int last = mg->nlevels - 1;
+        BottomCall c{};
+        c.nlev = nb;
+        for (int i = 0; i < nb; ++i) {
+          const int li = l + i;
+          const sttopt_level& Li = mg->levels[li];
+          const sttopt_level& Ci = mg->levels[li + 1];
+          PairLevel& P = c.lv[i];
+          P.op = op[li];
+          P.rows = rows;
+          P.b = i == 0 ? b : buf(Li.vb);
+          P.sb = n[li];
+          P.invd = invd(li);
+          P.sd = sd(li);
+          P.x2 = buf(Li.vx);
+          P.sx2 = n[li];
+          P.cmask = Ci.mask;
+          P.bc = buf(Ci.vb);
+          P.sbc = n[li + 1];
+          P.ec = li + 1 < last ? buf(Ci.vx2) : buf(Ci.vx);
+          P.sec = n[li + 1];
+          P.x4 = buf(Li.vx2);
+          P.sx4 = n[li];
+          P.omega = mg->omega;
+          P.active = active;
+        }
+        const sttopt_level& C = mg->levels[last];
+        c.n = n[last];
+        c.cinv = C.cinv;
+        c.scinv = C.cinv_stride;
+        c.cb = buf(C.vb);
+        c.scb = n[last];
+        c.cmask = C.mask;
+        c.cx = buf(C.vx);
+        c.scx = n[last];
+        if (int e = vcycle_bottom(c, bsync, berr, s)) return e;
+        bottom_used = true;
+        *out = buf(L.vx2);
+        return STTOPT_OK;
+      }
+    }
     if (l == mg->nlevels - 1) {
       T* x = buf(L.vx);
       if (int e = coarse_solve<T>(nl, rows, L.cinv, L.cinv_stride, b, nl, L.mask, x, nl,
@@ -966,6 +1060,11 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (remaining == 0) return STTOPT_OK;
 
   VCycle<T> vc(mg, s, sc.active);
+  if (g_bottom_levels > 0) {
+    if (int e = clear_bottom(sc, rows, s)) return e;
+    vc.bsync = sc.bottom;
+    vc.berr = sc.bottom_err;
+  }
   const int rz_tiles = vc.rz_tiles();
   if (int e = vc.run(r, n, z, n, sc.part[2], Ap)) return e;
   hipLaunchKernelGGL(fin_rz_kernel, rgrid, tblock, 0, s, rz_tiles, sc.part[2], sc.active,
@@ -1117,12 +1216,22 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
                     static_cast<const int*>(nullptr));
     STT_CHECK_LAUNCH("pcg last x update");
   }
-  // n_iter: the slowest row's count
+  // n_iter: the slowest row's count (and whether a wait of the persistent bottom pass
+  // gave up, read with it)
   std::vector<int> h(rows > 0 ? rows : 1);
+  int bottom_err = 0;
   hipError_t e = hipMemcpyAsync(h.data(), iters, sizeof(int) * rows, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(iters)");
+  if (vc.bottom_used) {
+    e = hipMemcpyAsync(&bottom_err, sc.bottom_err, sizeof(int), hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(bottom error)");
+  }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+  if (bottom_err != 0) {
+    set_error("sttopt_mgpcg: a wait of the persistent V-cycle bottom pass gave up");
+    return STTOPT_EHIP;
+  }
   int mx = 0;
   for (int i = 0; i < rows; ++i) mx = h[i] > mx ? h[i] : mx;
   if (g_apply_prof.on && !g_apply_prof.iter_of.empty()) {
@@ -1883,7 +1992,36 @@ int sttopt_vcycle(const sttopt_mg* mg, const double* r, int64_t r_stride, double
     return vc.run(r, r_stride, z, z_stride, nullptr);
   }
   VCycle<float> vc(mg, s, nullptr);
-  return vc.run(r, r_stride, z, z_stride, nullptr);
+  Scratch sc{};
+  if (g_bottom_levels > 0 && mg->scratch != nullptr) {
+    OpDesc op0;
+    op0.d = mg->d;
+    op0.nelx = mg->levels[0].nelx;
+    op0.nely = mg->levels[0].nely;
+    sc = carve(mg->scratch, mg->rows, partial_slots(op0));
+    if (int e = clear_bottom(sc, mg->rows, s)) return e;
+    vc.bsync = sc.bottom;
+    vc.berr = sc.bottom_err;
+  }
+  if (int e = vc.run(r, r_stride, z, z_stride, nullptr)) return e;
+  if (vc.bottom_used) {
+    int bottom_err = 0;
+    if (int e = read_device_int(sc.bottom_err, s, &bottom_err)) return e;
+    if (bottom_err != 0) {
+      set_error("sttopt_vcycle: a wait of the persistent bottom pass gave up");
+      return STTOPT_EHIP;
+    }
+  }
+  return STTOPT_OK;
+}
+
+int sttopt_vcycle_bottom(int32_t max_levels) {
+  if (max_levels < 0 || max_levels > kMaxBottomLevels) {
+    set_error("sttopt_vcycle_bottom: 0 to 3 levels");
+    return STTOPT_EINVAL;
+  }
+  g_bottom_levels = max_levels;
+  return STTOPT_OK;
 }
 
 int sttopt_level_apply(const sttopt_mg* mg, int32_t level, const void* v, int64_t v_stride,

@@ -738,6 +738,81 @@ def test_mgpcg_two_sweep_passes_equal_separate_sweeps(dev):
     assert err < 1e-9, err
 
 
+@pytest.mark.parametrize("d, max_levels", [(2, 1), (2, 2), (2, 3), (1, 3)])
+def test_persistent_bottom_equals_launch_by_launch(dev, d, max_levels):
+    """The bottom of the fp32 cycle as one persistent launch (the two-pass levels above
+    the coarsest and the coarsest solve as one work list with per-row counters) against
+    the same passes launch by launch, bit for bit: V-cycles on rows of different
+    densities, and whole MG-PCG solves (graph replay, rows retiring at different
+    iterations, stopped after 1..4 iterations and run to convergence), twice over, with
+    eager work between the calls; every chain length (1 to 3 levels) and d = 1, 2."""
+    import ctypes
+
+    from sttopt_amd import _lib, torch_fem, torch_mg
+
+    # 2 x 2 down to 7 x 6 (an even coarsest size at d = 1 and 2): node-stencil levels
+    # 2, 3 and 4, so each chain length takes a different start level
+    nelx, nely, rows = 224, 192, 3
+    if d == 2:
+        s = _setup(nelx, nely, dev)
+        mask_t, KE_t, edof_t, n = s["mask_t"], s["KE_t"], s["edof_t"], s["ndof"]
+    else:
+        nn = (nelx + 1) * (nely + 1)
+        mask = np.ones(nn, bool)
+        mask[-(nelx + 1):] = False
+        mask_t = torch.as_tensor(mask, device=dev)
+        KE_t = torch.as_tensor(fem_direct.diffusion_KE(), device=dev)
+        edof_t = torch.as_tensor(fem_direct.element_dofs(nelx, nely, 1), device=dev)
+        n = nn
+    rng = np.random.default_rng(40 + d)
+    x = torch.as_tensor(np.stack([binary_design(nelx, nely), rng.uniform(0.05, 1, nelx * nely),
+                                  np.full(nelx * nely, 0.5)]), dtype=torch.float64, device=dev)
+    dens = torch_fem.simp_density(x, EMIN, EMAX, PENAL)
+    levels = torch_mg.build_hierarchy(dens, edof_t, KE_t, mask_t, nelx, nely,
+                                      max_coarse_elements=48)
+    assert len(levels) == 6  # level 1 from the densities, levels 2-4 node stencils
+    lib = _lib.load()
+    old = torch_mg.SCHEDULE_OVERRIDE
+    torch_mg.SCHEDULE_OVERRIDE = "two_pass_narrow"
+    vc = torch_mg.VCycle(levels, dtype=torch.float32)
+    F = torch_fem.project(torch.as_tensor(rng.standard_normal((rows, n)), device=dev), mask_t)
+    r = torch_fem.project(torch.as_tensor(rng.standard_normal((rows, n)), device=dev), mask_t)
+
+    def solve(max_iter):
+        mg, _, work = vc._native(rows)
+        xo = torch.empty(rows, n, dtype=torch.float64, device=dev)
+        iters = torch.zeros(rows, dtype=torch.int32, device=dev)
+        resid = torch.zeros(rows, dtype=torch.float64, device=dev)
+        st = lib.sttopt_mgpcg(ctypes.byref(mg), F.data_ptr(), n, xo.data_ptr(), n, 1e-10,
+                              max_iter, 1, 1, 1, work.data_ptr(), iters.data_ptr(),
+                              resid.data_ptr(), ctypes.byref(ctypes.c_int32(0)),
+                              _lib.stream_handle(dev))
+        assert st in (_lib.OK, _lib.ENOCONV), _lib.last_error()
+        torch.cuda.synchronize(dev)
+        return st, xo.clone(), iters.tolist(), resid.clone()
+
+    def runs(bottom):
+        assert lib.sttopt_vcycle_bottom(bottom) == _lib.OK
+        out = [vc(r).clone()]
+        _ = (r * 2).sum().item()  # eager work and a synchronization between the calls
+        out += [solve(k) for k in (1, 2, 3, 4, 500)]
+        return out
+
+    try:
+        for _ in range(2):
+            a = runs(max_levels)
+            b = runs(0)
+            assert float(b[0].abs().max()) > 0
+            assert torch.equal(a[0], b[0])
+            for sa, sb in zip(a[1:], b[1:]):
+                assert sa[0] == sb[0] and sa[2] == sb[2]
+                assert torch.equal(sa[1], sb[1]) and torch.equal(sa[3], sb[3])
+            assert a[-1][0] == _lib.OK
+    finally:
+        lib.sttopt_vcycle_bottom(0)
+        torch_mg.SCHEDULE_OVERRIDE = old
+
+
 # ------------------------------------------------------- dense SPD inverse ----
 
 
