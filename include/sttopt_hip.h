@@ -183,6 +183,9 @@ typedef struct sttopt_mg {
   sttopt_level levels[STTOPT_MAX_LEVELS];
   /* device scratch sized by sttopt_mg_scratch_bytes                              */
   void* scratch;
+  /* optional device word of sttopt_spd_inverse_band_async for the coarsest inverse:
+   * sttopt_mgpcg and sttopt_mgpcg_batch return STTOPT_ENOTSPD when it is nonzero  */
+  const int32_t* cinv_status;
 } sttopt_mg;
 
 /* Upper-triangle copy of fp64 FULL element matrices, (A + A^T) / 2, in dtype (V-cycle
@@ -278,6 +281,17 @@ int sttopt_spd_inverse(int32_t n, int32_t rows, double* a, int64_t a_stride, dou
  * Replaces the same reference step (torch_mg.py:269-290, 448-451). */
 int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
                             int64_t a_stride, double* work, void* stream);
+
+/* sttopt_spd_inverse_band without the wait for its result: the launches are queued on
+ * the stream and the device word *not_spd (device memory) receives 1 when a matrix is
+ * not positive definite, else 0.  The caller reads it, or hands it to the solvers in
+ * sttopt_mg.cinv_status, which report STTOPT_ENOTSPD at their first readback.  Lets the
+ * host queue the rest of a hierarchy build and the solve while the inverse runs.
+ * Replaces the same reference step (torch_mg.py:269-290: the factorization's error is
+ * raised before the solve returns). */
+int sttopt_spd_inverse_band_async(int32_t n, int32_t band, int32_t rows, double* a,
+                                  int64_t a_stride, double* work, int32_t* not_spd,
+                                  void* stream);
 
 /* Grid transfers on the node grid (torch_mg._on_node_grid with _restrict_axis /
  * _interp_axis, torch_mg.py:75-109).  restrict: coarse = P_c (R fine);

@@ -95,6 +95,7 @@ class MG(ctypes.Structure):
         ("rdiag64_stride", c_int64),
         ("levels", Level * MAX_LEVELS),
         ("scratch", c_void_p),
+        ("cinv_status", c_void_p),
     ]
 
 
@@ -149,6 +150,8 @@ _SIGNATURES = {
                                       _P, c_int64, _P]),
     "sttopt_spd_inverse": (c_int32, [c_int32, c_int32, _P, c_int64, _P, _P]),
     "sttopt_spd_inverse_band": (c_int32, [c_int32, c_int32, c_int32, _P, c_int64, _P, _P]),
+    "sttopt_spd_inverse_band_async": (
+        c_int32, [c_int32, c_int32, c_int32, _P, c_int64, _P, _P, _P]),
     "sttopt_dense_assemble": (
         c_int32,
         [c_int32, c_int32, c_int32, c_int32, _P, c_int64, _P, c_int64, _P, c_int64, _P, _P,

@@ -434,17 +434,19 @@ int64_t sttopt_spd_inverse_work_count(int32_t n, int32_t rows) {
   return 2LL * rows * kSB * kSB + 2LL * rows * kSB * n + 1;
 }
 
-int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
-                            int64_t a_stride, double* work, void* stream) {
+int sttopt_spd_inverse_band_async(int32_t n, int32_t band, int32_t rows, double* a,
+                                  int64_t a_stride, double* work, int32_t* not_spd,
+                                  void* stream) {
   if (n <= 0 || rows < 0 || band < 0 || a == nullptr || work == nullptr ||
-      (rows > 1 && a_stride < (int64_t)n * n)) {
+      not_spd == nullptr || (rows > 1 && a_stride < (int64_t)n * n)) {
     set_error("sttopt_spd_inverse: invalid arguments");
     return STTOPT_EINVAL;
   }
-  if (rows == 0) return STTOPT_OK;
   hipStream_t s = (hipStream_t)stream;
+  if (int e = fill<int>(1, 1, 0, not_spd, 1, nullptr, s)) return e;
+  if (rows == 0) return STTOPT_OK;
   SweepWork w = carve_sweep(work, n, rows);
-  if (int e = fill<int>(1, 1, 0, w.bad, 1, nullptr, s)) return e;
+  w.bad = not_spd;
   const int nb = (n + kSB - 1) / kSB;
   // blocks off the diagonal that the band reaches; at least one, so that step K always
   // updates diagonal block K + 1, whose workgroup inverts it for the next step
@@ -464,9 +466,22 @@ int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
                        w.bad);
     STT_CHECK_LAUNCH("sweep");
   }
-  int bad = 0;
-  if (int e = read_device_int(w.bad, s, &bad)) return e;
-  if (bad) {
+  return STTOPT_OK;
+}
+
+int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
+                            int64_t a_stride, double* work, void* stream) {
+  if (n <= 0 || rows < 0 || work == nullptr) {
+    set_error("sttopt_spd_inverse: invalid arguments");
+    return STTOPT_EINVAL;
+  }
+  if (rows == 0) return STTOPT_OK;
+  int* bad = carve_sweep(work, n, rows).bad;
+  if (int e = sttopt_spd_inverse_band_async(n, band, rows, a, a_stride, work, bad, stream))
+    return e;
+  int v = 0;
+  if (int e = read_device_int(bad, (hipStream_t)stream, &v)) return e;
+  if (v) {
     set_error("sttopt_spd_inverse: a matrix is not positive definite");
     return STTOPT_ENOTSPD;
   }

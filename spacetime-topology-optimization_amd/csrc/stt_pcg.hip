@@ -285,13 +285,17 @@ __device__ __forceinline__ double safe_div(double num, double den) {
 }
 
 // Initial residual: b norm, relative residual, active flags.
+// counter[2] receives the coarsest inverse's status word (0 without one), which the host
+// reads with the active-row count.
 __global__ __launch_bounds__(kBlock) void fin_init_kernel(int tiles, const double* pbb,
                                                           const double* prr, double rtol,
                                                           double* bsafe, double* resid,
                                                           int* active, int* iters,
-                                                          int* counter) {
+                                                          int* counter,
+                                                          const int* cinv_status) {
   __shared__ double red[kBlock / kWave];
   const int row = blockIdx.x;
+  if (row == 0 && threadIdx.x == 0) counter[2] = cinv_status != nullptr ? *cinv_status : 0;
   const double bb = row_sum(pbb, tiles, row, red);
   __syncthreads();
   const double rr = row_sum(prr, tiles, row, red);
@@ -912,6 +916,26 @@ int read_device_int(const int* dev, hipStream_t s, int* out) {
   return STTOPT_OK;
 }
 
+// The active-row count and the coarsest inverse's status after fin_init, in one
+// readback (words 8 and 10 of the pinned block; the lagged checks use words 1 and 2).
+static int read_init(const int* counter, hipStream_t s, int* remaining) {
+  int* h = pinned_counter();
+  if (h == nullptr) {
+    set_error("hipHostMalloc of the readback word failed");
+    return STTOPT_EHIP;
+  }
+  hipError_t e = hipMemcpyAsync(h + 8, counter, 3 * sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(readback words)");
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+  *remaining = h[8];
+  if (h[10] != 0) {
+    set_error("sttopt_mgpcg: the coarsest operator is not positive definite");
+    return STTOPT_ENOTSPD;
+  }
+  return STTOPT_OK;
+}
+
 // Optional HIP-event timing of the outer iteration's K(rho)p launches (process-wide, off
 // by default): one event pair per launch, resolved after the solve's final readback.
 // Bytes per launch: 8 (2 ndof + nel) per row still active in that iteration.
@@ -1053,10 +1077,10 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (int e = fill<int>(2, 1, 0, sc.counter, 2, nullptr, s)) return e;
   hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4],
                      cold ? sc.part[4] : sc.part[1], rtol, sc.bsafe, resid, sc.active, iters,
-                     sc.counter);
+                     sc.counter, mg->cinv_status);
   STT_CHECK_LAUNCH("pcg init");
   int remaining = 0;
-  if (int e = read_device_int(sc.counter, s, &remaining)) return e;
+  if (int e = read_init(sc.counter, s, &remaining)) return e;
   if (remaining == 0) return STTOPT_OK;
 
   VCycle<T> vc(mg, s, sc.active);
@@ -1765,12 +1789,12 @@ static int mgpcg_batch_impl(const sttopt_batch_group* in, int ng, double rtol, i
     if (int e = fill<int>(2, 1, 0, g.sc.counter, 2, nullptr, s)) return e;
     hipLaunchKernelGGL(fin_init_kernel, dim3(g.rows), tblock, 0, s, g.tiles, g.sc.part[4],
                        cold ? g.sc.part[4] : g.sc.part[1], rtol, g.sc.bsafe, g.resid,
-                       g.sc.active, g.iters, g.sc.counter);
+                       g.sc.active, g.iters, g.sc.counter, g.mg->cinv_status);
     STT_CHECK_LAUNCH("batch init");
   }
   for (auto& g : G) {
     int rem = 0;
-    if (int e = read_device_int(g.sc.counter, s, &rem)) return e;
+    if (int e = read_init(g.sc.counter, s, &rem)) return e;
     remaining += rem;
   }
   if (remaining == 0) return STTOPT_OK;

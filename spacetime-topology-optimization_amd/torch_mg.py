@@ -294,6 +294,8 @@ class _Level:
     # device storage used by the kernels
     _soa: Tensor | None = field(default=None, repr=False)  # (rows, k*k, nel) fp64
     _cinv: Tensor | None = field(default=None, repr=False)  # (rows, n, n) fp64
+    # the inverse's not-positive-definite word on the device, until it has been read
+    _cinv_status: Tensor | None = field(default=None, repr=False)
     _dbatch: tuple = field(default=(), repr=False)
     _copies: dict = field(default_factory=dict, repr=False)
     # per-position matrices of the level-0 elements under one element of this level
@@ -341,6 +343,17 @@ class _Level:
 
     def bits(self) -> Tensor:
         return _grid.mask_bits(self.mask)
+
+    def spd_check(self) -> None:
+        """Raise the coarsest factorization's error if its inverse, queued without a wait
+        (`_dense_factor`), found a matrix that is not positive definite; reads the device
+        word once."""
+        status = self._cinv_status
+        if status is None:
+            return
+        if int(status.item()) != 0:
+            raise torch.linalg.LinAlgError(_NOT_SPD)
+        self._cinv_status = None
 
     def __call__(self, v: Tensor) -> Tensor:
         if not self.coarse:
@@ -473,14 +486,21 @@ def _assemble_dense(level: _Level, rows: int) -> Tensor:
     return dense
 
 
+_NOT_SPD = ("linalg.cholesky: The factorization could not be completed because the input is "
+            "not positive-definite (the coarsest multigrid operator).")
+
+
 def _dense_factor(level: _Level, rows: int) -> None:
     """Inverse of the coarsest level's dense projected operator (torch_mg._dense_cholesky,
     torch_mg.py:269-290; the inverse replaces the factor's two triangular solves per
     coarse solve, torch_mg.py:448-451).  The operator is assembled into the buffer that
     is then inverted in place; `chol` assembles it again if a caller asks for it.
 
-    :raises torch.linalg.LinAlgError: if the operator is not positive definite (the
-        reference's `torch.linalg.cholesky` raises the same)
+    The inverse is queued without a wait for its positive-definiteness check, so that the
+    host goes on queueing the build and the solve while it runs: the solvers read the
+    check with their first readback (`sttopt_mg.cinv_status`) and every other user of the
+    inverse calls `spd_check` first.  A matrix that is not positive definite raises
+    torch.linalg.LinAlgError (as the reference's `torch.linalg.cholesky`) from there.
     """
     dev = level.mask.device
     n = level.ndof
@@ -488,18 +508,16 @@ def _dense_factor(level: _Level, rows: int) -> None:
     lib = _lib.load()
     work = torch.empty(lib.sttopt_spd_inverse_work_count(n, rows), dtype=torch.float64,
                        device=dev)
+    status = torch.empty(1, dtype=torch.int32, device=dev)
     # the operator couples a node with its eight grid neighbours only: half-bandwidth
     # d (nnx + 1) + d - 1 in the node-major dof numbering
     band = level.d * (level.nelx + 2) + level.d - 1
-    status = lib.sttopt_spd_inverse_band(n, band, rows, inv.data_ptr(), n * n,
-                                         work.data_ptr(), _lib.stream_handle(dev))
-    if status == _lib.ENOTSPD:
-        raise torch.linalg.LinAlgError(
-            "linalg.cholesky: The factorization could not be completed because the input is "
-            "not positive-definite (the coarsest multigrid operator)."
-        )
-    _lib.check(status, "sttopt_spd_inverse")
+    _lib.check(lib.sttopt_spd_inverse_band_async(n, band, rows, inv.data_ptr(), n * n,
+                                                 work.data_ptr(), status.data_ptr(),
+                                                 _lib.stream_handle(dev)),
+               "sttopt_spd_inverse")
     level._cinv = inv
+    level._cinv_status = status
     level._dense = None
     level._chol = None
 
@@ -770,6 +788,9 @@ class VCycle:
             if lv._cinv is not None:
                 L.cinv = lv._cinv.data_ptr()
                 L.cinv_stride = 0 if shared else lv._cinv.stride(0)
+                if lv._cinv_status is not None:  # read by the solvers' first readback
+                    keep.append(lv._cinv_status)
+                    mg.cinv_status = lv._cinv_status.data_ptr()
             names = ["vb", "vx", "vx2", "vr"] + (["ve", "vw"] if self.gamma > 1 else [])
             for name in names:
                 buf = torch.empty(rows, lv.ndof, dtype=vd, device=dev)
@@ -794,6 +815,7 @@ class VCycle:
     def __call__(self, r: Tensor) -> Tensor:
         dev = _lib.require_device(r)
         lv0 = self.levels[0]
+        self.levels[-1].spd_check()
         batch = _grid.batch_shape(r.shape[:-1], lv0._dbatch)
         rows = max(_grid.numel(batch), 1)
         mg, _, _ = self._native(rows)
@@ -920,6 +942,9 @@ def native_mgpcg(vcycle: VCycle, b: Tensor, *, rtol: float, max_iter: int,
         per_row = iters.tolist()  # one read back
         info["row_iters"] = int(sum(per_row))
         info["row_iter_list"] = per_row
+    if status == _lib.ENOTSPD:
+        raise torch.linalg.LinAlgError(_NOT_SPD)
+    vcycle.levels[-1]._cinv_status = None  # checked by the solve
     if status == _lib.ENOCONV:
         raise torch_fem.CGConvergenceError(resid.reshape(batch) if batch else resid[0], int(max_iter), rtol)
     _lib.check(status, "sttopt_mgpcg")

@@ -102,6 +102,10 @@ def batch_solve(vcycles: list, rhs: list[Tensor], *, rtol: float, max_iter: int,
         info["n_iter"] = int(n_iter.value)
         info["row_iter_list"] = flat
         info["row_iters"] = int(sum(flat))
+    if status == _lib.ENOTSPD:
+        raise torch.linalg.LinAlgError(torch_mg._NOT_SPD)
+    for vc in vcycles:
+        vc.levels[-1]._cinv_status = None  # checked by the solve
     if status == _lib.ENOCONV:
         raise torch_fem.CGConvergenceError(torch.cat(resids), int(max_iter), rtol)
     _lib.check(status, "sttopt_mgpcg_batch")

@@ -939,6 +939,62 @@ def test_spd_inverse_reports_indefinite(dev):
     assert st == _lib.ENOTSPD and "not positive definite" in _lib.last_error()
 
 
+def test_spd_inverse_band_async_equals_the_waiting_call(dev):
+    """sttopt_spd_inverse_band_async queues the same sweep as sttopt_spd_inverse_band
+    (the same inverse, bit for bit) and leaves the positive-definiteness check in its
+    device word: 0 for SPD matrices, 1 when a pivot is not positive."""
+    from sttopt_amd import _lib
+
+    lib = _lib.load()
+    n, band, rows = 200, 21, 2
+    A = _banded_spd(n, band, rows, 3)
+    st, want = _spd_inverse_band(A, band, dev)
+    assert st == _lib.OK
+    for bad_row in (None, 1):
+        M = A.clone()
+        if bad_row is not None:
+            M[bad_row, 130, 130] = -1.0
+        out = M.to(dev).contiguous()
+        work = torch.empty(lib.sttopt_spd_inverse_work_count(n, rows), dtype=torch.float64,
+                           device=dev)
+        word = torch.full((1,), 7, dtype=torch.int32, device=dev)
+        st = lib.sttopt_spd_inverse_band_async(n, band, rows, out.data_ptr(), n * n,
+                                               work.data_ptr(), word.data_ptr(),
+                                               _lib.stream_handle(dev))
+        assert st == _lib.OK
+        if bad_row is None:
+            assert int(word.item()) == 0
+            assert torch.equal(out.cpu(), want.cpu())
+        else:
+            assert int(word.item()) == 1
+
+
+def test_solvers_raise_for_an_indefinite_coarsest_operator(dev):
+    """A hierarchy whose coarsest operator is not positive definite (negative stiffness)
+    raises torch.linalg.LinAlgError, as the reference's Cholesky factorization does: from
+    the MG-PCG solve (which reads the inverse's check with its first readback), from a
+    V-cycle application, and from the uneven batch solve."""
+    from sttopt_amd import torch_fem, torch_mg, uneven
+
+    nelx, nely = 24, 12
+    s = _setup(nelx, nely, dev)
+
+    def levels():
+        dens = torch.full((nelx * nely,), -1.0, dtype=torch.float64, device=dev)
+        return torch_mg.build_hierarchy(dens, s["edof_t"], s["KE_t"], s["mask_t"], nelx, nely,
+                                        max_coarse_elements=20)
+
+    b = torch_fem.project(s["F_t"], s["mask_t"])
+    with pytest.raises(torch.linalg.LinAlgError, match="not positive-definite"):
+        torch_mg.native_mgpcg(torch_mg.VCycle(levels(), dtype=torch.float32), b, rtol=1e-8,
+                              max_iter=50)
+    with pytest.raises(torch.linalg.LinAlgError, match="not positive-definite"):
+        torch_mg.VCycle(levels(), dtype=torch.float32)(b)
+    with pytest.raises(torch.linalg.LinAlgError, match="not positive-definite"):
+        uneven.batch_solve([torch_mg.VCycle(levels(), dtype=torch.float32)], [b[None]],
+                           rtol=1e-8, max_iter=50)
+
+
 @pytest.mark.parametrize("d, kx, ky", [(2, 2, 2), (2, 2, 1), (2, 1, 2), (1, 2, 2)])
 def test_param_diagonal_equals_the_diagonal_of_the_explicit_galerkin_matrices(dev, d, kx, ky):
     """sttopt_param_diagonal (the level's diagonal from the per-position tables and the
