@@ -36,6 +36,7 @@ from typing import Callable
 import numpy as np
 import torch
 from torch import Tensor
+from torch.utils.weak import WeakTensorKeyDictionary
 
 from . import _grid, _lib
 from . import torch_fem
@@ -553,6 +554,26 @@ def _param_diagonal(den64: Tensor, ptab: Tensor, cmask: Tensor, nelx: int, nely:
     return out
 
 
+# The coarse free-dof masks of a fine mask, per coarsening: (the fine mask's version,
+# the coarse mask).  Hierarchies built on one mask (every step of an optimization) share
+# them, and so their packed bits (`_grid.mask_bits`, cached per mask tensor).
+_COARSE_MASKS = WeakTensorKeyDictionary()
+
+
+def _coarse_mask(mask: Tensor, nelx: int, nely: int, kx: int, ky: int, d: int) -> Tensor:
+    """The free-dof mask of the kx x ky coarsening of a grid's mask: the nodes at every
+    kx-th column and ky-th row (torch_mg.py:376)."""
+    per = _COARSE_MASKS.get(mask)
+    key = (nelx, nely, kx, ky, d)
+    if per is not None and per[0] == mask._version and key in per[1]:
+        return per[1][key]
+    cmask = mask.reshape(nely + 1, nelx + 1, d)[::ky, ::kx].reshape(-1).contiguous()
+    if per is None or per[0] != mask._version:
+        per = _COARSE_MASKS[mask] = (mask._version, {})
+    per[1][key] = cmask
+    return cmask
+
+
 def build_hierarchy(
     density: Tensor,
     edofMat: Tensor,
@@ -665,7 +686,7 @@ def build_hierarchy(
             ptab = None  # the reaction term has no per-position form below level 1
         top.kx, top.ky = kx, ky
         cndof = d * (cnelx + 1) * (cnely + 1)
-        cmask = top.mask.reshape(top.nely + 1, top.nelx + 1, d)[::ky, ::kx].reshape(-1).contiguous()
+        cmask = _coarse_mask(top.mask, top.nelx, top.nely, kx, ky, d)
         # the first level of an fp32 cycle is applied from the level-0 densities
         # (PARAM_MAX_CHILDREN): its element matrices are formed only if something
         # reads them, and its diagonal comes from the per-position tables
