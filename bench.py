@@ -214,7 +214,8 @@ def step_bytes(p, info, rows):
       transfers            restrict 4 (n_l + n_l+1), prolongate 4 (2 n_l + n_l+1)
       coarsest             8 n_L^2 + 4 (2 n_L)                  dense inverse, b -> x
       post pass            8 n0 + 4 (3 n0 + n1 + e0)          r, x, e, 1/D, rho32 -> z
-      direction update     8 (4 n0) + 4 n0                    x, p, z -> x, p
+      direction update     8 (2 n0) + 4 n0 + 8 (3 n0) / 2     p, z -> p; x, p_prev -> x
+                           (the x updates of two iterations in one pass over x)
     Per solve: the first preconditioner application (the cycle without the residual
     update, 8 n0 less) and z -> p (12 n0), the cold start (24 n0); per step: the
     prologue 8 (2 e0 + n0 + R (e0 + n0)), the hierarchy (fp32 densities 12 e0, the
@@ -245,7 +246,8 @@ def step_bytes(p, info, rows):
 
     pre = 8 * 3 * n[0] + 4 * (4 * n[0] + e0)
     post = 8 * n[0] + 4 * (3 * n[0] + n[1] + e0)
-    per_iter = 8 * (2 * n[0] + e0) + pre + vcycle_below_0() + post + 8 * 4 * n[0] + 4 * n[0]
+    direction = 8 * 2 * n[0] + 4 * n[0] + 8 * 3 * n[0] // 2
+    per_iter = 8 * (2 * n[0] + e0) + pre + vcycle_below_0() + post + direction
     first = (pre - 8 * n[0]) + vcycle_below_0() + post + 12 * n[0] + 24 * n[0]
     build = 8 * (2 * e0 + n[0] + rows * (e0 + n[0]))          # prologue
     per_row_build = 12 * e0 + 8 * n[0]
@@ -468,10 +470,12 @@ def apply_roofline(torch, _lib, torch_fem, p, rows, dev, live, launches=50):
     """The fp64 K(rho)p apply of the outer PCG against its algorithmic bytes.  `live`:
     (seconds, bytes, launches) from HIP events around each of its launches in the timed
     steps -- the reported figure.  These are the launches from the second PCG iteration
-    on, which carry the direction update (p_new = z + beta p_old, x += alpha p_old): per
-    row still iterating, p_old, z (fp32), x and rho read and p_new, x and Ap written,
-    8 (5 ndof + nel) + 4 ndof bytes; per row that retired the iteration before, 24 ndof
-    (x += alpha p_old only).  Also timed in a separate loop on all rows: the plain apply
+    on, which carry the direction update (p_new = z + beta p_old) and, every other launch,
+    the x updates of two iterations (x += alpha_prev p_prev, then x += alpha p_old): per
+    row still iterating, p_old, z (fp32) and rho read and p_new and Ap written,
+    8 (3 ndof + nel) + 4 ndof bytes, 8 (6 ndof + nel) + 4 ndof in the launches that read
+    p_prev and read and write x; per row that retired the iteration before, 24 or 32 ndof
+    (its x update only).  Also timed in a separate loop on all rows: the plain apply
     (8 (2 ndof + nel) bytes per row) and the public operator() with its input
     projection."""
     args, keep, stream = _apply_args(torch, _lib, p, rows, dev, torch.float64, _lib.F64)

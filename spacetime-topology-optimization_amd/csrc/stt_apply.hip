@@ -58,7 +58,9 @@ struct StencilArgs {
   long long sbc;
   // kOperatorDir: u holds p_old; the staged (and applied) vector is p_new = z + beta p_old;
   // the tile's own nodes write p_new to dp and, for rows with iters == dk, take
-  // x += alpha p_old
+  // x += alpha p_old, preceded by x += alpha_prev p_prev when dpp is given (the update
+  // of iteration dk - 1, held back by the previous launch); with ddefer the rows still
+  // iterating hold theirs back instead (rows that retired in dk take it)
   double* dp;
   const void* dz;
   double* dx;
@@ -67,6 +69,9 @@ struct StencilArgs {
   const double* dbeta;
   const int* diters;
   int dk, dz_f32;
+  const double* dpp;
+  const double* dalpha_prev;
+  int ddefer;
   // kSmoothProlong: the coarse correction (row stride spe) on a grid of pcnnx nodes per row
   const T* pe;
   long long spe;
@@ -354,8 +359,11 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
   if (a.active != nullptr && a.active[row] == 0) {  // whole block: uniform
     if constexpr (kDir) {
       // a row that retired in iteration dk still takes that iteration's x += alpha p
+      // (and the held-back one of dk - 1)
       if (a.diters[row] == a.dk) {
         const double al = a.dalpha[row];
+        const bool two = a.dpp != nullptr;
+        const double alp = two ? a.dalpha_prev[row] : 0.0;
         double* xr = a.dx + row * a.sdx;
         const int gx = x0 + lane;
 #pragma unroll
@@ -365,8 +373,13 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
             const long long node = (long long)gy * a.nnx + gx;
             const Vec<double, D> pv = load_node<double, D>(u, node);
             Vec<double, D> xv = load_node<double, D>(xr, node);
+            if (two) {
+              const Vec<double, D> pq = load_node<double, D>(a.dpp + row * a.su, node);
 #pragma unroll
-            for (int q = 0; q < D; ++q) xv.v[q] += al * pv.v[q];
+              for (int q = 0; q < D; ++q) xv.v[q] = fma(alp, pq.v[q], xv.v[q]);
+            }
+#pragma unroll
+            for (int q = 0; q < D; ++q) xv.v[q] = fma(al, pv.v[q], xv.v[q]);
             store_node<double, D>(xr, node, xv);
           }
         }
@@ -378,7 +391,9 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
   // kOperatorDir: beta and alpha of this row, and whether it takes x += alpha p_old
   const double dbe = kDir ? a.dbeta[row] : 0.0;
   const double dal = kDir ? a.dalpha[row] : 0.0;
-  const bool dxup = kDir && a.diters[row] == a.dk;
+  const bool dxup = kDir && a.diters[row] == a.dk && !a.ddefer;
+  const bool dxtwo = dxup && a.dpp != nullptr;
+  const double dalp = dxtwo ? a.dalpha_prev[row] : 0.0;
   auto stage_node = [&](long long node, bool owned) -> Vec<T, D> {
     if constexpr (kDir) {
       // p_new = z + beta p_old, the expression of the unfused direction update
@@ -400,8 +415,13 @@ __device__ __forceinline__ void stencil_tile_body(const StencilArgs<T, TB, TY>& 
         if (dxup) {
           double* xr = a.dx + row * a.sdx;
           Vec<double, D> xv = load_node<double, D>(xr, node);
+          if (dxtwo) {
+            const Vec<double, D> pq = load_node<double, D>(a.dpp + row * a.su, node);
 #pragma unroll
-          for (int q = 0; q < D; ++q) xv.v[q] += dal * pv.v[q];
+            for (int q = 0; q < D; ++q) xv.v[q] = fma(dalp, pq.v[q], xv.v[q]);
+          }
+#pragma unroll
+          for (int q = 0; q < D; ++q) xv.v[q] = fma(dal, pv.v[q], xv.v[q]);
           store_node<double, D>(xr, node, xv);
         }
       }
@@ -1108,7 +1128,7 @@ int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
                  const void* z, int z_f32, double* x, long long xs, const double* alpha,
                  const double* beta, const int* iters, int k, double* Ap,
                  const int* active, double* partial, hipStream_t s, hipEvent_t start,
-                 hipEvent_t stop) {
+                 hipEvent_t stop, const double* p_prev, const double* alpha_prev, int defer) {
   if (rows <= 0 || op.nelx <= 0 || op.nely <= 0) return STTOPT_OK;
   const long long n = op.ndof();
   StencilArgs<double, double, double> a;
@@ -1125,7 +1145,12 @@ int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
               "alpha, beta, iteration counts and active flags");
     return STTOPT_EINVAL;
   }
+  if ((p_prev != nullptr) != (alpha_prev != nullptr) || (defer && p_prev != nullptr)) {
+    set_error("operator_dir: the held-back update needs p_prev and alpha_prev, and no deferral");
+    return STTOPT_EINVAL;
+  }
   if (op.d == 2 && (!pair_aligned(p_new, n, sizeof(double)) ||
+                    !pair_aligned(p_prev, n, sizeof(double)) ||
                     !pair_aligned(x, xs, sizeof(double)) ||
                     !pair_aligned(z, n, z_f32 ? sizeof(float) : sizeof(double)))) {
     set_error("operator_dir: d = 2 vectors need node-aligned pointers and even row strides");
@@ -1141,6 +1166,9 @@ int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
   a.dbeta = beta;
   a.diters = iters;
   a.dk = k;
+  a.dpp = p_prev;
+  a.dalpha_prev = alpha_prev;
+  a.ddefer = defer ? 1 : 0;
   const dim3 grid((unsigned)a.total), block(kBlock);
   auto launch = [&](auto kernel) {
     if (start != nullptr || stop != nullptr)

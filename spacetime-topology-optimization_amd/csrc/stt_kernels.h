@@ -105,12 +105,18 @@ int stencil(const OpDesc& op, int rows, int mode, bool proj_in, int dot, const T
 // p_new at free dofs and the per-tile p_new . Ap partials.  p_old and p_new are
 // different buffers (each tile reads p_old on its halo), all rows with stride n.
 // start / stop (optional): events that the launch itself records at the kernel's start
-// and end (no marker of their own in the stream).
+// and end (no marker of their own in the stream).  The x updates of two iterations can
+// share one pass over x: with defer, the rows still iterating hold iteration k's update
+// back (the rows that retired in k take it); the next launch passes p_prev = that p_old
+// and alpha_prev = its alpha, and takes x += alpha_prev p_prev, then x += alpha p_old,
+// as two fused multiply-adds, the same values as one update per launch.
 int operator_dir(const OpDesc& op, int rows, const double* p_old, double* p_new,
                  const void* z, int z_f32, double* x, long long xs, const double* alpha,
                  const double* beta, const int* iters, int k, double* Ap,
                  const int* active, double* partial, hipStream_t s,
-                 hipEvent_t start = nullptr, hipEvent_t stop = nullptr);
+                 hipEvent_t start = nullptr, hipEvent_t stop = nullptr,
+                 const double* p_prev = nullptr, const double* alpha_prev = nullptr,
+                 int defer = 0);
 
 // One damped-Jacobi sweep y = xp + w D^-1 (b - A xp) from xp = x + P e (P: the bilinear
 // 2 x 2 prolongation of the coarse correction e, added at the free dofs, the

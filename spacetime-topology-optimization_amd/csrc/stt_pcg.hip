@@ -27,7 +27,7 @@ struct Scratch {
   double* part[5];  // [rows][tiles] each: pAp, rr, rz, rzo, bb
   double* bsafe;    // [rows]
   double* rz;       // [rows]
-  double* alpha;    // [rows]
+  double* alpha;    // [2][rows]: alpha of iteration k in slot k & 1
   double* beta;     // [rows]
   int* active;      // [rows]
   int* counter;     // [1]
@@ -40,7 +40,7 @@ static long long align256(long long b) { return (b + 255) & ~255LL; }
 static long long scratch_bytes(int rows, int tiles) {
   long long b = 0;
   b += 5 * align256(8LL * rows * tiles);
-  b += 4 * align256(8LL * rows);
+  b += 3 * align256(8LL * rows) + align256(16LL * rows);
   b += align256(4LL * rows);
   b += align256(16);
   b += align256(4LL * (bottom_sync_words(rows) + 1));
@@ -66,7 +66,7 @@ static Scratch carve(void* base, int rows, int tiles) {
   s.rz = reinterpret_cast<double*>(p);
   p += align256(8LL * rows);
   s.alpha = reinterpret_cast<double*>(p);
-  p += align256(8LL * rows);
+  p += align256(16LL * rows);
   s.beta = reinterpret_cast<double*>(p);
   p += align256(8LL * rows);
   s.active = reinterpret_cast<int*>(p);
@@ -156,13 +156,16 @@ __global__ __launch_bounds__(kBlock) void pcg_update_kernel(
 }
 
 // For the rows that took iteration k (iters == k; k = *kdev when kdev is given): x +=
-// alpha p, then, for those still active and when update_p, p = z + beta p.
+// alpha p (after x += alpha_prev p_prev when p_prev is given: the update of iteration
+// k - 1 that the fused direction update held back), then, for those still active and
+// when update_p, p = z + beta p.
 template <int D, typename ZT>
 __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
     TileGeom g, long long n, const double* __restrict__ alpha, const double* __restrict__ beta,
     double* __restrict__ x, long long xs, double* __restrict__ p, const ZT* __restrict__ z,
     const int* __restrict__ active, const int* __restrict__ iters, int k, int update_p,
-    const int* __restrict__ kdev) {
+    const int* __restrict__ kdev, const double* __restrict__ alpha_prev,
+    const double* __restrict__ p_prev) {
   int row, tt, c, r0;
   tile_origin(g, row, tt, c, r0);
   if (kdev != nullptr) k = *kdev;  // the iteration fin_alpha advanced it to
@@ -170,6 +173,7 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
   const bool dir = update_p && active[row] != 0;
   const double a = alpha[row];
   const double b = dir ? beta[row] : 0.0;
+  const double ap = p_prev != nullptr ? alpha_prev[row] : 0.0;
   double* xr = x + row * xs;  // the caller's solution rows (stride xs)
   double* pr = p + row * n;
   const ZT* zr = z + row * n;
@@ -179,8 +183,13 @@ __global__ __launch_bounds__(kBlock) void pcg_pupdate_kernel(
       const long long node = (long long)(r0 + j) * g.nnx + c;
       Vec<double, D> pv = load_node<double, D>(pr, node);
       Vec<double, D> xv = load_node<double, D>(xr, node);
+      if (p_prev != nullptr) {
+        const Vec<double, D> pq = load_node<double, D>(p_prev + row * n, node);
 #pragma unroll
-      for (int q = 0; q < D; ++q) xv.v[q] += a * pv.v[q];
+        for (int q = 0; q < D; ++q) xv.v[q] = fma(ap, pq.v[q], xv.v[q]);
+      }
+#pragma unroll
+      for (int q = 0; q < D; ++q) xv.v[q] = fma(a, pv.v[q], xv.v[q]);
       store_node<double, D>(xr, node, xv);
       if (dir) {
         const Vec<ZT, D> zv = load_node<ZT, D>(zr, node);
@@ -1052,9 +1061,12 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   double* p = work + 2 * rows * n;
   double* Ap = work + 3 * rows * n;
   double* r2 = work + 4 * rows * n;  // the other residual buffer of the fused update
-  // the search direction of iteration k is in pbuf[k & 1]: the fused direction update
-  // (operator_dir) reads p_k on every tile's halo while writing p_{k+1}
-  double* pbuf[2] = {work + 5 * rows * n, p};
+  // the search direction of iteration k is in pbuf[k % 3]: the fused direction update
+  // (operator_dir) reads p_k on every tile's halo while writing p_{k+1}, and every other
+  // launch also reads p_{k-1}, whose x update the launch before held back (one pass over
+  // x per two iterations); alpha of iteration k is in slot k & 1
+  double* pbuf[3] = {work + 5 * rows * n, p, work + 6 * rows * n};
+  auto alpha_of = [&](int k) { return sc.alpha + (k & 1) * rows; };
   TileGeom g{op.nnx(), op.nny(), op.ntx(), tiles, tiles * rows, mg->d};
   const dim3 tblock(kBlock), rgrid(rows);
   *n_iter = 0;
@@ -1161,22 +1173,27 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
         t1 = g_apply_prof.pool[2 * pi + 1];
         g_apply_prof.iter_of.push_back(k);
       }
-      if (int e = operator_dir(op, rows, pbuf[(k - 1) & 1], pbuf[k & 1], z,
-                               sizeof(T) == sizeof(float), x, xs, sc.alpha, sc.beta, iters,
-                               k - 1, Ap, sc.active, sc.part[0], s, t0, t1))
+      // iteration k - 1's x update: held back when k - 1 is odd, taken with the held-back
+      // one of k - 2 when it is even
+      const bool two = ((k - 1) & 1) == 0;
+      if (int e = operator_dir(op, rows, pbuf[(k - 1) % 3], pbuf[k % 3], z,
+                               sizeof(T) == sizeof(float), x, xs, alpha_of(k - 1), sc.beta, iters,
+                               k - 1, Ap, sc.active, sc.part[0], s, t0, t1,
+                               two ? pbuf[(k - 2) % 3] : nullptr,
+                               two ? alpha_of(k - 2) : nullptr, two ? 0 : 1))
         return e;
     }
     hipLaunchKernelGGL(fin_alpha_kernel, rgrid, tblock, 0, s, tiles, sc.part[0], sc.active,
-                       sc.rz, sc.alpha, kdev);
+                       sc.rz, alpha_of(k), kdev);
     const bool fuse_now = fused && k < max_iter;
     const int parity = fused ? (k & 1) : 0;
     double* rcur = fused ? rbuf[(k - 1) & 1] : rbuf[0];  // the residual entering iteration k
     if (fuse_now) {
-      if (int e = vc.pair_pre_update(rcur, Ap, sc.alpha, rbuf[k & 1], sc.part[1],
+      if (int e = vc.pair_pre_update(rcur, Ap, alpha_of(k), rbuf[k & 1], sc.part[1],
                                      flexible ? z : static_cast<T*>(nullptr), sc.part[3]))
         return e;
     } else {
-      STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, sc.alpha, rcur, Ap,
+      STT_TILE_LAUNCH(pcg_update_kernel, T, g, s, n, alpha_of(k), rcur, Ap,
                       flexible ? z : static_cast<T*>(nullptr), sc.active, sc.part[1],
                       sc.part[3]);
     }
@@ -1235,9 +1252,13 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   }
   if (k_last > 0) {
     // the last iteration's x += alpha p (no direction update follows it)
-    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, sc.alpha, sc.beta, x, xs,
-                    pbuf[k_last & 1], z, sc.active, iters, k_last, 0,
-                    static_cast<const int*>(nullptr));
+    // (after the one of k_last - 1 when the last fused apply held that back)
+    const bool two = k_last >= 2 && (k_last & 1) == 0;
+    STT_TILE_LAUNCH(pcg_pupdate_kernel, T, g, s, n, alpha_of(k_last), sc.beta, x, xs,
+                    pbuf[k_last % 3], z, sc.active, iters, k_last, 0,
+                    static_cast<const int*>(nullptr),
+                    two ? static_cast<const double*>(alpha_of(k_last - 1)) : nullptr,
+                    two ? static_cast<const double*>(pbuf[(k_last - 1) % 3]) : nullptr);
     STT_CHECK_LAUNCH("pcg last x update");
   }
   // n_iter: the slowest row's count (and whether a wait of the persistent bottom pass
@@ -1260,17 +1281,20 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   for (int i = 0; i < rows; ++i) mx = h[i] > mx ? h[i] : mx;
   if (g_apply_prof.on && !g_apply_prof.iter_of.empty()) {
     // the fused launches (iteration 2 on; iteration 1's plain apply is not counted):
-    // algorithmic bytes per row still iterating: p_old, z, x and rho read, p_new, x and Ap
-    // written; per row that retired in the iteration before: p_old and x read, x written
+    // algorithmic bytes per row still iterating: p_old, z and rho read, p_new and Ap
+    // written, and in the launches that take two iterations' x updates (k - 1 even) also
+    // x and p_prev read and x written; per row that retired in the iteration before:
+    // p_old (and p_prev) and x read, x written
     const double nd = (double)n, ne = (double)op.nel();
-    const double fused = 8.0 * (5.0 * nd + ne) + (double)sizeof(T) * nd;
-    const double retiring = 8.0 * 3.0 * nd;
     for (size_t i = 0; i < g_apply_prof.iter_of.size(); ++i) {
       float ms = 0.0f;
       e = hipEventElapsedTime(&ms, g_apply_prof.pool[2 * i], g_apply_prof.pool[2 * i + 1]);
       if (e != hipSuccess) return hip_status(e, "hipEventElapsedTime");
       const int k = g_apply_prof.iter_of[i];
       if (k == 1) continue;
+      const bool two = ((k - 1) & 1) == 0;
+      const double fused = 8.0 * ((two ? 6.0 : 3.0) * nd + ne) + (double)sizeof(T) * nd;
+      const double retiring = 8.0 * (two ? 4.0 : 3.0) * nd;
       int act = 0, ret = 0;
       for (int r = 0; r < rows; ++r) {
         act += h[r] >= k ? 1 : 0;
@@ -1418,7 +1442,7 @@ __global__ __launch_bounds__(kBlock) void batch_pupdate_kernel(PupdateGroups gs)
       Vec<double, D> pv = load_node<double, D>(pr, node);
       Vec<double, D> xv = load_node<double, D>(xr, node);
 #pragma unroll
-      for (int q = 0; q < D; ++q) xv.v[q] += a * pv.v[q];
+      for (int q = 0; q < D; ++q) xv.v[q] = fma(a, pv.v[q], xv.v[q]);
       store_node<double, D>(xr, node, xv);
       if (dir) {
         const Vec<float, D> zv = load_node<float, D>(zr, node);
@@ -1933,7 +1957,8 @@ static int mgpcg_batch_impl(const sttopt_batch_group* in, int ng, double rtol, i
   if (k_last > 0) {
     for (auto& g : G) {
       STT_TILE_LAUNCH(pcg_pupdate_kernel, float, g.g, s, g.n, g.sc.alpha, g.sc.beta, g.x, g.xs,
-                      g.p, g.z, g.sc.active, g.iters, k_last, 0, static_cast<const int*>(nullptr));
+                      g.p, g.z, g.sc.active, g.iters, k_last, 0, static_cast<const int*>(nullptr),
+                      static_cast<const double*>(nullptr), static_cast<const double*>(nullptr));
       STT_CHECK_LAUNCH("batch last x update");
     }
   }
@@ -1973,9 +1998,9 @@ int64_t sttopt_mg_scratch_bytes(const sttopt_mg* mg) {
 int64_t sttopt_mgpcg_work_count(const sttopt_mg* mg) {
   if (mg == nullptr || mg->nlevels < 1) return 0;
   const long long n = (long long)(mg->levels[0].nelx + 1) * (mg->levels[0].nely + 1) * mg->d;
-  // r, z, p, Ap, the second residual buffer of the fused update and the second search
-  // direction buffer of the fused direction update
-  return 6LL * mg->rows * n;
+  // r, z, p, Ap, the second residual buffer of the fused update and the second and third
+  // search direction buffers of the fused direction update
+  return 7LL * mg->rows * n;
 }
 
 int sttopt_mgpcg_graphs(int32_t enable) {

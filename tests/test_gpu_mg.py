@@ -939,6 +939,69 @@ def test_spd_inverse_reports_indefinite(dev):
     assert st == _lib.ENOTSPD and "not positive definite" in _lib.last_error()
 
 
+def test_mgpcg_x_updates_equal_one_update_per_iteration(dev):
+    """sttopt_mgpcg's fused direction update takes the x updates of two iterations in one
+    pass over x (the odd iteration's held back, then both as two fused multiply-adds);
+    the uneven batch solver takes one per iteration.  Both give the same x bit for bit
+    for solves stopped after 1 to 6 iterations (the last update held back or not, rows
+    still iterating) and run to convergence (rows retiring at iterations of either
+    parity), with the same iteration counts."""
+    import ctypes
+
+    from sttopt_amd import _lib, torch_fem, torch_mg, uneven
+
+    nelx, nely, rows = 96, 48, 4
+    s = _setup(nelx, nely, dev)
+    n = s["ndof"]
+    rng = np.random.default_rng(71)
+    x = torch.as_tensor(np.stack([binary_design(nelx, nely)] +
+                                 [rng.uniform(0.05, 1, nelx * nely) for _ in range(rows - 1)]),
+                        dtype=torch.float64, device=dev)
+    levels = torch_mg.build_hierarchy(torch_fem.simp_density(x, EMIN, EMAX, PENAL), s["edof_t"],
+                                      s["KE_t"], s["mask_t"], nelx, nely,
+                                      max_coarse_elements=48)
+    vc = torch_mg.VCycle(levels, dtype=torch.float32)
+    F = torch_fem.project(torch.as_tensor(rng.standard_normal((rows, n)), device=dev),
+                          s["mask_t"])
+    lib = _lib.load()
+    stream = _lib.stream_handle(dev)
+
+    def single(max_iter):
+        mg, _, work = vc._native(rows)
+        xo = torch.empty(rows, n, dtype=torch.float64, device=dev)
+        iters = torch.zeros(rows, dtype=torch.int32, device=dev)
+        resid = torch.zeros(rows, dtype=torch.float64, device=dev)
+        st = lib.sttopt_mgpcg(ctypes.byref(mg), F.data_ptr(), n, xo.data_ptr(), n, 1e-10,
+                              max_iter, 1, torch_mg.CHECK_EVERY, 1, work.data_ptr(),
+                              iters.data_ptr(), resid.data_ptr(),
+                              ctypes.byref(ctypes.c_int32(0)), stream)
+        assert st in (_lib.OK, _lib.ENOCONV), _lib.last_error()
+        return st, xo.clone(), iters.tolist()
+
+    def batched(max_iter):
+        mg, _, work = vc._native(rows)
+        g = (uneven.BatchGroup * 1)()
+        xo = torch.empty(rows, n, dtype=torch.float64, device=dev)
+        iters = torch.zeros(rows, dtype=torch.int32, device=dev)
+        resid = torch.zeros(rows, dtype=torch.float64, device=dev)
+        g[0].mg = ctypes.addressof(mg)
+        g[0].b, g[0].b_stride = F.data_ptr(), n
+        g[0].x, g[0].x_stride = xo.data_ptr(), n
+        g[0].work = work.data_ptr()
+        g[0].iters, g[0].resid = iters.data_ptr(), resid.data_ptr()
+        st = lib.sttopt_mgpcg_batch(ctypes.addressof(g), 1, 1e-10, max_iter,
+                                    torch_mg.CHECK_EVERY, 1, ctypes.byref(ctypes.c_int32(0)),
+                                    stream)
+        assert st in (_lib.OK, _lib.ENOCONV), _lib.last_error()
+        return st, xo.clone(), iters.tolist()
+
+    for max_iter in (1, 2, 3, 4, 5, 6, 300):
+        a, b = single(max_iter), batched(max_iter)
+        assert a[0] == b[0] and a[2] == b[2], (max_iter, a[2], b[2])
+        assert torch.equal(a[1], b[1]), (max_iter, float((a[1] - b[1]).abs().max()))
+    assert a[0] == _lib.OK and len(set(a[2])) > 1  # converged, rows at different counts
+
+
 def test_spd_inverse_band_async_equals_the_waiting_call(dev):
     """sttopt_spd_inverse_band_async queues the same sweep as sttopt_spd_inverse_band
     (the same inverse, bit for bit) and leaves the positive-definiteness check in its
