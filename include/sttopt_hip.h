@@ -184,8 +184,14 @@ typedef struct sttopt_mg {
   /* device scratch sized by sttopt_mg_scratch_bytes                              */
   void* scratch;
   /* optional device word of sttopt_spd_inverse_band_async for the coarsest inverse:
-   * sttopt_mgpcg and sttopt_mgpcg_batch return STTOPT_ENOTSPD when it is nonzero  */
+   * sttopt_mgpcg and sttopt_mgpcg_batch read it with their final readback and return
+   * STTOPT_ENOTSPD when it is nonzero                                              */
   const int32_t* cinv_status;
+  /* optional hipEvent_t recorded after the coarsest inverse's launches when they were
+   * queued on another stream: every V-cycle makes its stream wait for it before its
+   * first use of the inverse, so the level passes above the coarsest run beside the
+   * inverse.  Null: the inverse is ordered on the solve's stream already.          */
+  void* cinv_ready;
 } sttopt_mg;
 
 /* Upper-triangle copy of fp64 FULL element matrices, (A + A^T) / 2, in dtype (V-cycle
@@ -285,7 +291,7 @@ int sttopt_spd_inverse_band(int32_t n, int32_t band, int32_t rows, double* a,
 /* sttopt_spd_inverse_band without the wait for its result: the launches are queued on
  * the stream and the device word *not_spd (device memory) receives 1 when a matrix is
  * not positive definite, else 0.  The caller reads it, or hands it to the solvers in
- * sttopt_mg.cinv_status, which report STTOPT_ENOTSPD at their first readback.  Lets the
+ * sttopt_mg.cinv_status, which report STTOPT_ENOTSPD with their final readback.  Lets the
  * host queue the rest of a hierarchy build and the solve while the inverse runs.
  * Replaces the same reference step (torch_mg.py:269-290: the factorization's error is
  * raised before the solve returns). */

@@ -96,6 +96,7 @@ class MG(ctypes.Structure):
         ("levels", Level * MAX_LEVELS),
         ("scratch", c_void_p),
         ("cinv_status", c_void_p),
+        ("cinv_ready", c_void_p),
     ]
 
 

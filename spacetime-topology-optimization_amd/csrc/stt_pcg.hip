@@ -294,17 +294,13 @@ __device__ __forceinline__ double safe_div(double num, double den) {
 }
 
 // Initial residual: b norm, relative residual, active flags.
-// counter[2] receives the coarsest inverse's status word (0 without one), which the host
-// reads with the active-row count.
 __global__ __launch_bounds__(kBlock) void fin_init_kernel(int tiles, const double* pbb,
                                                           const double* prr, double rtol,
                                                           double* bsafe, double* resid,
                                                           int* active, int* iters,
-                                                          int* counter,
-                                                          const int* cinv_status) {
+                                                          int* counter) {
   __shared__ double red[kBlock / kWave];
   const int row = blockIdx.x;
-  if (row == 0 && threadIdx.x == 0) counter[2] = cinv_status != nullptr ? *cinv_status : 0;
   const double bb = row_sum(pbb, tiles, row, red);
   __syncthreads();
   const double rr = row_sum(prr, tiles, row, red);
@@ -408,12 +404,16 @@ struct VCycle {
   unsigned* bsync = nullptr;
   int* berr = nullptr;
   bool bottom_used = false;
+  // the coarsest inverse's completion event (queued on another stream), waited for once
+  // before the first launch that reads the inverse; null once waited for
+  hipEvent_t cinv_ready = nullptr;
   int rows;
   OpDesc op[STTOPT_MAX_LEVELS];
   long long n[STTOPT_MAX_LEVELS];
 
   VCycle(const sttopt_mg* m, hipStream_t st, const int* act) : mg(m), s(st), active(act) {
     rows = m->rows;
+    cinv_ready = static_cast<hipEvent_t>(m->cinv_ready);
     for (int l = 0; l < m->nlevels; ++l) {
       const sttopt_level& L = m->levels[l];
       OpDesc& o = op[l];
@@ -456,6 +456,16 @@ struct VCycle {
   }
   // reduction partials per row of the r.z sums written by run()
   int rz_tiles() const { return pair() ? pair_tiles(op[0]) : op[0].tiles(); }
+
+  // Orders the inverse's launches before what s launches next (the first cycle runs on
+  // the solve's own stream, outside any graph capture, so later cycles need no wait).
+  int wait_cinv() {
+    if (cinv_ready == nullptr) return STTOPT_OK;
+    hipError_t e = hipStreamWaitEvent(s, cinv_ready, 0);
+    if (e != hipSuccess) return hip_status(e, "hipStreamWaitEvent(coarsest inverse)");
+    cinv_ready = nullptr;
+    return STTOPT_OK;
+  }
 
   const T* invd(int l) const { return static_cast<const T*>(mg->levels[l].invdiag); }
   long long sd(int l) const { return mg->levels[l].invdiag_stride; }
@@ -545,6 +555,7 @@ struct VCycle {
         c.cmask = C.mask;
         c.cx = buf(C.vx);
         c.scx = n[last];
+        if (int e = wait_cinv()) return e;
         if (int e = vcycle_bottom(c, bsync, berr, s)) return e;
         bottom_used = true;
         *out = buf(L.vx2);
@@ -553,6 +564,7 @@ struct VCycle {
     }
     if (l == mg->nlevels - 1) {
       T* x = buf(L.vx);
+      if (int e = wait_cinv()) return e;
       if (int e = coarse_solve<T>(nl, rows, L.cinv, L.cinv_stride, b, nl, L.mask, x, nl,
                                   active, s))
         return e;
@@ -724,6 +736,7 @@ struct VCycle {
     const long long nl = n[0];
     const int ns = mg->n_smooth;
     if (mg->nlevels == 1) {
+      if (int e = wait_cinv()) return e;
       if constexpr (sizeof(ZT) == sizeof(double)) {
         if (int e = coarse_solve<double>(nl, rows, L.cinv, L.cinv_stride, r, rs, L.mask, z, zs,
                                          active, s))
@@ -925,24 +938,50 @@ int read_device_int(const int* dev, hipStream_t s, int* out) {
   return STTOPT_OK;
 }
 
-// The active-row count and the coarsest inverse's status after fin_init, in one
-// readback (words 8 and 10 of the pinned block; the lagged checks use words 1 and 2).
+// The active-row count after fin_init (word 8 of the pinned block; the lagged checks use
+// words 1 and 2).
 static int read_init(const int* counter, hipStream_t s, int* remaining) {
   int* h = pinned_counter();
   if (h == nullptr) {
     set_error("hipHostMalloc of the readback word failed");
     return STTOPT_EHIP;
   }
-  hipError_t e = hipMemcpyAsync(h + 8, counter, 3 * sizeof(int), hipMemcpyDeviceToHost, s);
-  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(readback words)");
+  hipError_t e = hipMemcpyAsync(h + 8, counter, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(readback word)");
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
   *remaining = h[8];
-  if (h[10] != 0) {
-    set_error("sttopt_mgpcg: the coarsest operator is not positive definite");
-    return STTOPT_ENOTSPD;
-  }
   return STTOPT_OK;
+}
+
+// Queues the read of the coarsest inverse's status word into *out (0 without one), after
+// the stream has waited for the inverse; *out is valid once the stream is synchronized.
+static int queue_cinv_status(const sttopt_mg* mg, hipStream_t s, int* out) {
+  *out = 0;
+  if (mg->cinv_status == nullptr) return STTOPT_OK;
+  hipError_t e = hipSuccess;
+  if (mg->cinv_ready != nullptr) {
+    e = hipStreamWaitEvent(s, static_cast<hipEvent_t>(mg->cinv_ready), 0);
+    if (e != hipSuccess) return hip_status(e, "hipStreamWaitEvent(coarsest inverse)");
+  }
+  e = hipMemcpyAsync(out, mg->cinv_status, sizeof(int), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(coarsest inverse status)");
+  return STTOPT_OK;
+}
+
+static int cinv_status_error(int status) {
+  if (status == 0) return STTOPT_OK;
+  set_error("sttopt_mgpcg: the coarsest operator is not positive definite");
+  return STTOPT_ENOTSPD;
+}
+
+// The status alone (a solve with no row to iterate).
+static int read_cinv_status(const sttopt_mg* mg, hipStream_t s) {
+  int st = 0;
+  if (int e = queue_cinv_status(mg, s, &st)) return e;
+  hipError_t e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+  return cinv_status_error(st);
 }
 
 // Optional HIP-event timing of the outer iteration's K(rho)p launches (process-wide, off
@@ -1089,11 +1128,11 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   if (int e = fill<int>(2, 1, 0, sc.counter, 2, nullptr, s)) return e;
   hipLaunchKernelGGL(fin_init_kernel, rgrid, tblock, 0, s, tiles, sc.part[4],
                      cold ? sc.part[4] : sc.part[1], rtol, sc.bsafe, resid, sc.active, iters,
-                     sc.counter, mg->cinv_status);
+                     sc.counter);
   STT_CHECK_LAUNCH("pcg init");
   int remaining = 0;
   if (int e = read_init(sc.counter, s, &remaining)) return e;
-  if (remaining == 0) return STTOPT_OK;
+  if (remaining == 0) return read_cinv_status(mg, s);
 
   VCycle<T> vc(mg, s, sc.active);
   if (g_bottom_levels > 0) {
@@ -1264,15 +1303,17 @@ static int mgpcg_impl(const sttopt_mg* mg, const double* b, long long bs, double
   // n_iter: the slowest row's count (and whether a wait of the persistent bottom pass
   // gave up, read with it)
   std::vector<int> h(rows > 0 ? rows : 1);
-  int bottom_err = 0;
+  int bottom_err = 0, cinv_st = 0;
   hipError_t e = hipMemcpyAsync(h.data(), iters, sizeof(int) * rows, hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(iters)");
+  if (int ec = queue_cinv_status(mg, s, &cinv_st)) return ec;
   if (vc.bottom_used) {
     e = hipMemcpyAsync(&bottom_err, sc.bottom_err, sizeof(int), hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(bottom error)");
   }
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+  if (int ec = cinv_status_error(cinv_st)) return ec;
   if (bottom_err != 0) {
     set_error("sttopt_mgpcg: a wait of the persistent V-cycle bottom pass gave up");
     return STTOPT_EHIP;
@@ -1813,7 +1854,7 @@ static int mgpcg_batch_impl(const sttopt_batch_group* in, int ng, double rtol, i
     if (int e = fill<int>(2, 1, 0, g.sc.counter, 2, nullptr, s)) return e;
     hipLaunchKernelGGL(fin_init_kernel, dim3(g.rows), tblock, 0, s, g.tiles, g.sc.part[4],
                        cold ? g.sc.part[4] : g.sc.part[1], rtol, g.sc.bsafe, g.resid,
-                       g.sc.active, g.iters, g.sc.counter, g.mg->cinv_status);
+                       g.sc.active, g.iters, g.sc.counter);
     STT_CHECK_LAUNCH("batch init");
   }
   for (auto& g : G) {
@@ -1821,7 +1862,11 @@ static int mgpcg_batch_impl(const sttopt_batch_group* in, int ng, double rtol, i
     if (int e = read_init(g.sc.counter, s, &rem)) return e;
     remaining += rem;
   }
-  if (remaining == 0) return STTOPT_OK;
+  if (remaining == 0) {
+    for (auto& g : G)
+      if (int e = read_cinv_status(g.mg, s)) return e;
+    return STTOPT_OK;
+  }
   for (auto& g : G) {
     if (int e = g.vc->run(g.rbuf[0], g.n, g.z, g.n, g.sc.part[2], g.Ap)) return e;
     hipLaunchKernelGGL(fin_rz_kernel, dim3(g.rows), tblock, 0, s, g.vc->rz_tiles(),
@@ -1965,11 +2010,14 @@ static int mgpcg_batch_impl(const sttopt_batch_group* in, int ng, double rtol, i
   int mx = 0;
   for (auto& g : G) {
     std::vector<int> h(g.rows);
+    int cinv_st = 0;
     hipError_t e = hipMemcpyAsync(h.data(), g.iters, sizeof(int) * g.rows,
                                   hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return hip_status(e, "hipMemcpyAsync(iters)");
+    if (int ec = queue_cinv_status(g.mg, s, &cinv_st)) return ec;
     e = hipStreamSynchronize(s);
     if (e != hipSuccess) return hip_status(e, "hipStreamSynchronize");
+    if (int ec = cinv_status_error(cinv_st)) return ec;
     for (int v : h) mx = std::max(mx, v);
   }
   *n_iter = done ? mx : max_iter;

@@ -50,6 +50,13 @@ VCYCLE_DTYPE = torch.float32
 #: PCG iterations between host reads of the active-row count.
 CHECK_EVERY = 1
 
+#: Queue the coarsest inverse on a second stream of the device, so that what the host
+#: queues after it (the rest of the hierarchy's set-up and the solve's first V-cycle down
+#: to the coarsest level) runs beside it; every user of the inverse waits for it first.
+#: Same results either way.
+CINV_SIDE_STREAM = True
+_SIDE_STREAMS: dict = {}
+
 #: Coarse levels whose elements cover at most this many level-0 elements are applied
 #: in the V-cycle from the level-0 densities and fixed per-position matrices
 #: (keff_E = sum_g rho_g G_pos(g), exact by linearity of the Galerkin product) instead
@@ -297,6 +304,9 @@ class _Level:
     _cinv: Tensor | None = field(default=None, repr=False)  # (rows, n, n) fp64
     # the inverse's not-positive-definite word on the device, until it has been read
     _cinv_status: Tensor | None = field(default=None, repr=False)
+    # the event recorded after the inverse's launches on the side stream, until a user
+    # of the inverse has made its stream wait for it
+    _cinv_ready: object = field(default=None, repr=False)
     _dbatch: tuple = field(default=(), repr=False)
     _copies: dict = field(default_factory=dict, repr=False)
     # per-position matrices of the level-0 elements under one element of this level
@@ -345,10 +355,19 @@ class _Level:
     def bits(self) -> Tensor:
         return _grid.mask_bits(self.mask)
 
+    def cinv_wait(self) -> None:
+        """Make the current stream wait for the coarsest inverse when it was queued on
+        the side stream (`_dense_factor`)."""
+        ready = self._cinv_ready
+        if ready is not None:
+            torch.cuda.current_stream(self.mask.device).wait_event(ready)
+            self._cinv_ready = None
+
     def spd_check(self) -> None:
         """Raise the coarsest factorization's error if its inverse, queued without a wait
         (`_dense_factor`), found a matrix that is not positive definite; reads the device
         word once."""
+        self.cinv_wait()
         status = self._cinv_status
         if status is None:
             return
@@ -390,6 +409,7 @@ class _Level:
             dbatch = (int(rows.numel()),)
             if soa is not None:
                 soa = soa[rows]
+        self.cinv_wait()
         cinv = _select_rows(self._cinv, rows, 2) if self._cinv is not None and len(self._dbatch) == 1 else self._cinv
         k = 4 * self.d
         return dataclasses.replace(
@@ -491,6 +511,15 @@ _NOT_SPD = ("linalg.cholesky: The factorization could not be completed because t
             "not positive-definite (the coarsest multigrid operator).")
 
 
+def _side_stream(dev: torch.device) -> torch.cuda.Stream:
+    """The second stream of `dev` (default priority) that `_dense_factor` queues on."""
+    idx = dev.index if dev.index is not None else torch.cuda.current_device()
+    side = _SIDE_STREAMS.get(idx)
+    if side is None:
+        side = _SIDE_STREAMS[idx] = torch.cuda.Stream(device=torch.device("cuda", idx))
+    return side
+
+
 def _dense_factor(level: _Level, rows: int) -> None:
     """Inverse of the coarsest level's dense projected operator (torch_mg._dense_cholesky,
     torch_mg.py:269-290; the inverse replaces the factor's two triangular solves per
@@ -499,9 +528,13 @@ def _dense_factor(level: _Level, rows: int) -> None:
 
     The inverse is queued without a wait for its positive-definiteness check, so that the
     host goes on queueing the build and the solve while it runs: the solvers read the
-    check with their first readback (`sttopt_mg.cinv_status`) and every other user of the
+    check with their final readback (`sttopt_mg.cinv_status`) and every other user of the
     inverse calls `spd_check` first.  A matrix that is not positive definite raises
     torch.linalg.LinAlgError (as the reference's `torch.linalg.cholesky`) from there.
+    With `CINV_SIDE_STREAM` the inverse runs on a second stream of the device (its sweep
+    steps are bound by one workgroup each and leave most of the device free): the solvers'
+    V-cycles wait for its event (`sttopt_mg.cinv_ready`) before their first coarsest
+    solve, and the other users through `cinv_wait`.
     """
     dev = level.mask.device
     n = level.ndof
@@ -513,10 +546,20 @@ def _dense_factor(level: _Level, rows: int) -> None:
     # the operator couples a node with its eight grid neighbours only: half-bandwidth
     # d (nnx + 1) + d - 1 in the node-major dof numbering
     band = level.d * (level.nelx + 2) + level.d - 1
+    side = _side_stream(dev) if CINV_SIDE_STREAM else None
+    if side is not None:
+        side.wait_stream(torch.cuda.current_stream(dev))
+    stream = side.cuda_stream if side is not None else _lib.stream_handle(dev)
     _lib.check(lib.sttopt_spd_inverse_band_async(n, band, rows, inv.data_ptr(), n * n,
-                                                 work.data_ptr(), status.data_ptr(),
-                                                 _lib.stream_handle(dev)),
+                                                 work.data_ptr(), status.data_ptr(), stream),
                "sttopt_spd_inverse")
+    level._cinv_ready = None
+    if side is not None:
+        for t in (inv, work, status):  # allocated on the current stream, used on side
+            t.record_stream(side)
+        ready = torch.cuda.Event()
+        ready.record(side)
+        level._cinv_ready = ready
     level._cinv = inv
     level._cinv_status = status
     level._dense = None
@@ -809,9 +852,12 @@ class VCycle:
             if lv._cinv is not None:
                 L.cinv = lv._cinv.data_ptr()
                 L.cinv_stride = 0 if shared else lv._cinv.stride(0)
-                if lv._cinv_status is not None:  # read by the solvers' first readback
+                if lv._cinv_status is not None:  # read by the solvers' final readback
                     keep.append(lv._cinv_status)
                     mg.cinv_status = lv._cinv_status.data_ptr()
+                if lv._cinv_ready is not None:  # waited for before the first coarsest solve
+                    keep.append(lv._cinv_ready)
+                    mg.cinv_ready = lv._cinv_ready.cuda_event
             names = ["vb", "vx", "vx2", "vr"] + (["ve", "vw"] if self.gamma > 1 else [])
             for name in names:
                 buf = torch.empty(rows, lv.ndof, dtype=vd, device=dev)

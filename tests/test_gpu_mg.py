@@ -939,6 +939,52 @@ def test_spd_inverse_reports_indefinite(dev):
     assert st == _lib.ENOTSPD and "not positive definite" in _lib.last_error()
 
 
+def test_coarsest_inverse_on_the_side_stream_gives_the_same_results(dev):
+    """The coarsest inverse queued on a second stream (the V-cycle's passes above the
+    coarsest level running beside it, the first coarsest solve waiting for its event)
+    gives the solvers and the stand-alone V-cycle the same results bit for bit as the
+    inverse queued on the solve's stream: sttopt_mgpcg to convergence, the uneven batch
+    solver, and a V-cycle applied right after the build (each on a freshly built
+    hierarchy, so each first use of the inverse follows its launches)."""
+    from sttopt_amd import torch_fem, torch_mg, uneven
+
+    nelx, nely, rows = 128, 64, 3
+    s = _setup(nelx, nely, dev)
+    n = s["ndof"]
+    rng = np.random.default_rng(83)
+    x = torch.as_tensor(np.stack([binary_design(nelx, nely)] +
+                                 [rng.uniform(0.05, 1, nelx * nely) for _ in range(rows - 1)]),
+                        dtype=torch.float64, device=dev)
+    rho = torch_fem.simp_density(x, EMIN, EMAX, PENAL)
+    F = torch_fem.project(torch.as_tensor(rng.standard_normal((rows, n)), device=dev),
+                          s["mask_t"])
+
+    def fresh():
+        levels = torch_mg.build_hierarchy(rho, s["edof_t"], s["KE_t"], s["mask_t"], nelx,
+                                          nely)
+        return torch_mg.VCycle(levels, dtype=torch.float32)
+
+    def run():
+        info = {}
+        u, it = torch_mg.native_mgpcg(fresh(), F, rtol=1e-10, max_iter=400, info=info)
+        (ub,) = uneven.batch_solve([fresh()], [F], rtol=1e-10, max_iter=400)
+        z = fresh()(F)
+        torch.cuda.synchronize(dev)
+        return u.cpu(), it, info["row_iter_list"], ub.cpu(), z.cpu()
+
+    prev = torch_mg.CINV_SIDE_STREAM
+    try:
+        torch_mg.CINV_SIDE_STREAM = False
+        want = run()
+        torch_mg.CINV_SIDE_STREAM = True
+        got = run()
+    finally:
+        torch_mg.CINV_SIDE_STREAM = prev
+    assert want[1] == got[1] and want[2] == got[2]
+    for a, b in zip(want[:1] + want[3:], got[:1] + got[3:]):
+        assert torch.equal(a, b)
+
+
 def test_mgpcg_x_updates_equal_one_update_per_iteration(dev):
     """sttopt_mgpcg's fused direction update takes the x updates of two iterations in one
     pass over x (the odd iteration's held back, then both as two fused multiply-adds);
