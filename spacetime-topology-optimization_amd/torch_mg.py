@@ -549,14 +549,16 @@ def _dense_factor(level: _Level, rows: int) -> None:
     side = _side_stream(dev) if CINV_SIDE_STREAM else None
     if side is not None:
         side.wait_stream(torch.cuda.current_stream(dev))
+        # allocated on the current stream, used on side: marked before anything is
+        # queued there, so that no block returns to the allocator while side may use it
+        for t in (inv, work, status):
+            t.record_stream(side)
     stream = side.cuda_stream if side is not None else _lib.stream_handle(dev)
     _lib.check(lib.sttopt_spd_inverse_band_async(n, band, rows, inv.data_ptr(), n * n,
                                                  work.data_ptr(), status.data_ptr(), stream),
                "sttopt_spd_inverse")
     level._cinv_ready = None
     if side is not None:
-        for t in (inv, work, status):  # allocated on the current stream, used on side
-            t.record_stream(side)
         ready = torch.cuda.Event()
         ready.record(side)
         level._cinv_ready = ready
